@@ -1,0 +1,808 @@
+// Host orchestration of the energy/force pipeline and the C ABI (include/nnp_capi.h).
+//
+// Pipeline for a batch of structures (all kernels cover every atom of every structure):
+//   1. neighbour list      k_bin_atoms / k_scan_excl / k_fill_bins / k_neigh_cells (large cells)
+//                          k_neigh_allpairs (small cells), k_partner
+//   2. descriptors         k_symm<T, NE, NC>                 -> G
+//   3. networks            k_net_f64 | k_net_mixed per species -> E_i, dE/dG
+//   4. force terms         k_force<T, NE, NC>                 -> X (per neighbour entry)
+//   5. force assembly      k_assemble                          -> F (FP64)
+//   6. energy              per structure, sum of E_i in atom order (FP64), as
+//                          NNP::energy accumulates (nnp.cpp:394-400)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/nnp_capi.h"
+#include "device.hpp"
+#include "engine.hpp"
+#include "model.hpp"
+
+namespace atomnn {
+
+// launchers (symm.hip, net.hip, neigh.hip kernels)
+int grid_shape(int ne, int nc);
+size_t symm_lds(int shape, bool fp64, int maxp, int ldg);
+size_t force_lds(int shape, bool fp64, int maxp, int ldg);
+size_t net_lds_bytes(const DevNet& n, bool fp64);
+template <typename T>
+hipError_t launch_symm(int shape, const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
+                       const int* centres, int ncentres, const int* cnt, const int* meta, const double4* r, int maxp,
+                       T* G, int ldg, hipStream_t st);
+template <typename T>
+hipError_t launch_force(int shape, const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
+                        const int* centres, int ncentres, const int* cnt, const int* meta, const double4* r, int maxp,
+                        const T* dEdG, int ldg, T* X, hipStream_t st);
+template <typename T>
+hipError_t launch_assemble(const int* cnt, const int* partner, const T* X, int maxp, int ntot, double* F,
+                           hipStream_t st);
+hipError_t launch_net_f64(const DevPlan* plan, const DevNet& hn, const double* Wb, int s, const int* atoms, int M,
+                          const double* G, int ldg, int vac, double* eatom, double* dEdG, hipStream_t st);
+hipError_t launch_net_mixed(const DevPlan* plan, const DevNet& hn, const float* Wf, int s, const int* atoms, int M,
+                            const float* G, int ldg, int vac, double* eatom, float* dEdG, hipStream_t st);
+hipError_t launch_neighbors(const NeighborArgs& a, hipStream_t st);
+
+// Total energy per structure: one block per structure, fixed-order tree sum of E_i (FP64).
+__global__ __launch_bounds__(256) void k_energy(const double* __restrict__ eatom, const int* __restrict__ sstart,
+                                                double* __restrict__ E) {
+    __shared__ double part[256];
+    const int s = blockIdx.x, t = threadIdx.x;
+    const int a0 = sstart[s], a1 = sstart[s + 1];
+    const int n = a1 - a0, chunk = (n + 255) / 256;
+    double acc = 0.0;
+    for (int i = a0 + t * chunk; i < a0 + min(n, (t + 1) * chunk); ++i) acc += eatom[i];
+    part[t] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (t < o) part[t] += part[t + o];
+        __syncthreads();
+    }
+    if (t == 0) E[s] = part[0];
+}
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string g_err;
+
+void set_error(const std::string& s) { g_err = s; }
+const char* last_error() { return g_err.c_str(); }
+
+struct hip_failure : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define HIPCHK(expr)                                                                                          \
+    do {                                                                                                      \
+        hipError_t e_ = (expr);                                                                               \
+        if (e_ != hipSuccess)                                                                                 \
+            throw hip_failure(std::string("HIP error in ") + #expr + ": " + hipGetErrorString(e_));            \
+    } while (0)
+
+// ------------------------------------------------------------------ device buffers
+void DevMem::reserve(size_t bytes) {
+    if (bytes <= cap) return;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = std::max<size_t>(bytes, 256);
+    want = (want + 4095) & ~size_t(4095);
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+        p = nullptr;
+        throw std::bad_alloc();
+    }
+    cap = want;
+}
+void DevMem::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+}
+
+// ------------------------------------------------------------------ structures
+// Cell::init (cell.cpp:45-70) and the image table of NeighborList::build (neighbor.cpp:67-95)
+DevStruc make_struc(const nnp_structure& s, double rc, int atom0) {
+    DevStruc d;
+    std::memset(&d, 0, sizeof(d));
+    d.atom0 = atom0;
+    d.natoms = s.natoms;
+    double R[3][3] = {{0}};
+    if (s.cell)
+        for (int c = 0; c < 3; ++c)
+            for (int r = 0; r < 3; ++r) R[r][c] = s.cell[3 * c + r];
+    double fro = 0;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) fro += R[r][c] * R[r][c];
+    fro = std::sqrt(fro);
+    d.fro = fro;
+    d.periodic = fro > 0;
+    d.nimg = 1;
+    if (!d.periodic) return d;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) d.R[3 * r + c] = R[r][c];
+    const double c00 = R[1][1] * R[2][2] - R[1][2] * R[2][1];
+    const double c01 = R[1][2] * R[2][0] - R[1][0] * R[2][2];
+    const double c02 = R[1][0] * R[2][1] - R[1][1] * R[2][0];
+    const double det = R[0][0] * c00 + R[0][1] * c01 + R[0][2] * c02;
+    if (det == 0) throw std::invalid_argument("Cell::init: singular lattice");
+    const double id = 1.0 / det;
+    double Ri[3][3];
+    Ri[0][0] = c00 * id;
+    Ri[1][0] = c01 * id;
+    Ri[2][0] = c02 * id;
+    Ri[0][1] = (R[0][2] * R[2][1] - R[0][1] * R[2][2]) * id;
+    Ri[1][1] = (R[0][0] * R[2][2] - R[0][2] * R[2][0]) * id;
+    Ri[2][1] = (R[0][1] * R[2][0] - R[0][0] * R[2][1]) * id;
+    Ri[0][2] = (R[0][1] * R[1][2] - R[0][2] * R[1][1]) * id;
+    Ri[1][2] = (R[0][2] * R[1][0] - R[0][0] * R[1][2]) * id;
+    Ri[2][2] = (R[0][0] * R[1][1] - R[0][1] * R[1][0]) * id;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) d.RInv[3 * r + c] = Ri[r][c];
+    auto nrm = [](const double* v) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); };
+    double col[3][3];
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) col[c][r] = R[r][c];
+    double dmax = nrm(col[0]);
+    for (int c = 1; c < 3; ++c) dmax = std::min(dmax, nrm(col[c]));
+    dmax *= 0.5;
+    d.dMax2 = dmax * dmax;
+    // lattice vectors sorted by descending norm, stable (neighbor.cpp:71-74)
+    int perm[3] = {0, 1, 2};
+    for (int i = 1; i < 3; ++i) {
+        const int p = perm[i];
+        int j = i;
+        while (j > 0 && nrm(col[p]) > nrm(col[perm[j - 1]])) {
+            perm[j] = perm[j - 1];
+            --j;
+        }
+        perm[j] = p;
+    }
+    double Rv[3][3];
+    for (int q = 0; q < 3; ++q)
+        for (int r = 0; r < 3; ++r) Rv[q][r] = col[perm[q]][r];
+    for (int q = 0; q < 3; ++q)
+        for (int r = 0; r < 3; ++r) d.Rv[3 * q + r] = Rv[q][r];
+    auto cross = [](const double* a, const double* b, double* o) {
+        o[0] = a[1] * b[2] - a[2] * b[1];
+        o[1] = a[2] * b[0] - a[0] * b[2];
+        o[2] = a[0] * b[1] - a[1] * b[0];
+    };
+    double x01[3];
+    cross(Rv[0], Rv[1], x01);
+    d.shell[0] = (int)std::floor(2.0 * rc / nrm(Rv[0]));
+    d.shell[1] = (int)std::floor(2.0 * rc / (nrm(x01) / nrm(Rv[0])));
+    d.shell[2] = (int)std::floor(2.0 * rc / std::fabs((Rv[2][0] * x01[0] + Rv[2][1] * x01[1] + Rv[2][2] * x01[2]) /
+                                                      (nrm(Rv[0]) * nrm(Rv[1]))));
+    d.nimg = (2 * d.shell[0] + 1) * (2 * d.shell[1] + 1) * (2 * d.shell[2] + 1);
+    // cell list when every shell is 0 and the perpendicular heights hold at least one rc bin
+    if (d.shell[0] == 0 && d.shell[1] == 0 && d.shell[2] == 0 && s.natoms >= 256) {
+        const double vol = std::fabs(det);
+        bool ok = true;
+        for (int a = 0; a < 3; ++a) {
+            double xc[3];
+            cross(col[(a + 1) % 3], col[(a + 2) % 3], xc);
+            const double h = vol / nrm(xc);
+            d.nbin[a] = (int)std::floor(h / rc);
+            if (d.nbin[a] < 1) ok = false;
+        }
+        if (ok) d.mode = 1;
+    }
+    return d;
+}
+
+// ------------------------------------------------------------------ engine
+Engine::Engine(HostModel&& hm) : host(std::move(hm)) {
+    pb = build_plan(host);
+    shape = grid_shape(pb.plan.ne_max, pb.plan.nc_max);
+    HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    d_plan.reserve(sizeof(DevPlan));
+    HIPCHK(hipMemcpy(d_plan.p, &pb.plan, sizeof(DevPlan), hipMemcpyHostToDevice));
+    d_rfe.reserve(pb.rfun_eta.size() * 8);
+    d_rfr.reserve(pb.rfun_rs.size() * 8);
+    HIPCHK(hipMemcpy(d_rfe.p, pb.rfun_eta.data(), pb.rfun_eta.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_rfr.p, pb.rfun_rs.data(), pb.rfun_rs.size() * 8, hipMemcpyHostToDevice));
+    d_w64.reserve(pb.weights.size() * 8);
+    HIPCHK(hipMemcpy(d_w64.p, pb.weights.data(), pb.weights.size() * 8, hipMemcpyHostToDevice));
+    std::vector<float> wf(pb.weights.begin(), pb.weights.end());
+    d_w32.reserve(wf.size() * 4);
+    HIPCHK(hipMemcpy(d_w32.p, wf.data(), wf.size() * 4, hipMemcpyHostToDevice));
+}
+
+Engine::~Engine() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (int st = 0; st < ST_COUNT; ++st)
+        for (int q = 0; q < 2; ++q)
+            if (ev[st][q]) (void)hipEventDestroy(ev[st][q]);
+    d_E.release();
+    d_sstart.release();
+    for (DevMem* m : {&d_plan, &d_rfe, &d_rfr, &d_w64, &d_w32, &d_pos, &d_type, &d_astruc, &d_strucs, &d_centres,
+                      &d_cnt, &d_nbrj, &d_meta, &d_nbrr, &d_partner, &d_maxcnt, &d_abin, &d_bcount, &d_bstart,
+                      &d_bfill, &d_batoms, &d_G, &d_dEdG, &d_X, &d_eatom, &d_F, &d_spatoms})
+        m->release();
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+// Estimated entries per atom from the density, plus head room; refined after the first build.
+static int estimate_maxp(const std::vector<nnp_structure>& ss, double rc) {
+    double worst = 0;
+    for (const auto& s : ss) {
+        double rho = 0;
+        if (s.cell) {
+            const double* a = s.cell;
+            const double vol = std::fabs(a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+                                         a[2] * (a[3] * a[7] - a[4] * a[6]));
+            rho = vol > 0 ? s.natoms / vol : 0;
+        }
+        const double p = rho > 0 ? rho * 4.0 / 3.0 * M_PI * rc * rc * rc : (double)s.natoms;
+        worst = std::max(worst, p);
+    }
+    int m = (int)std::ceil(worst * 1.5 + 16);
+    m = (m + 31) / 32 * 32;
+    return std::max(64, std::min(m, 4096));
+}
+
+void Engine::upload(const std::vector<nnp_structure>& ss) {
+    resident_valid = false;
+    nstruc = (int)ss.size();
+    h_natoms.clear();
+    for (const auto& s : ss) h_natoms.push_back(s.natoms);
+    ntot = 0;
+    for (const auto& s : ss) {
+        if (s.natoms <= 0) throw std::invalid_argument("natoms must be positive");
+        if (!s.pos || !s.type) throw std::invalid_argument("structure without positions or types");
+        ntot += s.natoms;
+    }
+    h_pos.resize(3 * (size_t)ntot);
+    h_type.resize(ntot);
+    h_astruc.resize(ntot);
+    h_strucs.clear();
+    h_cell_centres.clear();
+    h_pair_centres.clear();
+    int a0 = 0, bins = 0;
+    for (int k = 0; k < nstruc; ++k) {
+        const auto& s = ss[k];
+        DevStruc d = make_struc(s, host.rc, a0);
+        if (d.mode == 1) {
+            d.bin0 = bins;
+            bins += d.nbin[0] * d.nbin[1] * d.nbin[2];
+        }
+        h_strucs.push_back(d);
+        std::memcpy(&h_pos[3 * (size_t)a0], s.pos, 3 * sizeof(double) * s.natoms);
+        for (int i = 0; i < s.natoms; ++i) {
+            const int t = s.type[i];
+            if (t < 0 || t >= host.T) throw std::invalid_argument("atom type out of range");
+            h_type[a0 + i] = t;
+            h_astruc[a0 + i] = k;
+            (d.mode == 1 ? h_cell_centres : h_pair_centres).push_back(a0 + i);
+        }
+        a0 += s.natoms;
+    }
+    nbins = bins;
+    // per-species atom lists for the network GEMMs
+    sp_off.assign(host.T + 1, 0);
+    for (int i = 0; i < ntot; ++i) sp_off[h_type[i] + 1]++;
+    for (int t = 0; t < host.T; ++t) sp_off[t + 1] += sp_off[t];
+    h_spatoms.resize(ntot);
+    {
+        std::vector<int> fill(sp_off.begin(), sp_off.end() - 1);
+        for (int i = 0; i < ntot; ++i) h_spatoms[fill[h_type[i]]++] = i;
+    }
+    h_centres = h_cell_centres;
+    h_centres.insert(h_centres.end(), h_pair_centres.begin(), h_pair_centres.end());
+    d_pos.reserve(h_pos.size() * 8);
+    d_type.reserve((size_t)ntot * 4);
+    d_astruc.reserve((size_t)ntot * 4);
+    d_strucs.reserve(h_strucs.size() * sizeof(DevStruc));
+    d_centres.reserve((size_t)ntot * 4);
+    d_spatoms.reserve((size_t)ntot * 4);
+    HIPCHK(hipMemcpyAsync(d_pos.p, h_pos.data(), h_pos.size() * 8, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(d_type.p, h_type.data(), (size_t)ntot * 4, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(d_astruc.p, h_astruc.data(), (size_t)ntot * 4, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(d_strucs.p, h_strucs.data(), h_strucs.size() * sizeof(DevStruc), hipMemcpyHostToDevice,
+                          stream));
+    HIPCHK(hipMemcpyAsync(d_centres.p, h_centres.data(), (size_t)ntot * 4, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(d_spatoms.p, h_spatoms.data(), (size_t)ntot * 4, hipMemcpyHostToDevice, stream));
+    if (maxp == 0) maxp = estimate_maxp(ss, host.rc);
+}
+
+void Engine::build_neighbors(bool all_images) {
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        const size_t ent = (size_t)ntot * maxp;
+        if (ent >= (size_t)1 << 31) throw std::invalid_argument("batch too large for 32-bit entry indices");
+        d_cnt.reserve((size_t)ntot * 4);
+        d_nbrj.reserve(ent * 4);
+        d_meta.reserve(ent * 4);
+        d_nbrr.reserve(ent * sizeof(double4));
+        d_partner.reserve(ent * 4);
+        d_maxcnt.reserve(64);
+        HIPCHK(hipMemsetAsync(d_maxcnt.p, 0, 4, stream));
+        NeighborArgs a;
+        a.S = (const DevStruc*)d_strucs.p;
+        a.atom_struc = (const int*)d_astruc.p;
+        a.pos = (const double*)d_pos.p;
+        a.type = (const int*)d_type.p;
+        a.ntot = ntot;
+        a.rc2 = host.rc * host.rc;
+        a.maxp = maxp;
+        a.cnt = (int*)d_cnt.p;
+        a.nbr_j = (int*)d_nbrj.p;
+        a.nbr_meta = (int*)d_meta.p;
+        a.nbr_r = (double4*)d_nbrr.p;
+        a.partner = (int*)d_partner.p;
+        a.maxcnt = (int*)d_maxcnt.p;
+        a.cell_centres = (const int*)d_centres.p;
+        a.ncell = (int)h_cell_centres.size();
+        a.pair_centres = (const int*)d_centres.p + h_cell_centres.size();
+        a.npair = (int)h_pair_centres.size();
+        a.nbins = nbins;
+        a.all_images = all_images ? 1 : 0;
+        if (nbins > 0) {
+            d_abin.reserve((size_t)ntot * 4);
+            d_bcount.reserve((size_t)nbins * 4);
+            d_bstart.reserve((size_t)nbins * 4);
+            d_bfill.reserve((size_t)nbins * 4);
+            d_batoms.reserve((size_t)ntot * 4);
+            a.atom_bin = (int*)d_abin.p;
+            a.bin_count = (int*)d_bcount.p;
+            a.bin_start = (int*)d_bstart.p;
+            a.bin_fill = (int*)d_bfill.p;
+            a.bin_atoms = (int*)d_batoms.p;
+        }
+        HIPCHK(launch_neighbors(a, stream));
+        int mc = 0;
+        HIPCHK(hipMemcpyAsync(&mc, d_maxcnt.p, 4, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        max_entries = mc;
+        if (mc <= maxp) return;
+        maxp = (mc + 31) / 32 * 32;
+    }
+    throw std::runtime_error("neighbour list did not converge");
+}
+
+template <typename T>
+void Engine::descriptors() {
+    const size_t lds = symm_lds(shape, sizeof(T) == 8, maxp, pb.plan.ldg);
+    if (lds > 65536) throw unsupported_error("too many neighbours per atom for the descriptor kernel's LDS");
+    d_G.reserve((size_t)ntot * pb.plan.ldg * sizeof(T));
+    HIPCHK(launch_symm<T>(shape, (const DevPlan*)d_plan.p, (const double*)d_rfe.p, (const double*)d_rfr.p,
+                          (const int*)d_type.p, (const int*)d_centres.p, ntot, (const int*)d_cnt.p,
+                          (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp, (T*)d_G.p, pb.plan.ldg, stream));
+}
+
+void Engine::networks(bool mixed, bool vac) {
+    d_eatom.reserve((size_t)ntot * 8);
+    d_dEdG.reserve((size_t)ntot * pb.plan.ldg * (mixed ? 4 : 8));
+    for (int s = 0; s < host.T; ++s) {
+        const int M = sp_off[s + 1] - sp_off[s];
+        const int* atoms = (const int*)d_spatoms.p + sp_off[s];
+        if (mixed)
+            HIPCHK(launch_net_mixed((const DevPlan*)d_plan.p, pb.plan.net[s], (const float*)d_w32.p, s, atoms, M,
+                                    (const float*)d_G.p, pb.plan.ldg, vac ? 1 : 0, (double*)d_eatom.p,
+                                    (float*)d_dEdG.p, stream));
+        else
+            HIPCHK(launch_net_f64((const DevPlan*)d_plan.p, pb.plan.net[s], (const double*)d_w64.p, s, atoms, M,
+                                  (const double*)d_G.p, pb.plan.ldg, vac ? 1 : 0, (double*)d_eatom.p,
+                                  (double*)d_dEdG.p, stream));
+    }
+}
+
+template <typename T>
+void Engine::forces() {
+    const size_t lds = force_lds(shape, sizeof(T) == 8, maxp, pb.plan.ldg);
+    if (lds > 65536) throw unsupported_error("too many neighbours per atom for the force kernel's LDS");
+    d_X.reserve((size_t)ntot * maxp * 4 * sizeof(T));
+    d_F.reserve((size_t)ntot * 3 * 8);
+    HIPCHK(launch_force<T>(shape, (const DevPlan*)d_plan.p, (const double*)d_rfe.p, (const double*)d_rfr.p,
+                           (const int*)d_type.p, (const int*)d_centres.p, ntot, (const int*)d_cnt.p,
+                           (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp, (const T*)d_dEdG.p, pb.plan.ldg,
+                           (T*)d_X.p, stream));
+    HIPCHK(launch_assemble<T>((const int*)d_cnt.p, (const int*)d_partner.p, (const T*)d_X.p, maxp, ntot,
+                              (double*)d_F.p, stream));
+}
+
+void Engine::run(const std::vector<nnp_structure>& ss, uint32_t flags) {
+    const bool mixed = flags & NNP_PREC_MIXED;
+    upload(ss);
+    build_neighbors(flags & NNP_ALL_IMAGES);
+    if (mixed) descriptors<float>();
+    else descriptors<double>();
+    networks(mixed, flags & NNP_VACUUM_ENERGY);
+    if (flags & NNP_FORCE) {
+        if (mixed) forces<float>();
+        else forces<double>();
+    }
+}
+
+void Engine::stage_begin(int s) {
+    if (!profiling) return;
+    if (!ev[s][0]) {
+        HIPCHK(hipEventCreate(&ev[s][0]));
+        HIPCHK(hipEventCreate(&ev[s][1]));
+    }
+    HIPCHK(hipEventRecord(ev[s][0], stream));
+}
+void Engine::stage_end(int s) {
+    if (!profiling) return;
+    HIPCHK(hipEventRecord(ev[s][1], stream));
+    pending[s] = true;
+}
+void Engine::profile_collect() {
+    for (int s = 0; s < ST_COUNT; ++s) {
+        if (!pending[s]) continue;
+        HIPCHK(hipEventSynchronize(ev[s][1]));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, ev[s][0], ev[s][1]));
+        stage_ms[s] += ms;
+        stage_calls[s] += 1;
+        pending[s] = false;
+    }
+}
+
+void Engine::energies() {
+    d_E.reserve((size_t)nstruc * 8);
+    hipLaunchKernelGGL(k_energy, dim3(nstruc), dim3(256), 0, stream, (const double*)d_eatom.p,
+                       (const int*)d_sstart.p, (double*)d_E.p);
+    HIPCHK(hipGetLastError());
+}
+
+// Upload a batch and size the neighbour rows once (with head room), so later passes run
+// without a host round trip.
+void Engine::resident_load(const std::vector<nnp_structure>& ss, uint32_t flags) {
+    upload(ss);
+    build_neighbors(flags & NNP_ALL_IMAGES);
+    const int want = (max_entries + max_entries / 8 + 8 + 31) / 32 * 32;
+    if (want > maxp) {
+        maxp = want;
+        build_neighbors(flags & NNP_ALL_IMAGES);
+    }
+    std::vector<int> st(nstruc + 1, 0);
+    for (int k = 0; k < nstruc; ++k) st[k + 1] = st[k] + ss[k].natoms;
+    d_sstart.reserve(st.size() * 4);
+    HIPCHK(hipMemcpyAsync(d_sstart.p, st.data(), st.size() * 4, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    resident_valid = true;
+}
+
+// One full pass on the resident batch, enqueued on the engine's stream (no host sync).
+void Engine::resident_run(uint32_t flags) {
+    const bool mixed = flags & NNP_PREC_MIXED;
+    NeighborArgs a;
+    a.S = (const DevStruc*)d_strucs.p;
+    a.atom_struc = (const int*)d_astruc.p;
+    a.pos = (const double*)d_pos.p;
+    a.type = (const int*)d_type.p;
+    a.ntot = ntot;
+    a.rc2 = host.rc * host.rc;
+    a.maxp = maxp;
+    a.cnt = (int*)d_cnt.p;
+    a.nbr_j = (int*)d_nbrj.p;
+    a.nbr_meta = (int*)d_meta.p;
+    a.nbr_r = (double4*)d_nbrr.p;
+    a.partner = (int*)d_partner.p;
+    a.maxcnt = (int*)d_maxcnt.p;
+    a.cell_centres = (const int*)d_centres.p;
+    a.ncell = (int)h_cell_centres.size();
+    a.pair_centres = (const int*)d_centres.p + h_cell_centres.size();
+    a.npair = (int)h_pair_centres.size();
+    a.nbins = nbins;
+    a.all_images = (flags & NNP_ALL_IMAGES) ? 1 : 0;
+    a.atom_bin = (int*)d_abin.p;
+    a.bin_count = (int*)d_bcount.p;
+    a.bin_start = (int*)d_bstart.p;
+    a.bin_fill = (int*)d_bfill.p;
+    a.bin_atoms = (int*)d_batoms.p;
+    stage_begin(ST_NEIGH);
+    HIPCHK(hipMemsetAsync(d_maxcnt.p, 0, 4, stream));
+    HIPCHK(launch_neighbors(a, stream));
+    stage_end(ST_NEIGH);
+    stage_begin(ST_SYMM);
+    if (mixed) descriptors<float>();
+    else descriptors<double>();
+    stage_end(ST_SYMM);
+    stage_begin(ST_NET);
+    networks(mixed, flags & NNP_VACUUM_ENERGY);
+    stage_end(ST_NET);
+    if (flags & NNP_FORCE) {
+        stage_begin(ST_FORCE);
+        const size_t lds = force_lds(shape, !mixed, maxp, pb.plan.ldg);
+        if (lds > 65536) throw unsupported_error("too many neighbours per atom for the force kernel's LDS");
+        d_X.reserve((size_t)ntot * maxp * 4 * (mixed ? 4 : 8));
+        d_F.reserve((size_t)ntot * 3 * 8);
+        if (mixed)
+            HIPCHK(launch_force<float>(shape, (const DevPlan*)d_plan.p, (const double*)d_rfe.p, (const double*)d_rfr.p,
+                                       (const int*)d_type.p, (const int*)d_centres.p, ntot, (const int*)d_cnt.p,
+                                       (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp,
+                                       (const float*)d_dEdG.p, pb.plan.ldg, (float*)d_X.p, stream));
+        else
+            HIPCHK(launch_force<double>(shape, (const DevPlan*)d_plan.p, (const double*)d_rfe.p,
+                                        (const double*)d_rfr.p, (const int*)d_type.p, (const int*)d_centres.p, ntot,
+                                        (const int*)d_cnt.p, (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp,
+                                        (const double*)d_dEdG.p, pb.plan.ldg, (double*)d_X.p, stream));
+        stage_end(ST_FORCE);
+        stage_begin(ST_ASSEMBLE);
+        if (mixed)
+            HIPCHK(launch_assemble<float>((const int*)d_cnt.p, (const int*)d_partner.p, (const float*)d_X.p, maxp,
+                                          ntot, (double*)d_F.p, stream));
+        else
+            HIPCHK(launch_assemble<double>((const int*)d_cnt.p, (const int*)d_partner.p, (const double*)d_X.p, maxp,
+                                           ntot, (double*)d_F.p, stream));
+        stage_end(ST_ASSEMBLE);
+    }
+    stage_begin(ST_ENERGY);
+    energies();
+    stage_end(ST_ENERGY);
+}
+
+void Engine::resident_fetch(uint32_t flags, nnp_result* res) {
+    int mc = 0;
+    HIPCHK(hipMemcpyAsync(&mc, d_maxcnt.p, 4, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    profile_collect();
+    if (mc > maxp) {  // the rows overflowed: rebuild them larger and redo the pass
+        maxp = (mc + mc / 8 + 8 + 31) / 32 * 32;
+        build_neighbors(flags & NNP_ALL_IMAGES);
+        resident_run(flags);
+        HIPCHK(hipStreamSynchronize(stream));
+        profile_collect();
+    }
+    h_E.resize(nstruc);
+    HIPCHK(hipMemcpyAsync(h_E.data(), d_E.p, (size_t)nstruc * 8, hipMemcpyDeviceToHost, stream));
+    h_eatom.resize(ntot);
+    HIPCHK(hipMemcpyAsync(h_eatom.data(), d_eatom.p, (size_t)ntot * 8, hipMemcpyDeviceToHost, stream));
+    if (flags & NNP_FORCE) {
+        h_F.resize(3 * (size_t)ntot);
+        HIPCHK(hipMemcpyAsync(h_F.data(), d_F.p, (size_t)ntot * 24, hipMemcpyDeviceToHost, stream));
+    }
+    HIPCHK(hipStreamSynchronize(stream));
+    int a0 = 0;
+    for (int k = 0; k < nstruc; ++k) {
+        const int n = h_natoms[k];
+        res[k].energy = h_E[k];
+        if (res[k].energy_atom) std::memcpy(res[k].energy_atom, &h_eatom[a0], (size_t)n * 8);
+        if ((flags & NNP_FORCE) && res[k].force) std::memcpy(res[k].force, &h_F[3 * (size_t)a0], (size_t)n * 24);
+        a0 += n;
+    }
+}
+
+void Engine::fetch(const std::vector<nnp_structure>& ss, uint32_t flags, nnp_result* res) {
+    h_eatom.resize(ntot);
+    HIPCHK(hipMemcpyAsync(h_eatom.data(), d_eatom.p, (size_t)ntot * 8, hipMemcpyDeviceToHost, stream));
+    if (flags & NNP_FORCE) {
+        h_F.resize(3 * (size_t)ntot);
+        HIPCHK(hipMemcpyAsync(h_F.data(), d_F.p, (size_t)ntot * 24, hipMemcpyDeviceToHost, stream));
+    }
+    HIPCHK(hipStreamSynchronize(stream));
+    int a0 = 0;
+    for (size_t k = 0; k < ss.size(); ++k) {
+        const int n = ss[k].natoms;
+        double e = 0;
+        for (int i = 0; i < n; ++i) e += h_eatom[a0 + i];  // nnp.cpp:394-399, atom order
+        res[k].energy = e;
+        if (res[k].energy_atom) std::memcpy(res[k].energy_atom, &h_eatom[a0], (size_t)n * 8);
+        if ((flags & NNP_FORCE) && res[k].force) std::memcpy(res[k].force, &h_F[3 * (size_t)a0], (size_t)n * 24);
+        a0 += n;
+    }
+}
+
+}  // namespace atomnn
+
+using namespace atomnn;
+
+struct nnp_model {
+    Engine* eng;
+};
+
+template <typename F>
+static int guarded(F&& f) {
+    try {
+        set_error("");
+        return f();
+    } catch (const io_error& e) {
+        set_error(e.what());
+        return NNP_EIO;
+    } catch (const unsupported_error& e) {
+        set_error(e.what());
+        return NNP_EUNSUPPORTED;
+    } catch (const hip_failure& e) {
+        set_error(e.what());
+        return NNP_EDEVICE;
+    } catch (const std::bad_alloc&) {
+        set_error("out of memory");
+        return NNP_ENOMEM;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return NNP_EINVAL;
+    }
+}
+
+extern "C" {
+
+const char* nnp_last_error(void) { return last_error(); }
+
+int nnp_load(const char* path, nnp_model** out) {
+    return guarded([&]() {
+        if (!path || !out) throw std::invalid_argument("nnp_load: null argument");
+        *out = nullptr;
+        HostModel hm = parse_ann(path);
+        auto* m = new nnp_model{nullptr};
+        try {
+            m->eng = new Engine(std::move(hm));
+        } catch (...) {
+            delete m;
+            throw;
+        }
+        *out = m;
+        return NNP_OK;
+    });
+}
+
+void nnp_free(nnp_model* m) {
+    if (!m) return;
+    delete m->eng;
+    delete m;
+}
+
+int nnp_ntypes(const nnp_model* m) { return m ? m->eng->host.T : NNP_EINVAL; }
+double nnp_rc(const nnp_model* m) { return m ? m->eng->host.rc : 0.0; }
+int nnp_ninput(const nnp_model* m, int t) {
+    if (!m || t < 0 || t >= m->eng->host.T) return NNP_EINVAL;
+    return m->eng->host.sp[t].nG;
+}
+int nnp_type_index(const nnp_model* m, const char* name) {
+    if (!m || !name) return -1;
+    return m->eng->host.index(name);
+}
+
+int64_t nnp_neighbors(nnp_model* m, const nnp_structure* s, int32_t* counts, int64_t cap, int32_t* index,
+                      int32_t* image, int32_t* is_min, double* r, double* d) {
+    int64_t total = 0;
+    const int rc = guarded([&]() {
+        if (!m || !s || !counts) throw std::invalid_argument("nnp_neighbors: null argument");
+        Engine& E = *m->eng;
+        std::vector<nnp_structure> ss(1, *s);
+        E.upload(ss);
+        E.build_neighbors(false);
+        const int n = s->natoms, mp = E.maxp;
+        std::vector<int> cnt(n), nj((size_t)n * mp), meta((size_t)n * mp);
+        std::vector<double4> rr((size_t)n * mp);
+        HIPCHK(hipMemcpy(cnt.data(), E.d_cnt.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(nj.data(), E.d_nbrj.p, (size_t)n * mp * 4, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(meta.data(), E.d_meta.p, (size_t)n * mp * 4, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(rr.data(), E.d_nbrr.p, (size_t)n * mp * sizeof(double4), hipMemcpyDeviceToHost));
+        const bool periodic = E.h_strucs[0].periodic;
+        for (int i = 0; i < n; ++i) {
+            counts[i] = cnt[i];
+            total += cnt[i];
+        }
+        if (total <= cap) {
+            int64_t q = 0;
+            for (int i = 0; i < n; ++i)
+                for (int k = 0; k < cnt[i]; ++k, ++q) {
+                    const size_t e = (size_t)i * mp + k;
+                    if (index) index[q] = nj[e];
+                    if (image) image[q] = periodic ? (meta[e] >> 8) : -1;
+                    if (is_min) is_min[q] = meta[e] & 1;
+                    if (r) {
+                        r[3 * q] = rr[e].x;
+                        r[3 * q + 1] = rr[e].y;
+                        r[3 * q + 2] = rr[e].z;
+                    }
+                    if (d) d[q] = rr[e].w;
+                }
+        }
+        return NNP_OK;
+    });
+    return rc < 0 ? rc : total;
+}
+
+int nnp_symm(nnp_model* m, const nnp_structure* s, double* G, int ldg) {
+    return guarded([&]() {
+        if (!m || !s || !G) throw std::invalid_argument("nnp_symm: null argument");
+        Engine& E = *m->eng;
+        std::vector<nnp_structure> ss(1, *s);
+        E.upload(ss);
+        E.build_neighbors(false);
+        E.descriptors<double>();
+        const int n = s->natoms, lg = E.pb.plan.ldg;
+        std::vector<double> g((size_t)n * lg);
+        HIPCHK(hipMemcpyAsync(g.data(), E.d_G.p, g.size() * 8, hipMemcpyDeviceToHost, E.stream));
+        HIPCHK(hipStreamSynchronize(E.stream));
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < ldg; ++k) G[(size_t)i * ldg + k] = k < lg ? g[(size_t)i * lg + k] : 0.0;
+        return NNP_OK;
+    });
+}
+
+int nnp_compute(nnp_model* m, const nnp_structure* s, uint32_t flags, nnp_result* r) {
+    return nnp_compute_batch(m, 1, s, flags, r);
+}
+
+int nnp_compute_batch(nnp_model* m, int n, const nnp_structure* s, uint32_t flags, nnp_result* r) {
+    return guarded([&]() {
+        if (!m || !s || !r || n <= 0) throw std::invalid_argument("nnp_compute: invalid argument");
+        Engine& E = *m->eng;
+        std::vector<nnp_structure> ss(s, s + n);
+        E.run(ss, flags);
+        E.fetch(ss, flags, r);
+        return NNP_OK;
+    });
+}
+
+int nnp_resident_load(nnp_model* m, int n, const nnp_structure* s, uint32_t flags) {
+    return guarded([&]() {
+        if (!m || !s || n <= 0) throw std::invalid_argument("nnp_resident_load: invalid argument");
+        m->eng->resident_load(std::vector<nnp_structure>(s, s + n), flags);
+        return NNP_OK;
+    });
+}
+
+int nnp_resident_run(nnp_model* m, uint32_t flags) {
+    return guarded([&]() {
+        if (!m || !m->eng->resident_valid) throw std::invalid_argument("nnp_resident_run: no resident batch");
+        m->eng->resident_run(flags);
+        return NNP_OK;
+    });
+}
+
+int nnp_resident_fetch(nnp_model* m, uint32_t flags, nnp_result* r) {
+    return guarded([&]() {
+        if (!m || !r || !m->eng->resident_valid) throw std::invalid_argument("nnp_resident_fetch: no resident batch");
+        m->eng->resident_fetch(flags, r);
+        return NNP_OK;
+    });
+}
+
+int nnp_sync(nnp_model* m) {
+    return guarded([&]() {
+        if (!m) throw std::invalid_argument("nnp_sync: null model");
+        HIPCHK(hipStreamSynchronize(m->eng->stream));
+        m->eng->profile_collect();
+        return NNP_OK;
+    });
+}
+
+int nnp_profile(nnp_model* m, int enable) {
+    return guarded([&]() {
+        if (!m) throw std::invalid_argument("nnp_profile: null model");
+        Engine& E = *m->eng;
+        HIPCHK(hipStreamSynchronize(E.stream));
+        E.profile_collect();
+        E.profiling = enable != 0;
+        for (int s = 0; s < Engine::ST_COUNT; ++s) {
+            E.stage_ms[s] = 0;
+            E.stage_calls[s] = 0;
+        }
+        return NNP_OK;
+    });
+}
+
+int nnp_profile_read(nnp_model* m, double* ms, int64_t* calls, int nstage) {
+    return guarded([&]() {
+        if (!m || !ms) throw std::invalid_argument("nnp_profile_read: invalid argument");
+        Engine& E = *m->eng;
+        for (int s = 0; s < nstage && s < Engine::ST_COUNT; ++s) {
+            ms[s] = E.stage_ms[s];
+            if (calls) calls[s] = E.stage_calls[s];
+        }
+        return Engine::ST_COUNT;
+    });
+}
+
+int nnp_info(nnp_model* m, int64_t* out, int nout) {
+    return guarded([&]() {
+        if (!m || !out) throw std::invalid_argument("nnp_info: invalid argument");
+        const Engine& E = *m->eng;
+        const int64_t v[] = {E.maxp, E.max_entries, E.ntot, E.nstruc, E.shape, E.pb.plan.ne_max, E.pb.plan.nc_max,
+                             E.pb.plan.ldg};
+        for (int k = 0; k < nout && k < (int)(sizeof(v) / sizeof(v[0])); ++k) out[k] = v[k];
+        return (int)(sizeof(v) / sizeof(v[0]));
+    });
+}
+
+}  // extern "C"

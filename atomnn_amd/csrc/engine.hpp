@@ -1,0 +1,114 @@
+// Engine state: the device-resident model and the reusable batch workspace.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/nnp_capi.h"
+#include "device.hpp"
+#include "model.hpp"
+
+namespace atomnn {
+
+void set_error(const std::string& s);
+const char* last_error();
+
+// A device allocation that only grows.
+struct DevMem {
+    void* p = nullptr;
+    size_t cap = 0;
+    void reserve(size_t bytes);
+    void release();
+};
+
+struct NeighborArgs {
+    const DevStruc* S;
+    const int* atom_struc;
+    const double* pos;
+    const int* type;
+    int ntot;
+    double rc2;
+    int maxp;
+    int* cnt;
+    int* nbr_j;
+    int* nbr_meta;
+    double4* nbr_r;
+    int* partner;
+    int* maxcnt;
+    const int* cell_centres;
+    int ncell;
+    const int* pair_centres;
+    int npair;
+    int nbins;
+    int* atom_bin = nullptr;
+    int* bin_count = nullptr;
+    int* bin_start = nullptr;
+    int* bin_fill = nullptr;
+    int* bin_atoms = nullptr;
+    int all_images;
+};
+
+DevStruc make_struc(const nnp_structure& s, double rc, int atom0);
+
+class Engine {
+  public:
+    explicit Engine(HostModel&& hm);
+    ~Engine();
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+
+    void upload(const std::vector<nnp_structure>& ss);
+    void build_neighbors(bool all_images);
+    template <typename T>
+    void descriptors();
+    void networks(bool mixed, bool vac);
+    template <typename T>
+    void forces();
+    void run(const std::vector<nnp_structure>& ss, uint32_t flags);
+    void fetch(const std::vector<nnp_structure>& ss, uint32_t flags, nnp_result* res);
+
+    // device-resident batches: upload once, enqueue passes, fetch results
+    void resident_load(const std::vector<nnp_structure>& ss, uint32_t flags);
+    void resident_run(uint32_t flags);
+    void resident_fetch(uint32_t flags, nnp_result* res);
+    void energies();
+
+    // stage timing with HIP events on the engine's stream
+    enum Stage { ST_NEIGH = 0, ST_SYMM, ST_NET, ST_FORCE, ST_ASSEMBLE, ST_ENERGY, ST_COUNT };
+    bool profiling = false;
+    hipEvent_t ev[ST_COUNT][2] = {};
+    double stage_ms[ST_COUNT] = {};
+    int64_t stage_calls[ST_COUNT] = {};
+    void stage_begin(int s);
+    void stage_end(int s);
+    void profile_collect();
+    bool pending[ST_COUNT] = {};
+    std::vector<int> h_natoms;  // atoms per structure of the current batch
+    bool resident_valid = false;
+    std::vector<double> h_E;
+    DevMem d_E, d_sstart;
+
+    HostModel host;
+    PlanBuffers pb;
+    int shape = 0;
+    hipStream_t stream = nullptr;
+    int maxp = 0;
+    int max_entries = 0;
+    int ntot = 0, nstruc = 0, nbins = 0;
+
+    // model
+    DevMem d_plan, d_rfe, d_rfr, d_w64, d_w32;
+    // batch
+    DevMem d_pos, d_type, d_astruc, d_strucs, d_centres, d_cnt, d_nbrj, d_meta, d_nbrr, d_partner, d_maxcnt;
+    DevMem d_abin, d_bcount, d_bstart, d_bfill, d_batoms;
+    DevMem d_G, d_dEdG, d_X, d_eatom, d_F, d_spatoms;
+
+    std::vector<double> h_pos, h_eatom, h_F;
+    std::vector<int> h_type, h_astruc, h_cell_centres, h_pair_centres, h_centres, h_spatoms, sp_off;
+    std::vector<DevStruc> h_strucs;
+};
+
+}  // namespace atomnn

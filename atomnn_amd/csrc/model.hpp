@@ -1,0 +1,137 @@
+// Host-side model of an AtomNN Behler-Parrinello potential and its device plan.
+//
+// HostModel is what the "ann" text file holds (grammar of AtomNN nnp/nnp.cpp:244-308);
+// DevPlan is the flattened, kernel-ready form uploaded once per model: per (centre
+// species, neighbour species) radial blocks, per (centre species, pair block) angular
+// blocks factorised into a grid of distinct radial exponents x distinct angular terms,
+// and zero-padded network weights for the MFMA GEMM kernels.
+#pragma once
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace atomnn {
+
+enum CutKind : int { CUT_STEP = 1, CUT_COS = 2, CUT_TANH = 3 };
+
+// Limits of the device plan.  The parser rejects potentials beyond them with NNP_EUNSUPPORTED.
+constexpr int kMaxSpecies = 8;
+constexpr int kMaxE = 8;       // distinct radial exponents per angular block
+constexpr int kMaxC = 8;       // distinct (zeta, lambda) terms per angular block
+constexpr int kMaxLayers = 8;  // network layers (hidden + output)
+constexpr int kMaxWidth = 256; // network layer width
+
+struct unsupported_error : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct io_error : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+struct CutoffSpec {
+    int kind = 0;
+    double rc = 0, rci = 0, pirci = 0, norm = 0;
+};
+
+struct RadialSpec {
+    CutoffSpec cut;
+    std::vector<double> rs, eta;
+};
+
+struct AngularSpec {
+    CutoffSpec cut;
+    int alpha = 0;
+    std::vector<double> eta, ietap, zeta;
+    std::vector<int> lambda;
+};
+
+struct NetSpec {
+    int nin = 0;
+    std::vector<int> nodes;  // hidden..., output (=1)
+    bool tanh_hidden = true;
+    std::vector<double> inpw, inpb, outw, outb;
+    std::vector<std::vector<double>> W, b;  // W[l]: nodes[l] x in(l), row-major
+};
+
+struct SpeciesSpec {
+    std::string name;
+    int hash = 0;
+    double evac = 0;
+    std::vector<RadialSpec> radial;    // [neighbour species]
+    std::vector<AngularSpec> angular;  // [lower-triangular pair block]
+    std::vector<int> offR, offA;
+    int nR = 0, nA = 0, nG = 0;
+    NetSpec net;
+};
+
+struct HostModel {
+    int T = 0;
+    double rc = 0;
+    std::vector<SpeciesSpec> sp;
+    int index(const std::string& name) const;  // -1 if absent
+};
+
+inline int lmat_index(int i, int j) { return (i >= j) ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; }
+int name_hash(const char* s);
+
+// Parse an ann file; throws std::invalid_argument / io_error / unsupported_error.
+HostModel parse_ann(const char* path);
+
+// ---------------------------------------------------------------- device plan (POD)
+struct DevRadial {
+    int n;       // functions in this block
+    int goff;    // descriptor offset
+    int fbase;   // first function in the plan's radial function arrays
+    int cut;
+    double rc, rci, pirci, norm;
+};
+
+struct DevAngular {
+    int goff;    // descriptor offset (nInputR + offsetA)
+    int alpha;
+    int ne, nc;  // distinct exponents / angular terms
+    int cut;
+    int pad_;
+    double rc, rci, pirci, norm;
+    double ietap[kMaxE];
+    double zeta[kMaxC];
+    double lambda[kMaxC];
+    int zint[kMaxC];          // integer zeta (>= 0) or -1
+    int fmap[kMaxE * kMaxC];  // grid cell (e, c) -> function index within the block, or -1
+};
+
+struct DevNet {
+    int L;                     // number of weight layers
+    int nin, nin_pad;
+    int width[kMaxLayers];     // real output widths
+    int wpad[kMaxLayers];      // output widths padded to 16
+    int64_t woff[kMaxLayers];  // offsets into the weight buffer (row-major wpad x in_pad)
+    int64_t boff[kMaxLayers];
+    int64_t inpw_off, inpb_off;
+    int tanh_hidden;
+    int pad_;
+    double outw, outb, evac;
+};
+
+struct DevPlan {
+    int T;
+    int ldg;  // max descriptor length over species
+    int ne_max, nc_max;
+    int nG[kMaxSpecies];
+    int nR[kMaxSpecies];
+    DevRadial rad[kMaxSpecies][kMaxSpecies];
+    DevAngular ang[kMaxSpecies][kMaxSpecies * (kMaxSpecies + 1) / 2];
+    DevNet net[kMaxSpecies];
+};
+
+struct PlanBuffers {
+    DevPlan plan;
+    std::vector<double> rfun_eta, rfun_rs;  // radial functions, all blocks
+    std::vector<double> weights;            // network weights/biases/scalings (padded)
+};
+
+PlanBuffers build_plan(const HostModel& m);
+
+}  // namespace atomnn

@@ -1,0 +1,405 @@
+// Neighbour-list kernels.
+//
+// The list reproduces AtomNN's NeighborList::build (struc/neighbor.cpp:59-171) entry for
+// entry: for centre i, neighbours j in ascending order; for each j the periodic images
+// r_ij - R_n in the order of the image table (a-major over the norm-sorted lattice vectors,
+// neighbor.cpp:87-95) relative to the minimum-image base Cell::diff(r_i, r_j)
+// (struc/cell.cpp:103-128); inclusive cutoff d^2 <= rc^2; self images only when
+// d^2 > 1e-6; the strictly closest image of each j != i (first on ties, and only below
+// ||R||_F) carries the min flag (neighbor.cpp:131-141).  Open structures have no images
+// and no min flags (neighbor.cpp:149-170).
+//
+// Two searches produce the same list:
+//  * all-pairs: one wavefront per centre sweeps j in chunks of 64 lanes; each lane tests
+//    the images of its j, a wave prefix sum places the kept entries in order.  Used for
+//    small cells (any image shell > 0) and small structures.
+//  * cell list: for structures whose image shells are all 0 (every cell height > 2 rc) the
+//    entry of j is exactly diff(r_i, r_j) when within rc; candidates come from the 27
+//    neighbouring fractional bins, and each wavefront sorts its kept candidates by j
+//    (bitonic sort in LDS) before writing them, so the order is the reference's.
+#include "device.hpp"
+
+namespace atomnn {
+
+// math/special.hpp:46-47 with z = 1: fmod(x, 1) = x - trunc(x) exactly for finite x.
+__device__ __forceinline__ double fmod1(double x) { return x - trunc(x); }
+__device__ __forceinline__ double wrap_half(double x) { return fmod1(fmod1(x + 0.5) + 1.0) - 0.5; }
+
+// Cell::diff (cell.cpp:103-128)
+__device__ __forceinline__ void cell_diff(const DevStruc& s, double dx, double dy, double dz, double& ox, double& oy,
+                                          double& oz) {
+    double f0 = s.RInv[0] * dx + s.RInv[1] * dy + s.RInv[2] * dz;
+    double f1 = s.RInv[3] * dx + s.RInv[4] * dy + s.RInv[5] * dz;
+    double f2 = s.RInv[6] * dx + s.RInv[7] * dy + s.RInv[8] * dz;
+    f0 = wrap_half(f0);
+    f1 = wrap_half(f1);
+    f2 = wrap_half(f2);
+    ox = s.R[0] * f0 + s.R[1] * f1 + s.R[2] * f2;
+    oy = s.R[3] * f0 + s.R[4] * f1 + s.R[5] * f2;
+    oz = s.R[6] * f0 + s.R[7] * f1 + s.R[8] * f2;
+    const double d2 = ox * ox + oy * oy + oz * oz;
+    if (d2 > s.dMax2) {
+        double bx = ox, by = oy, bz = oz, best = d2;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const int c = q >> 1;
+            const double sg = (q & 1) ? -1.0 : 1.0;
+            const double tx = ox + sg * s.R[c], ty = oy + sg * s.R[3 + c], tz = oz + sg * s.R[6 + c];
+            const double t2 = tx * tx + ty * ty + tz * tz;
+            if (t2 < best) {
+                best = t2;
+                bx = tx;
+                by = ty;
+                bz = tz;
+            }
+        }
+        ox = bx;
+        oy = by;
+        oz = bz;
+    }
+}
+
+// image vector R_n = a Rv0 + b Rv1 + c Rv2 for linear index n (neighbor.cpp:87-95)
+__device__ __forceinline__ void image_vec(const DevStruc& s, int n, double& x, double& y, double& z) {
+    const int w2 = 2 * s.shell[2] + 1, w1 = 2 * s.shell[1] + 1;
+    const int c = n % w2 - s.shell[2];
+    const int b = (n / w2) % w1 - s.shell[1];
+    const int a = n / (w2 * w1) - s.shell[0];
+    x = (double)a * s.Rv[0] + (double)b * s.Rv[3] + (double)c * s.Rv[6];
+    y = (double)a * s.Rv[1] + (double)b * s.Rv[4] + (double)c * s.Rv[7];
+    z = (double)a * s.Rv[2] + (double)b * s.Rv[5] + (double)c * s.Rv[8];
+}
+
+__device__ __forceinline__ int wave_excl_scan(int v, int lane, int& total) {
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int y = __shfl_up(x, o, kWave);
+        if (lane >= o) x += y;
+    }
+    total = __shfl(x, kWave - 1, kWave);
+    return x - v;
+}
+
+__device__ __forceinline__ void put_entry(int64_t slot, int j, int img, int typ, bool mn, double x, double y, double z,
+                                          double d, int* nbr_j, int* nbr_meta, double4* nbr_r) {
+    nbr_j[slot] = j;
+    nbr_meta[slot] = (img << 8) | (typ << 1) | (mn ? 1 : 0);
+    nbr_r[slot] = make_double4(x, y, z, d);
+}
+
+// ------------------------------------------------------------------ all-pairs search
+__global__ __launch_bounds__(kWave) void k_neigh_allpairs(const DevStruc* __restrict__ S, const int* __restrict__ atom_struc,
+                                                          const double* __restrict__ pos, const int* __restrict__ type,
+                                                          const int* __restrict__ centres, double rc2, int maxp,
+                                                          int* __restrict__ cnt, int* __restrict__ nbr_j,
+                                                          int* __restrict__ nbr_meta, double4* __restrict__ nbr_r,
+                                                          int* __restrict__ maxcnt) {
+    const int i = centres[blockIdx.x];
+    const DevStruc& s = S[atom_struc[i]];
+    const int lane = threadIdx.x;
+    const int li = i - s.atom0;
+    const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+    const int64_t row = (int64_t)i * maxp;
+    int written = 0;
+    for (int j0 = 0; j0 < s.natoms; j0 += kWave) {
+        const int j = j0 + lane;
+        const bool valid = j < s.natoms;
+        double bx = 0, by = 0, bz = 0;
+        int tj = 0;
+        int mine = 0;
+        double mind = s.fro;
+        int minn = -1;
+        if (valid) {
+            const int gj = s.atom0 + j;
+            tj = type[gj];
+            const double dx = xi - pos[3 * gj], dy = yi - pos[3 * gj + 1], dz = zi - pos[3 * gj + 2];
+            if (s.periodic) {
+                cell_diff(s, dx, dy, dz, bx, by, bz);
+            } else {
+                bx = dx;
+                by = dy;
+                bz = dz;
+            }
+            for (int n = 0; n < s.nimg; ++n) {
+                double ix = 0, iy = 0, iz = 0;
+                if (s.periodic) image_vec(s, n, ix, iy, iz);
+                const double rx = bx - ix, ry = by - iy, rz = bz - iz;
+                const double d2 = rx * rx + ry * ry + rz * rz;
+                const bool keep = (j != li) ? (d2 <= rc2) : (1e-6 < d2 && d2 <= rc2);
+                if (keep) {
+                    ++mine;
+                    const double d = sqrt(d2);
+                    if (j != li && s.periodic && d < mind) {
+                        mind = d;
+                        minn = n;
+                    }
+                }
+            }
+        }
+        int tot;
+        int at = written + wave_excl_scan(mine, lane, tot);
+        if (mine > 0) {
+            for (int n = 0; n < s.nimg; ++n) {
+                double ix = 0, iy = 0, iz = 0;
+                if (s.periodic) image_vec(s, n, ix, iy, iz);
+                const double rx = bx - ix, ry = by - iy, rz = bz - iz;
+                const double d2 = rx * rx + ry * ry + rz * rz;
+                const bool keep = (j != li) ? (d2 <= rc2) : (1e-6 < d2 && d2 <= rc2);
+                if (keep) {
+                    if (at < maxp) put_entry(row + at, j, n, tj, n == minn, rx, ry, rz, sqrt(d2), nbr_j, nbr_meta, nbr_r);
+                    ++at;
+                }
+            }
+        }
+        written += tot;
+    }
+    if (lane == 0) {
+        cnt[i] = written;
+        atomicMax(maxcnt, written);
+    }
+}
+
+// ------------------------------------------------------------------ cell list
+// Bin index of each atom (fractional coordinates wrapped into [0,1)).
+__global__ void k_bin_atoms(const DevStruc* __restrict__ S, const int* __restrict__ atom_struc,
+                            const double* __restrict__ pos, int natoms_total, int* __restrict__ atom_bin,
+                            int* __restrict__ bin_count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= natoms_total) return;
+    const DevStruc& s = S[atom_struc[i]];
+    if (s.mode != 1) {
+        atom_bin[i] = -1;
+        return;
+    }
+    const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+    int b[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        double f = s.RInv[3 * r] * x + s.RInv[3 * r + 1] * y + s.RInv[3 * r + 2] * z;
+        f -= floor(f);
+        int q = (int)(f * s.nbin[r]);
+        q = q < 0 ? 0 : (q >= s.nbin[r] ? s.nbin[r] - 1 : q);
+        b[r] = q;
+    }
+    const int bin = s.bin0 + (b[0] * s.nbin[1] + b[1]) * s.nbin[2] + b[2];
+    atom_bin[i] = bin;
+    atomicAdd(&bin_count[bin], 1);
+}
+
+// Scatter atoms into their bins.  Order inside a bin does not matter: candidates are sorted.
+__global__ void k_fill_bins(const int* __restrict__ atom_bin, int natoms_total, const int* __restrict__ bin_start,
+                            int* __restrict__ bin_fill, int* __restrict__ bin_atoms) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= natoms_total) return;
+    const int b = atom_bin[i];
+    if (b < 0) return;
+    const int slot = atomicAdd(&bin_fill[b], 1);
+    bin_atoms[bin_start[b] + slot] = i;
+}
+
+// One wavefront per centre: gather candidates from the 27 neighbouring bins, keep those with
+// |diff(r_i, r_j)| <= rc, sort the kept entries by j in LDS, write them in order.
+// LDS: vals double[cap][4] | keys int[cap] | skey int[n2cap] | perm int[n2cap], n2cap = 2^k >= cap.
+__global__ __launch_bounds__(kWave) void k_neigh_cells(const DevStruc* __restrict__ S, const int* __restrict__ atom_struc,
+                                                       const double* __restrict__ pos, const int* __restrict__ type,
+                                                       const int* __restrict__ centres, const int* __restrict__ atom_bin,
+                                                       const int* __restrict__ bin_start, const int* __restrict__ bin_count,
+                                                       const int* __restrict__ bin_atoms, double rc2, int maxp, int n2cap,
+                                                       int* __restrict__ cnt, int* __restrict__ nbr_j,
+                                                       int* __restrict__ nbr_meta, double4* __restrict__ nbr_r,
+                                                       int* __restrict__ maxcnt) {
+    extern __shared__ double lds_cells[];
+    const int cap = maxp;
+    double* vals = lds_cells;
+    int* keys = (int*)(vals + 4 * cap);
+    int* skey = keys + cap;
+    int* perm = skey + n2cap;
+    const int i = centres[blockIdx.x];
+    const DevStruc& s = S[atom_struc[i]];
+    const int lane = threadIdx.x;
+    const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+    const int myb = atom_bin[i] - s.bin0;
+    const int bz = myb % s.nbin[2], by = (myb / s.nbin[2]) % s.nbin[1], bx = myb / (s.nbin[2] * s.nbin[1]);
+    int kept = 0;
+    // neighbouring bins; along an axis with fewer than 3 bins every bin is visited once
+    const int n0 = s.nbin[0] < 3 ? s.nbin[0] : 3, n1 = s.nbin[1] < 3 ? s.nbin[1] : 3, n2 = s.nbin[2] < 3 ? s.nbin[2] : 3;
+    for (int o = 0; o < n0 * n1 * n2; ++o) {
+        const int o2 = o % n2, o1 = (o / n2) % n1, o0 = o / (n2 * n1);
+        const int cx = s.nbin[0] < 3 ? o0 : (bx + o0 - 1 + s.nbin[0]) % s.nbin[0];
+        const int cy = s.nbin[1] < 3 ? o1 : (by + o1 - 1 + s.nbin[1]) % s.nbin[1];
+        const int cz = s.nbin[2] < 3 ? o2 : (bz + o2 - 1 + s.nbin[2]) % s.nbin[2];
+        const int b = s.bin0 + (cx * s.nbin[1] + cy) * s.nbin[2] + cz;
+        const int start = bin_start[b], count = bin_count[b];
+        for (int c0 = 0; c0 < count; c0 += kWave) {
+            const int c = c0 + lane;
+            bool keep = false;
+            double rx = 0, ry = 0, rz = 0, d2 = 0;
+            int gj = 0;
+            if (c < count) {
+                gj = bin_atoms[start + c];
+                if (gj != i) {
+                    cell_diff(s, xi - pos[3 * gj], yi - pos[3 * gj + 1], zi - pos[3 * gj + 2], rx, ry, rz);
+                    d2 = rx * rx + ry * ry + rz * rz;
+                    keep = d2 <= rc2;
+                }
+            }
+            int tot;
+            const int at = kept + wave_excl_scan(keep ? 1 : 0, lane, tot);
+            if (keep && at < cap) {
+                keys[at] = gj - s.atom0;
+                vals[4 * at] = rx;
+                vals[4 * at + 1] = ry;
+                vals[4 * at + 2] = rz;
+                vals[4 * at + 3] = d2;
+            }
+            kept += tot;
+        }
+    }
+    __syncthreads();
+    const int P = kept < cap ? kept : cap;
+    int n2p = 1;
+    while (n2p < P) n2p <<= 1;
+    for (int q = lane; q < n2p; q += kWave) {
+        skey[q] = q < P ? keys[q] : 0x7fffffff;
+        perm[q] = q;
+    }
+    __syncthreads();
+    // bitonic sort of (key, slot) by key; keys are distinct atoms
+    for (int k = 2; k <= n2p; k <<= 1) {
+        for (int jj = k >> 1; jj > 0; jj >>= 1) {
+            for (int q = lane; q < n2p; q += kWave) {
+                const int p = q ^ jj;
+                if (p > q) {
+                    const bool up = (q & k) == 0;
+                    const int a = skey[q], b = skey[p];
+                    if ((a > b) == up) {
+                        skey[q] = b;
+                        skey[p] = a;
+                        const int t = perm[q];
+                        perm[q] = perm[p];
+                        perm[p] = t;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    const int64_t row = (int64_t)i * maxp;
+    for (int q = lane; q < P; q += kWave) {
+        const int src = perm[q];
+        const int j = keys[src];
+        const double d = sqrt(vals[4 * src + 3]);
+        // one image per j: it is the minimum image when below ||R||_F (neighbor.cpp:132-140)
+        put_entry(row + q, j, 0, type[s.atom0 + j], d < s.fro, vals[4 * src], vals[4 * src + 1], vals[4 * src + 2], d,
+                  nbr_j, nbr_meta, nbr_r);
+    }
+    if (lane == 0) {
+        cnt[i] = kept;
+        atomicMax(maxcnt, kept);
+    }
+}
+
+// Exclusive scan of n ints by one block (bin starts of the cell list).
+__global__ __launch_bounds__(1024) void k_scan_excl(const int* __restrict__ in, int* __restrict__ out, int n) {
+    __shared__ int part[1024];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < n; base += 1024) {
+        const int t = threadIdx.x;
+        const int v = base + t < n ? in[base + t] : 0;
+        part[t] = v;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            const int y = t >= o ? part[t - o] : 0;
+            __syncthreads();
+            part[t] += y;
+            __syncthreads();
+        }
+        if (base + t < n) out[base + t] = carry + part[t] - v;
+        __syncthreads();
+        if (t == 0) carry += part[1023];
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------ partners
+// For entry (i -> j, image n) the mirrored entry is (j -> i, image nimg-1-n): same distance,
+// opposite vector.  Rows are sorted by (j, n), so a binary search finds it.  In the
+// reference rule only min-image entries carry a reaction; with all_images every entry does.
+__global__ void k_partner(const DevStruc* __restrict__ S, const int* __restrict__ atom_struc,
+                          const int* __restrict__ cnt, const int* __restrict__ nbr_j, const int* __restrict__ nbr_meta,
+                          int maxp, int natoms_total, int all_images, int* __restrict__ partner) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)natoms_total * maxp) return;
+    const int i = (int)(e / maxp);
+    const int q = (int)(e % maxp);
+    const int P = cnt[i] < maxp ? cnt[i] : maxp;
+    if (q >= P) return;
+    const int meta = nbr_meta[e];
+    const bool mn = meta & 1;
+    const DevStruc& s = S[atom_struc[i]];
+    if (!(mn || (all_images != 0))) {
+        partner[e] = -1;
+        return;
+    }
+    const int img = meta >> 8;
+    const int want_img = s.periodic ? s.nimg - 1 - img : 0;
+    const int li = i - s.atom0;
+    const int j = nbr_j[e];
+    const int gj = s.atom0 + j;
+    const int64_t rowj = (int64_t)gj * maxp;
+    const int Pj = cnt[gj] < maxp ? cnt[gj] : maxp;
+    const int64_t want = (int64_t)li * s.nimg + want_img;
+    int lo = 0, hi = Pj;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const int64_t key = (int64_t)nbr_j[rowj + mid] * s.nimg + (nbr_meta[rowj + mid] >> 8);
+        if (key < want) lo = mid + 1;
+        else hi = mid;
+    }
+    int found = -1;
+    if (lo < Pj) {
+        const int64_t key = (int64_t)nbr_j[rowj + lo] * s.nimg + (nbr_meta[rowj + lo] >> 8);
+        if (key == want) found = (int)(rowj + lo);
+    }
+    partner[e] = found;
+}
+
+}  // namespace atomnn
+
+#include "engine.hpp"
+
+namespace atomnn {
+
+hipError_t launch_neighbors(const NeighborArgs& a, hipStream_t st) {
+    hipError_t e;
+    if (a.ncell > 0) {
+        if ((e = hipMemsetAsync(a.bin_count, 0, (size_t)a.nbins * 4, st)) != hipSuccess) return e;
+        if ((e = hipMemsetAsync(a.bin_fill, 0, (size_t)a.nbins * 4, st)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_bin_atoms, dim3((a.ntot + 255) / 256), dim3(256), 0, st, a.S, a.atom_struc, a.pos, a.ntot,
+                           a.atom_bin, a.bin_count);
+        hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, a.bin_count, a.bin_start, a.nbins);
+        hipLaunchKernelGGL(k_fill_bins, dim3((a.ntot + 255) / 256), dim3(256), 0, st, a.atom_bin, a.ntot, a.bin_start,
+                           a.bin_fill, a.bin_atoms);
+        int n2cap = 1;
+        while (n2cap < a.maxp) n2cap <<= 1;
+        const size_t lds = (size_t)a.maxp * 32 + (size_t)a.maxp * 4 + (size_t)n2cap * 8;
+        hipLaunchKernelGGL(k_neigh_cells, dim3(a.ncell), dim3(kWave), lds, st, a.S, a.atom_struc, a.pos, a.type,
+                           a.cell_centres, a.atom_bin, a.bin_start, a.bin_count, a.bin_atoms, a.rc2, a.maxp, n2cap,
+                           a.cnt, a.nbr_j, a.nbr_meta, a.nbr_r, a.maxcnt);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if (a.npair > 0) {
+        hipLaunchKernelGGL(k_neigh_allpairs, dim3(a.npair), dim3(kWave), 0, st, a.S, a.atom_struc, a.pos, a.type,
+                           a.pair_centres, a.rc2, a.maxp, a.cnt, a.nbr_j, a.nbr_meta, a.nbr_r, a.maxcnt);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    const int64_t ent = (int64_t)a.ntot * a.maxp;
+    hipLaunchKernelGGL(k_partner, dim3((unsigned)((ent + 255) / 256)), dim3(256), 0, st, a.S, a.atom_struc, a.cnt,
+                       a.nbr_j, a.nbr_meta, a.maxp, a.ntot, a.all_images, a.partner);
+    return hipGetLastError();
+}
+
+}  // namespace atomnn

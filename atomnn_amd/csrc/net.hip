@@ -1,0 +1,312 @@
+// Per-species network forward and input gradient, batched over all atoms of a species so
+// each layer is a GEMM on the matrix cores.
+//
+// Reference arithmetic (ml/nn.cpp):
+//   ins = inpw .* (G + inpb)                         fp/fpbp, nn.cpp:1194, 1218
+//   z_l = b_l + W_l a_{l-1};  a_l = tanh(z_l), d_l = 1 - a_l^2   (hidden, nn.cpp:209-216)
+//   last layer linear: a_L = z_L, d_L = 1            (nn.cpp:144-148, 1134-1137)
+//   out = outb + outw .* a_L                         (nn.cpp:1229-1230)
+//   dodz_L = outw .* d_L;  dodz_{l-1} = (dodz_l W_l) .* d_{l-1};  dE/dG = (dodz_0 W_0) .* inpw
+//                                                    DODZ::grad, nn.cpp:1620-1628
+// E_i = out_i (+ E_vac of the species with NNP_VACUUM_ENERGY, nnh.cpp:129).
+//
+// One wavefront handles a tile of 16 atoms.  FP64 form: v_mfma_f64_16x16x4f64, 16x16
+// output tiles, K in steps of 4; A lane l holds A[l%16][l/16], B lane l holds B[l/16][l%16],
+// and result register v of lane l is D[4v + l/16][l%16].  Mixed form: v_mfma_f32_16x16x32_bf16
+// with the activations and weights rounded to BF16 and FP32 accumulation; tanh and the
+// derivative products stay in FP32.  Weights are zero padded to multiples of 16, so padded units
+// carry exact zeros through both directions.
+#include "device.hpp"
+#include "model.hpp"
+
+namespace atomnn {
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
+
+// LDS: two activation buffers [16][maxw] and the derivatives of every layer [16][wpad[l]]
+size_t net_lds_bytes(const DevNet& n, bool fp64) {
+    int maxw = n.nin_pad, dsum = 0;
+    for (int l = 0; l < n.L; ++l) {
+        if (n.wpad[l] > maxw) maxw = n.wpad[l];
+        dsum += n.wpad[l];
+    }
+    return (size_t)(fp64 ? 8 : 4) * (size_t)(2 * 16 * maxw + 16 * dsum);
+}
+
+static __device__ __forceinline__ int net_maxw(const DevNet& n) {
+    int maxw = n.nin_pad;
+    for (int l = 0; l < n.L; ++l) maxw = n.wpad[l] > maxw ? n.wpad[l] : maxw;
+    return maxw;
+}
+
+// ------------------------------------------------------------------ FP64
+__global__ __launch_bounds__(kWave) void k_net_f64(const DevPlan* __restrict__ plan, const double* __restrict__ Wb,
+                                                   int s, const int* __restrict__ atoms, int M,
+                                                   const double* __restrict__ G, int ldg, int vac,
+                                                   double* __restrict__ eatom, double* __restrict__ dEdG) {
+    extern __shared__ __align__(16) double shd[];
+    const DevNet& N = plan->net[s];
+    const int lane = threadIdx.x;
+    const int r0 = blockIdx.x * 16;
+    const int maxw = net_maxw(N);
+    double* cur = shd;
+    double* nxt = shd + 16 * maxw;
+    double* der = shd + 32 * maxw;
+    const int li = lane & 15, lk = lane >> 4;
+    // scaled input
+    for (int idx = lane; idx < 16 * N.nin_pad; idx += kWave) {
+        const int r = idx / N.nin_pad, k = idx % N.nin_pad;
+        const int row = r0 + r;
+        double v = 0.0;
+        if (row < M && k < N.nin) {
+            const int at = atoms[row];
+            v = Wb[N.inpw_off + k] * (G[(int64_t)at * ldg + k] + Wb[N.inpb_off + k]);
+        }
+        cur[r * maxw + k] = v;
+    }
+    __syncthreads();
+    int in_pad = N.nin_pad;
+    int doff = 0;
+    for (int l = 0; l < N.L; ++l) {
+        const int out_pad = N.wpad[l];
+        const double* W = Wb + N.woff[l];
+        const double* b = Wb + N.boff[l];
+        const bool linear = (l == N.L - 1) || !N.tanh_hidden;
+        for (int jt = 0; jt < out_pad / 16; ++jt) {
+            d4_t acc = {0.0, 0.0, 0.0, 0.0};
+            for (int k0 = 0; k0 < in_pad; k0 += 4) {
+                const double a = cur[li * maxw + k0 + lk];
+                const double bv = W[(int64_t)(jt * 16 + li) * in_pad + k0 + lk];
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc, 0, 0, 0);
+            }
+            const int col = jt * 16 + li;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = 4 * v + lk;
+                const double z = acc[v] + b[col];
+                double av, dv;
+                if (linear) {
+                    av = z;
+                    dv = 1.0;
+                } else {
+                    av = tanh(z);
+                    dv = 1.0 - av * av;
+                }
+                nxt[row * maxw + col] = av;
+                der[doff + row * out_pad + col] = dv;
+            }
+        }
+        __syncthreads();
+        double* t = cur;
+        cur = nxt;
+        nxt = t;
+        in_pad = out_pad;
+        doff += 16 * out_pad;
+    }
+    // energies
+    if (lane < 16 && r0 + lane < M) {
+        const int at = atoms[r0 + lane];
+        eatom[at] = N.outb + cur[lane * maxw] * N.outw + (vac ? N.evac : 0.0);
+    }
+    // backward: delta_L = outw .* d_L (single output)
+    const int last = N.wpad[N.L - 1];
+    doff -= 16 * last;
+    for (int idx = lane; idx < 16 * last; idx += kWave) {
+        const int r = idx / last, o = idx % last;
+        nxt[r * maxw + o] = (o == 0) ? N.outw * der[doff + r * last + o] : 0.0;
+    }
+    __syncthreads();
+    {
+        double* t = cur;
+        cur = nxt;
+        nxt = t;
+    }
+    for (int l = N.L - 1; l >= 1; --l) {
+        const int K = N.wpad[l], J = N.wpad[l - 1];
+        const double* W = Wb + N.woff[l];  // wpad[l] x wpad[l-1], row-major
+        doff -= 16 * J;
+        for (int jt = 0; jt < J / 16; ++jt) {
+            d4_t acc = {0.0, 0.0, 0.0, 0.0};
+            for (int k0 = 0; k0 < K; k0 += 4) {
+                const double a = cur[li * maxw + k0 + lk];
+                const double bv = W[(int64_t)(k0 + lk) * J + jt * 16 + li];
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc, 0, 0, 0);
+            }
+            const int col = jt * 16 + li;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = 4 * v + lk;
+                nxt[row * maxw + col] = acc[v] * der[doff + row * J + col];
+            }
+        }
+        __syncthreads();
+        double* t = cur;
+        cur = nxt;
+        nxt = t;
+    }
+    // dE/dG = (delta_0 W_0) .* inpw
+    {
+        const int K = N.wpad[0], J = N.nin_pad;
+        const double* W = Wb + N.woff[0];
+        for (int jt = 0; jt < J / 16; ++jt) {
+            d4_t acc = {0.0, 0.0, 0.0, 0.0};
+            for (int k0 = 0; k0 < K; k0 += 4) {
+                const double a = cur[li * maxw + k0 + lk];
+                const double bv = W[(int64_t)(k0 + lk) * J + jt * 16 + li];
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc, 0, 0, 0);
+            }
+            const int col = jt * 16 + li;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = r0 + 4 * v + lk;
+                if (row < M && col < N.nin) dEdG[(int64_t)atoms[row] * ldg + col] = acc[v] * Wb[N.inpw_off + col];
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ mixed (BF16 MFMA, FP32 accumulate)
+// v_mfma_f32_16x16x32_bf16 (gfx950): A lane l holds A[l%16][8(l/16) .. 8(l/16)+7], B lane l
+// holds B[8(l/16) .. +7][l%16], result register v of lane l is D[4(l/16) + v][l%16].
+__device__ __forceinline__ __bf16 to_bf16(float x) { return (__bf16)x; }
+
+__global__ __launch_bounds__(kWave) void k_net_mixed(const DevPlan* __restrict__ plan, const float* __restrict__ Wf,
+                                                     int s, const int* __restrict__ atoms, int M,
+                                                     const float* __restrict__ G, int ldg, int vac,
+                                                     double* __restrict__ eatom, float* __restrict__ dEdG) {
+    extern __shared__ __align__(16) float shf[];
+    const DevNet& N = plan->net[s];
+    const int lane = threadIdx.x;
+    const int r0 = blockIdx.x * 16;
+    int maxw = N.nin_pad;
+    for (int l = 0; l < N.L; ++l) maxw = N.wpad[l] > maxw ? N.wpad[l] : maxw;
+    float* cur = shf;
+    float* nxt = shf + 16 * maxw;
+    float* der = shf + 32 * maxw;
+    const int li = lane & 15, lk = lane >> 4;
+    for (int idx = lane; idx < 16 * N.nin_pad; idx += kWave) {
+        const int r = idx / N.nin_pad, k = idx % N.nin_pad;
+        const int row = r0 + r;
+        float v = 0.f;
+        if (row < M && k < N.nin) {
+            const int at = atoms[row];
+            v = Wf[N.inpw_off + k] * (G[(int64_t)at * ldg + k] + Wf[N.inpb_off + k]);
+        }
+        cur[r * maxw + k] = v;
+    }
+    __syncthreads();
+    int in_pad = N.nin_pad;
+    int doff = 0;
+    for (int l = 0; l < N.L; ++l) {
+        const int out_pad = N.wpad[l];
+        const float* W = Wf + N.woff[l];
+        const float* b = Wf + N.boff[l];
+        const bool linear = (l == N.L - 1) || !N.tanh_hidden;
+        for (int jt = 0; jt < out_pad / 16; ++jt) {
+            f4_t acc = {0.f, 0.f, 0.f, 0.f};
+            for (int k0 = 0; k0 < in_pad; k0 += 32) {
+                bf8_t a, bv;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const int k = k0 + 8 * lk + q;
+                    const bool ok = k < in_pad;
+                    a[q] = to_bf16(ok ? cur[li * maxw + k] : 0.f);
+                    bv[q] = to_bf16(ok ? W[(int64_t)(jt * 16 + li) * in_pad + k] : 0.f);
+                }
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc, 0, 0, 0);
+            }
+            const int col = jt * 16 + li;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = 4 * lk + v;
+                const float z = acc[v] + b[col];
+                float av, dv;
+                if (linear) {
+                    av = z;
+                    dv = 1.f;
+                } else {
+                    av = tanhf(z);
+                    dv = 1.f - av * av;
+                }
+                nxt[row * maxw + col] = av;
+                der[doff + row * out_pad + col] = dv;
+            }
+        }
+        __syncthreads();
+        float* t = cur;
+        cur = nxt;
+        nxt = t;
+        in_pad = out_pad;
+        doff += 16 * out_pad;
+    }
+    if (lane < 16 && r0 + lane < M) {
+        const int at = atoms[r0 + lane];
+        eatom[at] = (double)N.outb + (double)cur[lane * maxw] * N.outw + (vac ? N.evac : 0.0);
+    }
+    const int last = N.wpad[N.L - 1];
+    doff -= 16 * last;
+    for (int idx = lane; idx < 16 * last; idx += kWave) {
+        const int r = idx / last, o = idx % last;
+        nxt[r * maxw + o] = (o == 0) ? (float)N.outw * der[doff + r * last + o] : 0.f;
+    }
+    __syncthreads();
+    {
+        float* t = cur;
+        cur = nxt;
+        nxt = t;
+    }
+    for (int l = N.L - 1; l >= 0; --l) {
+        const int K = N.wpad[l], J = (l == 0) ? N.nin_pad : N.wpad[l - 1];
+        const float* W = Wf + N.woff[l];
+        if (l > 0) doff -= 16 * J;
+        for (int jt = 0; jt < J / 16; ++jt) {
+            f4_t acc = {0.f, 0.f, 0.f, 0.f};
+            for (int k0 = 0; k0 < K; k0 += 32) {
+                bf8_t a, bv;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const int k = k0 + 8 * lk + q;
+                    const bool ok = k < K;
+                    a[q] = to_bf16(ok ? cur[li * maxw + k] : 0.f);
+                    bv[q] = to_bf16(ok ? W[(int64_t)k * J + jt * 16 + li] : 0.f);
+                }
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc, 0, 0, 0);
+            }
+            const int col = jt * 16 + li;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int rr = 4 * lk + v;
+                if (l > 0) {
+                    nxt[rr * maxw + col] = acc[v] * der[doff + rr * J + col];
+                } else {
+                    const int row = r0 + rr;
+                    if (row < M && col < N.nin) dEdG[(int64_t)atoms[row] * ldg + col] = acc[v] * Wf[N.inpw_off + col];
+                }
+            }
+        }
+        __syncthreads();
+        float* t = cur;
+        cur = nxt;
+        nxt = t;
+    }
+}
+
+hipError_t launch_net_f64(const DevPlan* plan, const DevNet& hn, const double* Wb, int s, const int* atoms, int M,
+                          const double* G, int ldg, int vac, double* eatom, double* dEdG, hipStream_t st) {
+    if (M <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_net_f64, dim3((M + 15) / 16), dim3(kWave), net_lds_bytes(hn, true), st, plan, Wb, s, atoms, M, G,
+                       ldg, vac, eatom, dEdG);
+    return hipGetLastError();
+}
+
+hipError_t launch_net_mixed(const DevPlan* plan, const DevNet& hn, const float* Wf, int s, const int* atoms, int M,
+                            const float* G, int ldg, int vac, double* eatom, float* dEdG, hipStream_t st) {
+    if (M <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_net_mixed, dim3((M + 15) / 16), dim3(kWave), net_lds_bytes(hn, false), st, plan, Wf, s, atoms,
+                       M, G, ldg, vac, eatom, dEdG);
+    return hipGetLastError();
+}
+
+}  // namespace atomnn

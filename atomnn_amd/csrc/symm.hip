@@ -1,0 +1,608 @@
+// Symmetry-function (descriptor) and force-term kernels.
+//
+// One wavefront owns one centre atom i and its P neighbour entries.  The entries are
+// staged in LDS and partitioned by neighbour species (a stable partition), so that every
+// angular block (a, b) of AtomNN's descriptor layout (nnh.cpp:95-121, LMat order
+// lmat.hpp:43-46) is a contiguous triangle (a == b) or rectangle (a != b) of pairs.  Each
+// block's pair space is cut into 64 contiguous, equal chunks, one per lane, walked row by
+// row so a lane keeps its row atom's data in registers.
+//
+// Radial G2 (BasisR GAUSSIAN, basis_radial.cpp:302-310, 371-380, 443):
+//   G[offR(b)+m] = norm * sum_j exp(-eta_m (d_j - rs_m)^2) fc(d_j)
+// Angular G4 (BasisA GAUSS, basis_angular.cpp:406-421, 456-507, 635-640):
+//   G[goff+m] = norm * sum_{j<k} |(1 + lambda_m cos_jk)/2|^zeta_m R_m(j) R_m(k),
+//   R_m(j) = fc(d_j) exp(-ietap_m d_j^alpha)
+// The block's functions are factorised into a grid of distinct radial exponents e and
+// distinct angular terms c (ietap_e x (zeta_c, lambda_c)), so the exponential is evaluated
+// once per neighbour and exponent instead of once per pair and function.  This changes the
+// rounding of the sum, not its terms.
+//
+// Forces: with dE/dG from the network, the kernel computes for every entry j of centre i
+//   X_ij = -dE_i/d r_ij   (r_ij = r_i - r_j - T, the entry's vector)
+// summing the radial term and every pair (j, k) that contains j.  The reference's force
+// loop (nnp.cpp:416-482) adds X_ij to F_i and subtracts it from F_j when the entry is the
+// minimum image of j; the assembly kernel does exactly that from X, without atomics.
+// Per pair (j, k) the kernel needs dE/dcos and dE/dd_j:
+//   dE/dcos = sum_c A'_c(cos) sum_e W_ec R_e(j) R_e(k)
+//   dE/dd_j = sum_c A_c(cos)  sum_e W_ec R'_e(j) R_e(k)
+//   X_j    -= dE/dd_j u_j + dE/dcos (u_k - cos u_j) / d_j
+// with W_ec = norm dE/dG[goff + fmap(e,c)].  A'_c = zeta lambda |cw|^(zeta-1) / 2 as in
+// basis_angular.cpp:498-501 (no sign of cw, which is >= 0 up to rounding).
+#include "device.hpp"
+#include "model.hpp"
+
+namespace atomnn {
+
+template <typename T>
+__device__ __forceinline__ void cutoff_fg(int kind, T d, T rc, T rci, T pirci, T& f, T& g) {
+    if (kind == CUT_COS) {  // basis.cpp:86-88, 106-108
+        if (d > rc) {
+            f = T(0);
+            g = T(0);
+        } else {
+            const T a = d * pirci;
+            f = T(0.5) * (cos(a) + T(1));
+            g = T(-0.5) * pirci * sin(a);
+        }
+    } else if (kind == CUT_TANH) {  // basis.cpp:89-92, 109-114
+        const T t = tanh(T(1) - d * rci);
+        f = (d > rc) ? T(0) : t * t * t;
+        g = (d < rc) ? T(-3) * t * t * (T(1) - t * t) * rci : T(0);
+    } else {  // STEP, basis.cpp:83-85, 103-105
+        f = (d > rc) ? T(0) : T(1);
+        g = T(0);
+    }
+}
+
+// math/special.cpp:111-120 (n >= 0)
+template <typename T>
+__device__ __forceinline__ T ipow(T x, int n) {
+    if (n == 0) return T(1);
+    if (x == T(0)) return T(0);
+    T yy = T(1), ww = x;
+    for (; n != 0; n >>= 1, ww *= ww)
+        if (n & 1) yy *= ww;
+    return yy;
+}
+
+template <typename T>
+__device__ __forceinline__ T rpow(T x, T e) {
+    return pow(x, e);
+}
+
+// |(1 + lambda cos)/2|^zeta (and the derivative factor zeta lambda |.|^(zeta-1) / 2)
+template <typename T>
+__device__ __forceinline__ void angular_term(T cs, T lam, T zeta, int zint, T& A, T& dA) {
+    const T acw = fabs(T(0.5) * (T(1) + lam * cs));
+    T p1;
+    if (zint >= 1) p1 = ipow(acw, zint - 1);
+    else if (zint == 0) p1 = T(0);  // zeta = 0: A = 1, dA = 0
+    else p1 = rpow(acw, zeta - T(1));
+    A = (zint == 0) ? T(1) : p1 * acw;
+    dA = T(0.5) * zeta * lam * p1;
+}
+
+template <typename T>
+__device__ __forceinline__ T angular_value(T cs, T lam, T zeta, int zint) {
+    const T acw = fabs(T(0.5) * (T(1) + lam * cs));
+    return zint >= 0 ? ipow(acw, zint) : rpow(acw, zeta);
+}
+
+// Row index of flat pair q in the strict upper triangle of an n x n matrix (row-major).
+__device__ __forceinline__ int tri_row(long long q, int n) {
+    const double b = 2.0 * n - 1.0;
+    int j = (int)floor((b - sqrt(b * b - 8.0 * (double)q)) * 0.5);
+    if (j < 0) j = 0;
+    auto start = [n](int r) { return (long long)r * (2LL * n - r - 1) / 2; };
+    while (j + 1 < n && start(j + 1) <= q) ++j;
+    while (j > 0 && start(j) > q) --j;
+    return j;
+}
+
+struct Segs {
+    int start[kMaxSpecies + 1];
+};
+
+// Stage the P entries of centre i into LDS, partitioned by neighbour species (stable).
+// su[4p..4p+3] = (ux, uy, uz, d); orig[p] = entry index in the reference order.
+template <typename T>
+__device__ __forceinline__ void stage_entries(int i, int P, int maxp, int ntypes, const int* __restrict__ nbr_meta,
+                                              const double4* __restrict__ nbr_r, T* su, int* orig, Segs& sg, int lane) {
+    int count[kMaxSpecies];
+#pragma unroll
+    for (int t = 0; t < kMaxSpecies; ++t) count[t] = 0;
+    const int64_t row = (int64_t)i * maxp;
+    for (int p0 = 0; p0 < P; p0 += kWave) {
+        const int p = p0 + lane;
+        const int t = p < P ? ((nbr_meta[row + p] >> 1) & 127) : -1;
+#pragma unroll
+        for (int s = 0; s < kMaxSpecies; ++s)
+            if (s < ntypes) count[s] += __popcll(__ballot(t == s));
+    }
+    sg.start[0] = 0;
+#pragma unroll
+    for (int s = 0; s < kMaxSpecies; ++s) sg.start[s + 1] = sg.start[s] + (s < ntypes ? count[s] : 0);
+    int fill[kMaxSpecies];
+#pragma unroll
+    for (int s = 0; s < kMaxSpecies; ++s) fill[s] = sg.start[s];
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int p0 = 0; p0 < P; p0 += kWave) {
+        const int p = p0 + lane;
+        int t = -1;
+        double4 r = make_double4(0, 0, 0, 1);
+        if (p < P) {
+            t = (nbr_meta[row + p] >> 1) & 127;
+            r = nbr_r[row + p];
+        }
+        int dst = -1;
+#pragma unroll
+        for (int s = 0; s < kMaxSpecies; ++s) {
+            if (s < ntypes) {
+                const unsigned long long m = __ballot(t == s);
+                if (t == s) dst = fill[s] + __popcll(m & lt);
+                fill[s] += __popcll(m);
+            }
+        }
+        if (p < P) {
+            const double inv = 1.0 / r.w;
+            su[4 * dst + 0] = T(r.x * inv);
+            su[4 * dst + 1] = T(r.y * inv);
+            su[4 * dst + 2] = T(r.z * inv);
+            su[4 * dst + 3] = T(r.w);
+            if (orig) orig[dst] = p;
+        }
+    }
+}
+
+// Radial exponent tables of one angular block for the slots [lo, hi): Rf (and Rd).
+template <typename T, int NE>
+__device__ __forceinline__ void radial_tables(const DevAngular& A, const T* su, int lo, int hi, T* Rf, T* Rd, int lane) {
+    const T rc = T(A.rc), rci = T(A.rci), pirci = T(A.pirci);
+    for (int p = lo + lane; p < hi; p += kWave) {
+        const T d = su[4 * p + 3];
+        T fc, gc;
+        cutoff_fg<T>(A.cut, d, rc, rci, pirci, fc, gc);
+        const T dp = ipow(d, A.alpha);
+        const T ddp = T(A.alpha) * dp / d;  // d(d^alpha)/dd
+#pragma unroll
+        for (int e = 0; e < NE; ++e) {
+            const T ie = T(A.ietap[e]);
+            const T ex = exp(-ie * dp);
+            Rf[p * NE + e] = fc * ex;
+            if (Rd) Rd[p * NE + e] = (gc - ie * ddp * fc) * ex;
+        }
+    }
+}
+
+// LDS sizes (bytes) of the two kernels for a given maxp.
+template <typename T, int NE, int NC>
+__host__ __device__ constexpr size_t symm_lds_bytes(int maxp, int ldg) {
+    return sizeof(T) * (size_t)(4 * maxp + NE * maxp + kWave * NE * NC) + sizeof(T) * (size_t)((ldg + 1) & ~1);
+}
+template <typename T, int NE, int NC>
+__host__ __device__ constexpr size_t force_lds_bytes(int maxp, int ldg) {
+    return sizeof(T) * (size_t)(4 * maxp + 2 * NE * maxp + 4 * maxp + 8 * kWave + ((ldg + 1) & ~1)) +
+           sizeof(int) * (size_t)(maxp + 2 * kWave);
+}
+
+// ------------------------------------------------------------------ descriptors
+template <typename T, int NE, int NC>
+__global__ __launch_bounds__(kWave) void k_symm(const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta,
+                                                const double* __restrict__ rfun_rs, const int* __restrict__ atom_type,
+                                                const int* __restrict__ centres, const int* __restrict__ cnt,
+                                                const int* __restrict__ nbr_meta, const double4* __restrict__ nbr_r,
+                                                int maxp, T* __restrict__ G, int ldg) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    T* su = (T*)lds_raw;
+    T* Rf = su + 4 * maxp;
+    T* red = Rf + NE * maxp;
+    T* Grow = red + kWave * NE * NC;
+    const int i = centres[blockIdx.x];
+    const int lane = threadIdx.x;
+    const int ti = atom_type[i];
+    const int P = min(cnt[i], maxp);
+    const int ntypes = plan->T;
+    Segs sg;
+    stage_entries<T>(i, P, maxp, ntypes, nbr_meta, nbr_r, su, nullptr, sg, lane);
+    for (int k = lane; k < ldg; k += kWave) Grow[k] = T(0);
+    __syncthreads();
+
+    // radial: one lane per function, summing over the neighbours of that species
+    for (int b = 0; b < ntypes; ++b) {
+        const DevRadial& R = plan->rad[ti][b];
+        const T rc = T(R.rc), rci = T(R.rci), pirci = T(R.pirci);
+        for (int f = lane; f < R.n; f += kWave) {
+            const T eta = T(rfun_eta[R.fbase + f]), rs = T(rfun_rs[R.fbase + f]);
+            T acc = T(0);
+            for (int p = sg.start[b]; p < sg.start[b + 1]; ++p) {
+                const T d = su[4 * p + 3];
+                T fc, gc;
+                cutoff_fg<T>(R.cut, d, rc, rci, pirci, fc, gc);
+                acc += exp(-eta * (d - rs) * (d - rs)) * fc;
+            }
+            Grow[R.goff + f] = acc * T(R.norm);
+        }
+    }
+
+    // angular blocks in LMat order
+    for (int bb = 0; bb < ntypes; ++bb) {
+        for (int a = 0; a <= bb; ++a) {
+            const DevAngular& A = plan->ang[ti][bb * (bb + 1) / 2 + a];
+            const int sa = sg.start[a], na = sg.start[a + 1] - sa;
+            const int sb = sg.start[bb], nb = sg.start[bb + 1] - sb;
+            const bool tri = (a == bb);
+            const long long Q = tri ? (long long)na * (na - 1) / 2 : (long long)na * nb;
+            if (Q == 0) continue;  // block stays zero
+            radial_tables<T, NE>(A, su, sa, sa + na, Rf, nullptr, lane);
+            if (!tri) radial_tables<T, NE>(A, su, sb, sb + nb, Rf, nullptr, lane);
+            __syncthreads();
+            T lam[NC], zet[NC];
+            int zin[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                lam[c] = T(A.lambda[c]);
+                zet[c] = T(A.zeta[c]);
+                zin[c] = A.zint[c];
+            }
+            T acc[NE][NC];
+#pragma unroll
+            for (int e = 0; e < NE; ++e)
+#pragma unroll
+                for (int c = 0; c < NC; ++c) acc[e][c] = T(0);
+            const long long L = (Q + kWave - 1) / kWave;
+            const long long q0 = (long long)lane * L, q1 = min(q0 + L, Q);
+            if (q0 < q1) {
+                int j, k;
+                if (tri) {
+                    j = tri_row(q0, na);
+                    k = j + 1 + (int)(q0 - (long long)j * (2LL * na - j - 1) / 2);
+                } else {
+                    j = (int)(q0 / nb);
+                    k = (int)(q0 % nb);
+                }
+                long long q = q0;
+                while (q < q1) {
+                    const int kend = tri ? na : nb;
+                    const int n = (int)min((long long)(kend - k), q1 - q);
+                    const int pj = sa + j;
+                    const T ujx = su[4 * pj], ujy = su[4 * pj + 1], ujz = su[4 * pj + 2];
+                    T rj[NE];
+#pragma unroll
+                    for (int e = 0; e < NE; ++e) rj[e] = Rf[pj * NE + e];
+                    for (int t = 0; t < n; ++t) {
+                        const int pk = sb + k + t;
+                        const T cs = ujx * su[4 * pk] + ujy * su[4 * pk + 1] + ujz * su[4 * pk + 2];
+                        T Ac[NC];
+#pragma unroll
+                        for (int c = 0; c < NC; ++c) Ac[c] = angular_value<T>(cs, lam[c], zet[c], zin[c]);
+#pragma unroll
+                        for (int e = 0; e < NE; ++e) {
+                            const T pe = rj[e] * Rf[pk * NE + e];
+#pragma unroll
+                            for (int c = 0; c < NC; ++c) acc[e][c] += Ac[c] * pe;
+                        }
+                    }
+                    q += n;
+                    ++j;
+                    k = tri ? j + 1 : 0;
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < NE; ++e)
+#pragma unroll
+                for (int c = 0; c < NC; ++c) red[(e * NC + c) * kWave + lane] = acc[e][c];
+            __syncthreads();
+            for (int t = lane; t < NE * NC; t += kWave) {
+                const int e = t / NC, c = t % NC;
+                const int f = A.fmap[e * kMaxC + c];
+                if (f >= 0) {
+                    T s = T(0);
+                    for (int l = 0; l < kWave; ++l) s += red[t * kWave + l];
+                    Grow[A.goff + f] = s * T(A.norm);
+                }
+            }
+            __syncthreads();
+        }
+    }
+    __syncthreads();
+    for (int k = lane; k < ldg; k += kWave) G[(int64_t)i * ldg + k] = Grow[k];
+}
+
+// ------------------------------------------------------------------ force terms
+template <typename T, int NE, int NC>
+__global__ __launch_bounds__(kWave) void k_force(const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta,
+                                                 const double* __restrict__ rfun_rs, const int* __restrict__ atom_type,
+                                                 const int* __restrict__ centres, const int* __restrict__ cnt,
+                                                 const int* __restrict__ nbr_meta, const double4* __restrict__ nbr_r,
+                                                 int maxp, const T* __restrict__ dEdG, int ldg, T* __restrict__ Xout) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    T* su = (T*)lds_raw;
+    T* Rf = su + 4 * maxp;
+    T* Rd = Rf + NE * maxp;
+    T* Xs = Rd + NE * maxp;           // [maxp][4]
+    T* slots = Xs + 4 * maxp;         // [2][kWave][4]
+    T* W = slots + 8 * kWave;         // dE/dG row
+    int* orig = (int*)(W + ((ldg + 1) & ~1));
+    int* slot_row = orig + maxp;      // [2][kWave]
+    const int i = centres[blockIdx.x];
+    const int lane = threadIdx.x;
+    const int ti = atom_type[i];
+    const int P = min(cnt[i], maxp);
+    const int ntypes = plan->T;
+    Segs sg;
+    stage_entries<T>(i, P, maxp, ntypes, nbr_meta, nbr_r, su, orig, sg, lane);
+    for (int k = lane; k < ldg; k += kWave) W[k] = dEdG[(int64_t)i * ldg + k];
+    __syncthreads();
+
+    // radial term, one lane per entry: X_j = -dE/dd_j u_j
+    for (int p = lane; p < P; p += kWave) {
+        int b = 0;
+        while (b + 1 < ntypes && p >= sg.start[b + 1]) ++b;
+        const DevRadial& R = plan->rad[ti][b];
+        const T d = su[4 * p + 3];
+        T fc, gc;
+        cutoff_fg<T>(R.cut, d, T(R.rc), T(R.rci), T(R.pirci), fc, gc);
+        T s = T(0);
+        for (int f = 0; f < R.n; ++f) {
+            const T eta = T(rfun_eta[R.fbase + f]), rs = T(rfun_rs[R.fbase + f]);
+            const T x = d - rs;
+            s += W[R.goff + f] * exp(-eta * x * x) * (T(-2) * eta * x * fc + gc);
+        }
+        s *= T(R.norm);
+        Xs[4 * p + 0] = -s * su[4 * p];
+        Xs[4 * p + 1] = -s * su[4 * p + 1];
+        Xs[4 * p + 2] = -s * su[4 * p + 2];
+    }
+    __syncthreads();
+
+    for (int bb = 0; bb < ntypes; ++bb) {
+        for (int a = 0; a <= bb; ++a) {
+            const DevAngular& A = plan->ang[ti][bb * (bb + 1) / 2 + a];
+            const int sa = sg.start[a], na = sg.start[a + 1] - sa;
+            const int sb = sg.start[bb], nb = sg.start[bb + 1] - sb;
+            const bool same = (a == bb);
+            if (same ? (na < 2) : (na == 0 || nb == 0)) continue;
+            radial_tables<T, NE>(A, su, sa, sa + na, Rf, Rd, lane);
+            if (!same) radial_tables<T, NE>(A, su, sb, sb + nb, Rf, Rd, lane);
+            T w[NE][NC];
+#pragma unroll
+            for (int e = 0; e < NE; ++e)
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    const int f = A.fmap[e * kMaxC + c];
+                    w[e][c] = f >= 0 ? W[A.goff + f] * T(A.norm) : T(0);
+                }
+            T lam[NC], zet[NC];
+            int zin[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                lam[c] = T(A.lambda[c]);
+                zet[c] = T(A.zeta[c]);
+                zin[c] = A.zint[c];
+            }
+            __syncthreads();
+            // ordered pairs: rows in one segment, columns in the other (or the same, minus the diagonal)
+            for (int pass = 0; pass < (same ? 1 : 2); ++pass) {
+                const int rs0 = pass == 0 ? sa : sb, nrow = pass == 0 ? na : nb;
+                const int cs0 = pass == 0 ? sb : sa, ncol = pass == 0 ? nb : na;
+                const int ncol_eff = same ? ncol - 1 : ncol;
+                const long long Q = (long long)nrow * ncol_eff;
+                const long long L = (Q + kWave - 1) / kWave;
+                const long long q0 = (long long)lane * L, q1 = min(q0 + L, Q);
+                slot_row[lane] = -1;
+                slot_row[kWave + lane] = -1;
+                if (q0 < q1) {
+                    int r = (int)(q0 / ncol_eff), c = (int)(q0 % ncol_eff);
+                    long long q = q0;
+                    while (q < q1) {
+                        const int n = (int)min((long long)(ncol_eff - c), q1 - q);
+                        const bool whole = (c == 0 && n == ncol_eff);
+                        const int pj = rs0 + r;
+                        const T ujx = su[4 * pj], ujy = su[4 * pj + 1], ujz = su[4 * pj + 2], dj = su[4 * pj + 3];
+                        T v[NE][NC], vd[NE][NC];
+#pragma unroll
+                        for (int e = 0; e < NE; ++e) {
+                            const T rfj = Rf[pj * NE + e], rdj = Rd[pj * NE + e];
+#pragma unroll
+                            for (int cc = 0; cc < NC; ++cc) {
+                                v[e][cc] = w[e][cc] * rfj;
+                                vd[e][cc] = w[e][cc] * rdj;
+                            }
+                        }
+                        T xx = T(0), xy = T(0), xz = T(0);
+                        for (int t = 0; t < n; ++t) {
+                            const int cc0 = c + t;
+                            const int k = (same && cc0 >= r) ? cc0 + 1 : cc0;
+                            const int pk = cs0 + k;
+                            const T ukx = su[4 * pk], uky = su[4 * pk + 1], ukz = su[4 * pk + 2];
+                            const T cs = ujx * ukx + ujy * uky + ujz * ukz;
+                            T tp[NC], tq[NC];
+#pragma unroll
+                            for (int cc = 0; cc < NC; ++cc) {
+                                tp[cc] = T(0);
+                                tq[cc] = T(0);
+                            }
+#pragma unroll
+                            for (int e = 0; e < NE; ++e) {
+                                const T rfk = Rf[pk * NE + e];
+#pragma unroll
+                                for (int cc = 0; cc < NC; ++cc) {
+                                    tp[cc] += v[e][cc] * rfk;
+                                    tq[cc] += vd[e][cc] * rfk;
+                                }
+                            }
+                            T dEdcos = T(0), dEdd = T(0);
+#pragma unroll
+                            for (int cc = 0; cc < NC; ++cc) {
+                                T Av, dAv;
+                                angular_term<T>(cs, lam[cc], zet[cc], zin[cc], Av, dAv);
+                                dEdcos += dAv * tp[cc];
+                                dEdd += Av * tq[cc];
+                            }
+                            const T g = dEdcos / dj;
+                            xx -= dEdd * ujx + g * (ukx - cs * ujx);
+                            xy -= dEdd * ujy + g * (uky - cs * ujy);
+                            xz -= dEdd * ujz + g * (ukz - cs * ujz);
+                        }
+                        if (whole) {
+                            Xs[4 * pj] += xx;
+                            Xs[4 * pj + 1] += xy;
+                            Xs[4 * pj + 2] += xz;
+                        } else {
+                            const int sl = (q == q0) ? 0 : 1;
+                            slot_row[sl * kWave + lane] = pj;
+                            slots[(sl * kWave + lane) * 4 + 0] = xx;
+                            slots[(sl * kWave + lane) * 4 + 1] = xy;
+                            slots[(sl * kWave + lane) * 4 + 2] = xz;
+                        }
+                        q += n;
+                        ++r;
+                        c = 0;
+                    }
+                }
+                __syncthreads();
+                // rows split between lanes: add the partial sums in lane order
+                for (int r = lane; r < nrow; r += kWave) {
+                    const int pj = rs0 + r;
+                    const long long lo = (long long)r * ncol_eff, hi = lo + ncol_eff - 1;
+                    const int lf = (int)(lo / L), ll = (int)min((long long)(kWave - 1), hi / L);
+                    for (int l = lf; l <= ll; ++l) {
+#pragma unroll
+                        for (int sl = 0; sl < 2; ++sl) {
+                            if (slot_row[sl * kWave + l] == pj) {
+                                Xs[4 * pj] += slots[(sl * kWave + l) * 4 + 0];
+                                Xs[4 * pj + 1] += slots[(sl * kWave + l) * 4 + 1];
+                                Xs[4 * pj + 2] += slots[(sl * kWave + l) * 4 + 2];
+                            }
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+    }
+    const int64_t row = (int64_t)i * maxp;
+    for (int p = lane; p < P; p += kWave) {
+        const int o = orig[p];
+        Xout[4 * (row + o) + 0] = Xs[4 * p];
+        Xout[4 * (row + o) + 1] = Xs[4 * p + 1];
+        Xout[4 * (row + o) + 2] = Xs[4 * p + 2];
+        Xout[4 * (row + o) + 3] = T(0);
+    }
+}
+
+// F_a = sum_s X[a,s] - sum_{s: partner(a,s) >= 0} X[partner(a,s)], accumulated in FP64.
+template <typename T>
+__global__ void k_assemble(const int* __restrict__ cnt, const int* __restrict__ partner, const T* __restrict__ X,
+                           int maxp, int natoms_total, double* __restrict__ F) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= natoms_total) return;
+    const int P = min(cnt[a], maxp);
+    const int64_t row = (int64_t)a * maxp;
+    double fx = 0, fy = 0, fz = 0;
+    for (int s = 0; s < P; ++s) {
+        fx += (double)X[4 * (row + s)];
+        fy += (double)X[4 * (row + s) + 1];
+        fz += (double)X[4 * (row + s) + 2];
+    }
+    for (int s = 0; s < P; ++s) {
+        const int p = partner[row + s];
+        if (p >= 0) {
+            fx -= (double)X[4 * (int64_t)p];
+            fy -= (double)X[4 * (int64_t)p + 1];
+            fz -= (double)X[4 * (int64_t)p + 2];
+        }
+    }
+    F[3 * (int64_t)a] = fx;
+    F[3 * (int64_t)a + 1] = fy;
+    F[3 * (int64_t)a + 2] = fz;
+}
+
+// ------------------------------------------------------------------ launchers
+template <typename T, int NE, int NC>
+static hipError_t launch_symm_t(const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
+                                const int* centres, int ncentres, const int* cnt, const int* meta, const double4* r,
+                                int maxp, T* G, int ldg, hipStream_t st) {
+    const size_t lds = symm_lds_bytes<T, NE, NC>(maxp, ldg);
+    hipLaunchKernelGGL((k_symm<T, NE, NC>), dim3(ncentres), dim3(kWave), lds, st, plan, rfe, rfr, atom_type, centres, cnt,
+                       meta, r, maxp, G, ldg);
+    return hipGetLastError();
+}
+
+template <typename T, int NE, int NC>
+static hipError_t launch_force_t(const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
+                                 const int* centres, int ncentres, const int* cnt, const int* meta, const double4* r,
+                                 int maxp, const T* dEdG, int ldg, T* X, hipStream_t st) {
+    const size_t lds = force_lds_bytes<T, NE, NC>(maxp, ldg);
+    hipLaunchKernelGGL((k_force<T, NE, NC>), dim3(ncentres), dim3(kWave), lds, st, plan, rfe, rfr, atom_type, centres,
+                       cnt, meta, r, maxp, dEdG, ldg, X);
+    return hipGetLastError();
+}
+
+// Grid shapes instantiated: (NE, NC) in {(2,4), (4,4), (8,8)}.
+int grid_shape(int ne, int nc) {
+    if (ne <= 2 && nc <= 4) return 0;
+    if (ne <= 4 && nc <= 4) return 1;
+    return 2;
+}
+
+size_t symm_lds(int shape, bool fp64, int maxp, int ldg) {
+    if (fp64) {
+        if (shape == 0) return symm_lds_bytes<double, 2, 4>(maxp, ldg);
+        if (shape == 1) return symm_lds_bytes<double, 4, 4>(maxp, ldg);
+        return symm_lds_bytes<double, 8, 8>(maxp, ldg);
+    }
+    if (shape == 0) return symm_lds_bytes<float, 2, 4>(maxp, ldg);
+    if (shape == 1) return symm_lds_bytes<float, 4, 4>(maxp, ldg);
+    return symm_lds_bytes<float, 8, 8>(maxp, ldg);
+}
+
+size_t force_lds(int shape, bool fp64, int maxp, int ldg) {
+    if (fp64) {
+        if (shape == 0) return force_lds_bytes<double, 2, 4>(maxp, ldg);
+        if (shape == 1) return force_lds_bytes<double, 4, 4>(maxp, ldg);
+        return force_lds_bytes<double, 8, 8>(maxp, ldg);
+    }
+    if (shape == 0) return force_lds_bytes<float, 2, 4>(maxp, ldg);
+    if (shape == 1) return force_lds_bytes<float, 4, 4>(maxp, ldg);
+    return force_lds_bytes<float, 8, 8>(maxp, ldg);
+}
+
+template <typename T>
+hipError_t launch_symm(int shape, const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
+                       const int* centres, int ncentres, const int* cnt, const int* meta, const double4* r, int maxp,
+                       T* G, int ldg, hipStream_t st) {
+    if (shape == 0) return launch_symm_t<T, 2, 4>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, G, ldg, st);
+    if (shape == 1) return launch_symm_t<T, 4, 4>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, G, ldg, st);
+    return launch_symm_t<T, 8, 8>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, G, ldg, st);
+}
+
+template <typename T>
+hipError_t launch_force(int shape, const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
+                        const int* centres, int ncentres, const int* cnt, const int* meta, const double4* r, int maxp,
+                        const T* dEdG, int ldg, T* X, hipStream_t st) {
+    if (shape == 0) return launch_force_t<T, 2, 4>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, dEdG, ldg, X, st);
+    if (shape == 1) return launch_force_t<T, 4, 4>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, dEdG, ldg, X, st);
+    return launch_force_t<T, 8, 8>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, dEdG, ldg, X, st);
+}
+
+template <typename T>
+hipError_t launch_assemble(const int* cnt, const int* partner, const T* X, int maxp, int ntot, double* F, hipStream_t st) {
+    hipLaunchKernelGGL((k_assemble<T>), dim3((ntot + 255) / 256), dim3(256), 0, st, cnt, partner, X, maxp, ntot, F);
+    return hipGetLastError();
+}
+
+template hipError_t launch_symm<double>(int, const DevPlan*, const double*, const double*, const int*, const int*, int,
+                                        const int*, const int*, const double4*, int, double*, int, hipStream_t);
+template hipError_t launch_symm<float>(int, const DevPlan*, const double*, const double*, const int*, const int*, int,
+                                       const int*, const int*, const double4*, int, float*, int, hipStream_t);
+template hipError_t launch_force<double>(int, const DevPlan*, const double*, const double*, const int*, const int*, int,
+                                         const int*, const int*, const double4*, int, const double*, int, double*,
+                                         hipStream_t);
+template hipError_t launch_force<float>(int, const DevPlan*, const double*, const double*, const int*, const int*, int,
+                                        const int*, const int*, const double4*, int, const float*, int, float*,
+                                        hipStream_t);
+template hipError_t launch_assemble<double>(const int*, const int*, const double*, int, int, double*, hipStream_t);
+template hipError_t launch_assemble<float>(const int*, const int*, const float*, int, int, double*, hipStream_t);
+
+}  // namespace atomnn

@@ -1,0 +1,187 @@
+"""Python mirror of AtomNN's NNP evaluation interface over the engine's C ABI.
+
+The engine is libatomnn_amd.so (HIP kernels for gfx950 behind include/nnp_capi.h).  This
+module only marshals arrays; every number comes from the GPU.  There is no CPU fallback:
+if the library or a GPU is missing, the calls raise.
+
+Reference interface mirrored (AtomNN src/):
+  NNP.read(path)          NNP::read                      nnp/nnp.cpp:182-191, 244-308
+  NNP.index(name)         NNP::index                     nnp/nnp.hpp:83-84
+  NNP.symm(struc)         NNP::init + NNP::symm          nnp/nnp.cpp:317-381
+  NNP.energy(struc)       NNP::energy (with E_vac)       nnp/nnp.cpp:390-402
+  NNP.force(struc)        NNP::force                     nnp/nnp.cpp:416-482
+  NNP.compute(struc)      NNP::compute (no E_vac)        nnp/nnp.cpp:496-566
+  NNP.neighbors(struc)    NeighborList::build            struc/neighbor.cpp:59-171
+Errors raise NNPError carrying the engine's code and message (the reference throws
+std::invalid_argument / std::runtime_error with the same texts).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from .synth import Structure
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libatomnn_amd.so")
+
+NNP_FORCE = 0x1
+NNP_ATOM_ENERGY = 0x2
+NNP_VACUUM_ENERGY = 0x4
+NNP_ALL_IMAGES = 0x8
+NNP_PREC_MIXED = 0x10
+
+ERRORS = {-22: "EINVAL", -5: "EIO", -12: "ENOMEM", -100: "EDEVICE", -101: "EBUDGET", -102: "EUNSUPPORTED",
+          -11: "EAGAIN"}
+
+
+class NNPError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class _Struc(ctypes.Structure):
+    _fields_ = [("natoms", ctypes.c_int32), ("type", ctypes.c_void_p), ("pos", ctypes.c_void_p),
+                ("cell", ctypes.c_void_p)]
+
+
+class _Result(ctypes.Structure):
+    _fields_ = [("energy", ctypes.c_double), ("energy_atom", ctypes.c_void_p), ("force", ctypes.c_void_p)]
+
+
+_lib = None
+
+
+def lib():
+    """Load the engine library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: the HIP engine must be built (__graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i32, i64, dp = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+        L.nnp_last_error.restype = ctypes.c_char_p
+        L.nnp_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp)]
+        L.nnp_load.restype = ctypes.c_int
+        L.nnp_free.argtypes = [vp]
+        L.nnp_ntypes.argtypes = [vp]
+        L.nnp_rc.argtypes = [vp]
+        L.nnp_rc.restype = dp
+        L.nnp_ninput.argtypes = [vp, ctypes.c_int]
+        L.nnp_type_index.argtypes = [vp, ctypes.c_char_p]
+        L.nnp_neighbors.argtypes = [vp, ctypes.POINTER(_Struc), vp, i64, vp, vp, vp, vp, vp]
+        L.nnp_neighbors.restype = i64
+        L.nnp_symm.argtypes = [vp, ctypes.POINTER(_Struc), vp, ctypes.c_int]
+        L.nnp_compute.argtypes = [vp, ctypes.POINTER(_Struc), ctypes.c_uint32, ctypes.POINTER(_Result)]
+        L.nnp_compute_batch.argtypes = [vp, ctypes.c_int, ctypes.POINTER(_Struc), ctypes.c_uint32,
+                                        ctypes.POINTER(_Result)]
+        _lib = L
+    return _lib
+
+
+def _check(code: int) -> int:
+    if code < 0:
+        raise NNPError(code, lib().nnp_last_error().decode())
+    return code
+
+
+class _Pinned:
+    """Keeps the numpy arrays of a marshalled structure alive for the duration of a call."""
+
+    def __init__(self, s: Structure):
+        self.t = np.ascontiguousarray(s.types, dtype=np.int32)
+        self.p = np.ascontiguousarray(s.pos, dtype=np.float64)
+        self.c = None if s.cell is None else np.ascontiguousarray(s.cell, dtype=np.float64).reshape(9)
+        self.c_struct = _Struc(int(self.t.shape[0]), self.t.ctypes.data, self.p.ctypes.data,
+                               None if self.c is None else self.c.ctypes.data)
+
+
+class NNP:
+    """A potential resident on the GPU."""
+
+    def __init__(self, handle, path: str):
+        self._h = handle
+        self.path = path
+
+    @staticmethod
+    def read(path: str) -> "NNP":
+        h = ctypes.c_void_p()
+        _check(lib().nnp_load(path.encode(), ctypes.byref(h)))
+        return NNP(h, path)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().nnp_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def ntypes(self) -> int:
+        return lib().nnp_ntypes(self._h)
+
+    @property
+    def rc(self) -> float:
+        return lib().nnp_rc(self._h)
+
+    def ninput(self, t: int) -> int:
+        return _check(lib().nnp_ninput(self._h, t))
+
+    def index(self, name: str) -> int:
+        return lib().nnp_type_index(self._h, name.encode())
+
+    def neighbors(self, s: Structure) -> dict:
+        ps = _Pinned(s)
+        n = s.natoms
+        counts = np.zeros(n, dtype=np.int32)
+        tot = _check(lib().nnp_neighbors(self._h, ctypes.byref(ps.c_struct), counts.ctypes.data, 0, None, None,
+                                         None, None, None))
+        idx = np.zeros(tot, dtype=np.int32)
+        img = np.zeros(tot, dtype=np.int32)
+        mn = np.zeros(tot, dtype=np.int32)
+        r = np.zeros((tot, 3))
+        d = np.zeros(tot)
+        _check(lib().nnp_neighbors(self._h, ctypes.byref(ps.c_struct), counts.ctypes.data, tot, idx.ctypes.data,
+                                   img.ctypes.data, mn.ctypes.data, r.ctypes.data, d.ctypes.data))
+        offsets = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+        return dict(counts=counts, offsets=offsets, index=idx, image=img, min=mn, r=r, d=d)
+
+    def symm(self, s: Structure) -> np.ndarray:
+        ps = _Pinned(s)
+        ldg = max(self.ninput(t) for t in range(self.ntypes))
+        G = np.zeros((s.natoms, ldg))
+        _check(lib().nnp_symm(self._h, ctypes.byref(ps.c_struct), G.ctypes.data, ldg))
+        return G
+
+    def compute(self, s: Structure, flags: int = NNP_FORCE | NNP_ATOM_ENERGY | NNP_VACUUM_ENERGY
+                ) -> Tuple[float, np.ndarray, Optional[np.ndarray]]:
+        (out,) = self.compute_batch([s], flags)
+        return out
+
+    def compute_batch(self, strucs: Sequence[Structure], flags: int = NNP_FORCE | NNP_ATOM_ENERGY | NNP_VACUUM_ENERGY
+                      ) -> List[Tuple[float, np.ndarray, Optional[np.ndarray]]]:
+        pins = [_Pinned(s) for s in strucs]
+        arr = (_Struc * len(strucs))(*[p.c_struct for p in pins])
+        eas = [np.zeros(s.natoms) for s in strucs]
+        fs = [np.zeros((s.natoms, 3)) for s in strucs]
+        res = (_Result * len(strucs))(*[_Result(0.0, ea.ctypes.data, f.ctypes.data) for ea, f in zip(eas, fs)])
+        _check(lib().nnp_compute_batch(self._h, len(strucs), arr, flags | NNP_ATOM_ENERGY, res))
+        out = []
+        for k in range(len(strucs)):
+            out.append((res[k].energy, eas[k], fs[k] if flags & NNP_FORCE else None))
+        return out
+
+    # reference-named conveniences
+    def energy(self, s: Structure) -> float:
+        return self.compute(s, NNP_VACUUM_ENERGY)[0]
+
+    def force(self, s: Structure) -> np.ndarray:
+        return self.compute(s, NNP_FORCE | NNP_VACUUM_ENERGY)[2]

@@ -1,0 +1,132 @@
+/*
+ * nnp_capi.h — C ABI of the MI355X Behler-Parrinello energy/force engine.
+ *
+ * Drop-in for AtomNN's CPU evaluation path (the nnp_compute tool): a structure goes in,
+ * the total energy, per-atom energies and forces come out.  Plain pointers and sizes
+ * only; no C++ types, no exceptions cross this boundary.  Every call returns 0 on
+ * success or a negative NNP_E* code; nnp_last_error() then holds a message (the text
+ * of the reference's exception where there is one).
+ *
+ * Reference interfaces replaced (paths relative to AtomNN's src/):
+ *   nnp_load            <- NNP::read(const char*, NNP&)            nnp/nnp.cpp:182-191, 244-308
+ *   nnp_type_index      <- NNP::index(const char*)                 nnp/nnp.hpp:83-84
+ *   nnp_ntypes/nnp_rc/nnp_ninput <- NNP::ntypes(), NNP::rc(), NNH::nInput()   nnp/nnp.hpp, nnp/nnh.hpp:91
+ *   nnp_neighbors       <- NeighborList::build(const Structure&, double)  struc/neighbor.cpp:59-171
+ *   nnp_symm            <- NNP::init + NNP::symm                   nnp/nnp.cpp:317-381
+ *   nnp_compute         <- NNP::symm + NNP::energy + NNP::force    nnp/nnp.cpp:334-482, as run by
+ *                          nnpce.cpp:275-279; with NNP_VACUUM_ENERGY clear it is NNP::compute
+ *                          (nnp/nnp.cpp:496-566), which omits the vacuum energy (nnp.cpp:509)
+ *   nnp_compute_batch   <- the per-structure loop of nnpce.cpp:273-281
+ *   nnp_free            <- NNP destructor
+ *
+ * Units and conventions follow the reference: positions in Angstrom, energies in the units
+ * of the potential file.  Lattice vectors are given as three consecutive 3-vectors
+ * a = cell[0..2], b = cell[3..5], c = cell[6..8] (the columns of AtomNN's R,
+ * struc/cell.hpp:22).  cell == NULL means a non-periodic structure (NeighborList::build's
+ * open branch, neighbor.cpp:149-170).  Species indices are the order of the "name" lines
+ * in the potential file (nnp/nnp.cpp:256-261).
+ *
+ * Threading: one host thread per model handle at a time.  The serving API (nnp_server_*)
+ * is internally synchronised.
+ */
+#ifndef ATOMNN_AMD_NNP_CAPI_H
+#define ATOMNN_AMD_NNP_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ---- */
+#define NNP_OK 0
+#define NNP_EINVAL (-22)   /* invalid argument / malformed potential file */
+#define NNP_EIO (-5)       /* file could not be opened or read */
+#define NNP_ENOMEM (-12)   /* device or host allocation failed */
+#define NNP_EDEVICE (-100) /* HIP runtime error */
+#define NNP_EBUDGET (-101) /* request does not fit the serving HBM budget */
+#define NNP_EUNSUPPORTED (-102) /* potential uses a form this engine does not implement */
+#define NNP_EAGAIN (-11)   /* serving: result not ready yet */
+
+/* ---- flags ---- */
+#define NNP_FORCE 0x1u          /* compute forces */
+#define NNP_ATOM_ENERGY 0x2u    /* write per-atom energies */
+#define NNP_VACUUM_ENERGY 0x4u  /* add the species vacuum energy per atom (NNP::energy, nnh.cpp:129) */
+#define NNP_ALL_IMAGES 0x8u     /* Newton reaction on every periodic image (true -dE/dr); default is
+                                   the reference rule: reaction on the minimum image only
+                                   (nnp.cpp:445, 471-478) */
+#define NNP_PREC_MIXED 0x10u    /* precision form: FP32 symmetry functions, BF16 network GEMMs
+                                   with FP32 accumulation, FP64 displacements and reductions */
+
+typedef struct nnp_model nnp_model;
+
+typedef struct {
+    int32_t natoms;
+    const int32_t* type; /* [natoms] species index */
+    const double* pos;   /* [natoms][3] Cartesian positions */
+    const double* cell;  /* [9] lattice vectors a,b,c consecutively, or NULL (open) */
+} nnp_structure;
+
+typedef struct {
+    double energy;       /* total energy */
+    double* energy_atom; /* [natoms] or NULL */
+    double* force;       /* [natoms][3] or NULL */
+} nnp_result;
+
+/* Last error message of the calling thread ("" if none). */
+const char* nnp_last_error(void);
+
+/* Parse an AtomNN "ann" potential file and upload it to the current HIP device. */
+int nnp_load(const char* ann_path, nnp_model** out);
+void nnp_free(nnp_model* m);
+
+int nnp_ntypes(const nnp_model* m);
+double nnp_rc(const nnp_model* m);
+int nnp_ninput(const nnp_model* m, int type);
+/* Species index of a name via the reference's string hash (str/string.cpp:17-28); -1 if absent. */
+int nnp_type_index(const nnp_model* m, const char* name);
+
+/* Neighbour list in the reference's order (NeighborList::build).  counts[natoms] receives the
+ * number of entries per atom.  If the total fits in `cap`, the entries are written in order:
+ * index[] (neighbour atom), image[] (linear image index in the reference's enumeration,
+ * -1 for open structures), is_min[] (the reference's min flag), r[][3] (vector from the
+ * neighbour to the centre) and d[] (its length).  Returns the total number of entries
+ * (>= 0) or a negative error code. */
+int64_t nnp_neighbors(nnp_model* m, const nnp_structure* s, int32_t* counts, int64_t cap, int32_t* index,
+                      int32_t* image, int32_t* is_min, double* r, double* d);
+
+/* Symmetry-function descriptors (NNP::symm): G[natoms][ldg], zero padded past nnp_ninput(type). */
+int nnp_symm(nnp_model* m, const nnp_structure* s, double* G, int ldg);
+
+/* Energy (and forces with NNP_FORCE) of one structure, synchronous. */
+int nnp_compute(nnp_model* m, const nnp_structure* s, uint32_t flags, nnp_result* r);
+
+/* n structures packed into shared launches; results per structure. */
+int nnp_compute_batch(nnp_model* m, int n, const nnp_structure* s, uint32_t flags, nnp_result* r);
+
+/* ---- device-resident batches (engine extension; no counterpart in the reference) ----
+ * For callers that evaluate the same structures repeatedly with inputs kept in HBM (benchmarks,
+ * serving, MD drivers): nnp_resident_load uploads a batch and sizes its neighbour rows,
+ * nnp_resident_run enqueues one full pass (neighbour list, descriptors, networks, forces,
+ * energies) on the model's stream without a host round trip, nnp_resident_fetch waits and copies
+ * the results out.  Any other compute call on the same model replaces the resident batch. */
+int nnp_resident_load(nnp_model* m, int n, const nnp_structure* s, uint32_t flags);
+int nnp_resident_run(nnp_model* m, uint32_t flags);
+int nnp_resident_fetch(nnp_model* m, uint32_t flags, nnp_result* r);
+int nnp_sync(nnp_model* m);
+
+/* ---- stage timing (engine extension) ----
+ * With profiling on, every pass brackets its stages with HIP events on the model's stream.
+ * nnp_profile_read returns accumulated milliseconds and call counts per stage:
+ * 0 neighbour list, 1 descriptors, 2 networks, 3 force terms, 4 force assembly, 5 energy. */
+#define NNP_NSTAGES 6
+int nnp_profile(nnp_model* m, int enable);
+int nnp_profile_read(nnp_model* m, double* ms, int64_t* calls, int nstage);
+/* Engine state: maxp, max entries, atoms, structures, kernel shape, NE, NC, descriptor width. */
+int nnp_info(nnp_model* m, int64_t* out, int nout);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ATOMNN_AMD_NNP_CAPI_H */

@@ -1,0 +1,157 @@
+"""Parity of the HIP engine against the CPU oracle (FP64, <= 1e-10 relative).
+
+Relative error conventions (plain FP64 form):
+  energy: |dE| / max(|E|, sum_i |E_i|)
+  forces: max_i |dF_i| / max_i |F_i|
+  descriptors: max |dG| / max |G| per structure
+Neighbour lists must match exactly in count, order, neighbour index, image index and min flag.
+"""
+import numpy as np
+import pytest
+
+from atomnn_amd import synth
+from atomnn_amd.nnp import NNP, NNP_ALL_IMAGES, NNP_FORCE, NNP_PREC_MIXED, NNP_VACUUM_ENERGY
+import oracle_binding as ob
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def rel_e(E, Eo, Eao):
+    return abs(E - Eo) / max(abs(Eo), np.abs(Eao).sum())
+
+
+def rel_f(F, Fo):
+    return np.linalg.norm(F - Fo, axis=1).max() / np.linalg.norm(Fo, axis=1).max()
+
+
+@pytest.fixture(scope="module")
+def models(potentials):
+    out = {k: (NNP.read(v), ob.OracleModel(v)) for k, v in potentials.items() if k in ("plain64", "plain4096",
+                                                                                      "mixed64x4096")}
+    yield out
+    for m, _ in out.values():
+        m.close()
+
+
+def small_batch(k=6):
+    return synth.batch512(k)
+
+
+def check_neighbors(m, s, rc):
+    got = m.neighbors(s)
+    ref = ob.neighbors(s.types, s.pos, s.cell, rc)
+    np.testing.assert_array_equal(got["counts"], ref["counts"])
+    np.testing.assert_array_equal(got["index"], ref["index"])
+    np.testing.assert_array_equal(got["min"], ref["min"])
+    if s.cell is not None and ref["shell"].max() > 0:
+        np.testing.assert_array_equal(got["image"], ref["image"])
+    np.testing.assert_allclose(got["r"], ref["r"], rtol=0, atol=1e-11)
+    np.testing.assert_allclose(got["d"], ref["d"], rtol=0, atol=1e-11)
+
+
+def test_neighbors_plain64(models):
+    m, o = models["plain64"]
+    check_neighbors(m, synth.plain64()[0], o.rc)
+
+
+def test_neighbors_cell_list_plain4096(models):
+    m, o = models["plain4096"]
+    check_neighbors(m, synth.plain4096()[0], o.rc)
+
+
+def test_neighbors_triclinic_small_cells(models):
+    m, o = models["plain4096"]
+    for s in small_batch(8):
+        check_neighbors(m, s, o.rc)
+
+
+@pytest.mark.parametrize("name", ["plain64", "mixed64x4096"])
+def test_symm_parity(models, name):
+    m, o = models[name]
+    if name == "plain64":
+        s = synth.plain64()[0]
+    else:  # 3 species, 300 atoms, periodic
+        cell = synth.cubic_cell(300, 0.06)
+        s = synth.Structure(np.array([i % 3 for i in range(300)], dtype=np.int32),
+                            synth.random_positions(300, cell, 1.8, 5), cell)
+    G = m.symm(s)
+    Go = o.symm(s.types, s.pos, s.cell)
+    assert np.abs(G - Go).max() / np.abs(Go).max() <= TOL
+
+
+def test_compute_plain64(models):
+    m, o = models["plain64"]
+    s = synth.plain64()[0]
+    E, Ea, F = m.compute(s)
+    Eo, Eao, Fo = o.compute(s.types, s.pos, s.cell)
+    assert rel_e(E, Eo, Eao) <= TOL
+    assert np.abs(Ea - Eao).max() / np.abs(Eao).max() <= TOL
+    assert rel_f(F, Fo) <= TOL
+
+
+def test_compute_plain4096(models):
+    m, o = models["plain4096"]
+    s = synth.plain4096()[0]
+    E, Ea, F = m.compute(s)
+    Eo, Eao, Fo = o.compute(s.types, s.pos, s.cell)
+    assert rel_e(E, Eo, Eao) <= TOL
+    assert rel_f(F, Fo) <= TOL
+
+
+def test_batch_parity_triclinic(models):
+    m, o = models["plain4096"]
+    strucs = small_batch(8)
+    res = m.compute_batch(strucs)
+    for s, (E, Ea, F) in zip(strucs, res):
+        Eo, Eao, Fo = o.compute(s.types, s.pos, s.cell)
+        assert rel_e(E, Eo, Eao) <= TOL
+        assert rel_f(F, Fo) <= TOL
+
+
+def test_batch_equals_single(models):
+    """Each structure's result is bit-identical alone and inside a batch."""
+    m, _ = models["plain4096"]
+    strucs = small_batch(5)
+    res = m.compute_batch(strucs)
+    for s, (E, Ea, F) in zip(strucs, res):
+        E1, Ea1, F1 = m.compute(s)
+        assert E1 == E
+        np.testing.assert_array_equal(F1, F)
+
+
+def test_all_images_and_no_vacuum(models):
+    m, o = models["plain64"]
+    s = synth.plain64()[0]
+    for flags in (NNP_FORCE | NNP_ALL_IMAGES, NNP_FORCE):
+        E, Ea, F = m.compute(s, flags)
+        Eo, Eao, Fo = o.compute(s.types, s.pos, s.cell, (ob.F_FORCE | (ob.F_ALL_IMAGES if flags & NNP_ALL_IMAGES
+                                                                        else 0)))
+        assert rel_e(E, Eo, Eao) <= TOL
+        assert rel_f(F, Fo) <= TOL
+
+
+def test_open_structure(models):
+    m, o = models["plain4096"]
+    rng = np.random.default_rng(3)
+    cell = synth.cubic_cell(200, 0.05)
+    pos = synth.random_positions(200, cell, 2.0, 11)
+    s = synth.Structure(rng.integers(0, 2, 200).astype(np.int32), pos, None)
+    E, Ea, F = m.compute(s)
+    Eo, Eao, Fo = o.compute(s.types, s.pos, None)
+    assert rel_e(E, Eo, Eao) <= TOL
+    assert rel_f(F, Fo) <= TOL
+
+
+def test_mixed_precision_accuracy(models):
+    """Mixed form vs FP64 oracle: <= 1 meV/atom energy, <= 20 meV/A RMS force."""
+    m, o = models["mixed64x4096"]
+    s = synth.mixed64x4096(1)[0]
+    E, Ea, F = m.compute(s, NNP_FORCE | NNP_VACUUM_ENERGY | NNP_PREC_MIXED)
+    Eo, Eao, Fo = o.compute(s.types, s.pos, s.cell)
+    e_err = abs(E - Eo) / s.natoms
+    f_rms = np.sqrt(np.mean(np.sum((F - Fo) ** 2, axis=1)))
+    print(f"mixed: dE/atom = {e_err * 1e3:.4f} meV, F rms = {f_rms * 1e3:.4f} meV/A")
+    assert e_err <= 1e-3
+    assert f_rms <= 20e-3
