@@ -59,10 +59,11 @@ template <typename T>
 __device__ __forceinline__ T ipow(T x, int n) {
     if (n == 0) return T(1);
     if (x == T(0)) return T(0);
+    int nn = n > 0 ? n : -n;
     T yy = T(1), ww = x;
-    for (; n != 0; n >>= 1, ww *= ww)
-        if (n & 1) yy *= ww;
-    return yy;
+    for (; nn != 0; nn >>= 1, ww *= ww)
+        if (nn & 1) yy *= ww;
+    return n > 0 ? yy : T(1) / yy;
 }
 
 template <typename T>
@@ -207,20 +208,49 @@ __global__ __launch_bounds__(kWave) void k_symm(const DevPlan* __restrict__ plan
     for (int k = lane; k < ldg; k += kWave) Grow[k] = T(0);
     __syncthreads();
 
-    // radial: one lane per function, summing over the neighbours of that species
-    for (int b = 0; b < ntypes; ++b) {
-        const DevRadial& R = plan->rad[ti][b];
-        const T rc = T(R.rc), rci = T(R.rci), pirci = T(R.pirci);
-        for (int f = lane; f < R.n; f += kWave) {
-            const T eta = T(rfun_eta[R.fbase + f]), rs = T(rfun_rs[R.fbase + f]);
-            T acc = T(0);
-            for (int p = sg.start[b]; p < sg.start[b + 1]; ++p) {
-                const T d = su[4 * p + 3];
+    // radial: cutoff once per neighbour, then (function, part) items over 64 lanes; the parts of
+    // a function are summed in a fixed order
+    {
+        T* fcr = Rf;    // [maxp] cutoff values (Rf is free until the angular blocks)
+        T* part = red;  // [64] partial sums
+        for (int b = 0; b < ntypes; ++b) {
+            const DevRadial& R = plan->rad[ti][b];
+            for (int p = sg.start[b] + lane; p < sg.start[b + 1]; p += kWave) {
                 T fc, gc;
-                cutoff_fg<T>(R.cut, d, rc, rci, pirci, fc, gc);
-                acc += exp(-eta * (d - rs) * (d - rs)) * fc;
+                cutoff_fg<T>(R.cut, su[4 * p + 3], T(R.rc), T(R.rci), T(R.pirci), fc, gc);
+                fcr[p] = fc;
             }
-            Grow[R.goff + f] = acc * T(R.norm);
+        }
+        __syncthreads();
+        int nrt = 0;
+        for (int b = 0; b < ntypes; ++b) nrt += plan->rad[ti][b].n;
+        const int nparts = nrt > 0 ? max(1, kWave / nrt) : 1;
+        for (int f0 = 0; f0 < nrt; f0 += kWave / nparts) {
+            const int item = lane;
+            const int f = f0 + item / nparts, prt = item % nparts;
+            T acc = T(0);
+            int goff = -1;
+            T norm = T(0);
+            if (item < (kWave / nparts) * nparts && f < nrt) {
+                int b = 0, fl = f;
+                while (fl >= plan->rad[ti][b].n) fl -= plan->rad[ti][b++].n;
+                const DevRadial& R = plan->rad[ti][b];
+                const T eta = T(rfun_eta[R.fbase + fl]), rs = T(rfun_rs[R.fbase + fl]);
+                for (int p = sg.start[b] + prt; p < sg.start[b + 1]; p += nparts) {
+                    const T x = su[4 * p + 3] - rs;
+                    acc += exp(-eta * x * x) * fcr[p];
+                }
+                goff = R.goff + fl;
+                norm = T(R.norm);
+            }
+            part[lane] = acc;
+            __syncthreads();
+            if (goff >= 0 && prt == 0) {
+                T s = T(0);
+                for (int q = 0; q < nparts; ++q) s += part[lane + q];
+                Grow[goff] = s * norm;
+            }
+            __syncthreads();
         }
     }
 
