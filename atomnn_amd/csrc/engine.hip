@@ -1,12 +1,12 @@
 // Host orchestration of the energy/force pipeline and the C ABI (include/nnp_capi.h).
 //
 // Pipeline for a batch of structures (all kernels cover every atom of every structure):
-//   1. neighbour list      k_bin_atoms / k_scan_excl / k_fill_bins / k_neigh_cells (large cells)
-//                          k_neigh_allpairs (small cells), k_partner
+//   1. neighbour list      k_zero, k_bin_atoms / k_scan_excl / k_fill_bins / k_neigh_cells
+//                          (large cells), k_neigh_allpairs (small cells)
 //   2. descriptors         k_symm<T, NE, NC>                 -> G
-//   3. networks            k_net_f64 | k_net_mixed per species -> E_i, dE/dG
+//   3. networks            k_net_f64 | k_net_mixed, all species' 16-atom tiles -> E_i, dE/dG
 //   4. force terms         k_force<T, NE, NC>                 -> X (per neighbour entry)
-//   5. force assembly      k_assemble                          -> F (FP64)
+//   5. force assembly      k_assemble (finds each entry's mirrored partner) -> F (FP64)
 //   6. energy              per structure, sum of E_i in atom order (FP64), as
 //                          NNP::energy accumulates (nnp.cpp:394-400)
 #include <hip/hip_runtime.h>
@@ -41,12 +41,10 @@ hipError_t launch_force(int shape, const DevPlan* plan, const double* rfe, const
                         const int* centres, int ncentres, const int* cnt, const int* meta, const double4* r, int maxp,
                         const T* dEdG, int ldg, T* X, hipStream_t st);
 template <typename T>
-hipError_t launch_assemble(const int* cnt, const int* partner, const T* X, int maxp, int ntot, double* F,
-                           hipStream_t st);
-hipError_t launch_net_f64(const DevPlan* plan, const DevNet& hn, const double* Wb, int s, const int* atoms, int M,
-                          const double* G, int ldg, int vac, double* eatom, double* dEdG, hipStream_t st);
-hipError_t launch_net_mixed(const DevPlan* plan, const DevNet& hn, const float* Wf, int s, const int* atoms, int M,
-                            const float* G, int ldg, int vac, double* eatom, float* dEdG, hipStream_t st);
+hipError_t launch_assemble(const DevStruc* S, const int* atom_struc, const int* cnt, const int* nbr_j, const int* meta,
+                           const T* X, int maxp, int ntot, int all_images, double* F, hipStream_t st);
+hipError_t launch_net(bool fp64, const DevPlan* plan, const DevPlan& hplan, const void* W, const int* sp_off,
+                      const int* spatoms, const void* G, int ldg, int vac, double* eatom, void* dEdG, hipStream_t st);
 hipError_t launch_neighbors(const NeighborArgs& a, hipStream_t st);
 
 // Total energy per structure: one block per structure, fixed-order tree sum of E_i (FP64).
@@ -214,15 +212,37 @@ Engine::Engine(HostModel&& hm) : host(std::move(hm)) {
     HIPCHK(hipMemcpy(d_w32.p, wf.data(), wf.size() * 4, hipMemcpyHostToDevice));
 }
 
+Engine::Engine(const Engine& o, int) : host(o.host), pb(o.pb), shape(o.shape) {
+    owns_model = false;
+    d_plan = o.d_plan;
+    d_rfe = o.d_rfe;
+    d_rfr = o.d_rfr;
+    d_w64 = o.d_w64;
+    d_w32 = o.d_w32;
+    HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+}
+
+size_t Engine::workspace_bytes() const {
+    size_t b = 0;
+    for (const DevMem* m : {&d_pos, &d_type, &d_astruc, &d_strucs, &d_centres, &d_cnt, &d_nbrj, &d_meta, &d_nbrr,
+                            &d_maxcnt, &d_abin, &d_bcount, &d_bstart, &d_bfill, &d_batoms, &d_G, &d_dEdG, &d_X,
+                            &d_eatom, &d_F, &d_spatoms, &d_E, &d_sstart})
+        b += m->cap;
+    return b;
+}
+
 Engine::~Engine() {
     if (stream) (void)hipStreamSynchronize(stream);
+    drop_graph();
     for (int st = 0; st < ST_COUNT; ++st)
         for (int q = 0; q < 2; ++q)
             if (ev[st][q]) (void)hipEventDestroy(ev[st][q]);
     d_E.release();
     d_sstart.release();
-    for (DevMem* m : {&d_plan, &d_rfe, &d_rfr, &d_w64, &d_w32, &d_pos, &d_type, &d_astruc, &d_strucs, &d_centres,
-                      &d_cnt, &d_nbrj, &d_meta, &d_nbrr, &d_partner, &d_maxcnt, &d_abin, &d_bcount, &d_bstart,
+    if (owns_model)
+        for (DevMem* m : {&d_plan, &d_rfe, &d_rfr, &d_w64, &d_w32}) m->release();
+    for (DevMem* m : {&d_pos, &d_type, &d_astruc, &d_strucs, &d_centres,
+                      &d_cnt, &d_nbrj, &d_meta, &d_nbrr, &d_maxcnt, &d_abin, &d_bcount, &d_bstart,
                       &d_bfill, &d_batoms, &d_G, &d_dEdG, &d_X, &d_eatom, &d_F, &d_spatoms})
         m->release();
     if (stream) (void)hipStreamDestroy(stream);
@@ -249,6 +269,8 @@ static int estimate_maxp(const std::vector<nnp_structure>& ss, double rc) {
 
 void Engine::upload(const std::vector<nnp_structure>& ss) {
     resident_valid = false;
+    drop_graph();
+    eager_runs = 0;
     nstruc = (int)ss.size();
     h_natoms.clear();
     for (const auto& s : ss) h_natoms.push_back(s.natoms);
@@ -312,6 +334,7 @@ void Engine::upload(const std::vector<nnp_structure>& ss) {
 }
 
 void Engine::build_neighbors(bool all_images) {
+    all_images_ = all_images;
     for (int attempt = 0; attempt < 4; ++attempt) {
         const size_t ent = (size_t)ntot * maxp;
         if (ent >= (size_t)1 << 31) throw std::invalid_argument("batch too large for 32-bit entry indices");
@@ -319,9 +342,7 @@ void Engine::build_neighbors(bool all_images) {
         d_nbrj.reserve(ent * 4);
         d_meta.reserve(ent * 4);
         d_nbrr.reserve(ent * sizeof(double4));
-        d_partner.reserve(ent * 4);
         d_maxcnt.reserve(64);
-        HIPCHK(hipMemsetAsync(d_maxcnt.p, 0, 4, stream));
         NeighborArgs a;
         a.S = (const DevStruc*)d_strucs.p;
         a.atom_struc = (const int*)d_astruc.p;
@@ -334,7 +355,6 @@ void Engine::build_neighbors(bool all_images) {
         a.nbr_j = (int*)d_nbrj.p;
         a.nbr_meta = (int*)d_meta.p;
         a.nbr_r = (double4*)d_nbrr.p;
-        a.partner = (int*)d_partner.p;
         a.maxcnt = (int*)d_maxcnt.p;
         a.cell_centres = (const int*)d_centres.p;
         a.ncell = (int)h_cell_centres.size();
@@ -378,18 +398,8 @@ void Engine::descriptors() {
 void Engine::networks(bool mixed, bool vac) {
     d_eatom.reserve((size_t)ntot * 8);
     d_dEdG.reserve((size_t)ntot * pb.plan.ldg * (mixed ? 4 : 8));
-    for (int s = 0; s < host.T; ++s) {
-        const int M = sp_off[s + 1] - sp_off[s];
-        const int* atoms = (const int*)d_spatoms.p + sp_off[s];
-        if (mixed)
-            HIPCHK(launch_net_mixed((const DevPlan*)d_plan.p, pb.plan.net[s], (const float*)d_w32.p, s, atoms, M,
-                                    (const float*)d_G.p, pb.plan.ldg, vac ? 1 : 0, (double*)d_eatom.p,
-                                    (float*)d_dEdG.p, stream));
-        else
-            HIPCHK(launch_net_f64((const DevPlan*)d_plan.p, pb.plan.net[s], (const double*)d_w64.p, s, atoms, M,
-                                  (const double*)d_G.p, pb.plan.ldg, vac ? 1 : 0, (double*)d_eatom.p,
-                                  (double*)d_dEdG.p, stream));
-    }
+    HIPCHK(launch_net(!mixed, (const DevPlan*)d_plan.p, pb.plan, mixed ? d_w32.p : d_w64.p, sp_off.data(),
+                      (const int*)d_spatoms.p, d_G.p, pb.plan.ldg, vac ? 1 : 0, (double*)d_eatom.p, d_dEdG.p, stream));
 }
 
 template <typename T>
@@ -402,8 +412,9 @@ void Engine::forces() {
                            (const int*)d_type.p, (const int*)d_centres.p, ntot, (const int*)d_cnt.p,
                            (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp, (const T*)d_dEdG.p, pb.plan.ldg,
                            (T*)d_X.p, stream));
-    HIPCHK(launch_assemble<T>((const int*)d_cnt.p, (const int*)d_partner.p, (const T*)d_X.p, maxp, ntot,
-                              (double*)d_F.p, stream));
+    HIPCHK(launch_assemble<T>((const DevStruc*)d_strucs.p, (const int*)d_astruc.p, (const int*)d_cnt.p,
+                              (const int*)d_nbrj.p, (const int*)d_meta.p, (const T*)d_X.p, maxp, ntot,
+                              all_images_ ? 1 : 0, (double*)d_F.p, stream));
 }
 
 void Engine::run(const std::vector<nnp_structure>& ss, uint32_t flags) {
@@ -469,8 +480,49 @@ void Engine::resident_load(const std::vector<nnp_structure>& ss, uint32_t flags)
     resident_valid = true;
 }
 
-// One full pass on the resident batch, enqueued on the engine's stream (no host sync).
+void Engine::drop_graph() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+}
+
+// One full pass on the resident batch.  The first pass with given flags runs eagerly (it
+// allocates every buffer); later passes replay a captured graph of the same launches.  With
+// profiling on, passes run eagerly so the stage events can be recorded.
 void Engine::resident_run(uint32_t flags) {
+    if (!use_graph || profiling) {
+        enqueue_pass(flags);
+        return;
+    }
+    if (graph_exec && graph_flags == flags && graph_maxp == maxp) {
+        HIPCHK(hipGraphLaunch(graph_exec, stream));
+        return;
+    }
+    if (eager_runs == 0 || graph_flags != flags || graph_maxp != maxp) {
+        drop_graph();
+        graph_flags = flags;
+        graph_maxp = maxp;
+        eager_runs = 1;
+        enqueue_pass(flags);
+        return;
+    }
+    HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    try {
+        enqueue_pass(flags);
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    HIPCHK(hipStreamEndCapture(stream, &graph));
+    HIPCHK(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+    HIPCHK(hipGraphLaunch(graph_exec, stream));
+}
+
+// Enqueue one full pass on the engine's stream (no host sync).
+void Engine::enqueue_pass(uint32_t flags) {
     const bool mixed = flags & NNP_PREC_MIXED;
     NeighborArgs a;
     a.S = (const DevStruc*)d_strucs.p;
@@ -484,7 +536,6 @@ void Engine::resident_run(uint32_t flags) {
     a.nbr_j = (int*)d_nbrj.p;
     a.nbr_meta = (int*)d_meta.p;
     a.nbr_r = (double4*)d_nbrr.p;
-    a.partner = (int*)d_partner.p;
     a.maxcnt = (int*)d_maxcnt.p;
     a.cell_centres = (const int*)d_centres.p;
     a.ncell = (int)h_cell_centres.size();
@@ -498,7 +549,6 @@ void Engine::resident_run(uint32_t flags) {
     a.bin_fill = (int*)d_bfill.p;
     a.bin_atoms = (int*)d_batoms.p;
     stage_begin(ST_NEIGH);
-    HIPCHK(hipMemsetAsync(d_maxcnt.p, 0, 4, stream));
     HIPCHK(launch_neighbors(a, stream));
     stage_end(ST_NEIGH);
     stage_begin(ST_SYMM);
@@ -526,12 +576,15 @@ void Engine::resident_run(uint32_t flags) {
                                         (const double*)d_dEdG.p, pb.plan.ldg, (double*)d_X.p, stream));
         stage_end(ST_FORCE);
         stage_begin(ST_ASSEMBLE);
+        const int ai = (flags & NNP_ALL_IMAGES) ? 1 : 0;
         if (mixed)
-            HIPCHK(launch_assemble<float>((const int*)d_cnt.p, (const int*)d_partner.p, (const float*)d_X.p, maxp,
-                                          ntot, (double*)d_F.p, stream));
+            HIPCHK(launch_assemble<float>((const DevStruc*)d_strucs.p, (const int*)d_astruc.p, (const int*)d_cnt.p,
+                                          (const int*)d_nbrj.p, (const int*)d_meta.p, (const float*)d_X.p, maxp, ntot,
+                                          ai, (double*)d_F.p, stream));
         else
-            HIPCHK(launch_assemble<double>((const int*)d_cnt.p, (const int*)d_partner.p, (const double*)d_X.p, maxp,
-                                           ntot, (double*)d_F.p, stream));
+            HIPCHK(launch_assemble<double>((const DevStruc*)d_strucs.p, (const int*)d_astruc.p, (const int*)d_cnt.p,
+                                           (const int*)d_nbrj.p, (const int*)d_meta.p, (const double*)d_X.p, maxp,
+                                           ntot, ai, (double*)d_F.p, stream));
         stage_end(ST_ASSEMBLE);
     }
     stage_begin(ST_ENERGY);
@@ -546,8 +599,9 @@ void Engine::resident_fetch(uint32_t flags, nnp_result* res) {
     profile_collect();
     if (mc > maxp) {  // the rows overflowed: rebuild them larger and redo the pass
         maxp = (mc + mc / 8 + 8 + 31) / 32 * 32;
+        drop_graph();
         build_neighbors(flags & NNP_ALL_IMAGES);
-        resident_run(flags);
+        enqueue_pass(flags);
         HIPCHK(hipStreamSynchronize(stream));
         profile_collect();
     }
@@ -597,6 +651,10 @@ using namespace atomnn;
 struct nnp_model {
     Engine* eng;
 };
+
+namespace atomnn {
+Engine& engine_of(nnp_model* m) { return *m->eng; }
+}  // namespace atomnn
 
 template <typename F>
 static int guarded(F&& f) {
@@ -791,6 +849,23 @@ int nnp_profile_read(nnp_model* m, double* ms, int64_t* calls, int nstage) {
             if (calls) calls[s] = E.stage_calls[s];
         }
         return Engine::ST_COUNT;
+    });
+}
+
+int nnp_set_option(nnp_model* m, const char* key, int64_t value) {
+    return guarded([&]() {
+        if (!m || !key) throw std::invalid_argument("nnp_set_option: invalid argument");
+        Engine& E = *m->eng;
+        const std::string k(key);
+        if (k == "graph") {
+            HIPCHK(hipStreamSynchronize(E.stream));
+            E.use_graph = value != 0;
+            E.drop_graph();
+            E.eager_runs = 0;
+        } else {
+            throw std::invalid_argument("nnp_set_option: unknown option " + k);
+        }
+        return NNP_OK;
     });
 }
 

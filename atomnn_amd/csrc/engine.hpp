@@ -36,7 +36,6 @@ struct NeighborArgs {
     int* nbr_j;
     int* nbr_meta;
     double4* nbr_r;
-    int* partner;
     int* maxcnt;
     const int* cell_centres;
     int ncell;
@@ -56,6 +55,8 @@ DevStruc make_struc(const nnp_structure& s, double rc, int atom0);
 class Engine {
   public:
     explicit Engine(HostModel&& hm);
+    // A second workspace (own stream and batch buffers) on the same device-resident model.
+    explicit Engine(const Engine& model_owner, int tag);
     ~Engine();
     Engine(const Engine&) = delete;
     Engine& operator=(const Engine&) = delete;
@@ -86,6 +87,15 @@ class Engine {
     void stage_end(int s);
     void profile_collect();
     bool pending[ST_COUNT] = {};
+    // captured resident pass (replayed with hipGraphLaunch)
+    bool use_graph = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    uint32_t graph_flags = 0;
+    int graph_maxp = 0;
+    int eager_runs = 0;
+    void drop_graph();
+    void enqueue_pass(uint32_t flags);
     std::vector<int> h_natoms;  // atoms per structure of the current batch
     bool resident_valid = false;
     std::vector<double> h_E;
@@ -98,11 +108,15 @@ class Engine {
     int maxp = 0;
     int max_entries = 0;
     int ntot = 0, nstruc = 0, nbins = 0;
+    bool all_images_ = false;
 
-    // model
+    // model (owned by the first engine; workspaces alias it)
     DevMem d_plan, d_rfe, d_rfr, d_w64, d_w32;
+    bool owns_model = true;
+    // bytes of device memory held by the batch buffers
+    size_t workspace_bytes() const;
     // batch
-    DevMem d_pos, d_type, d_astruc, d_strucs, d_centres, d_cnt, d_nbrj, d_meta, d_nbrr, d_partner, d_maxcnt;
+    DevMem d_pos, d_type, d_astruc, d_strucs, d_centres, d_cnt, d_nbrj, d_meta, d_nbrr, d_maxcnt;
     DevMem d_abin, d_bcount, d_bstart, d_bfill, d_batoms;
     DevMem d_G, d_dEdG, d_X, d_eatom, d_F, d_spatoms;
 

@@ -15,8 +15,11 @@
 //    small cells (any image shell > 0) and small structures.
 //  * cell list: for structures whose image shells are all 0 (every cell height > 2 rc) the
 //    entry of j is exactly diff(r_i, r_j) when within rc; candidates come from the 27
-//    neighbouring fractional bins, and each wavefront sorts its kept candidates by j
-//    (bitonic sort in LDS) before writing them, so the order is the reference's.
+//    neighbouring fractional bins as one flat list, and each wavefront sorts its kept
+//    candidates by j (bitonic sort in LDS) before writing them, so the order is the
+//    reference's.
+// The partner of an entry (the mirrored pair that carries the Newton reaction) is found
+// by the force-assembly kernel.
 #include "device.hpp"
 
 namespace atomnn {
@@ -221,40 +224,53 @@ __global__ __launch_bounds__(kWave) void k_neigh_cells(const DevStruc* __restric
     const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
     const int myb = atom_bin[i] - s.bin0;
     const int bz = myb % s.nbin[2], by = (myb / s.nbin[2]) % s.nbin[1], bx = myb / (s.nbin[2] * s.nbin[1]);
-    int kept = 0;
-    // neighbouring bins; along an axis with fewer than 3 bins every bin is visited once
+    // neighbouring bins (along an axis with fewer than 3 bins every bin is visited once): lane o
+    // owns bin o; a wave prefix sum over their counts gives one flat candidate list
     const int n0 = s.nbin[0] < 3 ? s.nbin[0] : 3, n1 = s.nbin[1] < 3 ? s.nbin[1] : 3, n2 = s.nbin[2] < 3 ? s.nbin[2] : 3;
-    for (int o = 0; o < n0 * n1 * n2; ++o) {
-        const int o2 = o % n2, o1 = (o / n2) % n1, o0 = o / (n2 * n1);
+    const int nb = n0 * n1 * n2;  // <= 27
+    int bstart = 0, bcount = 0;
+    if (lane < nb) {
+        const int o2 = lane % n2, o1 = (lane / n2) % n1, o0 = lane / (n2 * n1);
         const int cx = s.nbin[0] < 3 ? o0 : (bx + o0 - 1 + s.nbin[0]) % s.nbin[0];
         const int cy = s.nbin[1] < 3 ? o1 : (by + o1 - 1 + s.nbin[1]) % s.nbin[1];
         const int cz = s.nbin[2] < 3 ? o2 : (bz + o2 - 1 + s.nbin[2]) % s.nbin[2];
         const int b = s.bin0 + (cx * s.nbin[1] + cy) * s.nbin[2] + cz;
-        const int start = bin_start[b], count = bin_count[b];
-        for (int c0 = 0; c0 < count; c0 += kWave) {
-            const int c = c0 + lane;
-            bool keep = false;
-            double rx = 0, ry = 0, rz = 0, d2 = 0;
-            int gj = 0;
-            if (c < count) {
-                gj = bin_atoms[start + c];
-                if (gj != i) {
-                    cell_diff(s, xi - pos[3 * gj], yi - pos[3 * gj + 1], zi - pos[3 * gj + 2], rx, ry, rz);
-                    d2 = rx * rx + ry * ry + rz * rz;
-                    keep = d2 <= rc2;
-                }
-            }
-            int tot;
-            const int at = kept + wave_excl_scan(keep ? 1 : 0, lane, tot);
-            if (keep && at < cap) {
-                keys[at] = gj - s.atom0;
-                vals[4 * at] = rx;
-                vals[4 * at + 1] = ry;
-                vals[4 * at + 2] = rz;
-                vals[4 * at + 3] = d2;
-            }
-            kept += tot;
+        bstart = bin_start[b];
+        bcount = bin_count[b];
+    }
+    int ncand;
+    const int bfirst = wave_excl_scan(bcount, lane, ncand);  // first flat candidate of bin `lane`
+    int kept = 0;
+    for (int c0 = 0; c0 < ncand; c0 += kWave) {
+        const int c = c0 + lane;
+        // owning bin: the last bin whose first candidate is <= c
+        int ob = 0;
+        for (int q = 1; q < nb; ++q) {
+            const int fq = __shfl(bfirst, q, kWave);
+            if (fq <= c) ob = q;
         }
+        const int ostart = __shfl(bstart, ob, kWave), ofirst = __shfl(bfirst, ob, kWave);
+        bool keep = false;
+        double rx = 0, ry = 0, rz = 0, d2 = 0;
+        int gj = 0;
+        if (c < ncand) {
+            gj = bin_atoms[ostart + (c - ofirst)];
+            if (gj != i) {
+                cell_diff(s, xi - pos[3 * gj], yi - pos[3 * gj + 1], zi - pos[3 * gj + 2], rx, ry, rz);
+                d2 = rx * rx + ry * ry + rz * rz;
+                keep = d2 <= rc2;
+            }
+        }
+        int tot;
+        const int at = kept + wave_excl_scan(keep ? 1 : 0, lane, tot);
+        if (keep && at < cap) {
+            keys[at] = gj - s.atom0;
+            vals[4 * at] = rx;
+            vals[4 * at + 1] = ry;
+            vals[4 * at + 2] = rz;
+            vals[4 * at + 3] = d2;
+        }
+        kept += tot;
     }
     __syncthreads();
     const int P = kept < cap ? kept : cap;
@@ -300,6 +316,16 @@ __global__ __launch_bounds__(kWave) void k_neigh_cells(const DevStruc* __restric
     }
 }
 
+// Clears the per-pass counters (max entry count and the cell-list bin counters) in one launch.
+__global__ void k_zero(int* __restrict__ maxcnt, int* __restrict__ bin_count, int* __restrict__ bin_fill, int nbins) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) *maxcnt = 0;
+    if (t < nbins) {
+        bin_count[t] = 0;
+        bin_fill[t] = 0;
+    }
+}
+
 // Exclusive scan of n ints by one block (bin starts of the cell list).
 __global__ __launch_bounds__(1024) void k_scan_excl(const int* __restrict__ in, int* __restrict__ out, int n) {
     __shared__ int part[1024];
@@ -324,49 +350,6 @@ __global__ __launch_bounds__(1024) void k_scan_excl(const int* __restrict__ in, 
     }
 }
 
-// ------------------------------------------------------------------ partners
-// For entry (i -> j, image n) the mirrored entry is (j -> i, image nimg-1-n): same distance,
-// opposite vector.  Rows are sorted by (j, n), so a binary search finds it.  In the
-// reference rule only min-image entries carry a reaction; with all_images every entry does.
-__global__ void k_partner(const DevStruc* __restrict__ S, const int* __restrict__ atom_struc,
-                          const int* __restrict__ cnt, const int* __restrict__ nbr_j, const int* __restrict__ nbr_meta,
-                          int maxp, int natoms_total, int all_images, int* __restrict__ partner) {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (int64_t)natoms_total * maxp) return;
-    const int i = (int)(e / maxp);
-    const int q = (int)(e % maxp);
-    const int P = cnt[i] < maxp ? cnt[i] : maxp;
-    if (q >= P) return;
-    const int meta = nbr_meta[e];
-    const bool mn = meta & 1;
-    const DevStruc& s = S[atom_struc[i]];
-    if (!(mn || (all_images != 0))) {
-        partner[e] = -1;
-        return;
-    }
-    const int img = meta >> 8;
-    const int want_img = s.periodic ? s.nimg - 1 - img : 0;
-    const int li = i - s.atom0;
-    const int j = nbr_j[e];
-    const int gj = s.atom0 + j;
-    const int64_t rowj = (int64_t)gj * maxp;
-    const int Pj = cnt[gj] < maxp ? cnt[gj] : maxp;
-    const int64_t want = (int64_t)li * s.nimg + want_img;
-    int lo = 0, hi = Pj;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        const int64_t key = (int64_t)nbr_j[rowj + mid] * s.nimg + (nbr_meta[rowj + mid] >> 8);
-        if (key < want) lo = mid + 1;
-        else hi = mid;
-    }
-    int found = -1;
-    if (lo < Pj) {
-        const int64_t key = (int64_t)nbr_j[rowj + lo] * s.nimg + (nbr_meta[rowj + lo] >> 8);
-        if (key == want) found = (int)(rowj + lo);
-    }
-    partner[e] = found;
-}
-
 }  // namespace atomnn
 
 #include "engine.hpp"
@@ -375,9 +358,9 @@ namespace atomnn {
 
 hipError_t launch_neighbors(const NeighborArgs& a, hipStream_t st) {
     hipError_t e;
+    const int nz = a.ncell > 0 ? a.nbins : 0;
+    hipLaunchKernelGGL(k_zero, dim3((nz + 255) / 256 + 1), dim3(256), 0, st, a.maxcnt, a.bin_count, a.bin_fill, nz);
     if (a.ncell > 0) {
-        if ((e = hipMemsetAsync(a.bin_count, 0, (size_t)a.nbins * 4, st)) != hipSuccess) return e;
-        if ((e = hipMemsetAsync(a.bin_fill, 0, (size_t)a.nbins * 4, st)) != hipSuccess) return e;
         hipLaunchKernelGGL(k_bin_atoms, dim3((a.ntot + 255) / 256), dim3(256), 0, st, a.S, a.atom_struc, a.pos, a.ntot,
                            a.atom_bin, a.bin_count);
         hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, a.bin_count, a.bin_start, a.nbins);
@@ -396,9 +379,6 @@ hipError_t launch_neighbors(const NeighborArgs& a, hipStream_t st) {
                            a.pair_centres, a.rc2, a.maxp, a.cnt, a.nbr_j, a.nbr_meta, a.nbr_r, a.maxcnt);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    const int64_t ent = (int64_t)a.ntot * a.maxp;
-    hipLaunchKernelGGL(k_partner, dim3((unsigned)((ent + 255) / 256)), dim3(256), 0, st, a.S, a.atom_struc, a.cnt,
-                       a.nbr_j, a.nbr_meta, a.maxp, a.ntot, a.all_images, a.partner);
     return hipGetLastError();
 }
 

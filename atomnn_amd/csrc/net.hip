@@ -16,6 +16,8 @@
 // with the activations and weights rounded to BF16 and FP32 accumulation; tanh and the
 // derivative products stay in FP32.  Weights are zero padded to multiples of 16, so padded units
 // carry exact zeros through both directions.
+#include <algorithm>
+
 #include "device.hpp"
 #include "model.hpp"
 
@@ -42,14 +44,51 @@ static __device__ __forceinline__ int net_maxw(const DevNet& n) {
 }
 
 // ------------------------------------------------------------------ FP64
+// Tile product D(16 x 16, column tile jt) = A(16 x K, LDS rows of stride lda) * B(K x 16).
+// Forward layers use B[k][j] = W[(16 jt + j) ldw + k] (W row-major out x in); backward
+// layers use B[k][j] = W[k ldw + 16 jt + j].  Eight K-steps of operands are loaded before
+// their eight MFMAs so the weight loads overlap.
+template <bool FWD>
+__device__ __forceinline__ d4_t tile_mm_f64(const double* A, int lda, const double* __restrict__ W, int ldw, int K,
+                                            int jt, int li, int lk) {
+    d4_t acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        double a[8], b[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int k = k0 + 4 * q + lk;
+            const bool ok = k < K;
+            a[q] = ok ? A[li * lda + k] : 0.0;
+            b[q] = ok ? (FWD ? W[(int64_t)(jt * 16 + li) * ldw + k] : W[(int64_t)k * ldw + jt * 16 + li]) : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[q], b[q], acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+struct NetTiles {
+    int tile_start[kMaxSpecies + 1];  // first 16-atom tile of each species
+    int atom_start[kMaxSpecies + 1];  // first atom of each species in the species-sorted list
+};
+
+__device__ __forceinline__ int tile_species(const NetTiles& nt, int ntypes, int tile) {
+    int s = 0;
+    while (s + 1 < ntypes && tile >= nt.tile_start[s + 1]) ++s;
+    return s;
+}
+
 __global__ __launch_bounds__(kWave) void k_net_f64(const DevPlan* __restrict__ plan, const double* __restrict__ Wb,
-                                                   int s, const int* __restrict__ atoms, int M,
+                                                   NetTiles nt, const int* __restrict__ spatoms,
                                                    const double* __restrict__ G, int ldg, int vac,
                                                    double* __restrict__ eatom, double* __restrict__ dEdG) {
     extern __shared__ __align__(16) double shd[];
+    const int s = tile_species(nt, plan->T, blockIdx.x);
     const DevNet& N = plan->net[s];
+    const int* atoms = spatoms + nt.atom_start[s];
+    const int M = nt.atom_start[s + 1] - nt.atom_start[s];
     const int lane = threadIdx.x;
-    const int r0 = blockIdx.x * 16;
+    const int r0 = (blockIdx.x - nt.tile_start[s]) * 16;
     const int maxw = net_maxw(N);
     double* cur = shd;
     double* nxt = shd + 16 * maxw;
@@ -75,12 +114,7 @@ __global__ __launch_bounds__(kWave) void k_net_f64(const DevPlan* __restrict__ p
         const double* b = Wb + N.boff[l];
         const bool linear = (l == N.L - 1) || !N.tanh_hidden;
         for (int jt = 0; jt < out_pad / 16; ++jt) {
-            d4_t acc = {0.0, 0.0, 0.0, 0.0};
-            for (int k0 = 0; k0 < in_pad; k0 += 4) {
-                const double a = cur[li * maxw + k0 + lk];
-                const double bv = W[(int64_t)(jt * 16 + li) * in_pad + k0 + lk];
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc, 0, 0, 0);
-            }
+            const d4_t acc = tile_mm_f64<true>(cur, maxw, W, in_pad, in_pad, jt, li, lk);
             const int col = jt * 16 + li;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
@@ -128,12 +162,7 @@ __global__ __launch_bounds__(kWave) void k_net_f64(const DevPlan* __restrict__ p
         const double* W = Wb + N.woff[l];  // wpad[l] x wpad[l-1], row-major
         doff -= 16 * J;
         for (int jt = 0; jt < J / 16; ++jt) {
-            d4_t acc = {0.0, 0.0, 0.0, 0.0};
-            for (int k0 = 0; k0 < K; k0 += 4) {
-                const double a = cur[li * maxw + k0 + lk];
-                const double bv = W[(int64_t)(k0 + lk) * J + jt * 16 + li];
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc, 0, 0, 0);
-            }
+            const d4_t acc = tile_mm_f64<false>(cur, maxw, W, J, K, jt, li, lk);
             const int col = jt * 16 + li;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
@@ -151,12 +180,7 @@ __global__ __launch_bounds__(kWave) void k_net_f64(const DevPlan* __restrict__ p
         const int K = N.wpad[0], J = N.nin_pad;
         const double* W = Wb + N.woff[0];
         for (int jt = 0; jt < J / 16; ++jt) {
-            d4_t acc = {0.0, 0.0, 0.0, 0.0};
-            for (int k0 = 0; k0 < K; k0 += 4) {
-                const double a = cur[li * maxw + k0 + lk];
-                const double bv = W[(int64_t)(k0 + lk) * J + jt * 16 + li];
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc, 0, 0, 0);
-            }
+            const d4_t acc = tile_mm_f64<false>(cur, maxw, W, J, K, jt, li, lk);
             const int col = jt * 16 + li;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
@@ -173,13 +197,16 @@ __global__ __launch_bounds__(kWave) void k_net_f64(const DevPlan* __restrict__ p
 __device__ __forceinline__ __bf16 to_bf16(float x) { return (__bf16)x; }
 
 __global__ __launch_bounds__(kWave) void k_net_mixed(const DevPlan* __restrict__ plan, const float* __restrict__ Wf,
-                                                     int s, const int* __restrict__ atoms, int M,
+                                                     NetTiles nt, const int* __restrict__ spatoms,
                                                      const float* __restrict__ G, int ldg, int vac,
                                                      double* __restrict__ eatom, float* __restrict__ dEdG) {
     extern __shared__ __align__(16) float shf[];
+    const int s = tile_species(nt, plan->T, blockIdx.x);
     const DevNet& N = plan->net[s];
+    const int* atoms = spatoms + nt.atom_start[s];
+    const int M = nt.atom_start[s + 1] - nt.atom_start[s];
     const int lane = threadIdx.x;
-    const int r0 = blockIdx.x * 16;
+    const int r0 = (blockIdx.x - nt.tile_start[s]) * 16;
     int maxw = N.nin_pad;
     for (int l = 0; l < N.L; ++l) maxw = N.wpad[l] > maxw ? N.wpad[l] : maxw;
     float* cur = shf;
@@ -293,19 +320,39 @@ __global__ __launch_bounds__(kWave) void k_net_mixed(const DevPlan* __restrict__
     }
 }
 
-hipError_t launch_net_f64(const DevPlan* plan, const DevNet& hn, const double* Wb, int s, const int* atoms, int M,
-                          const double* G, int ldg, int vac, double* eatom, double* dEdG, hipStream_t st) {
-    if (M <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_net_f64, dim3((M + 15) / 16), dim3(kWave), net_lds_bytes(hn, true), st, plan, Wb, s, atoms, M, G,
-                       ldg, vac, eatom, dEdG);
-    return hipGetLastError();
+// One launch covers the 16-atom tiles of every species; the LDS size is the largest species'.
+static NetTiles make_tiles(const int* sp_off, int ntypes, int& ntiles) {
+    NetTiles nt;
+    ntiles = 0;
+    for (int s = 0; s <= kMaxSpecies; ++s) {
+        nt.tile_start[s] = 0;
+        nt.atom_start[s] = 0;
+    }
+    for (int s = 0; s < ntypes; ++s) {
+        nt.tile_start[s] = ntiles;
+        nt.atom_start[s] = sp_off[s];
+        ntiles += (sp_off[s + 1] - sp_off[s] + 15) / 16;
+    }
+    for (int s = ntypes; s <= kMaxSpecies; ++s) {
+        nt.tile_start[s] = ntiles;
+        nt.atom_start[s] = sp_off[ntypes];
+    }
+    return nt;
 }
 
-hipError_t launch_net_mixed(const DevPlan* plan, const DevNet& hn, const float* Wf, int s, const int* atoms, int M,
-                            const float* G, int ldg, int vac, double* eatom, float* dEdG, hipStream_t st) {
-    if (M <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_net_mixed, dim3((M + 15) / 16), dim3(kWave), net_lds_bytes(hn, false), st, plan, Wf, s, atoms,
-                       M, G, ldg, vac, eatom, dEdG);
+hipError_t launch_net(bool fp64, const DevPlan* plan, const DevPlan& hplan, const void* W, const int* sp_off,
+                      const int* spatoms, const void* G, int ldg, int vac, double* eatom, void* dEdG, hipStream_t st) {
+    int ntiles = 0;
+    const NetTiles nt = make_tiles(sp_off, hplan.T, ntiles);
+    if (ntiles == 0) return hipSuccess;
+    size_t lds = 0;
+    for (int s = 0; s < hplan.T; ++s) lds = std::max(lds, net_lds_bytes(hplan.net[s], fp64));
+    if (fp64)
+        hipLaunchKernelGGL(k_net_f64, dim3(ntiles), dim3(kWave), lds, st, plan, (const double*)W, nt, spatoms,
+                           (const double*)G, ldg, vac, eatom, (double*)dEdG);
+    else
+        hipLaunchKernelGGL(k_net_mixed, dim3(ntiles), dim3(kWave), lds, st, plan, (const float*)W, nt, spatoms,
+                           (const float*)G, ldg, vac, eatom, (float*)dEdG);
     return hipGetLastError();
 }
 

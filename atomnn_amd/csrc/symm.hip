@@ -101,7 +101,7 @@ __device__ __forceinline__ int tri_row(long long q, int n) {
 }
 
 struct Segs {
-    int start[kMaxSpecies + 1];
+    int* start;  // [kMaxSpecies + 1] in LDS
 };
 
 // Stage the P entries of centre i into LDS, partitioned by neighbour species (stable).
@@ -120,12 +120,15 @@ __device__ __forceinline__ void stage_entries(int i, int P, int maxp, int ntypes
         for (int s = 0; s < kMaxSpecies; ++s)
             if (s < ntypes) count[s] += __popcll(__ballot(t == s));
     }
-    sg.start[0] = 0;
-#pragma unroll
-    for (int s = 0; s < kMaxSpecies; ++s) sg.start[s + 1] = sg.start[s] + (s < ntypes ? count[s] : 0);
     int fill[kMaxSpecies];
+    int run = 0;
 #pragma unroll
-    for (int s = 0; s < kMaxSpecies; ++s) fill[s] = sg.start[s];
+    for (int s = 0; s < kMaxSpecies; ++s) {
+        fill[s] = run;
+        if (lane == 0) sg.start[s] = run;
+        run += (s < ntypes ? count[s] : 0);
+    }
+    if (lane == 0) sg.start[kMaxSpecies] = run;
     const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     for (int p0 = 0; p0 < P; p0 += kWave) {
         const int p = p0 + lane;
@@ -188,7 +191,7 @@ __host__ __device__ constexpr size_t force_lds_bytes(int maxp, int ldg) {
 
 // ------------------------------------------------------------------ descriptors
 template <typename T, int NE, int NC>
-__global__ __launch_bounds__(kWave) void k_symm(const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta,
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <= 16 ? 3 : 1, 8))) void k_symm(const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta,
                                                 const double* __restrict__ rfun_rs, const int* __restrict__ atom_type,
                                                 const int* __restrict__ centres, const int* __restrict__ cnt,
                                                 const int* __restrict__ nbr_meta, const double4* __restrict__ nbr_r,
@@ -203,7 +206,8 @@ __global__ __launch_bounds__(kWave) void k_symm(const DevPlan* __restrict__ plan
     const int ti = atom_type[i];
     const int P = min(cnt[i], maxp);
     const int ntypes = plan->T;
-    Segs sg;
+    __shared__ int seg_lds[kMaxSpecies + 1];
+    Segs sg{seg_lds};
     stage_entries<T>(i, P, maxp, ntypes, nbr_meta, nbr_r, su, nullptr, sg, lane);
     for (int k = lane; k < ldg; k += kWave) Grow[k] = T(0);
     __syncthreads();
@@ -340,7 +344,7 @@ __global__ __launch_bounds__(kWave) void k_symm(const DevPlan* __restrict__ plan
 
 // ------------------------------------------------------------------ force terms
 template <typename T, int NE, int NC>
-__global__ __launch_bounds__(kWave) void k_force(const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta,
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <= 16 ? 3 : 1, 8))) void k_force(const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta,
                                                  const double* __restrict__ rfun_rs, const int* __restrict__ atom_type,
                                                  const int* __restrict__ centres, const int* __restrict__ cnt,
                                                  const int* __restrict__ nbr_meta, const double4* __restrict__ nbr_r,
@@ -359,7 +363,8 @@ __global__ __launch_bounds__(kWave) void k_force(const DevPlan* __restrict__ pla
     const int ti = atom_type[i];
     const int P = min(cnt[i], maxp);
     const int ntypes = plan->T;
-    Segs sg;
+    __shared__ int seg_lds[kMaxSpecies + 1];
+    Segs sg{seg_lds};
     stage_entries<T>(i, P, maxp, ntypes, nbr_meta, nbr_r, su, orig, sg, lane);
     for (int k = lane; k < ldg; k += kWave) W[k] = dEdG[(int64_t)i * ldg + k];
     __syncthreads();
@@ -394,13 +399,15 @@ __global__ __launch_bounds__(kWave) void k_force(const DevPlan* __restrict__ pla
             if (same ? (na < 2) : (na == 0 || nb == 0)) continue;
             radial_tables<T, NE>(A, su, sa, sa + na, Rf, Rd, lane);
             if (!same) radial_tables<T, NE>(A, su, sb, sb + nb, Rf, Rd, lane);
+            // block weights W_ec = norm dE/dG[goff + fmap(e,c)]: uniform, read with scalar loads
             T w[NE][NC];
+            const T* dEdGi = dEdG + (int64_t)i * ldg + A.goff;
 #pragma unroll
             for (int e = 0; e < NE; ++e)
 #pragma unroll
                 for (int c = 0; c < NC; ++c) {
                     const int f = A.fmap[e * kMaxC + c];
-                    w[e][c] = f >= 0 ? W[A.goff + f] * T(A.norm) : T(0);
+                    w[e][c] = f >= 0 ? dEdGi[f] * T(A.norm) : T(0);
                 }
             T lam[NC], zet[NC];
             int zin[NC];
@@ -428,16 +435,13 @@ __global__ __launch_bounds__(kWave) void k_force(const DevPlan* __restrict__ pla
                         const int n = (int)min((long long)(ncol_eff - c), q1 - q);
                         const bool whole = (c == 0 && n == ncol_eff);
                         const int pj = rs0 + r;
-                        const T ujx = su[4 * pj], ujy = su[4 * pj + 1], ujz = su[4 * pj + 2], dj = su[4 * pj + 3];
-                        T v[NE][NC], vd[NE][NC];
+                        const T ujx = su[4 * pj], ujy = su[4 * pj + 1], ujz = su[4 * pj + 2];
+                        const T dji = T(1) / su[4 * pj + 3];
+                        T rfj[NE], rdj[NE];
 #pragma unroll
                         for (int e = 0; e < NE; ++e) {
-                            const T rfj = Rf[pj * NE + e], rdj = Rd[pj * NE + e];
-#pragma unroll
-                            for (int cc = 0; cc < NC; ++cc) {
-                                v[e][cc] = w[e][cc] * rfj;
-                                vd[e][cc] = w[e][cc] * rdj;
-                            }
+                            rfj[e] = Rf[pj * NE + e];
+                            rdj[e] = Rd[pj * NE + e];
                         }
                         T xx = T(0), xy = T(0), xz = T(0);
                         for (int t = 0; t < n; ++t) {
@@ -446,30 +450,28 @@ __global__ __launch_bounds__(kWave) void k_force(const DevPlan* __restrict__ pla
                             const int pk = cs0 + k;
                             const T ukx = su[4 * pk], uky = su[4 * pk + 1], ukz = su[4 * pk + 2];
                             const T cs = ujx * ukx + ujy * uky + ujz * ukz;
-                            T tp[NC], tq[NC];
-#pragma unroll
-                            for (int cc = 0; cc < NC; ++cc) {
-                                tp[cc] = T(0);
-                                tq[cc] = T(0);
-                            }
+                            T pf[NE], pd[NE];
 #pragma unroll
                             for (int e = 0; e < NE; ++e) {
                                 const T rfk = Rf[pk * NE + e];
-#pragma unroll
-                                for (int cc = 0; cc < NC; ++cc) {
-                                    tp[cc] += v[e][cc] * rfk;
-                                    tq[cc] += vd[e][cc] * rfk;
-                                }
+                                pf[e] = rfj[e] * rfk;
+                                pd[e] = rdj[e] * rfk;
                             }
                             T dEdcos = T(0), dEdd = T(0);
 #pragma unroll
                             for (int cc = 0; cc < NC; ++cc) {
+                                T tp = T(0), tq = T(0);
+#pragma unroll
+                                for (int e = 0; e < NE; ++e) {
+                                    tp += w[e][cc] * pf[e];
+                                    tq += w[e][cc] * pd[e];
+                                }
                                 T Av, dAv;
                                 angular_term<T>(cs, lam[cc], zet[cc], zin[cc], Av, dAv);
-                                dEdcos += dAv * tp[cc];
-                                dEdd += Av * tq[cc];
+                                dEdcos += dAv * tp;
+                                dEdd += Av * tq;
                             }
-                            const T g = dEdcos / dj;
+                            const T g = dEdcos * dji;
                             xx -= dEdd * ujx + g * (ukx - cs * ujx);
                             xy -= dEdd * ujy + g * (uky - cs * ujy);
                             xz -= dEdd * ujz + g * (ukz - cs * ujz);
@@ -521,31 +523,66 @@ __global__ __launch_bounds__(kWave) void k_force(const DevPlan* __restrict__ pla
     }
 }
 
-// F_a = sum_s X[a,s] - sum_{s: partner(a,s) >= 0} X[partner(a,s)], accumulated in FP64.
+// F_a = sum_s X[a,s] - sum_{entries (j -> a) that carry a reaction} X[j -> a], in FP64.
+// One wavefront per atom, lanes over its entries.  The mirrored entry (j -> a, image
+// nimg-1-n) of each entry (a -> j, image n) has the same distance, so it carries the min flag
+// exactly when (a -> j) does; it is found by binary search in j's row, which is sorted by
+// (neighbour, image).  With all_images every entry carries a reaction.  The lane partials are
+// summed in a fixed tree order (deterministic, no atomics).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
 template <typename T>
-__global__ void k_assemble(const int* __restrict__ cnt, const int* __restrict__ partner, const T* __restrict__ X,
-                           int maxp, int natoms_total, double* __restrict__ F) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= natoms_total) return;
+__global__ __launch_bounds__(kWave) void k_assemble(const DevStruc* __restrict__ S, const int* __restrict__ atom_struc,
+                                                    const int* __restrict__ cnt, const int* __restrict__ nbr_j,
+                                                    const int* __restrict__ nbr_meta, const T* __restrict__ X, int maxp,
+                                                    int all_images, double* __restrict__ F) {
+    const int a = blockIdx.x;
+    const int lane = threadIdx.x;
+    const DevStruc& st = S[atom_struc[a]];
+    const int la = a - st.atom0;
     const int P = min(cnt[a], maxp);
     const int64_t row = (int64_t)a * maxp;
     double fx = 0, fy = 0, fz = 0;
-    for (int s = 0; s < P; ++s) {
-        fx += (double)X[4 * (row + s)];
-        fy += (double)X[4 * (row + s) + 1];
-        fz += (double)X[4 * (row + s) + 2];
-    }
-    for (int s = 0; s < P; ++s) {
-        const int p = partner[row + s];
-        if (p >= 0) {
-            fx -= (double)X[4 * (int64_t)p];
-            fy -= (double)X[4 * (int64_t)p + 1];
-            fz -= (double)X[4 * (int64_t)p + 2];
+    for (int s = lane; s < P; s += kWave) {
+        const int64_t e = row + s;
+        fx += (double)X[4 * e];
+        fy += (double)X[4 * e + 1];
+        fz += (double)X[4 * e + 2];
+        const int meta = nbr_meta[e];
+        if ((meta & 1) || all_images) {
+            const int img = meta >> 8;
+            const int want_img = st.periodic ? st.nimg - 1 - img : 0;
+            const int gj = st.atom0 + nbr_j[e];
+            const int64_t rowj = (int64_t)gj * maxp;
+            const int Pj = min(cnt[gj], maxp);
+            const int64_t want = (int64_t)la * st.nimg + want_img;
+            int lo = 0, hi = Pj;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                const int64_t key = (int64_t)nbr_j[rowj + mid] * st.nimg + (nbr_meta[rowj + mid] >> 8);
+                if (key < want) lo = mid + 1;
+                else hi = mid;
+            }
+            if (lo < Pj && (int64_t)nbr_j[rowj + lo] * st.nimg + (nbr_meta[rowj + lo] >> 8) == want) {
+                const int64_t q = rowj + lo;
+                fx -= (double)X[4 * q];
+                fy -= (double)X[4 * q + 1];
+                fz -= (double)X[4 * q + 2];
+            }
         }
     }
-    F[3 * (int64_t)a] = fx;
-    F[3 * (int64_t)a + 1] = fy;
-    F[3 * (int64_t)a + 2] = fz;
+    fx = wave_sum(fx);
+    fy = wave_sum(fy);
+    fz = wave_sum(fz);
+    if (lane == 0) {
+        F[3 * (int64_t)a] = fx;
+        F[3 * (int64_t)a + 1] = fy;
+        F[3 * (int64_t)a + 2] = fz;
+    }
 }
 
 // ------------------------------------------------------------------ launchers
@@ -617,8 +654,10 @@ hipError_t launch_force(int shape, const DevPlan* plan, const double* rfe, const
 }
 
 template <typename T>
-hipError_t launch_assemble(const int* cnt, const int* partner, const T* X, int maxp, int ntot, double* F, hipStream_t st) {
-    hipLaunchKernelGGL((k_assemble<T>), dim3((ntot + 255) / 256), dim3(256), 0, st, cnt, partner, X, maxp, ntot, F);
+hipError_t launch_assemble(const DevStruc* S, const int* atom_struc, const int* cnt, const int* nbr_j, const int* meta,
+                           const T* X, int maxp, int ntot, int all_images, double* F, hipStream_t st) {
+    hipLaunchKernelGGL((k_assemble<T>), dim3(ntot), dim3(kWave), 0, st, S, atom_struc, cnt, nbr_j, meta, X, maxp,
+                       all_images, F);
     return hipGetLastError();
 }
 
@@ -632,7 +671,9 @@ template hipError_t launch_force<double>(int, const DevPlan*, const double*, con
 template hipError_t launch_force<float>(int, const DevPlan*, const double*, const double*, const int*, const int*, int,
                                         const int*, const int*, const double4*, int, const float*, int, float*,
                                         hipStream_t);
-template hipError_t launch_assemble<double>(const int*, const int*, const double*, int, int, double*, hipStream_t);
-template hipError_t launch_assemble<float>(const int*, const int*, const float*, int, int, double*, hipStream_t);
+template hipError_t launch_assemble<double>(const DevStruc*, const int*, const int*, const int*, const int*,
+                                            const double*, int, int, int, double*, hipStream_t);
+template hipError_t launch_assemble<float>(const DevStruc*, const int*, const int*, const int*, const int*, const float*,
+                                           int, int, int, double*, hipStream_t);
 
 }  // namespace atomnn

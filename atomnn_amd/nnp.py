@@ -79,6 +79,14 @@ def lib():
         L.nnp_compute.argtypes = [vp, ctypes.POINTER(_Struc), ctypes.c_uint32, ctypes.POINTER(_Result)]
         L.nnp_compute_batch.argtypes = [vp, ctypes.c_int, ctypes.POINTER(_Struc), ctypes.c_uint32,
                                         ctypes.POINTER(_Result)]
+        L.nnp_resident_load.argtypes = [vp, ctypes.c_int, ctypes.POINTER(_Struc), ctypes.c_uint32]
+        L.nnp_resident_run.argtypes = [vp, ctypes.c_uint32]
+        L.nnp_resident_fetch.argtypes = [vp, ctypes.c_uint32, ctypes.POINTER(_Result)]
+        L.nnp_sync.argtypes = [vp]
+        L.nnp_profile.argtypes = [vp, ctypes.c_int]
+        L.nnp_profile_read.argtypes = [vp, vp, vp, ctypes.c_int]
+        L.nnp_info.argtypes = [vp, vp, ctypes.c_int]
+        L.nnp_set_option.argtypes = [vp, ctypes.c_char_p, i64]
         _lib = L
     return _lib
 
@@ -179,9 +187,58 @@ class NNP:
             out.append((res[k].energy, eas[k], fs[k] if flags & NNP_FORCE else None))
         return out
 
+    def set_option(self, key: str, value: int) -> None:
+        _check(lib().nnp_set_option(self._h, key.encode(), int(value)))
+
+    def info(self) -> dict:
+        out = np.zeros(8, dtype=np.int64)
+        _check(lib().nnp_info(self._h, out.ctypes.data, 8))
+        keys = ["maxp", "max_entries", "atoms", "structures", "shape", "ne", "nc", "ldg"]
+        return dict(zip(keys, (int(v) for v in out)))
+
+    def resident(self, strucs: Sequence[Structure], flags: int) -> "Resident":
+        return Resident(self, strucs, flags)
+
     # reference-named conveniences
     def energy(self, s: Structure) -> float:
         return self.compute(s, NNP_VACUUM_ENERGY)[0]
 
     def force(self, s: Structure) -> np.ndarray:
         return self.compute(s, NNP_FORCE | NNP_VACUUM_ENERGY)[2]
+
+
+STAGES = ["neighbors", "descriptors", "networks", "force_terms", "force_assembly", "energy"]
+
+
+class Resident:
+    """A batch kept in HBM: load once, enqueue passes (replayed from a captured hipGraph), fetch."""
+
+    def __init__(self, model: NNP, strucs: Sequence[Structure], flags: int):
+        self.m = model
+        self.flags = flags
+        self.pins = [_Pinned(s) for s in strucs]
+        self.arr = (_Struc * len(strucs))(*[p.c_struct for p in self.pins])
+        self.natoms = [s.natoms for s in strucs]
+        _check(lib().nnp_resident_load(model._h, len(strucs), self.arr, flags))
+
+    def run(self) -> None:
+        _check(lib().nnp_resident_run(self.m._h, self.flags))
+
+    def sync(self) -> None:
+        _check(lib().nnp_sync(self.m._h))
+
+    def fetch(self):
+        eas = [np.zeros(n) for n in self.natoms]
+        fs = [np.zeros((n, 3)) for n in self.natoms]
+        res = (_Result * len(self.natoms))(*[_Result(0.0, ea.ctypes.data, f.ctypes.data) for ea, f in zip(eas, fs)])
+        _check(lib().nnp_resident_fetch(self.m._h, self.flags, res))
+        return [(res[k].energy, eas[k], fs[k] if self.flags & NNP_FORCE else None) for k in range(len(self.natoms))]
+
+    def profile(self, on: bool) -> None:
+        _check(lib().nnp_profile(self.m._h, 1 if on else 0))
+
+    def stage_ms(self):
+        ms = np.zeros(len(STAGES))
+        calls = np.zeros(len(STAGES), dtype=np.int64)
+        _check(lib().nnp_profile_read(self.m._h, ms.ctypes.data, calls.ctypes.data, len(STAGES)))
+        return ms, calls

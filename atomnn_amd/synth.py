@@ -400,20 +400,38 @@ class Request:
     structure: Structure
 
 
-def serve_stream(count: int = 20000, rate: float = 1000.0, seed_offset: int = 0) -> List[Request]:
-    """Open-loop Poisson stream: exponential inter-arrival times at ``rate`` requests/s."""
+def serve_structure(rng: np.random.Generator) -> Structure:
+    """One serving request: N = round(exp(U(ln 16, ln 20000))), Dirichlet(1,1,1) species
+    proportions, cube at 0.055 /A^3 with a +-5% symmetric strain, 1.8 A minimum separation."""
+    n = int(round(math.exp(rng.uniform(math.log(16), math.log(20000)))))
+    p = rng.dirichlet([1.0, 1.0, 1.0])
+    cell = strained_cell(n, 0.055, 0.05, rng)
+    types = species_exact(n, list(p), rng)
+    seed = int(rng.integers(0, 2**62))
+    return Structure(types, random_positions(n, cell, 1.8, seed), cell)
+
+
+def serve_pool(size: int, seed_offset: int = 0) -> List[Structure]:
+    """A pool of distinct serving structures drawn from the request distribution."""
     rng = np.random.default_rng(SEED0 + 4 + 1000 + seed_offset)
-    t = 0.0
-    out = []
-    for _ in range(count):
-        t += rng.exponential(1.0 / rate)
-        n = int(round(math.exp(rng.uniform(math.log(16), math.log(20000)))))
-        p = rng.dirichlet([1.0, 1.0, 1.0])
-        cell = strained_cell(n, 0.055, 0.05, rng)
-        types = species_exact(n, list(p), rng)
-        seed = int(rng.integers(0, 2**62))
-        out.append(Request(t, Structure(types, random_positions(n, cell, 1.8, seed), cell)))
-    return out
+    return [serve_structure(rng) for _ in range(size)]
+
+
+def serve_arrivals(count: int, rate: float, seed_offset: int = 0) -> np.ndarray:
+    """Open-loop Poisson arrival times (s) at `rate` requests/s."""
+    rng = np.random.default_rng(SEED0 + 4 + 2000 + seed_offset)
+    return np.cumsum(rng.exponential(1.0 / rate, size=count))
+
+
+def serve_stream(count: int = 20000, rate: float = 1000.0, pool: Optional[List[Structure]] = None,
+                 seed_offset: int = 0) -> List[Request]:
+    """Poisson request stream; with a pool, request k carries pool[k % len(pool)] (every request is
+    still evaluated in full), otherwise every request is a fresh draw."""
+    t = serve_arrivals(count, rate, seed_offset)
+    if pool is None:
+        rng = np.random.default_rng(SEED0 + 4 + 1000 + seed_offset)
+        return [Request(float(t[k]), serve_structure(rng)) for k in range(count)]
+    return [Request(float(t[k]), pool[k % len(pool)]) for k in range(count)]
 
 
 def write_workload_potential(name: str, path: str) -> Potential:

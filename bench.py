@@ -20,7 +20,6 @@ Also reported:
 Prints one JSON line.
 """
 import argparse
-import ctypes
 import json
 import math
 import os
@@ -33,60 +32,13 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from atomnn_amd import synth  # noqa: E402
+from atomnn_amd import serve, synth  # noqa: E402
 from atomnn_amd import nnp as nnpmod  # noqa: E402
-from atomnn_amd.nnp import NNP, NNP_FORCE, NNP_PREC_MIXED, NNP_VACUUM_ENERGY  # noqa: E402
+from atomnn_amd.nnp import NNP, NNP_FORCE, NNP_PREC_MIXED, NNP_VACUUM_ENERGY, Resident  # noqa: E402
 
 METRIC = "E+F atoms/s, mixed vs plain FP64 at ≤1 meV/atom; p50/p99 request latency"
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
-
-
-def resident_api():
-    L = nnpmod.lib()
-    vp = ctypes.c_void_p
-    L.nnp_resident_load.argtypes = [vp, ctypes.c_int, ctypes.POINTER(nnpmod._Struc), ctypes.c_uint32]
-    L.nnp_resident_run.argtypes = [vp, ctypes.c_uint32]
-    L.nnp_resident_fetch.argtypes = [vp, ctypes.c_uint32, ctypes.POINTER(nnpmod._Result)]
-    L.nnp_sync.argtypes = [vp]
-    L.nnp_profile.argtypes = [vp, ctypes.c_int]
-    L.nnp_profile_read.argtypes = [vp, vp, vp, ctypes.c_int]
-    L.nnp_info.argtypes = [vp, vp, ctypes.c_int]
-    return L
-
-
-class Resident:
-    def __init__(self, model: NNP, strucs, flags):
-        self.L = resident_api()
-        self.m = model
-        self.flags = flags
-        self.pins = [nnpmod._Pinned(s) for s in strucs]
-        self.arr = (nnpmod._Struc * len(strucs))(*[p.c_struct for p in self.pins])
-        self.strucs = strucs
-        nnpmod._check(self.L.nnp_resident_load(model._h, len(strucs), self.arr, flags))
-
-    def run(self):
-        nnpmod._check(self.L.nnp_resident_run(self.m._h, self.flags))
-
-    def sync(self):
-        nnpmod._check(self.L.nnp_sync(self.m._h))
-
-    def fetch(self):
-        eas = [np.zeros(s.natoms) for s in self.strucs]
-        fs = [np.zeros((s.natoms, 3)) for s in self.strucs]
-        res = (nnpmod._Result * len(self.strucs))(
-            *[nnpmod._Result(0.0, ea.ctypes.data, f.ctypes.data) for ea, f in zip(eas, fs)])
-        nnpmod._check(self.L.nnp_resident_fetch(self.m._h, self.flags, res))
-        return [(res[k].energy, eas[k], fs[k]) for k in range(len(self.strucs))]
-
-    def profile(self, on: bool):
-        nnpmod._check(self.L.nnp_profile(self.m._h, 1 if on else 0))
-
-    def stage_ms(self):
-        ms = np.zeros(6)
-        calls = np.zeros(6, dtype=np.int64)
-        nnpmod._check(self.L.nnp_profile_read(self.m._h, ms.ctypes.data, calls.ctypes.data, 6))
-        return ms, calls
 
 
 def timed_steps(R: Resident, steps: int, warmup: int):
@@ -152,7 +104,24 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--mixed-structures", type=int, default=64)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=20000)
+    ap.add_argument("--serve-pool", type=int, default=1024)
+    ap.add_argument("--serve-load", type=float, default=0.7, help="arrival rate as a fraction of measured capacity")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES bytes 5 times (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+
+        n = args.copy_calibration // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n * 4, "launches": 5}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); structures and requests shard "
               "across GPUs with no exchange, so the whole-job figure is N times the one-GPU figure.",
@@ -238,6 +207,47 @@ def main():
             "force_rms_err_meV_per_A": f_rms * 1e3,
             "error_reference": "plain FP64 GPU form (parity-tested against the FP64 CPU oracle at 1e-10)",
         }
+    if not args.no_serve:
+        spath = os.path.join(tmp, "serve20k.ann")
+        synth.workload_potential("serve20k").write(spath)
+        pool = synth.serve_pool(args.serve_pool)
+        mean_atoms = float(np.mean([p.natoms for p in pool]))
+        model_s = NNP.read(spath)
+        sflags = NNP_FORCE | NNP_VACUUM_ENERGY | NNP_PREC_MIXED
+        srv = serve.Server(model_s, sflags, lanes=3)
+        burst = serve.run_stream(srv, pool[:256], np.zeros(256))  # capacity probe: everything at t = 0
+        rate = args.serve_load * burst.atoms_per_s / mean_atoms
+        arrivals = synth.serve_arrivals(args.serve_requests, rate)
+        structs = [pool[k % len(pool)] for k in range(args.serve_requests)]
+        res = serve.run_stream(srv, structs, arrivals)
+        # accuracy of the served (mixed) results against FP64 on a sample of the pool
+        sample = [p for p in pool if p.natoms <= 6000][:24]
+        got = serve.run_stream(srv, sample, np.zeros(len(sample)), keep_results=True).results
+        ref = model_s.compute_batch(sample, NNP_FORCE | NNP_VACUUM_ENERGY)
+        e_err = max(abs(a[0] - b[0]) / s.natoms for a, b, s in zip(got, ref, sample))
+        f_rms = math.sqrt(sum(float(np.sum((a[2] - b[2]) ** 2)) for a, b in zip(got, ref)) /
+                          sum(s.natoms for s in sample))
+        srv.close()
+        out["serving"] = {
+            "workload": f"serve20k: {args.serve_requests} open-loop Poisson requests, atoms log-uniform 16-20000 "
+                        f"(pool of {args.serve_pool} distinct structures cycled), 3 species Dirichlet(1,1,1), "
+                        "0.055/A^3 +-5% strain, mixed precision, 3 lanes",
+            "atoms_per_s": res.atoms_per_s,
+            "requests_per_s": res.requests / res.wall_s,
+            "arrival_rate_per_s": rate,
+            "p50_ms": res.p50_ms,
+            "p99_ms": res.p99_ms,
+            "mean_ms": res.mean_ms,
+            "capacity_atoms_per_s": burst.atoms_per_s,
+            "batches": res.stats["batches"],
+            "max_concurrent_batches": res.stats["max_concurrent_batches"],
+            "peak_workspace_bytes": res.stats["peak_bytes"],
+            "max_energy_err_meV_per_atom": e_err * 1e3,
+            "force_rms_err_meV_per_A": f_rms * 1e3,
+            "error_reference": "plain FP64 GPU form on 24 pool structures",
+        }
+        out["ratio_mixed_serving_vs_plain_fp64"] = res.atoms_per_s / value
+        model_s.close()
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_binding as ob

@@ -104,6 +104,36 @@ int nnp_compute(nnp_model* m, const nnp_structure* s, uint32_t flags, nnp_result
 /* n structures packed into shared launches; results per structure. */
 int nnp_compute_batch(nnp_model* m, int n, const nnp_structure* s, uint32_t flags, nnp_result* r);
 
+/* ---- serving (engine extension; the reference has no server) ----
+ * A continuous-batching request server on one model: `lanes` concurrent batches, each on its
+ * own HIP stream and workspace, share `hbm_budget` bytes of device memory.  Free lanes take
+ * queued requests in arrival order and evaluate them as one packed batch; requests that arrive
+ * while other batches are on the GPU are admitted by the next free lane.  A request whose
+ * estimated footprint exceeds one lane's share of the budget is rejected with NNP_EBUDGET.
+ * nnp_server_submit copies the structure; the result buffers (energy_atom, force) must stay
+ * valid until nnp_server_wait returns for that request.  Destroy the server before its model.
+ * Results do not depend on which requests share a batch. */
+typedef struct nnp_server nnp_server;
+typedef struct {
+    uint32_t flags;               /* evaluation flags of every request (NNP_FORCE, NNP_PREC_MIXED, ...) */
+    int32_t lanes;                /* concurrent batches (default 3) */
+    int64_t hbm_budget;           /* bytes of batch workspaces over all lanes (default 16 GiB) */
+    int32_t max_batch_atoms;      /* atoms per batch (default 262144) */
+    int32_t max_batch_structures; /* structures per batch (default 4096) */
+} nnp_server_cfg;
+int nnp_server_create(nnp_model* m, const nnp_server_cfg* cfg, nnp_server** out);
+/* Returns the request id (>= 0) or a negative error code. */
+int64_t nnp_server_submit(nnp_server* s, const nnp_structure* st, nnp_result* r);
+/* NNP_OK when done, NNP_EAGAIN while pending, negative on error. */
+int nnp_server_poll(nnp_server* s, int64_t id);
+/* Blocks until the request is done; optional submit / completion times on nnp_server_now's clock. */
+int nnp_server_wait(nnp_server* s, int64_t id, double* t_submit, double* t_done);
+/* Seconds on a monotonic clock. */
+double nnp_server_now(void);
+/* completed requests, batches, max concurrent batches, workspace bytes, peak bytes, queued */
+int nnp_server_stats(nnp_server* s, int64_t* out, int nout);
+void nnp_server_destroy(nnp_server* s);
+
 /* ---- device-resident batches (engine extension; no counterpart in the reference) ----
  * For callers that evaluate the same structures repeatedly with inputs kept in HBM (benchmarks,
  * serving, MD drivers): nnp_resident_load uploads a batch and sizes its neighbour rows,
@@ -122,6 +152,8 @@ int nnp_sync(nnp_model* m);
 #define NNP_NSTAGES 6
 int nnp_profile(nnp_model* m, int enable);
 int nnp_profile_read(nnp_model* m, double* ms, int64_t* calls, int nstage);
+/* Engine options: "graph" (1 = replay resident passes from a captured hipGraph, the default). */
+int nnp_set_option(nnp_model* m, const char* key, int64_t value);
 /* Engine state: maxp, max entries, atoms, structures, kernel shape, NE, NC, descriptor width. */
 int nnp_info(nnp_model* m, int64_t* out, int nout);
 
