@@ -46,6 +46,12 @@ hipError_t launch_assemble(const DevStruc* S, const int* atom_struc, const int* 
 hipError_t launch_net(bool fp64, const DevPlan* plan, const DevPlan& hplan, const void* W, const int* sp_off,
                       const int* spatoms, const void* G, int ldg, int vac, double* eatom, void* dEdG, hipStream_t st);
 hipError_t launch_neighbors(const NeighborArgs& a, hipStream_t st);
+size_t fused_lds(int shape, bool fp64, int maxp, int ldg, int maxw, int dersum);
+template <typename T>
+hipError_t launch_fused(int shape, int nblocks, const DevPlan* plan, const double* rfe, const double* rfr, const T* Wt,
+                        const int* atom_type, const int* order, int ntot, int* work, const int* cnt, const int* meta,
+                        const double4* r, int maxp, int ldg, int maxw, int dersum, int vac, int do_force,
+                        double* eatom, T* X, hipStream_t st);
 
 // Total energy per structure: one block per structure, fixed-order tree sum of E_i (FP64).
 __global__ __launch_bounds__(256) void k_energy(const double* __restrict__ eatom, const int* __restrict__ sstart,
@@ -199,6 +205,9 @@ Engine::Engine(HostModel&& hm) : host(std::move(hm)) {
     pb = build_plan(host);
     shape = grid_shape(pb.plan.ne_max, pb.plan.nc_max);
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&num_cus, hipDeviceAttributeMultiprocessorCount, dev));
     d_plan.reserve(sizeof(DevPlan));
     HIPCHK(hipMemcpy(d_plan.p, &pb.plan, sizeof(DevPlan), hipMemcpyHostToDevice));
     d_rfe.reserve(pb.rfun_eta.size() * 8);
@@ -214,6 +223,10 @@ Engine::Engine(HostModel&& hm) : host(std::move(hm)) {
 
 Engine::Engine(const Engine& o, int) : host(o.host), pb(o.pb), shape(o.shape) {
     owns_model = false;
+    pipeline = o.pipeline;
+    fused_blocks_per_cu = o.fused_blocks_per_cu;
+    num_cus = o.num_cus;
+    tune_cache = o.tune_cache;
     d_plan = o.d_plan;
     d_rfe = o.d_rfe;
     d_rfr = o.d_rfr;
@@ -239,6 +252,7 @@ Engine::~Engine() {
             if (ev[st][q]) (void)hipEventDestroy(ev[st][q]);
     d_E.release();
     d_sstart.release();
+    d_work.release();
     if (owns_model)
         for (DevMem* m : {&d_plan, &d_rfe, &d_rfr, &d_w64, &d_w32}) m->release();
     for (DevMem* m : {&d_pos, &d_type, &d_astruc, &d_strucs, &d_centres,
@@ -356,6 +370,8 @@ void Engine::build_neighbors(bool all_images) {
         a.nbr_meta = (int*)d_meta.p;
         a.nbr_r = (double4*)d_nbrr.p;
         a.maxcnt = (int*)d_maxcnt.p;
+        d_work.reserve(64);
+        a.work = (int*)d_work.p;
         a.cell_centres = (const int*)d_centres.p;
         a.ncell = (int)h_cell_centres.size();
         a.pair_centres = (const int*)d_centres.p + h_cell_centres.size();
@@ -421,6 +437,21 @@ void Engine::run(const std::vector<nnp_structure>& ss, uint32_t flags) {
     const bool mixed = flags & NNP_PREC_MIXED;
     upload(ss);
     build_neighbors(flags & NNP_ALL_IMAGES);
+    if (pipeline == 1) {
+        fused(mixed, flags, fused_blocks(mixed, flags));
+        if (flags & NNP_FORCE) {
+            const int ai = (flags & NNP_ALL_IMAGES) ? 1 : 0;
+            if (mixed)
+                HIPCHK(launch_assemble<float>((const DevStruc*)d_strucs.p, (const int*)d_astruc.p, (const int*)d_cnt.p,
+                                              (const int*)d_nbrj.p, (const int*)d_meta.p, (const float*)d_X.p, maxp,
+                                              ntot, ai, (double*)d_F.p, stream));
+            else
+                HIPCHK(launch_assemble<double>((const DevStruc*)d_strucs.p, (const int*)d_astruc.p,
+                                               (const int*)d_cnt.p, (const int*)d_nbrj.p, (const int*)d_meta.p,
+                                               (const double*)d_X.p, maxp, ntot, ai, (double*)d_F.p, stream));
+        }
+        return;
+    }
     if (mixed) descriptors<float>();
     else descriptors<double>();
     networks(mixed, flags & NNP_VACUUM_ENERGY);
@@ -428,6 +459,90 @@ void Engine::run(const std::vector<nnp_structure>& ss, uint32_t flags) {
         if (mixed) forces<float>();
         else forces<double>();
     }
+}
+
+void Engine::net_dims(int& maxw, int& dersum) const {
+    maxw = 0;
+    dersum = 0;
+    for (int t = 0; t < pb.plan.T; ++t) {
+        const DevNet& n = pb.plan.net[t];
+        int ds = 0;
+        maxw = std::max(maxw, n.nin_pad);
+        for (int l = 0; l < n.L; ++l) {
+            maxw = std::max(maxw, n.wpad[l]);
+            ds += n.wpad[l];
+        }
+        dersum = std::max(dersum, ds);
+    }
+}
+
+// Enqueue the fused persistent kernel over the current batch (work counter must be zero:
+// k_zero in the neighbour launch clears it).
+void Engine::fused(bool mixed, uint32_t flags, int nblocks) {
+    int maxw, dersum;
+    net_dims(maxw, dersum);
+    if (fused_lds(shape, !mixed, maxp, pb.plan.ldg, maxw, dersum) > 65536)
+        throw unsupported_error("too many neighbours per atom for the fused kernel's LDS");
+    d_eatom.reserve((size_t)ntot * 8);
+    d_X.reserve((size_t)ntot * maxp * 4 * (mixed ? 4 : 8));
+    d_F.reserve((size_t)ntot * 3 * 8);
+    const int vac = (flags & NNP_VACUUM_ENERGY) ? 1 : 0, frc = (flags & NNP_FORCE) ? 1 : 0;
+    if (mixed)
+        HIPCHK(launch_fused<float>(shape, nblocks, (const DevPlan*)d_plan.p, (const double*)d_rfe.p,
+                                   (const double*)d_rfr.p, (const float*)d_w32.p, (const int*)d_type.p,
+                                   (const int*)d_spatoms.p, ntot, (int*)d_work.p, (const int*)d_cnt.p,
+                                   (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp, pb.plan.ldg, maxw, dersum,
+                                   vac, frc, (double*)d_eatom.p, (float*)d_X.p, stream));
+    else
+        HIPCHK(launch_fused<double>(shape, nblocks, (const DevPlan*)d_plan.p, (const double*)d_rfe.p,
+                                    (const double*)d_rfr.p, (const double*)d_w64.p, (const int*)d_type.p,
+                                    (const int*)d_spatoms.p, ntot, (int*)d_work.p, (const int*)d_cnt.p,
+                                    (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp, pb.plan.ldg, maxw, dersum,
+                                    vac, frc, (double*)d_eatom.p, (double*)d_X.p, stream));
+}
+
+// Persistent grid of the fused kernel.  The autotuner times the kernel on the current batch
+// for several blocks-per-CU counts the first time a (shape, precision, maxp, batch size)
+// combination is seen and keeps the fastest; the launch geometry does not change any result.
+int Engine::fused_blocks(bool mixed, uint32_t flags) {
+    if (fused_blocks_per_cu > 0) return std::min(ntot, num_cus * fused_blocks_per_cu);
+    int lg = 0;
+    while ((1 << (lg + 1)) <= ntot) ++lg;
+    const TuneKey key{shape, mixed ? 0 : 1, maxp, lg};
+    for (const auto& kv : tune_cache)
+        if (kv.first == key) return std::min(ntot, num_cus * kv.second);
+    const int cand[] = {1, 2, 3, 4, 6, 8, 12, 16};
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    last_tune.clear();
+    int best = 4;
+    double best_ms = 1e300;
+    for (int m : cand) {
+        const int nb = std::min(ntot, num_cus * m);
+        double tot = 0;
+        for (int rep = 0; rep < 3; ++rep) {
+            HIPCHK(hipMemsetAsync(d_work.p, 0, 4, stream));
+            HIPCHK(hipEventRecord(e0, stream));
+            fused(mixed, flags, nb);
+            HIPCHK(hipEventRecord(e1, stream));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            if (rep > 0) tot += ms;  // first repetition warms the caches
+        }
+        last_tune.push_back({m, tot / 2});
+        if (tot / 2 < best_ms) {
+            best_ms = tot / 2;
+            best = m;
+        }
+        if (nb == ntot) break;  // more blocks than atoms changes nothing
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIPCHK(hipMemsetAsync(d_work.p, 0, 4, stream));
+    tune_cache.push_back({key, best});
+    return std::min(ntot, num_cus * best);
 }
 
 void Engine::stage_begin(int s) {
@@ -476,6 +591,7 @@ void Engine::resident_load(const std::vector<nnp_structure>& ss, uint32_t flags)
     for (int k = 0; k < nstruc; ++k) st[k + 1] = st[k] + ss[k].natoms;
     d_sstart.reserve(st.size() * 4);
     HIPCHK(hipMemcpyAsync(d_sstart.p, st.data(), st.size() * 4, hipMemcpyHostToDevice, stream));
+    if (pipeline == 1) (void)fused_blocks(flags & NNP_PREC_MIXED, flags);  // tune before any capture
     HIPCHK(hipStreamSynchronize(stream));
     resident_valid = true;
 }
@@ -523,6 +639,7 @@ void Engine::resident_run(uint32_t flags) {
 
 // Enqueue one full pass on the engine's stream (no host sync).
 void Engine::enqueue_pass(uint32_t flags) {
+    const int fused_nb = pipeline == 1 ? fused_blocks(flags & NNP_PREC_MIXED, flags) : 0;
     const bool mixed = flags & NNP_PREC_MIXED;
     NeighborArgs a;
     a.S = (const DevStruc*)d_strucs.p;
@@ -537,6 +654,7 @@ void Engine::enqueue_pass(uint32_t flags) {
     a.nbr_meta = (int*)d_meta.p;
     a.nbr_r = (double4*)d_nbrr.p;
     a.maxcnt = (int*)d_maxcnt.p;
+    a.work = (int*)d_work.p;
     a.cell_centres = (const int*)d_centres.p;
     a.ncell = (int)h_cell_centres.size();
     a.pair_centres = (const int*)d_centres.p + h_cell_centres.size();
@@ -551,6 +669,28 @@ void Engine::enqueue_pass(uint32_t flags) {
     stage_begin(ST_NEIGH);
     HIPCHK(launch_neighbors(a, stream));
     stage_end(ST_NEIGH);
+    if (pipeline == 1) {
+        stage_begin(ST_SYMM);  // descriptors + networks + force terms in one kernel
+        fused(mixed, flags, fused_nb);
+        stage_end(ST_SYMM);
+        if (flags & NNP_FORCE) {
+            stage_begin(ST_ASSEMBLE);
+            const int ai = (flags & NNP_ALL_IMAGES) ? 1 : 0;
+            if (mixed)
+                HIPCHK(launch_assemble<float>((const DevStruc*)d_strucs.p, (const int*)d_astruc.p, (const int*)d_cnt.p,
+                                              (const int*)d_nbrj.p, (const int*)d_meta.p, (const float*)d_X.p, maxp,
+                                              ntot, ai, (double*)d_F.p, stream));
+            else
+                HIPCHK(launch_assemble<double>((const DevStruc*)d_strucs.p, (const int*)d_astruc.p,
+                                               (const int*)d_cnt.p, (const int*)d_nbrj.p, (const int*)d_meta.p,
+                                               (const double*)d_X.p, maxp, ntot, ai, (double*)d_F.p, stream));
+            stage_end(ST_ASSEMBLE);
+        }
+        stage_begin(ST_ENERGY);
+        energies();
+        stage_end(ST_ENERGY);
+        return;
+    }
     stage_begin(ST_SYMM);
     if (mixed) descriptors<float>();
     else descriptors<double>();
@@ -862,6 +1002,17 @@ int nnp_set_option(nnp_model* m, const char* key, int64_t value) {
             E.use_graph = value != 0;
             E.drop_graph();
             E.eager_runs = 0;
+        } else if (k == "pipeline") {
+            if (value != 0 && value != 1) throw std::invalid_argument("nnp_set_option: pipeline is 0 or 1");
+            HIPCHK(hipStreamSynchronize(E.stream));
+            E.pipeline = (int)value;
+            E.drop_graph();
+            E.eager_runs = 0;
+        } else if (k == "fused_blocks_per_cu") {
+            if (value < 0 || value > 64) throw std::invalid_argument("nnp_set_option: fused_blocks_per_cu in 0..64");
+            E.fused_blocks_per_cu = (int)value;
+            E.drop_graph();
+            E.eager_runs = 0;
         } else {
             throw std::invalid_argument("nnp_set_option: unknown option " + k);
         }
@@ -873,8 +1024,10 @@ int nnp_info(nnp_model* m, int64_t* out, int nout) {
     return guarded([&]() {
         if (!m || !out) throw std::invalid_argument("nnp_info: invalid argument");
         const Engine& E = *m->eng;
+        int64_t tuned = E.fused_blocks_per_cu;
+        if (!E.tune_cache.empty()) tuned = E.tune_cache.back().second;
         const int64_t v[] = {E.maxp, E.max_entries, E.ntot, E.nstruc, E.shape, E.pb.plan.ne_max, E.pb.plan.nc_max,
-                             E.pb.plan.ldg};
+                             E.pb.plan.ldg, E.pipeline, tuned, E.num_cus};
         for (int k = 0; k < nout && k < (int)(sizeof(v) / sizeof(v[0])); ++k) out[k] = v[k];
         return (int)(sizeof(v) / sizeof(v[0]));
     });

@@ -37,6 +37,7 @@ struct NeighborArgs {
     int* nbr_meta;
     double4* nbr_r;
     int* maxcnt;
+    int* work = nullptr;  // work counter of the fused persistent kernel (cleared with maxcnt)
     const int* cell_centres;
     int ncell;
     const int* pair_centres;
@@ -87,6 +88,23 @@ class Engine {
     void stage_end(int s);
     void profile_collect();
     bool pending[ST_COUNT] = {};
+    // pipeline: 0 = separate kernels (descriptors, MFMA networks, force terms),
+    //           1 = one persistent fused descriptor + network + force kernel
+    int pipeline = 0;
+    int fused_blocks_per_cu = 0;  // 0: chosen by the autotuner
+    int num_cus = 0;
+    DevMem d_work;
+    struct TuneKey {
+        int shape, fp64, maxp, ntot_log2;
+        bool operator==(const TuneKey& o) const {
+            return shape == o.shape && fp64 == o.fp64 && maxp == o.maxp && ntot_log2 == o.ntot_log2;
+        }
+    };
+    std::vector<std::pair<TuneKey, int>> tune_cache;  // key -> blocks per CU
+    std::vector<std::pair<int, double>> last_tune;    // candidates and their times (ms) of the last tuning
+    int fused_blocks(bool mixed, uint32_t flags);
+    void fused(bool mixed, uint32_t flags, int nblocks);
+    void net_dims(int& maxw, int& dersum) const;
     // captured resident pass (replayed with hipGraphLaunch)
     bool use_graph = true;
     hipGraph_t graph = nullptr;

@@ -371,6 +371,11 @@ PlanBuffers build_plan(const HostModel& m) {
                 for (int k = 0; k < in_pad; ++k)
                     W.push_back((o < out_real && k < in_real) ? net.W[l][(size_t)o * in_real + k] : 0.0);
             align();
+            dn.wtoff[l] = (int64_t)W.size();
+            for (int k = 0; k < in_pad; ++k)
+                for (int o = 0; o < out_pad; ++o)
+                    W.push_back((o < out_real && k < in_real) ? net.W[l][(size_t)o * in_real + k] : 0.0);
+            align();
             dn.boff[l] = (int64_t)W.size();
             for (int o = 0; o < out_pad; ++o) W.push_back(o < out_real ? net.b[l][o] : 0.0);
             in_real = out_real;

@@ -108,6 +108,7 @@ struct DevNet {
     int width[kMaxLayers];     // real output widths
     int wpad[kMaxLayers];      // output widths padded to 16
     int64_t woff[kMaxLayers];  // offsets into the weight buffer (row-major wpad x in_pad)
+    int64_t wtoff[kMaxLayers]; // the same weights transposed (in_pad x wpad)
     int64_t boff[kMaxLayers];
     int64_t inpw_off, inpb_off;
     int tanh_hidden;

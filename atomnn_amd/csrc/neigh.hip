@@ -317,9 +317,13 @@ __global__ __launch_bounds__(kWave) void k_neigh_cells(const DevStruc* __restric
 }
 
 // Clears the per-pass counters (max entry count and the cell-list bin counters) in one launch.
-__global__ void k_zero(int* __restrict__ maxcnt, int* __restrict__ bin_count, int* __restrict__ bin_fill, int nbins) {
+__global__ void k_zero(int* __restrict__ maxcnt, int* __restrict__ work, int* __restrict__ bin_count,
+                       int* __restrict__ bin_fill, int nbins) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t == 0) *maxcnt = 0;
+    if (t == 0) {
+        *maxcnt = 0;
+        if (work) *work = 0;
+    }
     if (t < nbins) {
         bin_count[t] = 0;
         bin_fill[t] = 0;
@@ -359,7 +363,8 @@ namespace atomnn {
 hipError_t launch_neighbors(const NeighborArgs& a, hipStream_t st) {
     hipError_t e;
     const int nz = a.ncell > 0 ? a.nbins : 0;
-    hipLaunchKernelGGL(k_zero, dim3((nz + 255) / 256 + 1), dim3(256), 0, st, a.maxcnt, a.bin_count, a.bin_fill, nz);
+    hipLaunchKernelGGL(k_zero, dim3((nz + 255) / 256 + 1), dim3(256), 0, st, a.maxcnt, a.work, a.bin_count, a.bin_fill,
+                       nz);
     if (a.ncell > 0) {
         hipLaunchKernelGGL(k_bin_atoms, dim3((a.ntot + 255) / 256), dim3(256), 0, st, a.S, a.atom_struc, a.pos, a.ntot,
                            a.atom_bin, a.bin_count);

@@ -178,40 +178,12 @@ __device__ __forceinline__ void radial_tables(const DevAngular& A, const T* su, 
     }
 }
 
-// LDS sizes (bytes) of the two kernels for a given maxp.
+// ------------------------------------------------------------------ per-atom bodies
+// Descriptors of one staged centre into Grow (LDS, zeroed by the caller).
 template <typename T, int NE, int NC>
-__host__ __device__ constexpr size_t symm_lds_bytes(int maxp, int ldg) {
-    return sizeof(T) * (size_t)(4 * maxp + NE * maxp + kWave * NE * NC) + sizeof(T) * (size_t)((ldg + 1) & ~1);
-}
-template <typename T, int NE, int NC>
-__host__ __device__ constexpr size_t force_lds_bytes(int maxp, int ldg) {
-    return sizeof(T) * (size_t)(4 * maxp + 2 * NE * maxp + 4 * maxp + 8 * kWave + ((ldg + 1) & ~1)) +
-           sizeof(int) * (size_t)(maxp + 2 * kWave);
-}
-
-// ------------------------------------------------------------------ descriptors
-template <typename T, int NE, int NC>
-__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <= 16 ? 3 : 1, 8))) void k_symm(const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta,
-                                                const double* __restrict__ rfun_rs, const int* __restrict__ atom_type,
-                                                const int* __restrict__ centres, const int* __restrict__ cnt,
-                                                const int* __restrict__ nbr_meta, const double4* __restrict__ nbr_r,
-                                                int maxp, T* __restrict__ G, int ldg) {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    T* su = (T*)lds_raw;
-    T* Rf = su + 4 * maxp;
-    T* red = Rf + NE * maxp;
-    T* Grow = red + kWave * NE * NC;
-    const int i = centres[blockIdx.x];
-    const int lane = threadIdx.x;
-    const int ti = atom_type[i];
-    const int P = min(cnt[i], maxp);
-    const int ntypes = plan->T;
-    __shared__ int seg_lds[kMaxSpecies + 1];
-    Segs sg{seg_lds};
-    stage_entries<T>(i, P, maxp, ntypes, nbr_meta, nbr_r, su, nullptr, sg, lane);
-    for (int k = lane; k < ldg; k += kWave) Grow[k] = T(0);
-    __syncthreads();
-
+__device__ __forceinline__ void symm_body(const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta,
+                                          const double* __restrict__ rfun_rs, int ti, int ntypes, const T* su, T* Rf,
+                                          T* red, T* Grow, const Segs& sg, int lane) {
     // radial: cutoff once per neighbour, then (function, part) items over 64 lanes; the parts of
     // a function are summed in a fixed order
     {
@@ -338,37 +310,15 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
             __syncthreads();
         }
     }
-    __syncthreads();
-    for (int k = lane; k < ldg; k += kWave) G[(int64_t)i * ldg + k] = Grow[k];
 }
 
-// ------------------------------------------------------------------ force terms
+// Force terms of one staged centre into Xs (LDS).  Wrad: dE/dG row in LDS (radial weights);
+// Wang: the same row in global memory (uniform, scalar loads) or LDS.
 template <typename T, int NE, int NC>
-__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <= 16 ? 3 : 1, 8))) void k_force(const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta,
-                                                 const double* __restrict__ rfun_rs, const int* __restrict__ atom_type,
-                                                 const int* __restrict__ centres, const int* __restrict__ cnt,
-                                                 const int* __restrict__ nbr_meta, const double4* __restrict__ nbr_r,
-                                                 int maxp, const T* __restrict__ dEdG, int ldg, T* __restrict__ Xout) {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    T* su = (T*)lds_raw;
-    T* Rf = su + 4 * maxp;
-    T* Rd = Rf + NE * maxp;
-    T* Xs = Rd + NE * maxp;           // [maxp][4]
-    T* slots = Xs + 4 * maxp;         // [2][kWave][4]
-    T* W = slots + 8 * kWave;         // dE/dG row
-    int* orig = (int*)(W + ((ldg + 1) & ~1));
-    int* slot_row = orig + maxp;      // [2][kWave]
-    const int i = centres[blockIdx.x];
-    const int lane = threadIdx.x;
-    const int ti = atom_type[i];
-    const int P = min(cnt[i], maxp);
-    const int ntypes = plan->T;
-    __shared__ int seg_lds[kMaxSpecies + 1];
-    Segs sg{seg_lds};
-    stage_entries<T>(i, P, maxp, ntypes, nbr_meta, nbr_r, su, orig, sg, lane);
-    for (int k = lane; k < ldg; k += kWave) W[k] = dEdG[(int64_t)i * ldg + k];
-    __syncthreads();
-
+__device__ __forceinline__ void force_body(const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta,
+                                           const double* __restrict__ rfun_rs, int ti, int ntypes, int P, const T* su,
+                                           T* Rf, T* Rd, T* Xs, T* slots, int* slot_row, const T* W, const T* Wang,
+                                           const Segs& sg, int lane) {
     // radial term, one lane per entry: X_j = -dE/dd_j u_j
     for (int p = lane; p < P; p += kWave) {
         int b = 0;
@@ -401,7 +351,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
             if (!same) radial_tables<T, NE>(A, su, sb, sb + nb, Rf, Rd, lane);
             // block weights W_ec = norm dE/dG[goff + fmap(e,c)]: uniform, read with scalar loads
             T w[NE][NC];
-            const T* dEdGi = dEdG + (int64_t)i * ldg + A.goff;
+            const T* dEdGi = Wang + A.goff;
 #pragma unroll
             for (int e = 0; e < NE; ++e)
 #pragma unroll
@@ -513,6 +463,73 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
             }
         }
     }
+}
+
+// LDS sizes (bytes) of the kernels for a given maxp.
+template <typename T, int NE, int NC>
+__host__ __device__ constexpr size_t symm_lds_bytes(int maxp, int ldg) {
+    return sizeof(T) * (size_t)(4 * maxp + NE * maxp + kWave * NE * NC) + sizeof(T) * (size_t)((ldg + 1) & ~1);
+}
+template <typename T, int NE, int NC>
+__host__ __device__ constexpr size_t force_lds_bytes(int maxp, int ldg) {
+    return sizeof(T) * (size_t)(4 * maxp + 2 * NE * maxp + 4 * maxp + 8 * kWave + ((ldg + 1) & ~1)) +
+           sizeof(int) * (size_t)(maxp + 2 * kWave);
+}
+
+// ------------------------------------------------------------------ descriptors
+template <typename T, int NE, int NC>
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <= 16 ? 3 : 1, 8))) void k_symm(
+    const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta, const double* __restrict__ rfun_rs,
+    const int* __restrict__ atom_type, const int* __restrict__ centres, const int* __restrict__ cnt,
+    const int* __restrict__ nbr_meta, const double4* __restrict__ nbr_r, int maxp, T* __restrict__ G, int ldg) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    T* su = (T*)lds_raw;
+    T* Rf = su + 4 * maxp;
+    T* red = Rf + NE * maxp;
+    T* Grow = red + kWave * NE * NC;
+    const int i = centres[blockIdx.x];
+    const int lane = threadIdx.x;
+    const int ti = atom_type[i];
+    const int P = min(cnt[i], maxp);
+    const int ntypes = plan->T;
+    __shared__ int seg_lds[kMaxSpecies + 1];
+    Segs sg{seg_lds};
+    stage_entries<T>(i, P, maxp, ntypes, nbr_meta, nbr_r, su, nullptr, sg, lane);
+    for (int k = lane; k < ldg; k += kWave) Grow[k] = T(0);
+    __syncthreads();
+    symm_body<T, NE, NC>(plan, rfun_eta, rfun_rs, ti, ntypes, su, Rf, red, Grow, sg, lane);
+    __syncthreads();
+    for (int k = lane; k < ldg; k += kWave) G[(int64_t)i * ldg + k] = Grow[k];
+}
+
+// ------------------------------------------------------------------ force terms
+template <typename T, int NE, int NC>
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <= 16 ? 3 : 1, 8))) void k_force(
+    const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta, const double* __restrict__ rfun_rs,
+    const int* __restrict__ atom_type, const int* __restrict__ centres, const int* __restrict__ cnt,
+    const int* __restrict__ nbr_meta, const double4* __restrict__ nbr_r, int maxp, const T* __restrict__ dEdG,
+    int ldg, T* __restrict__ Xout) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    T* su = (T*)lds_raw;
+    T* Rf = su + 4 * maxp;
+    T* Rd = Rf + NE * maxp;
+    T* Xs = Rd + NE * maxp;           // [maxp][4]
+    T* slots = Xs + 4 * maxp;         // [2][kWave][4]
+    T* W = slots + 8 * kWave;         // dE/dG row
+    int* orig = (int*)(W + ((ldg + 1) & ~1));
+    int* slot_row = orig + maxp;      // [2][kWave]
+    const int i = centres[blockIdx.x];
+    const int lane = threadIdx.x;
+    const int ti = atom_type[i];
+    const int P = min(cnt[i], maxp);
+    const int ntypes = plan->T;
+    __shared__ int seg_lds[kMaxSpecies + 1];
+    Segs sg{seg_lds};
+    stage_entries<T>(i, P, maxp, ntypes, nbr_meta, nbr_r, su, orig, sg, lane);
+    for (int k = lane; k < ldg; k += kWave) W[k] = dEdG[(int64_t)i * ldg + k];
+    __syncthreads();
+    force_body<T, NE, NC>(plan, rfun_eta, rfun_rs, ti, ntypes, P, su, Rf, Rd, Xs, slots, slot_row, W,
+                          dEdG + (int64_t)i * ldg, sg, lane);
     const int64_t row = (int64_t)i * maxp;
     for (int p = lane; p < P; p += kWave) {
         const int o = orig[p];
@@ -522,6 +539,188 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
         Xout[4 * (row + o) + 3] = T(0);
     }
 }
+
+// ------------------------------------------------------------------ fused persistent kernel
+// Network of one atom inside the wavefront (the batched form's fused launch): lanes own output
+// units; forward layers read the transposed weights (coalesced over lanes), backward layers the
+// row-major weights.  Same reference arithmetic as the MFMA kernels (net.hip): fp/fpbp and
+// DODZ::grad, ml/nn.cpp:1191-1233, 1620-1628.
+template <typename T>
+__device__ __forceinline__ double net_body(const DevNet& N, const T* __restrict__ Wt, const T* G, T* act0, T* act1,
+                                           T* der, T* dErow, int maxw, int vac, int lane) {
+    for (int k = lane; k < N.nin_pad; k += kWave)
+        act0[k] = k < N.nin ? Wt[N.inpw_off + k] * (G[k] + Wt[N.inpb_off + k]) : T(0);
+    __syncthreads();
+    T* cur = act0;
+    T* nxt = act1;
+    int in_pad = N.nin_pad, doff = 0;
+    for (int l = 0; l < N.L; ++l) {
+        const int out_pad = N.wpad[l];
+        const T* WT = Wt + N.wtoff[l];
+        const T* b = Wt + N.boff[l];
+        const bool linear = (l == N.L - 1) || !N.tanh_hidden;
+        for (int o = lane; o < out_pad; o += kWave) {
+            T z = b[o];
+            for (int k = 0; k < in_pad; ++k) z += WT[(int64_t)k * out_pad + o] * cur[k];
+            T a, d;
+            if (linear) {
+                a = z;
+                d = T(1);
+            } else {
+                a = tanh(z);
+                d = T(1) - a * a;
+            }
+            nxt[o] = a;
+            der[doff + o] = d;
+        }
+        __syncthreads();
+        T* t = cur;
+        cur = nxt;
+        nxt = t;
+        in_pad = out_pad;
+        doff += out_pad;
+    }
+    const double E = (double)N.outb + (double)cur[0] * N.outw + (vac ? N.evac : 0.0);
+    const int last = N.wpad[N.L - 1];
+    doff -= last;
+    for (int o = lane; o < last; o += kWave) nxt[o] = o == 0 ? T(N.outw) * der[doff] : T(0);
+    __syncthreads();
+    {
+        T* t = cur;
+        cur = nxt;
+        nxt = t;
+    }
+    for (int l = N.L - 1; l >= 0; --l) {
+        const int K = N.wpad[l], J = l == 0 ? N.nin_pad : N.wpad[l - 1];
+        const T* W = Wt + N.woff[l];  // K x J row-major
+        if (l > 0) doff -= J;
+        for (int j = lane; j < J; j += kWave) {
+            T sacc = T(0);
+            for (int k = 0; k < K; ++k) sacc += W[(int64_t)k * J + j] * cur[k];
+            if (l > 0) nxt[j] = sacc * der[doff + j];
+            else dErow[j] = j < N.nin ? sacc * Wt[N.inpw_off + j] : T(0);
+        }
+        __syncthreads();
+        T* t = cur;
+        cur = nxt;
+        nxt = t;
+    }
+    return E;
+}
+
+template <typename T, int NE, int NC>
+__host__ __device__ constexpr size_t fused_lds_bytes(int maxp, int ldg, int maxw, int dersum) {
+    return sizeof(T) * (size_t)(4 * maxp + 2 * NE * maxp + 4 * maxp + 8 * kWave + kWave * NE * NC +
+                                2 * ((ldg + 1) & ~1) + 2 * maxw + dersum) +
+           sizeof(int) * (size_t)(maxp + 2 * kWave);
+}
+
+// Persistent fused pass: each wavefront takes atoms from a work counter until the batch is
+// done (every wave reaches the exit), and for each atom stages its entries once, sums the
+// descriptors, runs its network in-wave and computes its per-entry force terms.  The order of
+// atoms (species-sorted) keeps a species' weights hot in the caches.
+template <typename T, int NE, int NC>
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <= 16 ? (sizeof(T) == 8 ? 2 : 3) : 1, 8))) void k_fused(
+    const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta, const double* __restrict__ rfun_rs,
+    const T* __restrict__ Wt, const int* __restrict__ atom_type, const int* __restrict__ order, int ntot,
+    int* __restrict__ work, const int* __restrict__ cnt, const int* __restrict__ nbr_meta,
+    const double4* __restrict__ nbr_r, int maxp, int ldg, int maxw, int dersum, int vac, int do_force,
+    double* __restrict__ eatom, T* __restrict__ Xout) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    T* su = (T*)lds_raw;
+    T* Rf = su + 4 * maxp;
+    T* Rd = Rf + NE * maxp;
+    T* Xs = Rd + NE * maxp;
+    T* slots = Xs + 4 * maxp;
+    T* red = slots + 8 * kWave;
+    T* Grow = red + kWave * NE * NC;
+    T* dErow = Grow + ((ldg + 1) & ~1);
+    T* act0 = dErow + ((ldg + 1) & ~1);
+    T* act1 = act0 + maxw;
+    T* der = act1 + maxw;
+    int* orig = (int*)(der + dersum);
+    int* slot_row = orig + maxp;
+    __shared__ int seg_lds[kMaxSpecies + 1];
+    __shared__ int next_atom;
+    Segs sg{seg_lds};
+    const int lane = threadIdx.x;
+    const int ntypes = plan->T;
+    for (;;) {
+        if (lane == 0) next_atom = atomicAdd(work, 1);
+        __syncthreads();
+        const int idx = next_atom;
+        __syncthreads();
+        if (idx >= ntot) break;
+        const int i = order[idx];
+        const int ti = atom_type[i];
+        const int P = min(cnt[i], maxp);
+        stage_entries<T>(i, P, maxp, ntypes, nbr_meta, nbr_r, su, orig, sg, lane);
+        for (int k = lane; k < ldg; k += kWave) Grow[k] = T(0);
+        __syncthreads();
+        symm_body<T, NE, NC>(plan, rfun_eta, rfun_rs, ti, ntypes, su, Rf, red, Grow, sg, lane);
+        __syncthreads();
+        const double E = net_body<T>(plan->net[ti], Wt, Grow, act0, act1, der, dErow, maxw, vac, lane);
+        if (lane == 0) eatom[i] = E;
+        if (do_force) {
+            force_body<T, NE, NC>(plan, rfun_eta, rfun_rs, ti, ntypes, P, su, Rf, Rd, Xs, slots, slot_row, dErow,
+                                  dErow, sg, lane);
+            const int64_t row = (int64_t)i * maxp;
+            for (int p = lane; p < P; p += kWave) {
+                const int o = orig[p];
+                Xout[4 * (row + o) + 0] = Xs[4 * p];
+                Xout[4 * (row + o) + 1] = Xs[4 * p + 1];
+                Xout[4 * (row + o) + 2] = Xs[4 * p + 2];
+                Xout[4 * (row + o) + 3] = T(0);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T, int NE, int NC>
+static hipError_t launch_fused_t(int nblocks, const DevPlan* plan, const double* rfe, const double* rfr, const T* Wt,
+                                 const int* atom_type, const int* order, int ntot, int* work, const int* cnt,
+                                 const int* meta, const double4* r, int maxp, int ldg, int maxw, int dersum, int vac,
+                                 int do_force, double* eatom, T* X, hipStream_t st) {
+    const size_t lds = fused_lds_bytes<T, NE, NC>(maxp, ldg, maxw, dersum);
+    if (lds > 65536) return hipErrorInvalidConfiguration;
+    hipLaunchKernelGGL((k_fused<T, NE, NC>), dim3(nblocks), dim3(kWave), lds, st, plan, rfe, rfr, Wt, atom_type, order,
+                       ntot, work, cnt, meta, r, maxp, ldg, maxw, dersum, vac, do_force, eatom, X);
+    return hipGetLastError();
+}
+
+size_t fused_lds(int shape, bool fp64, int maxp, int ldg, int maxw, int dersum) {
+    if (fp64) {
+        if (shape == 0) return fused_lds_bytes<double, 2, 4>(maxp, ldg, maxw, dersum);
+        if (shape == 1) return fused_lds_bytes<double, 4, 4>(maxp, ldg, maxw, dersum);
+        return fused_lds_bytes<double, 8, 8>(maxp, ldg, maxw, dersum);
+    }
+    if (shape == 0) return fused_lds_bytes<float, 2, 4>(maxp, ldg, maxw, dersum);
+    if (shape == 1) return fused_lds_bytes<float, 4, 4>(maxp, ldg, maxw, dersum);
+    return fused_lds_bytes<float, 8, 8>(maxp, ldg, maxw, dersum);
+}
+
+template <typename T>
+hipError_t launch_fused(int shape, int nblocks, const DevPlan* plan, const double* rfe, const double* rfr, const T* Wt,
+                        const int* atom_type, const int* order, int ntot, int* work, const int* cnt, const int* meta,
+                        const double4* r, int maxp, int ldg, int maxw, int dersum, int vac, int do_force,
+                        double* eatom, T* X, hipStream_t st) {
+    if (shape == 0)
+        return launch_fused_t<T, 2, 4>(nblocks, plan, rfe, rfr, Wt, atom_type, order, ntot, work, cnt, meta, r, maxp,
+                                       ldg, maxw, dersum, vac, do_force, eatom, X, st);
+    if (shape == 1)
+        return launch_fused_t<T, 4, 4>(nblocks, plan, rfe, rfr, Wt, atom_type, order, ntot, work, cnt, meta, r, maxp,
+                                       ldg, maxw, dersum, vac, do_force, eatom, X, st);
+    return launch_fused_t<T, 8, 8>(nblocks, plan, rfe, rfr, Wt, atom_type, order, ntot, work, cnt, meta, r, maxp, ldg,
+                                   maxw, dersum, vac, do_force, eatom, X, st);
+}
+
+template hipError_t launch_fused<double>(int, int, const DevPlan*, const double*, const double*, const double*,
+                                         const int*, const int*, int, int*, const int*, const int*, const double4*, int,
+                                         int, int, int, int, int, double*, double*, hipStream_t);
+template hipError_t launch_fused<float>(int, int, const DevPlan*, const double*, const double*, const float*,
+                                        const int*, const int*, int, int*, const int*, const int*, const double4*, int,
+                                        int, int, int, int, int, double*, float*, hipStream_t);
 
 // F_a = sum_s X[a,s] - sum_{entries (j -> a) that carry a reaction} X[j -> a], in FP64.
 // One wavefront per atom, lanes over its entries.  The mirrored entry (j -> a, image

@@ -152,9 +152,16 @@ int nnp_sync(nnp_model* m);
 #define NNP_NSTAGES 6
 int nnp_profile(nnp_model* m, int enable);
 int nnp_profile_read(nnp_model* m, double* ms, int64_t* calls, int nstage);
-/* Engine options: "graph" (1 = replay resident passes from a captured hipGraph, the default). */
+/* Engine options:
+ *   "graph"     1 = replay resident passes from a captured hipGraph (default), 0 = eager passes
+ *   "pipeline"  0 = separate kernels: descriptors, per-species MFMA network GEMMs, force terms
+ *                   (default);
+ *               1 = one persistent fused descriptor + network + force kernel (the batched form;
+ *                   networks evaluated in-wave, so results differ from pipeline 0 in the last bits)
+ *   "fused_blocks_per_cu"  persistent grid of the fused kernel; 0 = chosen by the autotuner */
 int nnp_set_option(nnp_model* m, const char* key, int64_t value);
-/* Engine state: maxp, max entries, atoms, structures, kernel shape, NE, NC, descriptor width. */
+/* Engine state: maxp, max entries, atoms, structures, kernel shape, NE, NC, descriptor width,
+ * pipeline, fused blocks per CU (tuned), compute units. */
 int nnp_info(nnp_model* m, int64_t* out, int nout);
 
 #ifdef __cplusplus
