@@ -35,11 +35,11 @@ size_t net_lds_bytes(const DevNet& n, bool fp64);
 template <typename T>
 hipError_t launch_symm(int shape, const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
                        const int* centres, int ncentres, const int* cnt, const int* meta, const double4* r, int maxp,
-                       T* G, int ldg, hipStream_t st);
+                       T* G, int ldg, int use_mom, int ntypes, hipStream_t st);
 template <typename T>
 hipError_t launch_force(int shape, const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
                         const int* centres, int ncentres, const int* cnt, const int* meta, const double4* r, int maxp,
-                        const T* dEdG, int ldg, T* X, hipStream_t st);
+                        const T* dEdG, int ldg, T* X, int use_mom, int ntypes, hipStream_t st);
 template <typename T>
 hipError_t launch_assemble(const DevStruc* S, const int* atom_struc, const int* cnt, const int* nbr_j, const int* meta,
                            const T* X, int maxp, int ntot, int all_images, double* F, hipStream_t st);
@@ -47,11 +47,12 @@ hipError_t launch_net(bool fp64, const DevPlan* plan, const DevPlan& hplan, cons
                       const int* spatoms, const void* G, int ldg, int vac, double* eatom, void* dEdG, hipStream_t st);
 hipError_t launch_neighbors(const NeighborArgs& a, hipStream_t st);
 size_t fused_lds(int shape, bool fp64, int maxp, int ldg, int maxw, int dersum);
+size_t mom_lds(int shape, bool fp64, int maxp, int ntypes);
 template <typename T>
 hipError_t launch_fused(int shape, int nblocks, const DevPlan* plan, const double* rfe, const double* rfr, const T* Wt,
                         const int* atom_type, const int* order, int ntot, int* work, const int* cnt, const int* meta,
                         const double4* r, int maxp, int ldg, int maxw, int dersum, int vac, int do_force,
-                        double* eatom, T* X, hipStream_t st);
+                        double* eatom, T* X, int use_mom, int ntypes, hipStream_t st);
 
 // Total energy per structure: one block per structure, fixed-order tree sum of E_i (FP64).
 __global__ __launch_bounds__(256) void k_energy(const double* __restrict__ eatom, const int* __restrict__ sstart,
@@ -225,6 +226,7 @@ Engine::Engine(const Engine& o, int) : host(o.host), pb(o.pb), shape(o.shape) {
     owns_model = false;
     pipeline = o.pipeline;
     fused_blocks_per_cu = o.fused_blocks_per_cu;
+    moments = o.moments;
     num_cus = o.num_cus;
     tune_cache = o.tune_cache;
     d_plan = o.d_plan;
@@ -401,14 +403,25 @@ void Engine::build_neighbors(bool all_images) {
     throw std::runtime_error("neighbour list did not converge");
 }
 
+// Whether a kernel whose own LDS is base_lds bytes runs the moment expansion for qualifying
+// species: the option is on, some species qualifies and the moment tables fit in 64 KiB.
+int Engine::mom_flag(size_t base_lds, bool fp64) {
+    int any = 0;
+    for (int t = 0; t < pb.plan.T; ++t) any |= pb.plan.mom_ok[t];
+    last_mom = (moments && any && base_lds + mom_lds(shape, fp64, maxp, pb.plan.T) <= 65536) ? 1 : 0;
+    return last_mom;
+}
+
 template <typename T>
 void Engine::descriptors() {
     const size_t lds = symm_lds(shape, sizeof(T) == 8, maxp, pb.plan.ldg);
     if (lds > 65536) throw unsupported_error("too many neighbours per atom for the descriptor kernel's LDS");
+    const int mom = mom_flag(lds, sizeof(T) == 8);
     d_G.reserve((size_t)ntot * pb.plan.ldg * sizeof(T));
     HIPCHK(launch_symm<T>(shape, (const DevPlan*)d_plan.p, (const double*)d_rfe.p, (const double*)d_rfr.p,
                           (const int*)d_type.p, (const int*)d_centres.p, ntot, (const int*)d_cnt.p,
-                          (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp, (T*)d_G.p, pb.plan.ldg, stream));
+                          (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp, (T*)d_G.p, pb.plan.ldg, mom,
+                          pb.plan.T, stream));
 }
 
 void Engine::networks(bool mixed, bool vac) {
@@ -422,12 +435,13 @@ template <typename T>
 void Engine::forces() {
     const size_t lds = force_lds(shape, sizeof(T) == 8, maxp, pb.plan.ldg);
     if (lds > 65536) throw unsupported_error("too many neighbours per atom for the force kernel's LDS");
+    const int mom = mom_flag(lds, sizeof(T) == 8);
     d_X.reserve((size_t)ntot * maxp * 4 * sizeof(T));
     d_F.reserve((size_t)ntot * 3 * 8);
     HIPCHK(launch_force<T>(shape, (const DevPlan*)d_plan.p, (const double*)d_rfe.p, (const double*)d_rfr.p,
                            (const int*)d_type.p, (const int*)d_centres.p, ntot, (const int*)d_cnt.p,
                            (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp, (const T*)d_dEdG.p, pb.plan.ldg,
-                           (T*)d_X.p, stream));
+                           (T*)d_X.p, mom, pb.plan.T, stream));
     HIPCHK(launch_assemble<T>((const DevStruc*)d_strucs.p, (const int*)d_astruc.p, (const int*)d_cnt.p,
                               (const int*)d_nbrj.p, (const int*)d_meta.p, (const T*)d_X.p, maxp, ntot,
                               all_images_ ? 1 : 0, (double*)d_F.p, stream));
@@ -481,8 +495,9 @@ void Engine::net_dims(int& maxw, int& dersum) const {
 void Engine::fused(bool mixed, uint32_t flags, int nblocks) {
     int maxw, dersum;
     net_dims(maxw, dersum);
-    if (fused_lds(shape, !mixed, maxp, pb.plan.ldg, maxw, dersum) > 65536)
-        throw unsupported_error("too many neighbours per atom for the fused kernel's LDS");
+    const size_t lds = fused_lds(shape, !mixed, maxp, pb.plan.ldg, maxw, dersum);
+    if (lds > 65536) throw unsupported_error("too many neighbours per atom for the fused kernel's LDS");
+    const int mom = mom_flag(lds, !mixed);
     d_eatom.reserve((size_t)ntot * 8);
     d_X.reserve((size_t)ntot * maxp * 4 * (mixed ? 4 : 8));
     d_F.reserve((size_t)ntot * 3 * 8);
@@ -492,13 +507,13 @@ void Engine::fused(bool mixed, uint32_t flags, int nblocks) {
                                    (const double*)d_rfr.p, (const float*)d_w32.p, (const int*)d_type.p,
                                    (const int*)d_spatoms.p, ntot, (int*)d_work.p, (const int*)d_cnt.p,
                                    (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp, pb.plan.ldg, maxw, dersum,
-                                   vac, frc, (double*)d_eatom.p, (float*)d_X.p, stream));
+                                   vac, frc, (double*)d_eatom.p, (float*)d_X.p, mom, pb.plan.T, stream));
     else
         HIPCHK(launch_fused<double>(shape, nblocks, (const DevPlan*)d_plan.p, (const double*)d_rfe.p,
                                     (const double*)d_rfr.p, (const double*)d_w64.p, (const int*)d_type.p,
                                     (const int*)d_spatoms.p, ntot, (int*)d_work.p, (const int*)d_cnt.p,
                                     (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp, pb.plan.ldg, maxw, dersum,
-                                    vac, frc, (double*)d_eatom.p, (double*)d_X.p, stream));
+                                    vac, frc, (double*)d_eatom.p, (double*)d_X.p, mom, pb.plan.T, stream));
 }
 
 // Persistent grid of the fused kernel.  The autotuner times the kernel on the current batch
@@ -702,18 +717,20 @@ void Engine::enqueue_pass(uint32_t flags) {
         stage_begin(ST_FORCE);
         const size_t lds = force_lds(shape, !mixed, maxp, pb.plan.ldg);
         if (lds > 65536) throw unsupported_error("too many neighbours per atom for the force kernel's LDS");
+        const int mom = mom_flag(lds, !mixed);
         d_X.reserve((size_t)ntot * maxp * 4 * (mixed ? 4 : 8));
         d_F.reserve((size_t)ntot * 3 * 8);
         if (mixed)
             HIPCHK(launch_force<float>(shape, (const DevPlan*)d_plan.p, (const double*)d_rfe.p, (const double*)d_rfr.p,
                                        (const int*)d_type.p, (const int*)d_centres.p, ntot, (const int*)d_cnt.p,
                                        (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp,
-                                       (const float*)d_dEdG.p, pb.plan.ldg, (float*)d_X.p, stream));
+                                       (const float*)d_dEdG.p, pb.plan.ldg, (float*)d_X.p, mom, pb.plan.T, stream));
         else
             HIPCHK(launch_force<double>(shape, (const DevPlan*)d_plan.p, (const double*)d_rfe.p,
                                         (const double*)d_rfr.p, (const int*)d_type.p, (const int*)d_centres.p, ntot,
                                         (const int*)d_cnt.p, (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp,
-                                        (const double*)d_dEdG.p, pb.plan.ldg, (double*)d_X.p, stream));
+                                        (const double*)d_dEdG.p, pb.plan.ldg, (double*)d_X.p, mom, pb.plan.T,
+                                        stream));
         stage_end(ST_FORCE);
         stage_begin(ST_ASSEMBLE);
         const int ai = (flags & NNP_ALL_IMAGES) ? 1 : 0;
@@ -1008,6 +1025,13 @@ int nnp_set_option(nnp_model* m, const char* key, int64_t value) {
             E.pipeline = (int)value;
             E.drop_graph();
             E.eager_runs = 0;
+        } else if (k == "moments") {
+            if (value != 0 && value != 1) throw std::invalid_argument("nnp_set_option: moments is 0 or 1");
+            HIPCHK(hipStreamSynchronize(E.stream));
+            E.moments = (int)value;
+            E.tune_cache.clear();
+            E.drop_graph();
+            E.eager_runs = 0;
         } else if (k == "fused_blocks_per_cu") {
             if (value < 0 || value > 64) throw std::invalid_argument("nnp_set_option: fused_blocks_per_cu in 0..64");
             E.fused_blocks_per_cu = (int)value;
@@ -1027,7 +1051,7 @@ int nnp_info(nnp_model* m, int64_t* out, int nout) {
         int64_t tuned = E.fused_blocks_per_cu;
         if (!E.tune_cache.empty()) tuned = E.tune_cache.back().second;
         const int64_t v[] = {E.maxp, E.max_entries, E.ntot, E.nstruc, E.shape, E.pb.plan.ne_max, E.pb.plan.nc_max,
-                             E.pb.plan.ldg, E.pipeline, tuned, E.num_cus};
+                             E.pb.plan.ldg, E.pipeline, tuned, E.num_cus, E.last_mom};
         for (int k = 0; k < nout && k < (int)(sizeof(v) / sizeof(v[0])); ++k) out[k] = v[k];
         return (int)(sizeof(v) / sizeof(v[0]));
     });

@@ -92,6 +92,11 @@ class Engine {
     //           1 = one persistent fused descriptor + network + force kernel
     int pipeline = 0;
     int fused_blocks_per_cu = 0;  // 0: chosen by the autotuner
+    // moments: 1 = angular sums by the moment expansion where the model qualifies (integer
+    // zeta <= 4, |lambda| <= 1, one shared exponent table) and the LDS fits; 0 = triplet loops
+    int moments = 1;
+    int last_mom = 0;  // whether the last pass used the moment expansion
+    int mom_flag(size_t base_lds, bool fp64);
     int num_cus = 0;
     DevMem d_work;
     struct TuneKey {

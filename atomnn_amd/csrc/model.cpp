@@ -341,6 +341,20 @@ PlanBuffers build_plan(const HostModel& m) {
             if (d.ne > P.ne_max) P.ne_max = d.ne;
             if (d.nc > P.nc_max) P.nc_max = d.nc;
         }
+        // moment expansion eligibility of this centre species
+        {
+            bool ok = true;
+            const DevAngular& A0 = P.ang[i][0];
+            for (int b = 0; b < nblk && ok; ++b) {
+                const DevAngular& A = P.ang[i][b];
+                if (A.ne != A0.ne || A.alpha != A0.alpha || A.cut != A0.cut || A.rc != A0.rc) ok = false;
+                for (int e = 0; e < A.ne && ok; ++e)
+                    if (A.ietap[e] != A0.ietap[e]) ok = false;
+                for (int c = 0; c < A.nc && ok; ++c)
+                    if (A.zint[c] < 0 || A.zint[c] > 4 || std::fabs(A.lambda[c]) > 1.0) ok = false;
+            }
+            P.mom_ok[i] = ok ? 1 : 0;
+        }
         // network: weights padded (outputs to 16, inputs to 16) and stored row-major
         const NetSpec& net = s.net;
         DevNet& dn = P.net[i];

@@ -120,6 +120,9 @@ struct DevPlan {
     int T;
     int ldg;  // max descriptor length over species
     int ne_max, nc_max;
+    // moment path per centre species: every angular block has integer zeta in [0, 4], |lambda| <= 1,
+    // and all blocks share one exponent table, alpha and cutoff (see symm.hip)
+    int mom_ok[kMaxSpecies];
     int nG[kMaxSpecies];
     int nR[kMaxSpecies];
     DevRadial rad[kMaxSpecies][kMaxSpecies];

@@ -30,6 +30,7 @@
 // basis_angular.cpp:498-501 (no sign of cw, which is >= 0 up to rounding).
 #include "device.hpp"
 #include "model.hpp"
+#include "moments.hpp"
 
 namespace atomnn {
 
@@ -182,8 +183,8 @@ __device__ __forceinline__ void radial_tables(const DevAngular& A, const T* su, 
 // Descriptors of one staged centre into Grow (LDS, zeroed by the caller).
 template <typename T, int NE, int NC>
 __device__ __forceinline__ void symm_body(const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta,
-                                          const double* __restrict__ rfun_rs, int ti, int ntypes, const T* su, T* Rf,
-                                          T* red, T* Grow, const Segs& sg, int lane) {
+                                          const double* __restrict__ rfun_rs, int ti, int ntypes, int P, int maxp,
+                                          const T* su, T* Rf, T* red, T* Grow, const Segs& sg, T* mom, int lane) {
     // radial: cutoff once per neighbour, then (function, part) items over 64 lanes; the parts of
     // a function are summed in a fixed order
     {
@@ -230,7 +231,20 @@ __device__ __forceinline__ void symm_body(const DevPlan* __restrict__ plan, cons
         }
     }
 
-    // angular blocks in LMat order
+    // angular blocks: moment expansion when the centre species allows it
+    if (mom && plan->mom_ok[ti]) {
+        T* mono = mom;
+        T* Mom = mono + kMomK * (maxp + 1);
+        T* D = Mom + ntypes * NE * kMomK;
+        T* S = D + ntypes * NE;
+        radial_tables<T, NE>(plan->ang[ti][0], su, 0, P, Rf, nullptr, lane);
+        __syncthreads();
+        build_moments<T, NE>(su, Rf, P, maxp, ntypes, sg.start, mono, Mom, D, lane);
+        symm_angular_moments<T, NE>(plan, ti, ntypes, Mom, D, S, Grow, lane);
+        __syncthreads();
+        return;
+    }
+    // angular blocks in LMat order, pair by pair
     for (int bb = 0; bb < ntypes; ++bb) {
         for (int a = 0; a <= bb; ++a) {
             const DevAngular& A = plan->ang[ti][bb * (bb + 1) / 2 + a];
@@ -316,9 +330,9 @@ __device__ __forceinline__ void symm_body(const DevPlan* __restrict__ plan, cons
 // Wang: the same row in global memory (uniform, scalar loads) or LDS.
 template <typename T, int NE, int NC>
 __device__ __forceinline__ void force_body(const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta,
-                                           const double* __restrict__ rfun_rs, int ti, int ntypes, int P, const T* su,
-                                           T* Rf, T* Rd, T* Xs, T* slots, int* slot_row, const T* W, const T* Wang,
-                                           const Segs& sg, int lane) {
+                                           const double* __restrict__ rfun_rs, int ti, int ntypes, int P, int maxp,
+                                           const T* su, T* Rf, T* Rd, T* Xs, T* slots, int* slot_row, const T* W,
+                                           const T* Wang, const Segs& sg, T* mom, int lane) {
     // radial term, one lane per entry: X_j = -dE/dd_j u_j
     for (int p = lane; p < P; p += kWave) {
         int b = 0;
@@ -340,6 +354,24 @@ __device__ __forceinline__ void force_body(const DevPlan* __restrict__ plan, con
     }
     __syncthreads();
 
+    if (mom && plan->mom_ok[ti]) {
+        T* mono = mom;
+        T* Mom = mono + kMomK * (maxp + 1);
+        T* D = Mom + ntypes * NE * kMomK;
+        T* wtab = D + ntypes * NE;
+        T* Lam = wtab + (ntypes * (ntypes + 1) / 2) * NE * (kMomZ + 1);
+        T* Lamp = Lam + ntypes * NE * kMomK;
+        T* Lx = Lamp + ntypes * NE * kMomK;
+        T* Ly = Lx + ntypes * NE * kMomK3;
+        T* Lz = Ly + ntypes * NE * kMomK3;
+        radial_tables<T, NE>(plan->ang[ti][0], su, 0, P, Rf, Rd, lane);
+        __syncthreads();
+        build_moments<T, NE>(su, Rf, P, maxp, ntypes, sg.start, mono, Mom, D, lane);
+        force_angular_moments<T, NE>(plan, ti, ntypes, P, maxp, su, Rf, Rd, W, sg.start, mono, Mom, wtab, Lam, Lamp,
+                                     Lx, Ly, Lz, Xs, lane);
+        __syncthreads();
+        return;
+    }
     for (int bb = 0; bb < ntypes; ++bb) {
         for (int a = 0; a <= bb; ++a) {
             const DevAngular& A = plan->ang[ti][bb * (bb + 1) / 2 + a];
@@ -466,6 +498,7 @@ __device__ __forceinline__ void force_body(const DevPlan* __restrict__ plan, con
 }
 
 // LDS sizes (bytes) of the kernels for a given maxp.
+// Each kernel's LDS: its own arrays, then (moment path) the moment region of mom_lds_elems.
 template <typename T, int NE, int NC>
 __host__ __device__ constexpr size_t symm_lds_bytes(int maxp, int ldg) {
     return sizeof(T) * (size_t)(4 * maxp + NE * maxp + kWave * NE * NC) + sizeof(T) * (size_t)((ldg + 1) & ~1);
@@ -475,18 +508,24 @@ __host__ __device__ constexpr size_t force_lds_bytes(int maxp, int ldg) {
     return sizeof(T) * (size_t)(4 * maxp + 2 * NE * maxp + 4 * maxp + 8 * kWave + ((ldg + 1) & ~1)) +
            sizeof(int) * (size_t)(maxp + 2 * kWave);
 }
+template <typename T, int NE>
+__host__ __device__ constexpr size_t mom_lds_bytes(int maxp, int ntypes) {
+    return sizeof(T) * (size_t)mom_lds_elems<NE>(maxp, ntypes);
+}
 
 // ------------------------------------------------------------------ descriptors
 template <typename T, int NE, int NC>
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <= 16 ? 3 : 1, 8))) void k_symm(
     const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta, const double* __restrict__ rfun_rs,
     const int* __restrict__ atom_type, const int* __restrict__ centres, const int* __restrict__ cnt,
-    const int* __restrict__ nbr_meta, const double4* __restrict__ nbr_r, int maxp, T* __restrict__ G, int ldg) {
+    const int* __restrict__ nbr_meta, const double4* __restrict__ nbr_r, int maxp, T* __restrict__ G, int ldg,
+    int use_mom) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     T* su = (T*)lds_raw;
     T* Rf = su + 4 * maxp;
     T* red = Rf + NE * maxp;
     T* Grow = red + kWave * NE * NC;
+    T* mom = use_mom ? Grow + ((ldg + 1) & ~1) : nullptr;
     const int i = centres[blockIdx.x];
     const int lane = threadIdx.x;
     const int ti = atom_type[i];
@@ -497,7 +536,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
     stage_entries<T>(i, P, maxp, ntypes, nbr_meta, nbr_r, su, nullptr, sg, lane);
     for (int k = lane; k < ldg; k += kWave) Grow[k] = T(0);
     __syncthreads();
-    symm_body<T, NE, NC>(plan, rfun_eta, rfun_rs, ti, ntypes, su, Rf, red, Grow, sg, lane);
+    symm_body<T, NE, NC>(plan, rfun_eta, rfun_rs, ti, ntypes, P, maxp, su, Rf, red, Grow, sg, mom, lane);
     __syncthreads();
     for (int k = lane; k < ldg; k += kWave) G[(int64_t)i * ldg + k] = Grow[k];
 }
@@ -508,7 +547,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
     const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta, const double* __restrict__ rfun_rs,
     const int* __restrict__ atom_type, const int* __restrict__ centres, const int* __restrict__ cnt,
     const int* __restrict__ nbr_meta, const double4* __restrict__ nbr_r, int maxp, const T* __restrict__ dEdG,
-    int ldg, T* __restrict__ Xout) {
+    int ldg, T* __restrict__ Xout, int use_mom) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     T* su = (T*)lds_raw;
     T* Rf = su + 4 * maxp;
@@ -518,6 +557,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
     T* W = slots + 8 * kWave;         // dE/dG row
     int* orig = (int*)(W + ((ldg + 1) & ~1));
     int* slot_row = orig + maxp;      // [2][kWave]
+    T* mom = use_mom ? (T*)(slot_row + ((2 * kWave + 3) & ~3)) : nullptr;
     const int i = centres[blockIdx.x];
     const int lane = threadIdx.x;
     const int ti = atom_type[i];
@@ -528,8 +568,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
     stage_entries<T>(i, P, maxp, ntypes, nbr_meta, nbr_r, su, orig, sg, lane);
     for (int k = lane; k < ldg; k += kWave) W[k] = dEdG[(int64_t)i * ldg + k];
     __syncthreads();
-    force_body<T, NE, NC>(plan, rfun_eta, rfun_rs, ti, ntypes, P, su, Rf, Rd, Xs, slots, slot_row, W,
-                          dEdG + (int64_t)i * ldg, sg, lane);
+    force_body<T, NE, NC>(plan, rfun_eta, rfun_rs, ti, ntypes, P, maxp, su, Rf, Rd, Xs, slots, slot_row, W,
+                          dEdG + (int64_t)i * ldg, sg, mom, lane);
     const int64_t row = (int64_t)i * maxp;
     for (int p = lane; p < P; p += kWave) {
         const int o = orig[p];
@@ -625,7 +665,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
     const T* __restrict__ Wt, const int* __restrict__ atom_type, const int* __restrict__ order, int ntot,
     int* __restrict__ work, const int* __restrict__ cnt, const int* __restrict__ nbr_meta,
     const double4* __restrict__ nbr_r, int maxp, int ldg, int maxw, int dersum, int vac, int do_force,
-    double* __restrict__ eatom, T* __restrict__ Xout) {
+    double* __restrict__ eatom, T* __restrict__ Xout, int use_mom) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     T* su = (T*)lds_raw;
     T* Rf = su + 4 * maxp;
@@ -640,6 +680,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
     T* der = act1 + maxw;
     int* orig = (int*)(der + dersum);
     int* slot_row = orig + maxp;
+    T* mom = use_mom ? (T*)(slot_row + ((2 * kWave + 3) & ~3)) : nullptr;
     __shared__ int seg_lds[kMaxSpecies + 1];
     __shared__ int next_atom;
     Segs sg{seg_lds};
@@ -657,13 +698,13 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
         stage_entries<T>(i, P, maxp, ntypes, nbr_meta, nbr_r, su, orig, sg, lane);
         for (int k = lane; k < ldg; k += kWave) Grow[k] = T(0);
         __syncthreads();
-        symm_body<T, NE, NC>(plan, rfun_eta, rfun_rs, ti, ntypes, su, Rf, red, Grow, sg, lane);
+        symm_body<T, NE, NC>(plan, rfun_eta, rfun_rs, ti, ntypes, P, maxp, su, Rf, red, Grow, sg, mom, lane);
         __syncthreads();
         const double E = net_body<T>(plan->net[ti], Wt, Grow, act0, act1, der, dErow, maxw, vac, lane);
         if (lane == 0) eatom[i] = E;
         if (do_force) {
-            force_body<T, NE, NC>(plan, rfun_eta, rfun_rs, ti, ntypes, P, su, Rf, Rd, Xs, slots, slot_row, dErow,
-                                  dErow, sg, lane);
+            force_body<T, NE, NC>(plan, rfun_eta, rfun_rs, ti, ntypes, P, maxp, su, Rf, Rd, Xs, slots, slot_row,
+                                  dErow, dErow, sg, mom, lane);
             const int64_t row = (int64_t)i * maxp;
             for (int p = lane; p < P; p += kWave) {
                 const int o = orig[p];
@@ -681,12 +722,20 @@ template <typename T, int NE, int NC>
 static hipError_t launch_fused_t(int nblocks, const DevPlan* plan, const double* rfe, const double* rfr, const T* Wt,
                                  const int* atom_type, const int* order, int ntot, int* work, const int* cnt,
                                  const int* meta, const double4* r, int maxp, int ldg, int maxw, int dersum, int vac,
-                                 int do_force, double* eatom, T* X, hipStream_t st) {
-    const size_t lds = fused_lds_bytes<T, NE, NC>(maxp, ldg, maxw, dersum);
+                                 int do_force, double* eatom, T* X, int use_mom, int ntypes, hipStream_t st) {
+    const size_t lds = fused_lds_bytes<T, NE, NC>(maxp, ldg, maxw, dersum) + 16 +
+                       (use_mom ? mom_lds_bytes<T, NE>(maxp, ntypes) : 0);
     if (lds > 65536) return hipErrorInvalidConfiguration;
     hipLaunchKernelGGL((k_fused<T, NE, NC>), dim3(nblocks), dim3(kWave), lds, st, plan, rfe, rfr, Wt, atom_type, order,
-                       ntot, work, cnt, meta, r, maxp, ldg, maxw, dersum, vac, do_force, eatom, X);
+                       ntot, work, cnt, meta, r, maxp, ldg, maxw, dersum, vac, do_force, eatom, X, use_mom);
     return hipGetLastError();
+}
+
+size_t mom_lds(int shape, bool fp64, int maxp, int ntypes) {
+    const int ne = shape == 0 ? 2 : (shape == 1 ? 4 : 8);
+    return (fp64 ? 8 : 4) * (size_t)(ne == 2 ? mom_lds_elems<2>(maxp, ntypes)
+                                             : (ne == 4 ? mom_lds_elems<4>(maxp, ntypes) : mom_lds_elems<8>(maxp, ntypes))) +
+           16;
 }
 
 size_t fused_lds(int shape, bool fp64, int maxp, int ldg, int maxw, int dersum) {
@@ -704,23 +753,23 @@ template <typename T>
 hipError_t launch_fused(int shape, int nblocks, const DevPlan* plan, const double* rfe, const double* rfr, const T* Wt,
                         const int* atom_type, const int* order, int ntot, int* work, const int* cnt, const int* meta,
                         const double4* r, int maxp, int ldg, int maxw, int dersum, int vac, int do_force,
-                        double* eatom, T* X, hipStream_t st) {
+                        double* eatom, T* X, int use_mom, int ntypes, hipStream_t st) {
     if (shape == 0)
         return launch_fused_t<T, 2, 4>(nblocks, plan, rfe, rfr, Wt, atom_type, order, ntot, work, cnt, meta, r, maxp,
-                                       ldg, maxw, dersum, vac, do_force, eatom, X, st);
+                                       ldg, maxw, dersum, vac, do_force, eatom, X, use_mom, ntypes, st);
     if (shape == 1)
         return launch_fused_t<T, 4, 4>(nblocks, plan, rfe, rfr, Wt, atom_type, order, ntot, work, cnt, meta, r, maxp,
-                                       ldg, maxw, dersum, vac, do_force, eatom, X, st);
+                                       ldg, maxw, dersum, vac, do_force, eatom, X, use_mom, ntypes, st);
     return launch_fused_t<T, 8, 8>(nblocks, plan, rfe, rfr, Wt, atom_type, order, ntot, work, cnt, meta, r, maxp, ldg,
-                                   maxw, dersum, vac, do_force, eatom, X, st);
+                                   maxw, dersum, vac, do_force, eatom, X, use_mom, ntypes, st);
 }
 
 template hipError_t launch_fused<double>(int, int, const DevPlan*, const double*, const double*, const double*,
                                          const int*, const int*, int, int*, const int*, const int*, const double4*, int,
-                                         int, int, int, int, int, double*, double*, hipStream_t);
+                                         int, int, int, int, int, double*, double*, int, int, hipStream_t);
 template hipError_t launch_fused<float>(int, int, const DevPlan*, const double*, const double*, const float*,
                                         const int*, const int*, int, int*, const int*, const int*, const double4*, int,
-                                        int, int, int, int, int, double*, float*, hipStream_t);
+                                        int, int, int, int, int, double*, float*, int, int, hipStream_t);
 
 // F_a = sum_s X[a,s] - sum_{entries (j -> a) that carry a reaction} X[j -> a], in FP64.
 // One wavefront per atom, lanes over its entries.  The mirrored entry (j -> a, image
@@ -788,20 +837,23 @@ __global__ __launch_bounds__(kWave) void k_assemble(const DevStruc* __restrict__
 template <typename T, int NE, int NC>
 static hipError_t launch_symm_t(const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
                                 const int* centres, int ncentres, const int* cnt, const int* meta, const double4* r,
-                                int maxp, T* G, int ldg, hipStream_t st) {
-    const size_t lds = symm_lds_bytes<T, NE, NC>(maxp, ldg);
+                                int maxp, T* G, int ldg, int use_mom, int ntypes, hipStream_t st) {
+    const size_t lds = symm_lds_bytes<T, NE, NC>(maxp, ldg) + (use_mom ? mom_lds_bytes<T, NE>(maxp, ntypes) : 0);
+    if (lds > 65536) return hipErrorInvalidConfiguration;
     hipLaunchKernelGGL((k_symm<T, NE, NC>), dim3(ncentres), dim3(kWave), lds, st, plan, rfe, rfr, atom_type, centres, cnt,
-                       meta, r, maxp, G, ldg);
+                       meta, r, maxp, G, ldg, use_mom);
     return hipGetLastError();
 }
 
 template <typename T, int NE, int NC>
 static hipError_t launch_force_t(const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
                                  const int* centres, int ncentres, const int* cnt, const int* meta, const double4* r,
-                                 int maxp, const T* dEdG, int ldg, T* X, hipStream_t st) {
-    const size_t lds = force_lds_bytes<T, NE, NC>(maxp, ldg);
+                                 int maxp, const T* dEdG, int ldg, T* X, int use_mom, int ntypes, hipStream_t st) {
+    const size_t lds = force_lds_bytes<T, NE, NC>(maxp, ldg) + 16 +
+                       (use_mom ? mom_lds_bytes<T, NE>(maxp, ntypes) : 0);
+    if (lds > 65536) return hipErrorInvalidConfiguration;
     hipLaunchKernelGGL((k_force<T, NE, NC>), dim3(ncentres), dim3(kWave), lds, st, plan, rfe, rfr, atom_type, centres,
-                       cnt, meta, r, maxp, dEdG, ldg, X);
+                       cnt, meta, r, maxp, dEdG, ldg, X, use_mom);
     return hipGetLastError();
 }
 
@@ -837,19 +889,29 @@ size_t force_lds(int shape, bool fp64, int maxp, int ldg) {
 template <typename T>
 hipError_t launch_symm(int shape, const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
                        const int* centres, int ncentres, const int* cnt, const int* meta, const double4* r, int maxp,
-                       T* G, int ldg, hipStream_t st) {
-    if (shape == 0) return launch_symm_t<T, 2, 4>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, G, ldg, st);
-    if (shape == 1) return launch_symm_t<T, 4, 4>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, G, ldg, st);
-    return launch_symm_t<T, 8, 8>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, G, ldg, st);
+                       T* G, int ldg, int use_mom, int ntypes, hipStream_t st) {
+    if (shape == 0)
+        return launch_symm_t<T, 2, 4>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, G, ldg, use_mom,
+                                      ntypes, st);
+    if (shape == 1)
+        return launch_symm_t<T, 4, 4>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, G, ldg, use_mom,
+                                      ntypes, st);
+    return launch_symm_t<T, 8, 8>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, G, ldg, use_mom,
+                                  ntypes, st);
 }
 
 template <typename T>
 hipError_t launch_force(int shape, const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
                         const int* centres, int ncentres, const int* cnt, const int* meta, const double4* r, int maxp,
-                        const T* dEdG, int ldg, T* X, hipStream_t st) {
-    if (shape == 0) return launch_force_t<T, 2, 4>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, dEdG, ldg, X, st);
-    if (shape == 1) return launch_force_t<T, 4, 4>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, dEdG, ldg, X, st);
-    return launch_force_t<T, 8, 8>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, dEdG, ldg, X, st);
+                        const T* dEdG, int ldg, T* X, int use_mom, int ntypes, hipStream_t st) {
+    if (shape == 0)
+        return launch_force_t<T, 2, 4>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, dEdG, ldg, X,
+                                       use_mom, ntypes, st);
+    if (shape == 1)
+        return launch_force_t<T, 4, 4>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, dEdG, ldg, X,
+                                       use_mom, ntypes, st);
+    return launch_force_t<T, 8, 8>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, dEdG, ldg, X,
+                                   use_mom, ntypes, st);
 }
 
 template <typename T>
@@ -861,15 +923,16 @@ hipError_t launch_assemble(const DevStruc* S, const int* atom_struc, const int* 
 }
 
 template hipError_t launch_symm<double>(int, const DevPlan*, const double*, const double*, const int*, const int*, int,
-                                        const int*, const int*, const double4*, int, double*, int, hipStream_t);
-template hipError_t launch_symm<float>(int, const DevPlan*, const double*, const double*, const int*, const int*, int,
-                                       const int*, const int*, const double4*, int, float*, int, hipStream_t);
-template hipError_t launch_force<double>(int, const DevPlan*, const double*, const double*, const int*, const int*, int,
-                                         const int*, const int*, const double4*, int, const double*, int, double*,
-                                         hipStream_t);
-template hipError_t launch_force<float>(int, const DevPlan*, const double*, const double*, const int*, const int*, int,
-                                        const int*, const int*, const double4*, int, const float*, int, float*,
+                                        const int*, const int*, const double4*, int, double*, int, int, int,
                                         hipStream_t);
+template hipError_t launch_symm<float>(int, const DevPlan*, const double*, const double*, const int*, const int*, int,
+                                       const int*, const int*, const double4*, int, float*, int, int, int, hipStream_t);
+template hipError_t launch_force<double>(int, const DevPlan*, const double*, const double*, const int*, const int*, int,
+                                         const int*, const int*, const double4*, int, const double*, int, double*, int,
+                                         int, hipStream_t);
+template hipError_t launch_force<float>(int, const DevPlan*, const double*, const double*, const int*, const int*, int,
+                                        const int*, const int*, const double4*, int, const float*, int, float*, int,
+                                        int, hipStream_t);
 template hipError_t launch_assemble<double>(const DevStruc*, const int*, const int*, const int*, const int*,
                                             const double*, int, int, int, double*, hipStream_t);
 template hipError_t launch_assemble<float>(const DevStruc*, const int*, const int*, const int*, const int*, const float*,

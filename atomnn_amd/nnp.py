@@ -191,10 +191,10 @@ class NNP:
         _check(lib().nnp_set_option(self._h, key.encode(), int(value)))
 
     def info(self) -> dict:
-        out = np.zeros(11, dtype=np.int64)
-        _check(lib().nnp_info(self._h, out.ctypes.data, 11))
+        out = np.zeros(12, dtype=np.int64)
+        _check(lib().nnp_info(self._h, out.ctypes.data, 12))
         keys = ["maxp", "max_entries", "atoms", "structures", "shape", "ne", "nc", "ldg", "pipeline",
-                "fused_blocks_per_cu", "compute_units"]
+                "fused_blocks_per_cu", "compute_units", "moments"]
         return dict(zip(keys, (int(v) for v in out)))
 
     def resident(self, strucs: Sequence[Structure], flags: int) -> "Resident":

@@ -23,12 +23,13 @@ def rel_f(F, Fo):
     return np.linalg.norm(F - Fo, axis=1).max() / np.linalg.norm(Fo, axis=1).max()
 
 
-@pytest.fixture(scope="module")
-def fused_models(potentials):
+@pytest.fixture(scope="module", params=[0, 1], ids=["triplets", "moments"])
+def fused_models(potentials, request):
     out = {}
     for k in ("plain64", "plain4096", "mixed64x4096"):
         m = NNP.read(potentials[k])
         m.set_option("pipeline", 1)
+        m.set_option("moments", request.param)
         out[k] = (m, ob.OracleModel(potentials[k]))
     yield out
     for m, _ in out.values():

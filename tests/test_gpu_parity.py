@@ -26,10 +26,13 @@ def rel_f(F, Fo):
     return np.linalg.norm(F - Fo, axis=1).max() / np.linalg.norm(Fo, axis=1).max()
 
 
-@pytest.fixture(scope="module")
-def models(potentials):
+@pytest.fixture(scope="module", params=[0, 1], ids=["triplets", "moments"])
+def models(potentials, request):
+    """Every test runs with the G4 sums by triplet loops (0) and by the moment expansion (1)."""
     out = {k: (NNP.read(v), ob.OracleModel(v)) for k, v in potentials.items() if k in ("plain64", "plain4096",
                                                                                       "mixed64x4096")}
+    for m, _ in out.values():
+        m.set_option("moments", request.param)
     yield out
     for m, _ in out.values():
         m.close()
@@ -155,3 +158,24 @@ def test_mixed_precision_accuracy(models):
     print(f"mixed: dE/atom = {e_err * 1e3:.4f} meV, F rms = {f_rms * 1e3:.4f} meV/A")
     assert e_err <= 1e-3
     assert f_rms <= 20e-3
+
+
+def test_moment_path_engagement(models, potentials, request):
+    """Qualifying models use the moment expansion exactly when it is on; a model with
+    non-integer zeta never does and still matches the oracle."""
+    mode = request.node.callspec.params["models"]
+    m, _ = models["plain64"]
+    m.compute(synth.plain64()[0])
+    assert m.info()["moments"] == mode
+    frac = NNP.read(potentials["frac_zeta"])
+    try:
+        frac.set_option("moments", mode)
+        o = ob.OracleModel(potentials["frac_zeta"])
+        s = small_batch(2)[1]
+        E, Ea, F = frac.compute(s)
+        assert frac.info()["moments"] == 0
+        Eo, Eao, Fo = o.compute(s.types, s.pos, s.cell)
+        assert rel_e(E, Eo, Eao) <= TOL
+        assert rel_f(F, Fo) <= TOL
+    finally:
+        frac.close()
