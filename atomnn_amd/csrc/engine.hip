@@ -29,15 +29,15 @@ namespace atomnn {
 
 // launchers (symm.hip, net.hip, neigh.hip kernels)
 int grid_shape(int ne, int nc);
-size_t symm_lds(int shape, bool fp64, int maxp, int ldg);
-size_t force_lds(int shape, bool fp64, int maxp, int ldg);
+size_t symm_lds(int shape, bool fp64, bool ns, int maxp, int ldg);
+size_t force_lds(int shape, bool fp64, bool ns, int maxp, int ldg);
 size_t net_lds_bytes(const DevNet& n, bool fp64);
 template <typename T>
-hipError_t launch_symm(int shape, const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
+hipError_t launch_symm(int shape, int ns, const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
                        const int* centres, int ncentres, const int* cnt, const int* meta, const double4* r, int maxp,
                        T* G, int ldg, int use_mom, int ntypes, hipStream_t st);
 template <typename T>
-hipError_t launch_force(int shape, const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
+hipError_t launch_force(int shape, int ns, const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
                         const int* centres, int ncentres, const int* cnt, const int* meta, const double4* r, int maxp,
                         const T* dEdG, int ldg, T* X, int use_mom, int ntypes, hipStream_t st);
 template <typename T>
@@ -46,10 +46,10 @@ hipError_t launch_assemble(const DevStruc* S, const int* atom_struc, const int* 
 hipError_t launch_net(bool fp64, const DevPlan* plan, const DevPlan& hplan, const void* W, const int* sp_off,
                       const int* spatoms, const void* G, int ldg, int vac, double* eatom, void* dEdG, hipStream_t st);
 hipError_t launch_neighbors(const NeighborArgs& a, hipStream_t st);
-size_t fused_lds(int shape, bool fp64, int maxp, int ldg, int maxw, int dersum);
+size_t fused_lds(int shape, bool fp64, bool ns, int maxp, int ldg, int maxw, int dersum);
 size_t mom_lds(int shape, bool fp64, int maxp, int ntypes);
 template <typename T>
-hipError_t launch_fused(int shape, int nblocks, const DevPlan* plan, const double* rfe, const double* rfr, const T* Wt,
+hipError_t launch_fused(int shape, int ns, int nblocks, const DevPlan* plan, const double* rfe, const double* rfr, const T* Wt,
                         const int* atom_type, const int* order, int ntot, int* work, const int* cnt, const int* meta,
                         const double4* r, int maxp, int ldg, int maxw, int dersum, int vac, int do_force,
                         double* eatom, T* X, int use_mom, int ntypes, hipStream_t st);
@@ -403,22 +403,24 @@ void Engine::build_neighbors(bool all_images) {
     throw std::runtime_error("neighbour list did not converge");
 }
 
-// Whether a kernel whose own LDS is base_lds bytes runs the moment expansion for qualifying
-// species: the option is on, some species qualifies and the moment tables fit in 64 KiB.
-int Engine::mom_flag(size_t base_lds, bool fp64) {
+// The moment bits (of `bits`: 1 descriptors, 2 forces) a kernel whose own LDS is base_lds bytes
+// runs with: the option's bits, when some species qualifies and the moment tables fit in 64 KiB.
+int Engine::mom_flag(size_t base_lds, bool fp64, int bits) {
     int any = 0;
     for (int t = 0; t < pb.plan.T; ++t) any |= pb.plan.mom_ok[t];
-    last_mom = (moments && any && base_lds + mom_lds(shape, fp64, maxp, pb.plan.T) <= 65536) ? 1 : 0;
-    return last_mom;
+    const int use = moments & bits;
+    const int got = (use && any && base_lds + mom_lds(shape, fp64, maxp, pb.plan.T) <= 65536) ? use : 0;
+    last_mom |= got;
+    return got;
 }
 
 template <typename T>
 void Engine::descriptors() {
-    const size_t lds = symm_lds(shape, sizeof(T) == 8, maxp, pb.plan.ldg);
+    const size_t lds = symm_lds(shape, sizeof(T) == 8, pb.plan.generic, maxp, pb.plan.ldg);
     if (lds > 65536) throw unsupported_error("too many neighbours per atom for the descriptor kernel's LDS");
-    const int mom = mom_flag(lds, sizeof(T) == 8);
+    const int mom = mom_flag(lds, sizeof(T) == 8, 1) ? 1 : 0;
     d_G.reserve((size_t)ntot * pb.plan.ldg * sizeof(T));
-    HIPCHK(launch_symm<T>(shape, (const DevPlan*)d_plan.p, (const double*)d_rfe.p, (const double*)d_rfr.p,
+    HIPCHK(launch_symm<T>(shape, pb.plan.generic, (const DevPlan*)d_plan.p, (const double*)d_rfe.p, (const double*)d_rfr.p,
                           (const int*)d_type.p, (const int*)d_centres.p, ntot, (const int*)d_cnt.p,
                           (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp, (T*)d_G.p, pb.plan.ldg, mom,
                           pb.plan.T, stream));
@@ -433,12 +435,12 @@ void Engine::networks(bool mixed, bool vac) {
 
 template <typename T>
 void Engine::forces() {
-    const size_t lds = force_lds(shape, sizeof(T) == 8, maxp, pb.plan.ldg);
+    const size_t lds = force_lds(shape, sizeof(T) == 8, pb.plan.generic, maxp, pb.plan.ldg);
     if (lds > 65536) throw unsupported_error("too many neighbours per atom for the force kernel's LDS");
-    const int mom = mom_flag(lds, sizeof(T) == 8);
+    const int mom = mom_flag(lds, sizeof(T) == 8, 2) ? 1 : 0;
     d_X.reserve((size_t)ntot * maxp * 4 * sizeof(T));
     d_F.reserve((size_t)ntot * 3 * 8);
-    HIPCHK(launch_force<T>(shape, (const DevPlan*)d_plan.p, (const double*)d_rfe.p, (const double*)d_rfr.p,
+    HIPCHK(launch_force<T>(shape, pb.plan.generic, (const DevPlan*)d_plan.p, (const double*)d_rfe.p, (const double*)d_rfr.p,
                            (const int*)d_type.p, (const int*)d_centres.p, ntot, (const int*)d_cnt.p,
                            (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp, (const T*)d_dEdG.p, pb.plan.ldg,
                            (T*)d_X.p, mom, pb.plan.T, stream));
@@ -449,6 +451,7 @@ void Engine::forces() {
 
 void Engine::run(const std::vector<nnp_structure>& ss, uint32_t flags) {
     const bool mixed = flags & NNP_PREC_MIXED;
+    last_mom = 0;
     upload(ss);
     build_neighbors(flags & NNP_ALL_IMAGES);
     if (pipeline == 1) {
@@ -495,21 +498,21 @@ void Engine::net_dims(int& maxw, int& dersum) const {
 void Engine::fused(bool mixed, uint32_t flags, int nblocks) {
     int maxw, dersum;
     net_dims(maxw, dersum);
-    const size_t lds = fused_lds(shape, !mixed, maxp, pb.plan.ldg, maxw, dersum);
+    const size_t lds = fused_lds(shape, !mixed, pb.plan.generic, maxp, pb.plan.ldg, maxw, dersum);
     if (lds > 65536) throw unsupported_error("too many neighbours per atom for the fused kernel's LDS");
-    const int mom = mom_flag(lds, !mixed);
+    const int mom = mom_flag(lds, !mixed, (flags & NNP_FORCE) ? 3 : 1);
     d_eatom.reserve((size_t)ntot * 8);
     d_X.reserve((size_t)ntot * maxp * 4 * (mixed ? 4 : 8));
     d_F.reserve((size_t)ntot * 3 * 8);
     const int vac = (flags & NNP_VACUUM_ENERGY) ? 1 : 0, frc = (flags & NNP_FORCE) ? 1 : 0;
     if (mixed)
-        HIPCHK(launch_fused<float>(shape, nblocks, (const DevPlan*)d_plan.p, (const double*)d_rfe.p,
+        HIPCHK(launch_fused<float>(shape, pb.plan.generic, nblocks, (const DevPlan*)d_plan.p, (const double*)d_rfe.p,
                                    (const double*)d_rfr.p, (const float*)d_w32.p, (const int*)d_type.p,
                                    (const int*)d_spatoms.p, ntot, (int*)d_work.p, (const int*)d_cnt.p,
                                    (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp, pb.plan.ldg, maxw, dersum,
                                    vac, frc, (double*)d_eatom.p, (float*)d_X.p, mom, pb.plan.T, stream));
     else
-        HIPCHK(launch_fused<double>(shape, nblocks, (const DevPlan*)d_plan.p, (const double*)d_rfe.p,
+        HIPCHK(launch_fused<double>(shape, pb.plan.generic, nblocks, (const DevPlan*)d_plan.p, (const double*)d_rfe.p,
                                     (const double*)d_rfr.p, (const double*)d_w64.p, (const int*)d_type.p,
                                     (const int*)d_spatoms.p, ntot, (int*)d_work.p, (const int*)d_cnt.p,
                                     (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp, pb.plan.ldg, maxw, dersum,
@@ -523,9 +526,19 @@ int Engine::fused_blocks(bool mixed, uint32_t flags) {
     if (fused_blocks_per_cu > 0) return std::min(ntot, num_cus * fused_blocks_per_cu);
     int lg = 0;
     while ((1 << (lg + 1)) <= ntot) ++lg;
-    const TuneKey key{shape, mixed ? 0 : 1, maxp, lg};
+    const TuneKey key{shape, mixed ? 0 : 1, lg};
     for (const auto& kv : tune_cache)
         if (kv.first == key) return std::min(ntot, num_cus * kv.second);
+    // small batches, and serving lanes: the nearest tuned batch size of this shape, else 4
+    if (!may_tune || ntot < 4 * num_cus) {
+        int best = 4, dist = 1 << 30;
+        for (const auto& kv : tune_cache)
+            if (kv.first.shape == key.shape && kv.first.fp64 == key.fp64 && std::abs(kv.first.ntot_log2 - lg) < dist) {
+                dist = std::abs(kv.first.ntot_log2 - lg);
+                best = kv.second;
+            }
+        return std::min(ntot, num_cus * best);
+    }
     const int cand[] = {1, 2, 3, 4, 6, 8, 12, 16};
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
@@ -654,6 +667,7 @@ void Engine::resident_run(uint32_t flags) {
 
 // Enqueue one full pass on the engine's stream (no host sync).
 void Engine::enqueue_pass(uint32_t flags) {
+    last_mom = 0;
     const int fused_nb = pipeline == 1 ? fused_blocks(flags & NNP_PREC_MIXED, flags) : 0;
     const bool mixed = flags & NNP_PREC_MIXED;
     NeighborArgs a;
@@ -715,18 +729,18 @@ void Engine::enqueue_pass(uint32_t flags) {
     stage_end(ST_NET);
     if (flags & NNP_FORCE) {
         stage_begin(ST_FORCE);
-        const size_t lds = force_lds(shape, !mixed, maxp, pb.plan.ldg);
+        const size_t lds = force_lds(shape, !mixed, pb.plan.generic, maxp, pb.plan.ldg);
         if (lds > 65536) throw unsupported_error("too many neighbours per atom for the force kernel's LDS");
-        const int mom = mom_flag(lds, !mixed);
+        const int mom = mom_flag(lds, !mixed, 2) ? 1 : 0;
         d_X.reserve((size_t)ntot * maxp * 4 * (mixed ? 4 : 8));
         d_F.reserve((size_t)ntot * 3 * 8);
         if (mixed)
-            HIPCHK(launch_force<float>(shape, (const DevPlan*)d_plan.p, (const double*)d_rfe.p, (const double*)d_rfr.p,
+            HIPCHK(launch_force<float>(shape, pb.plan.generic, (const DevPlan*)d_plan.p, (const double*)d_rfe.p, (const double*)d_rfr.p,
                                        (const int*)d_type.p, (const int*)d_centres.p, ntot, (const int*)d_cnt.p,
                                        (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp,
                                        (const float*)d_dEdG.p, pb.plan.ldg, (float*)d_X.p, mom, pb.plan.T, stream));
         else
-            HIPCHK(launch_force<double>(shape, (const DevPlan*)d_plan.p, (const double*)d_rfe.p,
+            HIPCHK(launch_force<double>(shape, pb.plan.generic, (const DevPlan*)d_plan.p, (const double*)d_rfe.p,
                                         (const double*)d_rfr.p, (const int*)d_type.p, (const int*)d_centres.p, ntot,
                                         (const int*)d_cnt.p, (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp,
                                         (const double*)d_dEdG.p, pb.plan.ldg, (double*)d_X.p, mom, pb.plan.T,
@@ -1026,7 +1040,7 @@ int nnp_set_option(nnp_model* m, const char* key, int64_t value) {
             E.drop_graph();
             E.eager_runs = 0;
         } else if (k == "moments") {
-            if (value != 0 && value != 1) throw std::invalid_argument("nnp_set_option: moments is 0 or 1");
+            if (value < 0 || value > 3) throw std::invalid_argument("nnp_set_option: moments is 0..3");
             HIPCHK(hipStreamSynchronize(E.stream));
             E.moments = (int)value;
             E.tune_cache.clear();

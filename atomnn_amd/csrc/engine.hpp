@@ -92,19 +92,25 @@ class Engine {
     //           1 = one persistent fused descriptor + network + force kernel
     int pipeline = 0;
     int fused_blocks_per_cu = 0;  // 0: chosen by the autotuner
-    // moments: 1 = angular sums by the moment expansion where the model qualifies (integer
-    // zeta <= 4, |lambda| <= 1, one shared exponent table) and the LDS fits; 0 = triplet loops
+    // moments: bit 0 = descriptor angular sums, bit 1 = angular force terms by the moment
+    // expansion, where the model qualifies (integer zeta <= 4, |lambda| <= 1, one shared
+    // exponent table and separable distance factor) and the LDS fits; clear bits use the
+    // triplet loops.  Default: descriptors only (the triplet force loop is the faster one at
+    // 45-70 neighbours per atom).
     int moments = 1;
-    int last_mom = 0;  // whether the last pass used the moment expansion
-    int mom_flag(size_t base_lds, bool fp64);
+    int last_mom = 0;  // the moment bits the last pass used
+    int mom_flag(size_t base_lds, bool fp64, int bits);
     int num_cus = 0;
     DevMem d_work;
+    // Autotuned persistent grid of the fused kernel per (shape, precision, log2 batch atoms).
+    // Serving lanes read the cache but do not tune (concurrent lanes would time each other).
     struct TuneKey {
-        int shape, fp64, maxp, ntot_log2;
+        int shape, fp64, ntot_log2;
         bool operator==(const TuneKey& o) const {
-            return shape == o.shape && fp64 == o.fp64 && maxp == o.maxp && ntot_log2 == o.ntot_log2;
+            return shape == o.shape && fp64 == o.fp64 && ntot_log2 == o.ntot_log2;
         }
     };
+    bool may_tune = true;
     std::vector<std::pair<TuneKey, int>> tune_cache;  // key -> blocks per CU
     std::vector<std::pair<int, double>> last_tune;    // candidates and their times (ms) of the last tuning
     int fused_blocks(bool mixed, uint32_t flags);

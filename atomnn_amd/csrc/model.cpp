@@ -109,8 +109,12 @@ void read_radial(Lines& in, RadialSpec& r) {
     const std::string phi = in.tok(4);
     const int n = std::atoi(in.tok(5).c_str());
     r.cut = make_cutoff(in.tok(1), in.tok(2), std::atof(in.tok(3).c_str()));
-    if (phi != "GAUSSIAN")
-        throw unsupported_error("radial function " + phi + " is not implemented by this engine (GAUSSIAN only)");
+    static const char* names[] = {"GAUSSIAN", "TANH", "SOFTPLUS", "LOGCOSH", "SWISH", "MISH"};
+    r.phi = -1;
+    for (int k = 0; k < 6; ++k)
+        if (phi == names[k]) r.phi = k;
+    if (r.phi < 0)  // basis_radial.cpp:70
+        throw std::invalid_argument("BasisR(rc,cutoff::Name,cutoff::Norm,int,PhiRN): invalid radial function type");
     if (n < 0) throw std::invalid_argument("Basis::resize(int): invalid number of functions.");
     r.rs.resize(n);
     r.eta.resize(n);
@@ -127,8 +131,13 @@ void read_angular(Lines& in, AngularSpec& a) {
     const int n = std::atoi(in.tok(5).c_str());
     a.alpha = std::atoi(in.tok(6).c_str());
     a.cut = make_cutoff(in.tok(1), in.tok(2), std::atof(in.tok(3).c_str()));
-    if (phi != "GAUSS")
-        throw unsupported_error("angular function " + phi + " is not implemented by this engine (GAUSS only)");
+    if (phi == "GAUSS") a.phi = ANG_GAUSS;
+    else if (phi == "IPOWP") a.phi = ANG_IPOWP;
+    else if (phi == "IPOWS") a.phi = ANG_IPOWS;
+    else if (phi == "SECHP") a.phi = ANG_SECHP;
+    else if (phi == "SECHS") a.phi = ANG_SECHS;
+    else  // basis_angular.cpp:62
+        throw std::invalid_argument("BasisA(rc,cutoff::Name,cutoff::Norm,int,PhiAN): invalid angular function type");
     if (n < 0) throw std::invalid_argument("Basis::resize(int): invalid number of functions.");
     a.eta.resize(n);
     a.zeta.resize(n);
@@ -152,14 +161,12 @@ void read_net(Lines& in, NetSpec& net) {
         if (v <= 0) throw std::invalid_argument("ANN::resize(const ANNP&,int,const std::vector<int>&): Invalid layer size.");
     in.next("neuron");
     const std::string neuron = in.tok(1);
-    if (neuron == "TANH") net.tanh_hidden = true;
-    else if (neuron == "LINEAR") net.tanh_hidden = false;
-    else if (neuron == "SIGMOID" || neuron == "ISRU" || neuron == "ARCTAN" || neuron == "SOFTSIGN" ||
-             neuron == "SOFTPLUS" || neuron == "RELU" || neuron == "ELU" || neuron == "GELU" || neuron == "SWISH" ||
-             neuron == "MISH" || neuron == "TANHRE" || neuron == "LOGCOSH" || neuron == "IQUAD" || neuron == "IFABS" ||
-             neuron == "SINSIG")
-        throw unsupported_error("neuron " + neuron + " is not implemented by this engine (TANH, LINEAR)");
-    else throw std::invalid_argument("ANN::read(FILE*,ANN&): Invalid neuron.");
+    static const char* names[] = {"LINEAR", "SIGMOID", "TANH", "ISRU", "ARCTAN", "SOFTSIGN", "RELU", "SOFTPLUS", "ELU",
+                                  "GELU", "SWISH", "MISH", "TANHRE", "LOGCOSH", "IQUAD", "IFABS", "SINSIG"};
+    net.neuron = 0;
+    for (int k = 0; k < 17; ++k)
+        if (neuron == names[k]) net.neuron = k + 1;  // nn.cpp:115-134
+    if (net.neuron == 0) throw std::invalid_argument("ANN::read(FILE*,ANN&): Invalid neuron.");  // nn.cpp:1376
     net.nin = nodes[0];
     net.nodes.assign(nodes.begin() + 1, nodes.end());
     if (net.nodes.back() != 1) throw unsupported_error("a potential network must have exactly one output");
@@ -286,6 +293,8 @@ PlanBuffers build_plan(const HostModel& m) {
             d.goff = s.offR[j];
             d.fbase = (int)pb.rfun_eta.size();
             d.cut = r.cut.kind;
+            d.phi = r.phi;
+            if (r.phi != RAD_GAUSSIAN) P.generic = 1;
             d.rc = r.cut.rc;
             d.rci = r.cut.rci;
             d.pirci = r.cut.pirci;
@@ -301,6 +310,8 @@ PlanBuffers build_plan(const HostModel& m) {
             d.goff = s.nR + s.offA[b];
             d.alpha = a.alpha;
             d.cut = a.cut.kind;
+            d.phi = a.phi;
+            if (a.phi != ANG_GAUSS) P.generic = 1;
             d.rc = a.cut.rc;
             d.rci = a.cut.rci;
             d.pirci = a.cut.pirci;
@@ -347,7 +358,9 @@ PlanBuffers build_plan(const HostModel& m) {
             const DevAngular& A0 = P.ang[i][0];
             for (int b = 0; b < nblk && ok; ++b) {
                 const DevAngular& A = P.ang[i][b];
-                if (A.ne != A0.ne || A.alpha != A0.alpha || A.cut != A0.cut || A.rc != A0.rc) ok = false;
+                if (A.ne != A0.ne || A.alpha != A0.alpha || A.cut != A0.cut || A.rc != A0.rc || A.phi != A0.phi ||
+                    !angular_separable(A.phi))
+                    ok = false;
                 for (int e = 0; e < A.ne && ok; ++e)
                     if (A.ietap[e] != A0.ietap[e]) ok = false;
                 for (int c = 0; c < A.nc && ok; ++c)
@@ -361,7 +374,8 @@ PlanBuffers build_plan(const HostModel& m) {
         dn.L = (int)net.nodes.size();
         dn.nin = net.nin;
         dn.nin_pad = pad_to(net.nin, 16);
-        dn.tanh_hidden = net.tanh_hidden ? 1 : 0;
+        dn.act = net.neuron;
+        if (net.neuron != NEU_TANH && net.neuron != NEU_LINEAR) P.generic = 1;
         dn.outw = net.outw[0];
         dn.outb = net.outb[0];
         dn.evac = s.evac;

@@ -16,6 +16,18 @@ namespace atomnn {
 
 enum CutKind : int { CUT_STEP = 1, CUT_COS = 2, CUT_TANH = 3 };
 
+// Radial functions phi(x), x = -eta (r - rs) (basis_radial.cpp:24-32, 302-359).
+enum RadialKind : int { RAD_GAUSSIAN = 0, RAD_TANH, RAD_SOFTPLUS, RAD_LOGCOSH, RAD_SWISH, RAD_MISH };
+// Angular distance factors of x = ietap r^alpha (basis_angular.cpp:24-30, 415-441).  The first
+// three factorise over the two legs of a triplet (separable); IPOWS and SECHS do not.
+enum AngularKind : int { ANG_GAUSS = 0, ANG_IPOWP, ANG_SECHP, ANG_IPOWS, ANG_SECHS };
+inline bool angular_separable(int k) { return k <= ANG_SECHP; }
+// Activations, numbered as the reference's Neuron::Type (ml/nn.hpp:84-103).
+enum NeuronKind : int {
+    NEU_LINEAR = 1, NEU_SIGMOID, NEU_TANH, NEU_ISRU, NEU_ARCTAN, NEU_SOFTSIGN, NEU_RELU, NEU_SOFTPLUS, NEU_ELU,
+    NEU_GELU, NEU_SWISH, NEU_MISH, NEU_TANHRE, NEU_LOGCOSH, NEU_IQUAD, NEU_IFABS, NEU_SINSIG
+};
+
 // Limits of the device plan.  The parser rejects potentials beyond them with NNP_EUNSUPPORTED.
 constexpr int kMaxSpecies = 8;
 constexpr int kMaxE = 8;       // distinct radial exponents per angular block
@@ -37,11 +49,13 @@ struct CutoffSpec {
 
 struct RadialSpec {
     CutoffSpec cut;
+    int phi = RAD_GAUSSIAN;
     std::vector<double> rs, eta;
 };
 
 struct AngularSpec {
     CutoffSpec cut;
+    int phi = ANG_GAUSS;
     int alpha = 0;
     std::vector<double> eta, ietap, zeta;
     std::vector<int> lambda;
@@ -50,7 +64,7 @@ struct AngularSpec {
 struct NetSpec {
     int nin = 0;
     std::vector<int> nodes;  // hidden..., output (=1)
-    bool tanh_hidden = true;
+    int neuron = NEU_TANH;  // hidden layers; the output layer is linear
     std::vector<double> inpw, inpb, outw, outb;
     std::vector<std::vector<double>> W, b;  // W[l]: nodes[l] x in(l), row-major
 };
@@ -85,6 +99,8 @@ struct DevRadial {
     int goff;    // descriptor offset
     int fbase;   // first function in the plan's radial function arrays
     int cut;
+    int phi;     // RadialKind
+    int pad_;
     double rc, rci, pirci, norm;
 };
 
@@ -93,7 +109,7 @@ struct DevAngular {
     int alpha;
     int ne, nc;  // distinct exponents / angular terms
     int cut;
-    int pad_;
+    int phi;     // AngularKind
     double rc, rci, pirci, norm;
     double ietap[kMaxE];
     double zeta[kMaxC];
@@ -111,7 +127,7 @@ struct DevNet {
     int64_t wtoff[kMaxLayers]; // the same weights transposed (in_pad x wpad)
     int64_t boff[kMaxLayers];
     int64_t inpw_off, inpb_off;
-    int tanh_hidden;
+    int act;                   // NeuronKind of the hidden layers
     int pad_;
     double outw, outb, evac;
 };
@@ -123,6 +139,9 @@ struct DevPlan {
     // moment path per centre species: every angular block has integer zeta in [0, 4], |lambda| <= 1,
     // and all blocks share one exponent table, alpha and cutoff (see symm.hip)
     int mom_ok[kMaxSpecies];
+    // the model uses a radial function other than GAUSSIAN, an angular factor other than GAUSS
+    // or an activation other than tanh / linear: the kernels' generic instantiations run it
+    int generic;
     int nG[kMaxSpecies];
     int nR[kMaxSpecies];
     DevRadial rad[kMaxSpecies][kMaxSpecies];

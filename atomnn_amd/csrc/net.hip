@@ -3,7 +3,8 @@
 //
 // Reference arithmetic (ml/nn.cpp):
 //   ins = inpw .* (G + inpb)                         fp/fpbp, nn.cpp:1194, 1218
-//   z_l = b_l + W_l a_{l-1};  a_l = tanh(z_l), d_l = 1 - a_l^2   (hidden, nn.cpp:209-216)
+//   z_l = b_l + W_l a_{l-1};  a_l = act(z_l), d_l = act'(z_l)   (hidden layers: the potential's
+//                                                    neuron, AFFPBP nn.cpp:144-766)
 //   last layer linear: a_L = z_L, d_L = 1            (nn.cpp:144-148, 1134-1137)
 //   out = outb + outw .* a_L                         (nn.cpp:1229-1230)
 //   dodz_L = outw .* d_L;  dodz_{l-1} = (dodz_l W_l) .* d_{l-1};  dE/dG = (dodz_0 W_0) .* inpw
@@ -14,11 +15,13 @@
 // output tiles, K in steps of 4; A lane l holds A[l%16][l/16], B lane l holds B[l/16][l%16],
 // and result register v of lane l is D[4v + l/16][l%16].  Mixed form: v_mfma_f32_16x16x32_bf16
 // with the activations and weights rounded to BF16 and FP32 accumulation; tanh and the
-// derivative products stay in FP32.  Weights are zero padded to multiples of 16, so padded units
-// carry exact zeros through both directions.
+// derivative products stay in FP32.  Weights are zero padded to multiples of 16: a padded unit
+// may carry act(0) != 0 (SIGMOID), but its outgoing weights are zero, so it adds nothing in
+// either direction.
 #include <algorithm>
 
 #include "device.hpp"
+#include "basis.hpp"
 #include "model.hpp"
 
 namespace atomnn {
@@ -112,7 +115,7 @@ __global__ __launch_bounds__(kWave) void k_net_f64(const DevPlan* __restrict__ p
         const int out_pad = N.wpad[l];
         const double* W = Wb + N.woff[l];
         const double* b = Wb + N.boff[l];
-        const bool linear = (l == N.L - 1) || !N.tanh_hidden;
+        const int act = (l == N.L - 1) ? NEU_LINEAR : N.act;
         for (int jt = 0; jt < out_pad / 16; ++jt) {
             const d4_t acc = tile_mm_f64<true>(cur, maxw, W, in_pad, in_pad, jt, li, lk);
             const int col = jt * 16 + li;
@@ -121,13 +124,7 @@ __global__ __launch_bounds__(kWave) void k_net_f64(const DevPlan* __restrict__ p
                 const int row = 4 * v + lk;
                 const double z = acc[v] + b[col];
                 double av, dv;
-                if (linear) {
-                    av = z;
-                    dv = 1.0;
-                } else {
-                    av = tanh(z);
-                    dv = 1.0 - av * av;
-                }
+                act_eval<double>(act, z, av, dv);
                 nxt[row * maxw + col] = av;
                 der[doff + row * out_pad + col] = dv;
             }
@@ -230,7 +227,7 @@ __global__ __launch_bounds__(kWave) void k_net_mixed(const DevPlan* __restrict__
         const int out_pad = N.wpad[l];
         const float* W = Wf + N.woff[l];
         const float* b = Wf + N.boff[l];
-        const bool linear = (l == N.L - 1) || !N.tanh_hidden;
+        const int act = (l == N.L - 1) ? NEU_LINEAR : N.act;
         for (int jt = 0; jt < out_pad / 16; ++jt) {
             f4_t acc = {0.f, 0.f, 0.f, 0.f};
             for (int k0 = 0; k0 < in_pad; k0 += 32) {
@@ -250,13 +247,7 @@ __global__ __launch_bounds__(kWave) void k_net_mixed(const DevPlan* __restrict__
                 const int row = 4 * lk + v;
                 const float z = acc[v] + b[col];
                 float av, dv;
-                if (linear) {
-                    av = z;
-                    dv = 1.f;
-                } else {
-                    av = tanhf(z);
-                    dv = 1.f - av * av;
-                }
+                act_eval<float>(act, z, av, dv);
                 nxt[row * maxw + col] = av;
                 der[doff + row * out_pad + col] = dv;
             }

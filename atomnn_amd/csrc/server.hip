@@ -55,7 +55,10 @@ class Server {
         if (cfg_.max_batch_atoms <= 0) cfg_.max_batch_atoms = 1 << 18;
         if (cfg_.max_batch_structures <= 0) cfg_.max_batch_structures = 4096;
         lane_budget_ = (size_t)(cfg_.hbm_budget / cfg_.lanes);
-        for (int k = 0; k < cfg_.lanes; ++k) lanes_.emplace_back(new Engine(owner_, k));
+        for (int k = 0; k < cfg_.lanes; ++k) {
+            lanes_.emplace_back(new Engine(owner_, k));
+            lanes_.back()->may_tune = false;
+        }
         for (int k = 0; k < cfg_.lanes; ++k) workers_.emplace_back(&Server::worker, this, k);
     }
 

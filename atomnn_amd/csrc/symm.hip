@@ -30,6 +30,7 @@
 // basis_angular.cpp:498-501 (no sign of cw, which is >= 0 up to rounding).
 #include "device.hpp"
 #include "model.hpp"
+#include "basis.hpp"
 #include "moments.hpp"
 
 namespace atomnn {
@@ -160,7 +161,7 @@ __device__ __forceinline__ void stage_entries(int i, int P, int maxp, int ntypes
 }
 
 // Radial exponent tables of one angular block for the slots [lo, hi): Rf (and Rd).
-template <typename T, int NE>
+template <typename T, int NE, bool GEN>
 __device__ __forceinline__ void radial_tables(const DevAngular& A, const T* su, int lo, int hi, T* Rf, T* Rd, int lane) {
     const T rc = T(A.rc), rci = T(A.rci), pirci = T(A.pirci);
     for (int p = lo + lane; p < hi; p += kWave) {
@@ -169,22 +170,57 @@ __device__ __forceinline__ void radial_tables(const DevAngular& A, const T* su, 
         cutoff_fg<T>(A.cut, d, rc, rci, pirci, fc, gc);
         const T dp = ipow(d, A.alpha);
         const T ddp = T(A.alpha) * dp / d;  // d(d^alpha)/dd
+        if (!GEN || A.phi == ANG_GAUSS) {
+#pragma unroll
+            for (int e = 0; e < NE; ++e) {
+                const T ie = T(A.ietap[e]);
+                const T ex = exp(-ie * dp);
+                Rf[p * NE + e] = fc * ex;
+                if (Rd) Rd[p * NE + e] = (gc - ie * ddp * fc) * ex;
+            }
+        } else {  // IPOWP, SECHP: leg factor h(ietap d^alpha)
+#pragma unroll
+            for (int e = 0; e < NE; ++e) {
+                const T ie = T(A.ietap[e]);
+                T h, dh;
+                ang_leg<T>(A.phi, ie * dp, h, dh);
+                Rf[p * NE + e] = fc * h;
+                if (Rd) Rd[p * NE + e] = gc * h + fc * dh * ie * ddp;
+            }
+        }
+    }
+}
+
+// Entry tables of a non-separable angular block (IPOWS, SECHS) for the slots [lo, hi):
+// Rf[p][e] = a(ietap_e d^alpha), Rd[p][e] = ietap_e d(d^alpha)/dd, nsx[2p] = fc, nsx[2p+1] = fc'.
+template <typename T, int NE>
+__device__ __forceinline__ void ns_tables(const DevAngular& A, const T* su, int lo, int hi, T* Rf, T* Rd, T* nsx,
+                                          int lane) {
+    const T rc = T(A.rc), rci = T(A.rci), pirci = T(A.pirci);
+    for (int p = lo + lane; p < hi; p += kWave) {
+        const T d = su[4 * p + 3];
+        T fc, gc;
+        cutoff_fg<T>(A.cut, d, rc, rci, pirci, fc, gc);
+        const T dp = ipow(d, A.alpha);
+        const T ddp = T(A.alpha) * dp / d;
 #pragma unroll
         for (int e = 0; e < NE; ++e) {
             const T ie = T(A.ietap[e]);
-            const T ex = exp(-ie * dp);
-            Rf[p * NE + e] = fc * ex;
-            if (Rd) Rd[p * NE + e] = (gc - ie * ddp * fc) * ex;
+            Rf[p * NE + e] = ang_entry<T>(A.phi, ie * dp);
+            if (Rd) Rd[p * NE + e] = ie * ddp;
         }
+        nsx[2 * p] = fc;
+        nsx[2 * p + 1] = gc;
     }
 }
 
 // ------------------------------------------------------------------ per-atom bodies
 // Descriptors of one staged centre into Grow (LDS, zeroed by the caller).
-template <typename T, int NE, int NC>
+template <typename T, int NE, int NC, bool GEN>
 __device__ __forceinline__ void symm_body(const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta,
                                           const double* __restrict__ rfun_rs, int ti, int ntypes, int P, int maxp,
-                                          const T* su, T* Rf, T* red, T* Grow, const Segs& sg, T* mom, int lane) {
+                                          const T* su, T* Rf, T* red, T* Grow, const Segs& sg, T* nsx, T* mom,
+                                          int lane) {
     // radial: cutoff once per neighbour, then (function, part) items over 64 lanes; the parts of
     // a function are summed in a fixed order
     {
@@ -213,9 +249,17 @@ __device__ __forceinline__ void symm_body(const DevPlan* __restrict__ plan, cons
                 while (fl >= plan->rad[ti][b].n) fl -= plan->rad[ti][b++].n;
                 const DevRadial& R = plan->rad[ti][b];
                 const T eta = T(rfun_eta[R.fbase + fl]), rs = T(rfun_rs[R.fbase + fl]);
-                for (int p = sg.start[b] + prt; p < sg.start[b + 1]; p += nparts) {
-                    const T x = su[4 * p + 3] - rs;
-                    acc += exp(-eta * x * x) * fcr[p];
+                if (!GEN || R.phi == RAD_GAUSSIAN) {
+                    for (int p = sg.start[b] + prt; p < sg.start[b + 1]; p += nparts) {
+                        const T x = su[4 * p + 3] - rs;
+                        acc += exp(-eta * x * x) * fcr[p];
+                    }
+                } else {
+                    for (int p = sg.start[b] + prt; p < sg.start[b + 1]; p += nparts) {
+                        T v, dv;
+                        rad_phi<T>(R.phi, -eta * (su[4 * p + 3] - rs), v, dv);
+                        acc += v * fcr[p];
+                    }
                 }
                 goff = R.goff + fl;
                 norm = T(R.norm);
@@ -237,7 +281,7 @@ __device__ __forceinline__ void symm_body(const DevPlan* __restrict__ plan, cons
         T* Mom = mono + kMomK * (maxp + 1);
         T* D = Mom + ntypes * NE * kMomK;
         T* S = D + ntypes * NE;
-        radial_tables<T, NE>(plan->ang[ti][0], su, 0, P, Rf, nullptr, lane);
+        radial_tables<T, NE, GEN>(plan->ang[ti][0], su, 0, P, Rf, nullptr, lane);
         __syncthreads();
         build_moments<T, NE>(su, Rf, P, maxp, ntypes, sg.start, mono, Mom, D, lane);
         symm_angular_moments<T, NE>(plan, ti, ntypes, Mom, D, S, Grow, lane);
@@ -253,8 +297,14 @@ __device__ __forceinline__ void symm_body(const DevPlan* __restrict__ plan, cons
             const bool tri = (a == bb);
             const long long Q = tri ? (long long)na * (na - 1) / 2 : (long long)na * nb;
             if (Q == 0) continue;  // block stays zero
-            radial_tables<T, NE>(A, su, sa, sa + na, Rf, nullptr, lane);
-            if (!tri) radial_tables<T, NE>(A, su, sb, sb + nb, Rf, nullptr, lane);
+            const bool nsb = GEN && !ang_separable(A.phi);
+            if (nsb) {
+                ns_tables<T, NE>(A, su, sa, sa + na, Rf, nullptr, nsx, lane);
+                if (!tri) ns_tables<T, NE>(A, su, sb, sb + nb, Rf, nullptr, nsx, lane);
+            } else {
+                radial_tables<T, NE, GEN>(A, su, sa, sa + na, Rf, nullptr, lane);
+                if (!tri) radial_tables<T, NE, GEN>(A, su, sb, sb + nb, Rf, nullptr, lane);
+            }
             __syncthreads();
             T lam[NC], zet[NC];
             int zin[NC];
@@ -289,15 +339,24 @@ __device__ __forceinline__ void symm_body(const DevPlan* __restrict__ plan, cons
                     T rj[NE];
 #pragma unroll
                     for (int e = 0; e < NE; ++e) rj[e] = Rf[pj * NE + e];
+                    const T fcj = nsb ? nsx[2 * pj] : T(0);
                     for (int t = 0; t < n; ++t) {
                         const int pk = sb + k + t;
                         const T cs = ujx * su[4 * pk] + ujy * su[4 * pk + 1] + ujz * su[4 * pk + 2];
                         T Ac[NC];
 #pragma unroll
                         for (int c = 0; c < NC; ++c) Ac[c] = angular_value<T>(cs, lam[c], zet[c], zin[c]);
+                        const T fcjk = nsb ? fcj * nsx[2 * pk] : T(0);
 #pragma unroll
                         for (int e = 0; e < NE; ++e) {
-                            const T pe = rj[e] * Rf[pk * NE + e];
+                            T pe;
+                            if (nsb) {
+                                T F, tt;
+                                ang_pair<T>(A.phi, rj[e], Rf[pk * NE + e], F, tt);
+                                pe = fcjk * F;
+                            } else {
+                                pe = rj[e] * Rf[pk * NE + e];
+                            }
 #pragma unroll
                             for (int c = 0; c < NC; ++c) acc[e][c] += Ac[c] * pe;
                         }
@@ -328,11 +387,11 @@ __device__ __forceinline__ void symm_body(const DevPlan* __restrict__ plan, cons
 
 // Force terms of one staged centre into Xs (LDS).  Wrad: dE/dG row in LDS (radial weights);
 // Wang: the same row in global memory (uniform, scalar loads) or LDS.
-template <typename T, int NE, int NC>
+template <typename T, int NE, int NC, bool GEN>
 __device__ __forceinline__ void force_body(const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta,
                                            const double* __restrict__ rfun_rs, int ti, int ntypes, int P, int maxp,
                                            const T* su, T* Rf, T* Rd, T* Xs, T* slots, int* slot_row, const T* W,
-                                           const T* Wang, const Segs& sg, T* mom, int lane) {
+                                           const T* Wang, const Segs& sg, T* nsx, T* mom, int lane) {
     // radial term, one lane per entry: X_j = -dE/dd_j u_j
     for (int p = lane; p < P; p += kWave) {
         int b = 0;
@@ -342,10 +401,20 @@ __device__ __forceinline__ void force_body(const DevPlan* __restrict__ plan, con
         T fc, gc;
         cutoff_fg<T>(R.cut, d, T(R.rc), T(R.rci), T(R.pirci), fc, gc);
         T s = T(0);
-        for (int f = 0; f < R.n; ++f) {
-            const T eta = T(rfun_eta[R.fbase + f]), rs = T(rfun_rs[R.fbase + f]);
-            const T x = d - rs;
-            s += W[R.goff + f] * exp(-eta * x * x) * (T(-2) * eta * x * fc + gc);
+        if (!GEN || R.phi == RAD_GAUSSIAN) {
+            for (int f = 0; f < R.n; ++f) {
+                const T eta = T(rfun_eta[R.fbase + f]), rs = T(rfun_rs[R.fbase + f]);
+                const T x = d - rs;
+                s += W[R.goff + f] * exp(-eta * x * x) * (T(-2) * eta * x * fc + gc);
+            }
+        } else {
+            // d/dd [phi(-eta (d - rs)) fc(d)] = -eta phi' fc + phi fc'
+            for (int f = 0; f < R.n; ++f) {
+                const T eta = T(rfun_eta[R.fbase + f]), rs = T(rfun_rs[R.fbase + f]);
+                T v, dv;
+                rad_phi<T>(R.phi, -eta * (d - rs), v, dv);
+                s += W[R.goff + f] * (-eta * dv * fc + v * gc);
+            }
         }
         s *= T(R.norm);
         Xs[4 * p + 0] = -s * su[4 * p];
@@ -364,7 +433,7 @@ __device__ __forceinline__ void force_body(const DevPlan* __restrict__ plan, con
         T* Lx = Lamp + ntypes * NE * kMomK;
         T* Ly = Lx + ntypes * NE * kMomK3;
         T* Lz = Ly + ntypes * NE * kMomK3;
-        radial_tables<T, NE>(plan->ang[ti][0], su, 0, P, Rf, Rd, lane);
+        radial_tables<T, NE, GEN>(plan->ang[ti][0], su, 0, P, Rf, Rd, lane);
         __syncthreads();
         build_moments<T, NE>(su, Rf, P, maxp, ntypes, sg.start, mono, Mom, D, lane);
         force_angular_moments<T, NE>(plan, ti, ntypes, P, maxp, su, Rf, Rd, W, sg.start, mono, Mom, wtab, Lam, Lamp,
@@ -379,8 +448,14 @@ __device__ __forceinline__ void force_body(const DevPlan* __restrict__ plan, con
             const int sb = sg.start[bb], nb = sg.start[bb + 1] - sb;
             const bool same = (a == bb);
             if (same ? (na < 2) : (na == 0 || nb == 0)) continue;
-            radial_tables<T, NE>(A, su, sa, sa + na, Rf, Rd, lane);
-            if (!same) radial_tables<T, NE>(A, su, sb, sb + nb, Rf, Rd, lane);
+            const bool nsb = GEN && !ang_separable(A.phi);
+            if (nsb) {
+                ns_tables<T, NE>(A, su, sa, sa + na, Rf, Rd, nsx, lane);
+                if (!same) ns_tables<T, NE>(A, su, sb, sb + nb, Rf, Rd, nsx, lane);
+            } else {
+                radial_tables<T, NE, GEN>(A, su, sa, sa + na, Rf, Rd, lane);
+                if (!same) radial_tables<T, NE, GEN>(A, su, sb, sb + nb, Rf, Rd, lane);
+            }
             // block weights W_ec = norm dE/dG[goff + fmap(e,c)]: uniform, read with scalar loads
             T w[NE][NC];
             const T* dEdGi = Wang + A.goff;
@@ -426,6 +501,7 @@ __device__ __forceinline__ void force_body(const DevPlan* __restrict__ plan, con
                             rdj[e] = Rd[pj * NE + e];
                         }
                         T xx = T(0), xy = T(0), xz = T(0);
+                        const T fcj = nsb ? nsx[2 * pj] : T(0), gcj = nsb ? nsx[2 * pj + 1] : T(0);
                         for (int t = 0; t < n; ++t) {
                             const int cc0 = c + t;
                             const int k = (same && cc0 >= r) ? cc0 + 1 : cc0;
@@ -433,11 +509,23 @@ __device__ __forceinline__ void force_body(const DevPlan* __restrict__ plan, con
                             const T ukx = su[4 * pk], uky = su[4 * pk + 1], ukz = su[4 * pk + 2];
                             const T cs = ujx * ukx + ujy * uky + ujz * ukz;
                             T pf[NE], pd[NE];
+                            if (nsb) {
+                                // F(x_j, x_k) fc_j fc_k and its d/dd_j: fc_k F (fc_j' - fc_j t dx_j/dd_j)
+                                const T fck = nsx[2 * pk];
 #pragma unroll
-                            for (int e = 0; e < NE; ++e) {
-                                const T rfk = Rf[pk * NE + e];
-                                pf[e] = rfj[e] * rfk;
-                                pd[e] = rdj[e] * rfk;
+                                for (int e = 0; e < NE; ++e) {
+                                    T F, tt;
+                                    ang_pair<T>(A.phi, rfj[e], Rf[pk * NE + e], F, tt);
+                                    pf[e] = fcj * fck * F;
+                                    pd[e] = fck * F * (gcj - fcj * tt * rdj[e]);
+                                }
+                            } else {
+#pragma unroll
+                                for (int e = 0; e < NE; ++e) {
+                                    const T rfk = Rf[pk * NE + e];
+                                    pf[e] = rfj[e] * rfk;
+                                    pd[e] = rdj[e] * rfk;
+                                }
                             }
                             T dEdcos = T(0), dEdd = T(0);
 #pragma unroll
@@ -499,14 +587,18 @@ __device__ __forceinline__ void force_body(const DevPlan* __restrict__ plan, con
 
 // LDS sizes (bytes) of the kernels for a given maxp.
 // Each kernel's LDS: its own arrays, then (moment path) the moment region of mom_lds_elems.
-template <typename T, int NE, int NC>
+// GEN: the kernels for models beyond GAUSSIAN / GAUSS / tanh (other radial functions, angular
+// distance factors and activations); they add the per-entry cutoff table nsx [maxp][2] that the
+// non-separable angular factors use.  GEN = false keeps the common case's register budget.
+template <typename T, int NE, int NC, bool GEN>
 __host__ __device__ constexpr size_t symm_lds_bytes(int maxp, int ldg) {
-    return sizeof(T) * (size_t)(4 * maxp + NE * maxp + kWave * NE * NC) + sizeof(T) * (size_t)((ldg + 1) & ~1);
+    return sizeof(T) * (size_t)(4 * maxp + NE * maxp + kWave * NE * NC) + sizeof(T) * (size_t)((ldg + 1) & ~1) +
+           (GEN ? sizeof(T) * 2 * (size_t)maxp : 0);
 }
-template <typename T, int NE, int NC>
+template <typename T, int NE, int NC, bool GEN>
 __host__ __device__ constexpr size_t force_lds_bytes(int maxp, int ldg) {
     return sizeof(T) * (size_t)(4 * maxp + 2 * NE * maxp + 4 * maxp + 8 * kWave + ((ldg + 1) & ~1)) +
-           sizeof(int) * (size_t)(maxp + 2 * kWave);
+           sizeof(int) * (size_t)(maxp + ((2 * kWave + 3) & ~3)) + (GEN ? sizeof(T) * 2 * (size_t)maxp : 0);
 }
 template <typename T, int NE>
 __host__ __device__ constexpr size_t mom_lds_bytes(int maxp, int ntypes) {
@@ -514,7 +606,7 @@ __host__ __device__ constexpr size_t mom_lds_bytes(int maxp, int ntypes) {
 }
 
 // ------------------------------------------------------------------ descriptors
-template <typename T, int NE, int NC>
+template <typename T, int NE, int NC, bool GEN>
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <= 16 ? 3 : 1, 8))) void k_symm(
     const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta, const double* __restrict__ rfun_rs,
     const int* __restrict__ atom_type, const int* __restrict__ centres, const int* __restrict__ cnt,
@@ -525,7 +617,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
     T* Rf = su + 4 * maxp;
     T* red = Rf + NE * maxp;
     T* Grow = red + kWave * NE * NC;
-    T* mom = use_mom ? Grow + ((ldg + 1) & ~1) : nullptr;
+    T* nsx = Grow + ((ldg + 1) & ~1);
+    T* mom = use_mom ? nsx + (GEN ? 2 * maxp : 0) : nullptr;
     const int i = centres[blockIdx.x];
     const int lane = threadIdx.x;
     const int ti = atom_type[i];
@@ -536,13 +629,13 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
     stage_entries<T>(i, P, maxp, ntypes, nbr_meta, nbr_r, su, nullptr, sg, lane);
     for (int k = lane; k < ldg; k += kWave) Grow[k] = T(0);
     __syncthreads();
-    symm_body<T, NE, NC>(plan, rfun_eta, rfun_rs, ti, ntypes, P, maxp, su, Rf, red, Grow, sg, mom, lane);
+    symm_body<T, NE, NC, GEN>(plan, rfun_eta, rfun_rs, ti, ntypes, P, maxp, su, Rf, red, Grow, sg, nsx, mom, lane);
     __syncthreads();
     for (int k = lane; k < ldg; k += kWave) G[(int64_t)i * ldg + k] = Grow[k];
 }
 
 // ------------------------------------------------------------------ force terms
-template <typename T, int NE, int NC>
+template <typename T, int NE, int NC, bool GEN>
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <= 16 ? 3 : 1, 8))) void k_force(
     const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta, const double* __restrict__ rfun_rs,
     const int* __restrict__ atom_type, const int* __restrict__ centres, const int* __restrict__ cnt,
@@ -557,7 +650,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
     T* W = slots + 8 * kWave;         // dE/dG row
     int* orig = (int*)(W + ((ldg + 1) & ~1));
     int* slot_row = orig + maxp;      // [2][kWave]
-    T* mom = use_mom ? (T*)(slot_row + ((2 * kWave + 3) & ~3)) : nullptr;
+    T* nsx = (T*)(slot_row + ((2 * kWave + 3) & ~3));
+    T* mom = use_mom ? nsx + (GEN ? 2 * maxp : 0) : nullptr;
     const int i = centres[blockIdx.x];
     const int lane = threadIdx.x;
     const int ti = atom_type[i];
@@ -568,8 +662,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
     stage_entries<T>(i, P, maxp, ntypes, nbr_meta, nbr_r, su, orig, sg, lane);
     for (int k = lane; k < ldg; k += kWave) W[k] = dEdG[(int64_t)i * ldg + k];
     __syncthreads();
-    force_body<T, NE, NC>(plan, rfun_eta, rfun_rs, ti, ntypes, P, maxp, su, Rf, Rd, Xs, slots, slot_row, W,
-                          dEdG + (int64_t)i * ldg, sg, mom, lane);
+    force_body<T, NE, NC, GEN>(plan, rfun_eta, rfun_rs, ti, ntypes, P, maxp, su, Rf, Rd, Xs, slots, slot_row, W,
+                              dEdG + (int64_t)i * ldg, sg, nsx, mom, lane);
     const int64_t row = (int64_t)i * maxp;
     for (int p = lane; p < P; p += kWave) {
         const int o = orig[p];
@@ -585,7 +679,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
 // units; forward layers read the transposed weights (coalesced over lanes), backward layers the
 // row-major weights.  Same reference arithmetic as the MFMA kernels (net.hip): fp/fpbp and
 // DODZ::grad, ml/nn.cpp:1191-1233, 1620-1628.
-template <typename T>
+template <typename T, bool GEN>
 __device__ __forceinline__ double net_body(const DevNet& N, const T* __restrict__ Wt, const T* G, T* act0, T* act1,
                                            T* der, T* dErow, int maxw, int vac, int lane) {
     for (int k = lane; k < N.nin_pad; k += kWave)
@@ -598,12 +692,14 @@ __device__ __forceinline__ double net_body(const DevNet& N, const T* __restrict_
         const int out_pad = N.wpad[l];
         const T* WT = Wt + N.wtoff[l];
         const T* b = Wt + N.boff[l];
-        const bool linear = (l == N.L - 1) || !N.tanh_hidden;
+        const int act = (l == N.L - 1) ? NEU_LINEAR : N.act;
         for (int o = lane; o < out_pad; o += kWave) {
             T z = b[o];
             for (int k = 0; k < in_pad; ++k) z += WT[(int64_t)k * out_pad + o] * cur[k];
             T a, d;
-            if (linear) {
+            if (GEN) {
+                act_eval<T>(act, z, a, d);
+            } else if (act == NEU_LINEAR) {
                 a = z;
                 d = T(1);
             } else {
@@ -648,18 +744,18 @@ __device__ __forceinline__ double net_body(const DevNet& N, const T* __restrict_
     return E;
 }
 
-template <typename T, int NE, int NC>
+template <typename T, int NE, int NC, bool GEN>
 __host__ __device__ constexpr size_t fused_lds_bytes(int maxp, int ldg, int maxw, int dersum) {
     return sizeof(T) * (size_t)(4 * maxp + 2 * NE * maxp + 4 * maxp + 8 * kWave + kWave * NE * NC +
                                 2 * ((ldg + 1) & ~1) + 2 * maxw + dersum) +
-           sizeof(int) * (size_t)(maxp + 2 * kWave);
+           sizeof(int) * (size_t)(maxp + ((2 * kWave + 3) & ~3)) + (GEN ? sizeof(T) * 2 * (size_t)maxp : 0);
 }
 
 // Persistent fused pass: each wavefront takes atoms from a work counter until the batch is
 // done (every wave reaches the exit), and for each atom stages its entries once, sums the
 // descriptors, runs its network in-wave and computes its per-entry force terms.  The order of
 // atoms (species-sorted) keeps a species' weights hot in the caches.
-template <typename T, int NE, int NC>
+template <typename T, int NE, int NC, bool GEN>
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <= 16 ? (sizeof(T) == 8 ? 2 : 3) : 1, 8))) void k_fused(
     const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta, const double* __restrict__ rfun_rs,
     const T* __restrict__ Wt, const int* __restrict__ atom_type, const int* __restrict__ order, int ntot,
@@ -680,7 +776,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
     T* der = act1 + maxw;
     int* orig = (int*)(der + dersum);
     int* slot_row = orig + maxp;
-    T* mom = use_mom ? (T*)(slot_row + ((2 * kWave + 3) & ~3)) : nullptr;
+    T* nsx = (T*)(slot_row + ((2 * kWave + 3) & ~3));
+    T* mom = use_mom ? nsx + (GEN ? 2 * maxp : 0) : nullptr;
     __shared__ int seg_lds[kMaxSpecies + 1];
     __shared__ int next_atom;
     Segs sg{seg_lds};
@@ -698,13 +795,14 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
         stage_entries<T>(i, P, maxp, ntypes, nbr_meta, nbr_r, su, orig, sg, lane);
         for (int k = lane; k < ldg; k += kWave) Grow[k] = T(0);
         __syncthreads();
-        symm_body<T, NE, NC>(plan, rfun_eta, rfun_rs, ti, ntypes, P, maxp, su, Rf, red, Grow, sg, mom, lane);
+        symm_body<T, NE, NC, GEN>(plan, rfun_eta, rfun_rs, ti, ntypes, P, maxp, su, Rf, red, Grow, sg, nsx,
+                                  (use_mom & 1) ? mom : nullptr, lane);
         __syncthreads();
-        const double E = net_body<T>(plan->net[ti], Wt, Grow, act0, act1, der, dErow, maxw, vac, lane);
+        const double E = net_body<T, GEN>(plan->net[ti], Wt, Grow, act0, act1, der, dErow, maxw, vac, lane);
         if (lane == 0) eatom[i] = E;
         if (do_force) {
-            force_body<T, NE, NC>(plan, rfun_eta, rfun_rs, ti, ntypes, P, maxp, su, Rf, Rd, Xs, slots, slot_row,
-                                  dErow, dErow, sg, mom, lane);
+            force_body<T, NE, NC, GEN>(plan, rfun_eta, rfun_rs, ti, ntypes, P, maxp, su, Rf, Rd, Xs, slots,
+                                       slot_row, dErow, dErow, sg, nsx, (use_mom & 2) ? mom : nullptr, lane);
             const int64_t row = (int64_t)i * maxp;
             for (int p = lane; p < P; p += kWave) {
                 const int o = orig[p];
@@ -718,56 +816,71 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
     }
 }
 
-template <typename T, int NE, int NC>
+template <typename T, int NE, int NC, bool GEN>
 static hipError_t launch_fused_t(int nblocks, const DevPlan* plan, const double* rfe, const double* rfr, const T* Wt,
                                  const int* atom_type, const int* order, int ntot, int* work, const int* cnt,
                                  const int* meta, const double4* r, int maxp, int ldg, int maxw, int dersum, int vac,
                                  int do_force, double* eatom, T* X, int use_mom, int ntypes, hipStream_t st) {
-    const size_t lds = fused_lds_bytes<T, NE, NC>(maxp, ldg, maxw, dersum) + 16 +
+    const size_t lds = fused_lds_bytes<T, NE, NC, GEN>(maxp, ldg, maxw, dersum) +
                        (use_mom ? mom_lds_bytes<T, NE>(maxp, ntypes) : 0);
     if (lds > 65536) return hipErrorInvalidConfiguration;
-    hipLaunchKernelGGL((k_fused<T, NE, NC>), dim3(nblocks), dim3(kWave), lds, st, plan, rfe, rfr, Wt, atom_type, order,
-                       ntot, work, cnt, meta, r, maxp, ldg, maxw, dersum, vac, do_force, eatom, X, use_mom);
+    hipLaunchKernelGGL((k_fused<T, NE, NC, GEN>), dim3(nblocks), dim3(kWave), lds, st, plan, rfe, rfr, Wt, atom_type,
+                       order, ntot, work, cnt, meta, r, maxp, ldg, maxw, dersum, vac, do_force, eatom, X, use_mom);
     return hipGetLastError();
 }
 
 size_t mom_lds(int shape, bool fp64, int maxp, int ntypes) {
     const int ne = shape == 0 ? 2 : (shape == 1 ? 4 : 8);
     return (fp64 ? 8 : 4) * (size_t)(ne == 2 ? mom_lds_elems<2>(maxp, ntypes)
-                                             : (ne == 4 ? mom_lds_elems<4>(maxp, ntypes) : mom_lds_elems<8>(maxp, ntypes))) +
-           16;
+                                             : (ne == 4 ? mom_lds_elems<4>(maxp, ntypes) : mom_lds_elems<8>(maxp, ntypes)));
 }
 
-size_t fused_lds(int shape, bool fp64, int maxp, int ldg, int maxw, int dersum) {
-    if (fp64) {
-        if (shape == 0) return fused_lds_bytes<double, 2, 4>(maxp, ldg, maxw, dersum);
-        if (shape == 1) return fused_lds_bytes<double, 4, 4>(maxp, ldg, maxw, dersum);
-        return fused_lds_bytes<double, 8, 8>(maxp, ldg, maxw, dersum);
-    }
-    if (shape == 0) return fused_lds_bytes<float, 2, 4>(maxp, ldg, maxw, dersum);
-    if (shape == 1) return fused_lds_bytes<float, 4, 4>(maxp, ldg, maxw, dersum);
-    return fused_lds_bytes<float, 8, 8>(maxp, ldg, maxw, dersum);
+template <typename T, bool GEN>
+static size_t fused_lds_t(int shape, int maxp, int ldg, int maxw, int dersum) {
+    if (shape == 0) return fused_lds_bytes<T, 2, 4, GEN>(maxp, ldg, maxw, dersum);
+    if (shape == 1) return fused_lds_bytes<T, 4, 4, GEN>(maxp, ldg, maxw, dersum);
+    return fused_lds_bytes<T, 8, 8, GEN>(maxp, ldg, maxw, dersum);
+}
+
+size_t fused_lds(int shape, bool fp64, bool ns, int maxp, int ldg, int maxw, int dersum) {
+    if (fp64) return ns ? fused_lds_t<double, true>(shape, maxp, ldg, maxw, dersum)
+                        : fused_lds_t<double, false>(shape, maxp, ldg, maxw, dersum);
+    return ns ? fused_lds_t<float, true>(shape, maxp, ldg, maxw, dersum)
+              : fused_lds_t<float, false>(shape, maxp, ldg, maxw, dersum);
+}
+
+template <typename T, bool GEN>
+static hipError_t launch_fused_s(int shape, int nblocks, const DevPlan* plan, const double* rfe, const double* rfr,
+                                 const T* Wt, const int* atom_type, const int* order, int ntot, int* work,
+                                 const int* cnt, const int* meta, const double4* r, int maxp, int ldg, int maxw,
+                                 int dersum, int vac, int do_force, double* eatom, T* X, int use_mom, int ntypes,
+                                 hipStream_t st) {
+    if (shape == 0)
+        return launch_fused_t<T, 2, 4, GEN>(nblocks, plan, rfe, rfr, Wt, atom_type, order, ntot, work, cnt, meta, r,
+                                           maxp, ldg, maxw, dersum, vac, do_force, eatom, X, use_mom, ntypes, st);
+    if (shape == 1)
+        return launch_fused_t<T, 4, 4, GEN>(nblocks, plan, rfe, rfr, Wt, atom_type, order, ntot, work, cnt, meta, r,
+                                           maxp, ldg, maxw, dersum, vac, do_force, eatom, X, use_mom, ntypes, st);
+    return launch_fused_t<T, 8, 8, GEN>(nblocks, plan, rfe, rfr, Wt, atom_type, order, ntot, work, cnt, meta, r, maxp,
+                                       ldg, maxw, dersum, vac, do_force, eatom, X, use_mom, ntypes, st);
 }
 
 template <typename T>
-hipError_t launch_fused(int shape, int nblocks, const DevPlan* plan, const double* rfe, const double* rfr, const T* Wt,
-                        const int* atom_type, const int* order, int ntot, int* work, const int* cnt, const int* meta,
-                        const double4* r, int maxp, int ldg, int maxw, int dersum, int vac, int do_force,
-                        double* eatom, T* X, int use_mom, int ntypes, hipStream_t st) {
-    if (shape == 0)
-        return launch_fused_t<T, 2, 4>(nblocks, plan, rfe, rfr, Wt, atom_type, order, ntot, work, cnt, meta, r, maxp,
-                                       ldg, maxw, dersum, vac, do_force, eatom, X, use_mom, ntypes, st);
-    if (shape == 1)
-        return launch_fused_t<T, 4, 4>(nblocks, plan, rfe, rfr, Wt, atom_type, order, ntot, work, cnt, meta, r, maxp,
-                                       ldg, maxw, dersum, vac, do_force, eatom, X, use_mom, ntypes, st);
-    return launch_fused_t<T, 8, 8>(nblocks, plan, rfe, rfr, Wt, atom_type, order, ntot, work, cnt, meta, r, maxp, ldg,
-                                   maxw, dersum, vac, do_force, eatom, X, use_mom, ntypes, st);
+hipError_t launch_fused(int shape, int ns, int nblocks, const DevPlan* plan, const double* rfe, const double* rfr,
+                        const T* Wt, const int* atom_type, const int* order, int ntot, int* work, const int* cnt,
+                        const int* meta, const double4* r, int maxp, int ldg, int maxw, int dersum, int vac,
+                        int do_force, double* eatom, T* X, int use_mom, int ntypes, hipStream_t st) {
+    if (ns)
+        return launch_fused_s<T, true>(shape, nblocks, plan, rfe, rfr, Wt, atom_type, order, ntot, work, cnt, meta, r,
+                                       maxp, ldg, maxw, dersum, vac, do_force, eatom, X, use_mom, ntypes, st);
+    return launch_fused_s<T, false>(shape, nblocks, plan, rfe, rfr, Wt, atom_type, order, ntot, work, cnt, meta, r,
+                                    maxp, ldg, maxw, dersum, vac, do_force, eatom, X, use_mom, ntypes, st);
 }
 
-template hipError_t launch_fused<double>(int, int, const DevPlan*, const double*, const double*, const double*,
+template hipError_t launch_fused<double>(int, int, int, const DevPlan*, const double*, const double*, const double*,
                                          const int*, const int*, int, int*, const int*, const int*, const double4*, int,
                                          int, int, int, int, int, double*, double*, int, int, hipStream_t);
-template hipError_t launch_fused<float>(int, int, const DevPlan*, const double*, const double*, const float*,
+template hipError_t launch_fused<float>(int, int, int, const DevPlan*, const double*, const double*, const float*,
                                         const int*, const int*, int, int*, const int*, const int*, const double4*, int,
                                         int, int, int, int, int, double*, float*, int, int, hipStream_t);
 
@@ -834,26 +947,25 @@ __global__ __launch_bounds__(kWave) void k_assemble(const DevStruc* __restrict__
 }
 
 // ------------------------------------------------------------------ launchers
-template <typename T, int NE, int NC>
+template <typename T, int NE, int NC, bool GEN>
 static hipError_t launch_symm_t(const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
                                 const int* centres, int ncentres, const int* cnt, const int* meta, const double4* r,
                                 int maxp, T* G, int ldg, int use_mom, int ntypes, hipStream_t st) {
-    const size_t lds = symm_lds_bytes<T, NE, NC>(maxp, ldg) + (use_mom ? mom_lds_bytes<T, NE>(maxp, ntypes) : 0);
+    const size_t lds = symm_lds_bytes<T, NE, NC, GEN>(maxp, ldg) + (use_mom ? mom_lds_bytes<T, NE>(maxp, ntypes) : 0);
     if (lds > 65536) return hipErrorInvalidConfiguration;
-    hipLaunchKernelGGL((k_symm<T, NE, NC>), dim3(ncentres), dim3(kWave), lds, st, plan, rfe, rfr, atom_type, centres, cnt,
-                       meta, r, maxp, G, ldg, use_mom);
+    hipLaunchKernelGGL((k_symm<T, NE, NC, GEN>), dim3(ncentres), dim3(kWave), lds, st, plan, rfe, rfr, atom_type,
+                       centres, cnt, meta, r, maxp, G, ldg, use_mom);
     return hipGetLastError();
 }
 
-template <typename T, int NE, int NC>
+template <typename T, int NE, int NC, bool GEN>
 static hipError_t launch_force_t(const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
                                  const int* centres, int ncentres, const int* cnt, const int* meta, const double4* r,
                                  int maxp, const T* dEdG, int ldg, T* X, int use_mom, int ntypes, hipStream_t st) {
-    const size_t lds = force_lds_bytes<T, NE, NC>(maxp, ldg) + 16 +
-                       (use_mom ? mom_lds_bytes<T, NE>(maxp, ntypes) : 0);
+    const size_t lds = force_lds_bytes<T, NE, NC, GEN>(maxp, ldg) + (use_mom ? mom_lds_bytes<T, NE>(maxp, ntypes) : 0);
     if (lds > 65536) return hipErrorInvalidConfiguration;
-    hipLaunchKernelGGL((k_force<T, NE, NC>), dim3(ncentres), dim3(kWave), lds, st, plan, rfe, rfr, atom_type, centres,
-                       cnt, meta, r, maxp, dEdG, ldg, X, use_mom);
+    hipLaunchKernelGGL((k_force<T, NE, NC, GEN>), dim3(ncentres), dim3(kWave), lds, st, plan, rfe, rfr, atom_type,
+                       centres, cnt, meta, r, maxp, dEdG, ldg, X, use_mom);
     return hipGetLastError();
 }
 
@@ -864,54 +976,79 @@ int grid_shape(int ne, int nc) {
     return 2;
 }
 
-size_t symm_lds(int shape, bool fp64, int maxp, int ldg) {
-    if (fp64) {
-        if (shape == 0) return symm_lds_bytes<double, 2, 4>(maxp, ldg);
-        if (shape == 1) return symm_lds_bytes<double, 4, 4>(maxp, ldg);
-        return symm_lds_bytes<double, 8, 8>(maxp, ldg);
-    }
-    if (shape == 0) return symm_lds_bytes<float, 2, 4>(maxp, ldg);
-    if (shape == 1) return symm_lds_bytes<float, 4, 4>(maxp, ldg);
-    return symm_lds_bytes<float, 8, 8>(maxp, ldg);
+template <typename T, bool GEN>
+static size_t symm_lds_t(int shape, int maxp, int ldg) {
+    if (shape == 0) return symm_lds_bytes<T, 2, 4, GEN>(maxp, ldg);
+    if (shape == 1) return symm_lds_bytes<T, 4, 4, GEN>(maxp, ldg);
+    return symm_lds_bytes<T, 8, 8, GEN>(maxp, ldg);
+}
+template <typename T, bool GEN>
+static size_t force_lds_t(int shape, int maxp, int ldg) {
+    if (shape == 0) return force_lds_bytes<T, 2, 4, GEN>(maxp, ldg);
+    if (shape == 1) return force_lds_bytes<T, 4, 4, GEN>(maxp, ldg);
+    return force_lds_bytes<T, 8, 8, GEN>(maxp, ldg);
 }
 
-size_t force_lds(int shape, bool fp64, int maxp, int ldg) {
-    if (fp64) {
-        if (shape == 0) return force_lds_bytes<double, 2, 4>(maxp, ldg);
-        if (shape == 1) return force_lds_bytes<double, 4, 4>(maxp, ldg);
-        return force_lds_bytes<double, 8, 8>(maxp, ldg);
-    }
-    if (shape == 0) return force_lds_bytes<float, 2, 4>(maxp, ldg);
-    if (shape == 1) return force_lds_bytes<float, 4, 4>(maxp, ldg);
-    return force_lds_bytes<float, 8, 8>(maxp, ldg);
+size_t symm_lds(int shape, bool fp64, bool ns, int maxp, int ldg) {
+    if (fp64) return ns ? symm_lds_t<double, true>(shape, maxp, ldg) : symm_lds_t<double, false>(shape, maxp, ldg);
+    return ns ? symm_lds_t<float, true>(shape, maxp, ldg) : symm_lds_t<float, false>(shape, maxp, ldg);
+}
+
+size_t force_lds(int shape, bool fp64, bool ns, int maxp, int ldg) {
+    if (fp64) return ns ? force_lds_t<double, true>(shape, maxp, ldg) : force_lds_t<double, false>(shape, maxp, ldg);
+    return ns ? force_lds_t<float, true>(shape, maxp, ldg) : force_lds_t<float, false>(shape, maxp, ldg);
+}
+
+template <typename T, bool GEN>
+static hipError_t launch_symm_s(int shape, const DevPlan* plan, const double* rfe, const double* rfr,
+                                const int* atom_type, const int* centres, int ncentres, const int* cnt, const int* meta,
+                                const double4* r, int maxp, T* G, int ldg, int use_mom, int ntypes, hipStream_t st) {
+    if (shape == 0)
+        return launch_symm_t<T, 2, 4, GEN>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, G, ldg,
+                                          use_mom, ntypes, st);
+    if (shape == 1)
+        return launch_symm_t<T, 4, 4, GEN>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, G, ldg,
+                                          use_mom, ntypes, st);
+    return launch_symm_t<T, 8, 8, GEN>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, G, ldg, use_mom,
+                                      ntypes, st);
 }
 
 template <typename T>
-hipError_t launch_symm(int shape, const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
-                       const int* centres, int ncentres, const int* cnt, const int* meta, const double4* r, int maxp,
-                       T* G, int ldg, int use_mom, int ntypes, hipStream_t st) {
-    if (shape == 0)
-        return launch_symm_t<T, 2, 4>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, G, ldg, use_mom,
-                                      ntypes, st);
-    if (shape == 1)
-        return launch_symm_t<T, 4, 4>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, G, ldg, use_mom,
-                                      ntypes, st);
-    return launch_symm_t<T, 8, 8>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, G, ldg, use_mom,
-                                  ntypes, st);
-}
-
-template <typename T>
-hipError_t launch_force(int shape, const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
-                        const int* centres, int ncentres, const int* cnt, const int* meta, const double4* r, int maxp,
-                        const T* dEdG, int ldg, T* X, int use_mom, int ntypes, hipStream_t st) {
-    if (shape == 0)
-        return launch_force_t<T, 2, 4>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, dEdG, ldg, X,
-                                       use_mom, ntypes, st);
-    if (shape == 1)
-        return launch_force_t<T, 4, 4>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, dEdG, ldg, X,
-                                       use_mom, ntypes, st);
-    return launch_force_t<T, 8, 8>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, dEdG, ldg, X,
+hipError_t launch_symm(int shape, int ns, const DevPlan* plan, const double* rfe, const double* rfr,
+                       const int* atom_type, const int* centres, int ncentres, const int* cnt, const int* meta,
+                       const double4* r, int maxp, T* G, int ldg, int use_mom, int ntypes, hipStream_t st) {
+    if (ns)
+        return launch_symm_s<T, true>(shape, plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, G, ldg,
+                                      use_mom, ntypes, st);
+    return launch_symm_s<T, false>(shape, plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, G, ldg,
                                    use_mom, ntypes, st);
+}
+
+template <typename T, bool GEN>
+static hipError_t launch_force_s(int shape, const DevPlan* plan, const double* rfe, const double* rfr,
+                                 const int* atom_type, const int* centres, int ncentres, const int* cnt,
+                                 const int* meta, const double4* r, int maxp, const T* dEdG, int ldg, T* X,
+                                 int use_mom, int ntypes, hipStream_t st) {
+    if (shape == 0)
+        return launch_force_t<T, 2, 4, GEN>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, dEdG, ldg,
+                                           X, use_mom, ntypes, st);
+    if (shape == 1)
+        return launch_force_t<T, 4, 4, GEN>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, dEdG, ldg,
+                                           X, use_mom, ntypes, st);
+    return launch_force_t<T, 8, 8, GEN>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, dEdG, ldg, X,
+                                       use_mom, ntypes, st);
+}
+
+template <typename T>
+hipError_t launch_force(int shape, int ns, const DevPlan* plan, const double* rfe, const double* rfr,
+                        const int* atom_type, const int* centres, int ncentres, const int* cnt, const int* meta,
+                        const double4* r, int maxp, const T* dEdG, int ldg, T* X, int use_mom, int ntypes,
+                        hipStream_t st) {
+    if (ns)
+        return launch_force_s<T, true>(shape, plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, dEdG,
+                                       ldg, X, use_mom, ntypes, st);
+    return launch_force_s<T, false>(shape, plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, dEdG, ldg,
+                                    X, use_mom, ntypes, st);
 }
 
 template <typename T>
@@ -922,17 +1059,18 @@ hipError_t launch_assemble(const DevStruc* S, const int* atom_struc, const int* 
     return hipGetLastError();
 }
 
-template hipError_t launch_symm<double>(int, const DevPlan*, const double*, const double*, const int*, const int*, int,
-                                        const int*, const int*, const double4*, int, double*, int, int, int,
+template hipError_t launch_symm<double>(int, int, const DevPlan*, const double*, const double*, const int*, const int*,
+                                        int, const int*, const int*, const double4*, int, double*, int, int, int,
                                         hipStream_t);
-template hipError_t launch_symm<float>(int, const DevPlan*, const double*, const double*, const int*, const int*, int,
-                                       const int*, const int*, const double4*, int, float*, int, int, int, hipStream_t);
-template hipError_t launch_force<double>(int, const DevPlan*, const double*, const double*, const int*, const int*, int,
-                                         const int*, const int*, const double4*, int, const double*, int, double*, int,
-                                         int, hipStream_t);
-template hipError_t launch_force<float>(int, const DevPlan*, const double*, const double*, const int*, const int*, int,
-                                        const int*, const int*, const double4*, int, const float*, int, float*, int,
-                                        int, hipStream_t);
+template hipError_t launch_symm<float>(int, int, const DevPlan*, const double*, const double*, const int*, const int*,
+                                       int, const int*, const int*, const double4*, int, float*, int, int, int,
+                                       hipStream_t);
+template hipError_t launch_force<double>(int, int, const DevPlan*, const double*, const double*, const int*,
+                                         const int*, int, const int*, const int*, const double4*, int, const double*,
+                                         int, double*, int, int, hipStream_t);
+template hipError_t launch_force<float>(int, int, const DevPlan*, const double*, const double*, const int*, const int*,
+                                        int, const int*, const int*, const double4*, int, const float*, int, float*,
+                                        int, int, hipStream_t);
 template hipError_t launch_assemble<double>(const DevStruc*, const int*, const int*, const int*, const int*,
                                             const double*, int, int, int, double*, hipStream_t);
 template hipError_t launch_assemble<float>(const DevStruc*, const int*, const int*, const int*, const int*, const float*,

@@ -119,6 +119,10 @@ class Potential:
     neuron: str = "TANH"
     norm: str = "UNIT"
     cut: str = "COS"
+    phi_r: str = "GAUSSIAN"  # radial function (basis_radial.cpp:24-32)
+    phi_a: str = "GAUSS"     # angular distance function (basis_angular.cpp:24-30)
+    outw: Dict[int, float] = field(default_factory=dict)  # output scaling per species (default 1)
+    outb: Dict[int, float] = field(default_factory=dict)  # output shift per species (default 0)
 
     @property
     def ntypes(self) -> int:
@@ -141,14 +145,14 @@ class Potential:
                 for j in range(T):
                     f.write(f"basis_radial {self.species[j]}\n")
                     fr = self.radial[(i, j)]
-                    f.write(f"BasisR {self.norm} {self.cut} {self.rc:f} GAUSSIAN {len(fr)}\n")
+                    f.write(f"BasisR {self.norm} {self.cut} {self.rc:f} {self.phi_r} {len(fr)}\n")
                     for rs, eta in fr:
                         f.write(f"\t{rs:f} {eta:f}\n")
                 for j in range(T):
                     for k in range(j, T):
                         f.write(f"basis_angular {self.species[j]} {self.species[k]}\n")
                         alpha, fa = self.angular[(i, j, k)]
-                        f.write(f"BasisA {self.norm} {self.cut} {self.rc:f} GAUSS {len(fa)} {alpha}\n")
+                        f.write(f"BasisA {self.norm} {self.cut} {self.rc:f} {self.phi_a} {len(fa)} {alpha}\n")
                         for eta, zeta, lam in fa:
                             f.write(f"\t{eta:f} {zeta:f} {lam}\n")
             for s in range(T):
@@ -157,9 +161,9 @@ class Potential:
                 f.write("nn " + " ".join(str(v) for v in lay) + " \n")
                 f.write(f"neuron {self.neuron}\n")
                 f.write("inpw " + "".join(f"{v:.15f} " for v in self.inpw[s]) + "\n")
-                f.write("outw " + f"{1.0:.15f} " + "\n")
+                f.write("outw " + f"{self.outw.get(s, 1.0):.15f} " + "\n")
                 f.write("inpb " + "".join(f"{v:.15f} " for v in self.inpb[s]) + "\n")
-                f.write("outb " + f"{0.0:.15f} " + "\n")
+                f.write("outb " + f"{self.outb.get(s, 0.0):.15f} " + "\n")
                 for l, b in enumerate(self.biases[s]):
                     f.write(f"bias[{l + 1}] " + "".join(f"{v:.15f} " for v in b) + "\n")
                 for l, W in enumerate(self.weights[s]):
@@ -179,7 +183,8 @@ class Potential:
             inpw={s: np.vectorize(q15)(v) for s, v in self.inpw.items()},
             inpb={s: np.vectorize(q15)(v) for s, v in self.inpb.items()},
             evac={s: q6(v) for s, v in self.evac.items()},
-            neuron=self.neuron, norm=self.norm, cut=self.cut)
+            neuron=self.neuron, norm=self.norm, cut=self.cut, phi_r=self.phi_r, phi_a=self.phi_a,
+            outw={s: q15(v) for s, v in self.outw.items()}, outb={s: q15(v) for s, v in self.outb.items()})
 
     def ninput(self, s: int) -> int:
         T = self.ntypes

@@ -20,6 +20,11 @@
 //                          no-duplicate and supercell invariants.
 //   cutoff / G2 / G4 ..... pinned: closed forms (dimer, symmetric trimer), integral
 //                          of fc, finite differences of every derivative.
+//   basis variants ....... pinned: radial TANH/SOFTPLUS/LOGCOSH/SWISH/MISH against dimer closed
+//                          forms, angular IPOWP/IPOWS/SECHP/SECHS against an asymmetric trimer,
+//                          forces of each by finite differences (tests/test_oracle_variants.py).
+//   neurons .............. pinned: all 17 activations against closed forms on a one-unit
+//                          network, dO/dI and forces by finite differences.
 //   descriptor layout .... pinned: hand-built 3-species cases (LMat block order).
 //   ANN fp / DODZ ........ pinned: closed-form one-hot networks, finite differences.
 //   energy / force ....... pinned: F = -dE/dr by finite differences on L > 2rc cells,
@@ -82,6 +87,23 @@ static double powint(double x, int n) {
         if (nn & 1) yy *= ww;
     return (n > 0) ? yy : 1.0 / yy;
 }
+
+// math/special.cpp:131-135 — the reference's log(1+x): 2 atanh(y), y = x/(x+2), by its odd
+// series truncated after the y^9 term (not the library log1p; error ~2e-7 at x = 1)
+static double logp1(double x) {
+    const double y = x / (x + 2.0);
+    const double y2 = y * y;
+    return 2.0 * y * (1.0 + y2 * (1.0 / 3.0 + y2 * (1.0 / 5.0 + y2 * (1.0 / 7.0 + y2 * 1.0 / 9.0))));
+}
+
+// math/special.cpp:71-79 — sech from one exponential of -|x|
+static double sech(double x) {
+    const double e = std::exp(-std::fabs(x));
+    return 2.0 * e / (1.0 + e * e);
+}
+
+// math/const.hpp:15 (the reference's truncated ln 2)
+static const double LOG2 = 0.693147180559945;
 
 // str/string.cpp:17-28 — polynomial string hash (int arithmetic, mod 2^31-1)
 static int str_hash(const char* s) {
@@ -265,8 +287,27 @@ static void neighbor_build(const Cell& cell, int n, const V3* pos, const int* ty
 // ---------------------------------------------------------------- basis
 enum CutName { CUT_UNKNOWN = 0, CUT_STEP, CUT_COS, CUT_TANH };  // nnp/cutoff.hpp:56-61
 enum CutNorm { NORM_UNKNOWN = 0, NORM_UNIT, NORM_VOL };          // nnp/cutoff.hpp:29-33
-enum PhiR { PHIR_UNKNOWN = 0, PHIR_GAUSSIAN };                   // basis_radial.cpp:25-33 (GAUSSIAN only)
-enum PhiA { PHIA_UNKNOWN = 0, PHIA_GAUSS };                      // basis_angular (GAUSS only)
+// basis_radial.cpp:24-32 and basis_angular.cpp:24-30: function names
+enum PhiR { PHIR_UNKNOWN = 0, PHIR_GAUSSIAN, PHIR_TANH, PHIR_SOFTPLUS, PHIR_LOGCOSH, PHIR_SWISH, PHIR_MISH };
+enum PhiA { PHIA_UNKNOWN = 0, PHIA_GAUSS, PHIA_IPOWP, PHIA_IPOWS, PHIA_SECHP, PHIA_SECHS };
+
+static PhiR phir_read(const std::string& s) {
+    if (s == "GAUSSIAN") return PHIR_GAUSSIAN;
+    if (s == "TANH") return PHIR_TANH;
+    if (s == "SOFTPLUS") return PHIR_SOFTPLUS;
+    if (s == "LOGCOSH") return PHIR_LOGCOSH;
+    if (s == "SWISH") return PHIR_SWISH;
+    if (s == "MISH") return PHIR_MISH;
+    return PHIR_UNKNOWN;
+}
+static PhiA phia_read(const std::string& s) {
+    if (s == "GAUSS") return PHIA_GAUSS;
+    if (s == "IPOWP") return PHIA_IPOWP;
+    if (s == "IPOWS") return PHIA_IPOWS;
+    if (s == "SECHP") return PHIA_SECHP;
+    if (s == "SECHS") return PHIA_SECHS;
+    return PHIA_UNKNOWN;
+}
 
 static CutName cut_read(const std::string& s) {  // cutoff.cpp:42-47
     if (s == "STEP") return CUT_STEP;
@@ -332,17 +373,100 @@ struct BasisR : Basis {
     PhiR phi = PHIR_UNKNOWN;
     std::vector<double> rs, eta;
     int size() const { return (int)eta.size(); }
-    // basis_radial.cpp:302-310 (GAUSSIAN)
+
+    // Radial function of x = -eta (r - rs) and its derivative with respect to x.
+    // basis_radial.cpp:302-359 (values) and 371-438 (derivatives, there multiplied by -eta).
+    // SWISH and MISH at x = 0 exactly: the reference leaves the previous value in place
+    // (basis_radial.cpp:341-357 assign only for x > 0 and x < 0); here the function's value 0.
+    static void phi_eval(PhiR f, double x, double& v, double& dv) {
+        switch (f) {
+            case PHIR_TANH: {  // 0.5 (tanh(x) + 1)
+                const double t = std::tanh(x);
+                v = 0.5 * (t + 1.0);
+                dv = 0.5 * (1.0 - t * t);
+            } break;
+            case PHIR_SOFTPLUS: {  // log(1 + e^x), stable split at 0
+                if (x > 0.0) {
+                    const double e = std::exp(-x);
+                    v = x + logp1(e);
+                    dv = 1.0 / (1.0 + e);
+                } else {
+                    const double e = std::exp(x);
+                    v = logp1(e);
+                    dv = e / (1.0 + e);
+                }
+            } break;
+            case PHIR_LOGCOSH: {  // x + log(1 + e^-2x)/2 (x > 0), log(1 + e^2x)/2 (x <= 0)
+                if (x > 0.0) {
+                    const double e = std::exp(-2.0 * x);
+                    v = x + 0.5 * logp1(e);
+                    dv = 1.0 / (1.0 + e);
+                } else {
+                    const double e = std::exp(2.0 * x);
+                    v = 0.5 * logp1(e);
+                    dv = e / (1.0 + e);
+                }
+            } break;
+            case PHIR_SWISH: {  // x sigmoid(x)
+                if (x > 0.0) {
+                    const double e = std::exp(-x), den = 1.0 / (1.0 + e);
+                    v = x * den;
+                    dv = (1.0 + e * (1.0 + x)) * den * den;
+                } else if (x < 0.0) {
+                    const double e = std::exp(x), den = 1.0 / (1.0 + e);
+                    v = x * e * den;
+                    dv = (1.0 + x + e) * e * den * den;
+                } else {
+                    v = 0.0;
+                    dv = 0.5;
+                }
+            } break;
+            case PHIR_MISH: {  // x tanh(softplus(x))
+                if (x > 0.0) {
+                    const double e = std::exp(-x), den = 1.0 / (2.0 * e * (e + 1.0) + 1.0);
+                    v = x * (2.0 * e + 1.0) * den;
+                    dv = (1.0 + e * (4.0 + e * (4.0 * x + 6.0 + e * 4.0 * (x + 1.0)))) * den * den;
+                } else if (x < 0.0) {
+                    const double e = std::exp(x), den = 1.0 / (e * (e + 2.0) + 2.0);
+                    v = x * e * (e + 2.0) * den;
+                    dv = (e * (e * (e + 4.0) + 4.0 * x + 6.0) + 4.0 * (x + 1.0)) * den * den * e;
+                } else {
+                    v = 0.0;
+                    dv = 0.6;
+                }
+            } break;
+            default: throw std::invalid_argument("BasisR::symm(double): Invalid symmetry function.");
+        }
+    }
+
+    // basis_radial.cpp:302-359
     void symm(double dr, double* out) const {
         const double cutv = cut_func(dr) * norm;
-        for (int i = 0; i < size(); ++i) out[i] = std::exp(-eta[i] * (dr - rs[i]) * (dr - rs[i])) * cutv;
+        if (phi == PHIR_GAUSSIAN) {
+            for (int i = 0; i < size(); ++i) out[i] = std::exp(-eta[i] * (dr - rs[i]) * (dr - rs[i])) * cutv;
+            return;
+        }
+        for (int i = 0; i < size(); ++i) {
+            double v, dv;
+            phi_eval(phi, -eta[i] * (dr - rs[i]), v, dv);
+            out[i] = v * cutv;
+        }
     }
-    // basis_radial.cpp:371-380, 443 (GAUSSIAN)
+    // basis_radial.cpp:371-443: amp = -norm sum_i dEdG_i d/dr[phi_i(r) fc(r)]
     double force(double dr, const double* dEdG) const {
         double amp = 0;
         const double cutv = cut_func(dr), gcut = cut_grad(dr);
-        for (int i = 0; i < size(); ++i)
-            amp -= dEdG[i] * std::exp(-eta[i] * (dr - rs[i]) * (dr - rs[i])) * (-2.0 * eta[i] * (dr - rs[i]) * cutv + gcut);
+        if (phi == PHIR_GAUSSIAN) {
+            for (int i = 0; i < size(); ++i)
+                amp -= dEdG[i] * std::exp(-eta[i] * (dr - rs[i]) * (dr - rs[i])) *
+                       (-2.0 * eta[i] * (dr - rs[i]) * cutv + gcut);
+            return amp * norm;
+        }
+        for (int i = 0; i < size(); ++i) {
+            double v, dv;
+            phi_eval(phi, -eta[i] * (dr - rs[i]), v, dv);
+            amp -= dEdG[i] * (-eta[i] * dv * cutv + v * gcut);
+        }
         return amp * norm;
     }
 };
@@ -358,15 +482,54 @@ struct BasisA : Basis {
         ietap.resize(eta.size());
         for (int i = 0; i < size(); ++i) ietap[i] = powint(1.0 / eta[i], alpha);
     }
-    // basis_angular.cpp:406-421 (GAUSS); no fc(r_jk), factor 2^-zeta
+    // Distance factor of function i for xj = ietap r_ij^alpha, xk = ietap r_ik^alpha, and the
+    // factor tj with d(factor)/d(xj) = -tj factor (tk likewise):
+    //   GAUSS exp(-(xj+xk)): t = 1;  IPOWP 1/((1+xj)(1+xk)): tj = 1/(1+xj);
+    //   IPOWS 1/(1+xj+xk): t = the factor;  SECHP sech(xj) sech(xk): tj = tanh(xj);
+    //   SECHS sech(xj+xk): t = tanh(xj+xk).  tanh is taken as sqrt(1 - sech^2).
+    // basis_angular.cpp:415-441 (values), 478-630 (the derivative factors).
+    void dist_factor(double xj, double xk, double& f, double& tj, double& tk) const {
+        switch (phi) {
+            case PHIA_GAUSS:
+                f = std::exp(-(xj + xk));
+                tj = tk = 1.0;
+                break;
+            case PHIA_IPOWP: {
+                const double dj = 1.0 / (1.0 + xj), dk = 1.0 / (1.0 + xk);
+                f = dj * dk;
+                tj = dj;
+                tk = dk;
+            } break;
+            case PHIA_IPOWS:
+                f = 1.0 / (1.0 + (xj + xk));
+                tj = tk = f;
+                break;
+            case PHIA_SECHP: {
+                const double sj = sech(xj), sk = sech(xk);
+                f = sj * sk;
+                tj = std::sqrt(1.0 - sj * sj);
+                tk = std::sqrt(1.0 - sk * sk);
+            } break;
+            case PHIA_SECHS: {
+                const double sv = sech(xj + xk);
+                f = sv;
+                tj = tk = std::sqrt(1.0 - sv * sv);
+            } break;
+            default: throw std::invalid_argument("BasisA::symm(double): Invalid symmetry function.");
+        }
+    }
+    // basis_angular.cpp:406-441; no fc(r_jk), factor 2^-zeta
     void symm(double cs, const double dr[3], double* out) const {
         const double c0 = cut_func(dr[0]), c1 = cut_func(dr[1]);
         const double cprod = c0 * c1 * norm;
         const double rijp = powint(dr[0], alpha), rikp = powint(dr[1], alpha);
-        for (int i = 0; i < size(); ++i)
-            out[i] = std::pow(std::fabs(0.5 * (1.0 + lambda[i] * cs)), zeta[i]) * cprod * std::exp(-ietap[i] * (rijp + rikp));
+        for (int i = 0; i < size(); ++i) {
+            double f, tj, tk;
+            dist_factor(ietap[i] * rijp, ietap[i] * rikp, f, tj, tk);
+            out[i] = std::pow(std::fabs(0.5 * (1.0 + lambda[i] * cs)), zeta[i]) * cprod * f;
+        }
     }
-    // basis_angular.cpp:456-507, 635-640 (GAUSS)
+    // basis_angular.cpp:456-640
     void force(double& phiv, double* etav, double cs, const double dr[3], const double* dEdG) const {
         const double c[2] = {cut_func(dr[0]), cut_func(dr[1])};
         const double g[2] = {cut_grad(dr[0]), cut_grad(dr[1])};
@@ -377,13 +540,14 @@ struct BasisA : Basis {
         const double dij = alpha * rijp / dr[0] * c[0];
         const double dik = alpha * rikp / dr[1] * c[1];
         for (int i = 0; i < size(); ++i) {
-            const double expf = std::exp(-ietap[i] * (rijp + rikp));
+            double f, tj, tk;
+            dist_factor(ietap[i] * rijp, ietap[i] * rikp, f, tj, tk);
             const double cw = std::fabs(0.5 * (1.0 + lambda[i] * cs));
-            const double gangle = std::pow(cw, zeta[i] - 1.0) * dEdG[i] * expf;
+            const double gangle = std::pow(cw, zeta[i] - 1.0) * dEdG[i] * f;
             const double fangle = cw * gangle;
             phiv -= 0.5 * zeta[i] * lambda[i] * cprod * gangle;
-            etav[0] -= fangle * (-dij * ietap[i] + g[0]) * c[1];
-            etav[1] -= fangle * (-dik * ietap[i] + g[1]) * c[0];
+            etav[0] -= fangle * (-dij * ietap[i] * tj + g[0]) * c[1];
+            etav[1] -= fangle * (-dik * ietap[i] * tk + g[1]) * c[0];
         }
         phiv *= norm;
         etav[0] *= norm;
@@ -393,7 +557,136 @@ struct BasisA : Basis {
 };
 
 // ---------------------------------------------------------------- network
-enum Neuron { NEURON_UNKNOWN = 0, NEURON_LINEAR, NEURON_TANH };
+// ml/nn.hpp:84-103 and nn.cpp:115-134: activation names
+enum Neuron {
+    NEURON_UNKNOWN = 0, NEURON_LINEAR, NEURON_SIGMOID, NEURON_TANH, NEURON_ISRU, NEURON_ARCTAN, NEURON_SOFTSIGN,
+    NEURON_RELU, NEURON_SOFTPLUS, NEURON_ELU, NEURON_GELU, NEURON_SWISH, NEURON_MISH, NEURON_TANHRE, NEURON_LOGCOSH,
+    NEURON_IQUAD, NEURON_IFABS, NEURON_SINSIG
+};
+
+static Neuron neuron_read(const std::string& s) {
+    static const char* names[] = {"LINEAR", "SIGMOID", "TANH", "ISRU", "ARCTAN", "SOFTSIGN", "RELU", "SOFTPLUS", "ELU",
+                                  "GELU", "SWISH", "MISH", "TANHRE", "LOGCOSH", "IQUAD", "IFABS", "SINSIG"};
+    for (int k = 0; k < 17; ++k)
+        if (s == names[k]) return (Neuron)(k + 1);
+    return NEURON_UNKNOWN;
+}
+
+// Activation a(z) and derivative d(z) of the hidden layers (AFFPBP, ml/nn.cpp:144-766).
+static void activate(Neuron n, double z, double& a, double& d) {
+    switch (n) {
+        case NEURON_LINEAR: a = z; d = 1.0; return;                       // 144-148
+        case NEURON_SIGMOID: {                                            // 168-183
+            const double e = std::exp(-std::fabs(z)), fr = 1.0 / (1.0 + e);
+            a = z >= 0 ? fr : e * fr;
+            d = e * fr * fr;
+        } return;
+        case NEURON_TANH: a = std::tanh(z); d = 1.0 - a * a; return;      // 209-216
+        case NEURON_ISRU: {                                               // 233-240
+            const double r = 1.0 / std::sqrt(1.0 + z * z);
+            a = z * r;
+            d = r * r * r;
+        } return;
+        case NEURON_ARCTAN:                                               // 258-264
+            a = (2.0 / PI) * std::atan(z);
+            d = (2.0 / PI) / (1.0 + z * z);
+            return;
+        case NEURON_SOFTSIGN: {                                           // 281-288
+            const double r = 1.0 / (1.0 + std::fabs(z));
+            a = z * r;
+            d = r * r;
+        } return;
+        case NEURON_RELU: a = z > 0.0 ? z : 0.0; d = z > 0.0 ? 1.0 : 0.0; return;  // 309-321
+        case NEURON_SOFTPLUS:                                             // 350-366, shifted by -ln 2
+            if (z >= 0.0) {
+                const double e = std::exp(-z);
+                a = z + logp1(e) - LOG2;
+                d = 1.0 / (1.0 + e);
+            } else {
+                const double e = std::exp(z);
+                a = logp1(e) - LOG2;
+                d = e / (1.0 + e);
+            }
+            return;
+        case NEURON_ELU:                                                  // 397-409
+            if (z >= 0.0) {
+                a = z;
+                d = 1.0;
+            } else {
+                const double e = std::exp(z);
+                a = e - 1.0;
+                d = e;
+            }
+            return;
+        case NEURON_GELU:                                                 // 446-469: z sigmoid(1.702 z)
+        case NEURON_SWISH: {                                              // 510-525: z sigmoid(z)
+            const double X = (n == NEURON_GELU) ? 1.702 : 1.0;
+            if (z > 0.0) {
+                const double e = std::exp(-X * z), den = 1.0 / (1.0 + e);
+                a = z * den;
+                d = (1.0 + e * (1.0 + X * z)) * den * den;
+            } else {
+                const double e = std::exp(X * z), den = 1.0 / (1.0 + e);
+                a = z * e * den;
+                d = e * (X * z + 1.0 + e) * den * den;
+            }
+        } return;
+        case NEURON_MISH:                                                 // 557-572
+            if (z > 0.0) {
+                const double e = std::exp(-z), den = 1.0 / (2.0 * e * (e + 1.0) + 1.0);
+                a = z * (2.0 * e + 1.0) * den;
+                d = (1.0 + e * (4.0 + e * (4.0 * z + 6.0 + e * 4.0 * (z + 1.0)))) * den * den;
+            } else {
+                const double e = std::exp(z), den = 1.0 / (e * (e + 2.0) + 2.0);
+                a = z * e * (e + 2.0) * den;
+                d = e * (e * (e * (e + 4.0) + 4.0 * z + 6.0) + 4.0 * (z + 1.0)) * den * den;
+            }
+            return;
+        case NEURON_TANHRE:                                               // 602-614
+            if (z > 0) {
+                a = z;
+                d = 1.0;
+            } else {
+                a = std::tanh(z);
+                d = 1.0 - a * a;
+            }
+            return;
+        case NEURON_LOGCOSH:                                              // 639-652, shifted by -ln 2 / 2
+            if (z >= 0.0) {
+                const double e = std::exp(-2.0 * z);
+                a = z + 0.5 * (logp1(e) - LOG2);
+                d = 1.0 / (e + 1.0);
+            } else {
+                const double e = std::exp(2.0 * z);
+                a = 0.5 * (logp1(e) - LOG2);
+                d = e / (e + 1.0);
+            }
+            return;
+        case NEURON_IQUAD: {                                              // 678-685
+            const double r = 1.0 / std::sqrt(1.0 + z * z);
+            a = 0.5 * z * (1.0 + z * r);
+            d = 0.5 * (1.0 + z * (z * z + 2.0) * r * r * r);
+        } return;
+        case NEURON_IFABS:                                                // 706-720
+            if (z > 0) {
+                const double r = 1.0 / (1.0 + z);
+                a = 0.5 * z * (1.0 + z * r);
+                d = 1.0 - 0.5 * r * r;
+            } else {
+                const double r = 1.0 / (1.0 - z);
+                a = 0.5 * z * (1.0 + z * r);
+                d = 0.5 * r * r;
+            }
+            return;
+        case NEURON_SINSIG: {                                             // 743-754
+            const double e = std::exp(-z), den = 1.0 / (1.0 + e);
+            const double sv = std::sin(0.5 * PI * den), cv = std::cos(0.5 * PI * den);
+            a = z * sv;
+            d = sv + 0.5 * PI * z * e * cv * den * den;
+        } return;
+        default: throw std::invalid_argument("ANN: invalid neuron");
+    }
+}
 
 // ml/nn.cpp:969-1138 (resize), 1191-1233 (fp/fpbp), 1620-1628 (DODZ::grad)
 struct ANN {
@@ -408,14 +701,11 @@ struct ANN {
 
     // activation of layer l: hidden layers use `neuron`, the last layer is linear (nn.cpp:1134-1137)
     void act(int l, double z, double& a, double& d) const {
-        const bool last = (l == nlayer() - 1);
-        if (last || neuron == NEURON_LINEAR) {  // nn.cpp:144-148
+        if (l == nlayer() - 1) {
             a = z;
             d = 1.0;
-        } else {  // nn.cpp:209-216
-            const double t = std::tanh(z);
-            a = t;
-            d = 1.0 - t * t;
+        } else {
+            activate(neuron, z, a, d);
         }
     }
     // forward (fpbp: activations and derivatives); returns out[0]
@@ -539,12 +829,13 @@ static void read_basisR(LineReader& lr, BasisR& b) {
     const CutNorm nm = norm_read(lr.at(1, "BasisR norm"));
     const CutName cn = cut_read(lr.at(2, "BasisR cutoff"));
     const double rc = std::atof(lr.at(3, "BasisR rc").c_str());
-    const std::string phi = lr.at(4, "BasisR function");
+    const PhiR phi = phir_read(lr.at(4, "BasisR function"));
     const int n = std::atoi(lr.at(5, "BasisR size").c_str());
-    if (phi != "GAUSSIAN") throw std::invalid_argument("BasisR: unsupported radial function " + phi);
-    if (n < 0) throw std::invalid_argument("Basis::resize(int): invalid number of functions.");
     b.init(rc, cn, nm);
-    b.phi = PHIR_GAUSSIAN;
+    if (phi == PHIR_UNKNOWN)  // basis_radial.cpp:70
+        throw std::invalid_argument("BasisR(rc,cutoff::Name,cutoff::Norm,int,PhiRN): invalid radial function type");
+    if (n < 0) throw std::invalid_argument("Basis::resize(int): invalid number of functions.");
+    b.phi = phi;
     b.rs.resize(n);
     b.eta.resize(n);
     for (int i = 0; i < n; ++i) {
@@ -560,13 +851,14 @@ static void read_basisA(LineReader& lr, BasisA& b) {
     const CutNorm nm = norm_read(lr.at(1, "BasisA norm"));
     const CutName cn = cut_read(lr.at(2, "BasisA cutoff"));
     const double rc = std::atof(lr.at(3, "BasisA rc").c_str());
-    const std::string phi = lr.at(4, "BasisA function");
+    const PhiA phi = phia_read(lr.at(4, "BasisA function"));
     const int n = std::atoi(lr.at(5, "BasisA size").c_str());
     const int alpha = std::atoi(lr.at(6, "BasisA alpha").c_str());
-    if (phi != "GAUSS") throw std::invalid_argument("BasisA: unsupported angular function " + phi);
-    if (n < 0) throw std::invalid_argument("Basis::resize(int): invalid number of functions.");
     b.init(rc, cn, nm);
-    b.phi = PHIA_GAUSS;
+    if (phi == PHIA_UNKNOWN)  // basis_angular.cpp:62
+        throw std::invalid_argument("BasisA(rc,cutoff::Name,cutoff::Norm,int,PhiAN): invalid angular function type");
+    if (n < 0) throw std::invalid_argument("Basis::resize(int): invalid number of functions.");
+    b.phi = phi;
     b.eta.resize(n);
     b.zeta.resize(n);
     b.lambda.resize(n);
@@ -589,10 +881,8 @@ static void read_ann(LineReader& lr, ANN& nn) {
     for (int v : nodes)
         if (v <= 0) throw std::invalid_argument("ANN::resize: Invalid layer size.");
     lr.next_line("neuron");
-    const std::string neuron = lr.at(1, "neuron");
-    if (neuron == "TANH") nn.neuron = NEURON_TANH;
-    else if (neuron == "LINEAR") nn.neuron = NEURON_LINEAR;
-    else throw std::invalid_argument("ANN::read(FILE*,ANN&): Invalid neuron (unsupported: " + neuron + ")");
+    nn.neuron = neuron_read(lr.at(1, "neuron"));
+    if (nn.neuron == NEURON_UNKNOWN) throw std::invalid_argument("ANN::read(FILE*,ANN&): Invalid neuron.");  // nn.cpp:1376
     nn.nin = nodes[0];
     nn.nodes.assign(nodes.begin() + 1, nodes.end());
     if (nn.nodes.back() != 1) throw std::invalid_argument("ANN: the potential needs exactly one output");

@@ -23,7 +23,7 @@ def rel_f(F, Fo):
     return np.linalg.norm(F - Fo, axis=1).max() / np.linalg.norm(Fo, axis=1).max()
 
 
-@pytest.fixture(scope="module", params=[0, 1], ids=["triplets", "moments"])
+@pytest.fixture(scope="module", params=[0, 3], ids=["triplets", "moments"])
 def fused_models(potentials, request):
     out = {}
     for k in ("plain64", "plain4096", "mixed64x4096"):

@@ -26,9 +26,10 @@ def rel_f(F, Fo):
     return np.linalg.norm(F - Fo, axis=1).max() / np.linalg.norm(Fo, axis=1).max()
 
 
-@pytest.fixture(scope="module", params=[0, 1], ids=["triplets", "moments"])
+@pytest.fixture(scope="module", params=[0, 3], ids=["triplets", "moments"])
 def models(potentials, request):
-    """Every test runs with the G4 sums by triplet loops (0) and by the moment expansion (1)."""
+    """Every test runs with the G4 sums and force terms by triplet loops (0) and by the moment
+    expansion (3: descriptors and forces)."""
     out = {k: (NNP.read(v), ob.OracleModel(v)) for k, v in potentials.items() if k in ("plain64", "plain4096",
                                                                                       "mixed64x4096")}
     for m, _ in out.values():
