@@ -29,29 +29,29 @@ namespace atomnn {
 
 // launchers (symm.hip, net.hip, neigh.hip kernels)
 int grid_shape(int ne, int nc);
-size_t symm_lds(int shape, bool fp64, bool ns, int maxp, int ldg);
-size_t force_lds(int shape, bool fp64, bool ns, int maxp, int ldg);
+size_t symm_lds(int shape, bool fp64, bool ns, int lmaxp, int ldg);
+size_t force_lds(int shape, bool fp64, bool ns, int lmaxp, int ldg);
 size_t net_lds_bytes(const DevNet& n, bool fp64);
 template <typename T>
 hipError_t launch_symm(int shape, int ns, const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
                        const int* centres, int ncentres, const int* cnt, const int* meta, const double4* r, int maxp,
-                       T* G, int ldg, int use_mom, int ntypes, hipStream_t st);
+                       int lmaxp, T* G, int ldg, int use_mom, int ntypes, hipStream_t st);
 template <typename T>
 hipError_t launch_force(int shape, int ns, const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
                         const int* centres, int ncentres, const int* cnt, const int* meta, const double4* r, int maxp,
-                        const T* dEdG, int ldg, T* X, int use_mom, int ntypes, hipStream_t st);
+                        int lmaxp, const T* dEdG, int ldg, T* X, int use_mom, int ntypes, hipStream_t st);
 template <typename T>
 hipError_t launch_assemble(const DevStruc* S, const int* atom_struc, const int* cnt, const int* nbr_j, const int* meta,
                            const T* X, int maxp, int ntot, int all_images, double* F, hipStream_t st);
 hipError_t launch_net(bool fp64, const DevPlan* plan, const DevPlan& hplan, const void* W, const int* sp_off,
                       const int* spatoms, const void* G, int ldg, int vac, double* eatom, void* dEdG, hipStream_t st);
 hipError_t launch_neighbors(const NeighborArgs& a, hipStream_t st);
-size_t fused_lds(int shape, bool fp64, bool ns, int maxp, int ldg, int maxw, int dersum);
-size_t mom_lds(int shape, bool fp64, int maxp, int ntypes);
+size_t fused_lds(int shape, bool fp64, bool ns, int lmaxp, int ldg, int maxw, int dersum);
+size_t mom_lds(int shape, bool fp64, int lmaxp, int ntypes, int bits);
 template <typename T>
 hipError_t launch_fused(int shape, int ns, int nblocks, const DevPlan* plan, const double* rfe, const double* rfr, const T* Wt,
                         const int* atom_type, const int* order, int ntot, int* work, const int* cnt, const int* meta,
-                        const double4* r, int maxp, int ldg, int maxw, int dersum, int vac, int do_force,
+                        const double4* r, int maxp, int lmaxp, int ldg, int maxw, int dersum, int vac, int do_force,
                         double* eatom, T* X, int use_mom, int ntypes, hipStream_t st);
 
 // Total energy per structure: one block per structure, fixed-order tree sum of E_i (FP64).
@@ -397,7 +397,10 @@ void Engine::build_neighbors(bool all_images) {
         HIPCHK(hipMemcpyAsync(&mc, d_maxcnt.p, 4, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
         max_entries = mc;
-        if (mc <= maxp) return;
+        if (mc <= maxp) {
+            lmaxp = std::min(maxp, std::max(8, (mc + 7) / 8 * 8));
+            return;
+        }
         maxp = (mc + 31) / 32 * 32;
     }
     throw std::runtime_error("neighbour list did not converge");
@@ -409,20 +412,20 @@ int Engine::mom_flag(size_t base_lds, bool fp64, int bits) {
     int any = 0;
     for (int t = 0; t < pb.plan.T; ++t) any |= pb.plan.mom_ok[t];
     const int use = moments & bits;
-    const int got = (use && any && base_lds + mom_lds(shape, fp64, maxp, pb.plan.T) <= 65536) ? use : 0;
+    const int got = (use && any && base_lds + mom_lds(shape, fp64, lmaxp, pb.plan.T, use) <= 65536) ? use : 0;
     last_mom |= got;
     return got;
 }
 
 template <typename T>
 void Engine::descriptors() {
-    const size_t lds = symm_lds(shape, sizeof(T) == 8, pb.plan.generic, maxp, pb.plan.ldg);
+    const size_t lds = symm_lds(shape, sizeof(T) == 8, pb.plan.generic, lmaxp, pb.plan.ldg);
     if (lds > 65536) throw unsupported_error("too many neighbours per atom for the descriptor kernel's LDS");
     const int mom = mom_flag(lds, sizeof(T) == 8, 1) ? 1 : 0;
     d_G.reserve((size_t)ntot * pb.plan.ldg * sizeof(T));
     HIPCHK(launch_symm<T>(shape, pb.plan.generic, (const DevPlan*)d_plan.p, (const double*)d_rfe.p, (const double*)d_rfr.p,
                           (const int*)d_type.p, (const int*)d_centres.p, ntot, (const int*)d_cnt.p,
-                          (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp, (T*)d_G.p, pb.plan.ldg, mom,
+                          (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp, lmaxp, (T*)d_G.p, pb.plan.ldg, mom,
                           pb.plan.T, stream));
 }
 
@@ -435,14 +438,14 @@ void Engine::networks(bool mixed, bool vac) {
 
 template <typename T>
 void Engine::forces() {
-    const size_t lds = force_lds(shape, sizeof(T) == 8, pb.plan.generic, maxp, pb.plan.ldg);
+    const size_t lds = force_lds(shape, sizeof(T) == 8, pb.plan.generic, lmaxp, pb.plan.ldg);
     if (lds > 65536) throw unsupported_error("too many neighbours per atom for the force kernel's LDS");
     const int mom = mom_flag(lds, sizeof(T) == 8, 2) ? 1 : 0;
     d_X.reserve((size_t)ntot * maxp * 4 * sizeof(T));
     d_F.reserve((size_t)ntot * 3 * 8);
     HIPCHK(launch_force<T>(shape, pb.plan.generic, (const DevPlan*)d_plan.p, (const double*)d_rfe.p, (const double*)d_rfr.p,
                            (const int*)d_type.p, (const int*)d_centres.p, ntot, (const int*)d_cnt.p,
-                           (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp, (const T*)d_dEdG.p, pb.plan.ldg,
+                           (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp, lmaxp, (const T*)d_dEdG.p, pb.plan.ldg,
                            (T*)d_X.p, mom, pb.plan.T, stream));
     HIPCHK(launch_assemble<T>((const DevStruc*)d_strucs.p, (const int*)d_astruc.p, (const int*)d_cnt.p,
                               (const int*)d_nbrj.p, (const int*)d_meta.p, (const T*)d_X.p, maxp, ntot,
@@ -498,7 +501,7 @@ void Engine::net_dims(int& maxw, int& dersum) const {
 void Engine::fused(bool mixed, uint32_t flags, int nblocks) {
     int maxw, dersum;
     net_dims(maxw, dersum);
-    const size_t lds = fused_lds(shape, !mixed, pb.plan.generic, maxp, pb.plan.ldg, maxw, dersum);
+    const size_t lds = fused_lds(shape, !mixed, pb.plan.generic, lmaxp, pb.plan.ldg, maxw, dersum);
     if (lds > 65536) throw unsupported_error("too many neighbours per atom for the fused kernel's LDS");
     const int mom = mom_flag(lds, !mixed, (flags & NNP_FORCE) ? 3 : 1);
     d_eatom.reserve((size_t)ntot * 8);
@@ -509,13 +512,13 @@ void Engine::fused(bool mixed, uint32_t flags, int nblocks) {
         HIPCHK(launch_fused<float>(shape, pb.plan.generic, nblocks, (const DevPlan*)d_plan.p, (const double*)d_rfe.p,
                                    (const double*)d_rfr.p, (const float*)d_w32.p, (const int*)d_type.p,
                                    (const int*)d_spatoms.p, ntot, (int*)d_work.p, (const int*)d_cnt.p,
-                                   (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp, pb.plan.ldg, maxw, dersum,
+                                   (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp, lmaxp, pb.plan.ldg, maxw, dersum,
                                    vac, frc, (double*)d_eatom.p, (float*)d_X.p, mom, pb.plan.T, stream));
     else
         HIPCHK(launch_fused<double>(shape, pb.plan.generic, nblocks, (const DevPlan*)d_plan.p, (const double*)d_rfe.p,
                                     (const double*)d_rfr.p, (const double*)d_w64.p, (const int*)d_type.p,
                                     (const int*)d_spatoms.p, ntot, (int*)d_work.p, (const int*)d_cnt.p,
-                                    (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp, pb.plan.ldg, maxw, dersum,
+                                    (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp, lmaxp, pb.plan.ldg, maxw, dersum,
                                     vac, frc, (double*)d_eatom.p, (double*)d_X.p, mom, pb.plan.T, stream));
 }
 
@@ -615,6 +618,8 @@ void Engine::resident_load(const std::vector<nnp_structure>& ss, uint32_t flags)
         maxp = want;
         build_neighbors(flags & NNP_ALL_IMAGES);
     }
+    // LDS rows with a little head room over the realised maximum (positions may move)
+    lmaxp = std::min(maxp, (max_entries + max_entries / 16 + 4 + 7) / 8 * 8);
     std::vector<int> st(nstruc + 1, 0);
     for (int k = 0; k < nstruc; ++k) st[k + 1] = st[k] + ss[k].natoms;
     d_sstart.reserve(st.size() * 4);
@@ -639,14 +644,15 @@ void Engine::resident_run(uint32_t flags) {
         enqueue_pass(flags);
         return;
     }
-    if (graph_exec && graph_flags == flags && graph_maxp == maxp) {
+    if (graph_exec && graph_flags == flags && graph_maxp == maxp && graph_lmaxp == lmaxp) {
         HIPCHK(hipGraphLaunch(graph_exec, stream));
         return;
     }
-    if (eager_runs == 0 || graph_flags != flags || graph_maxp != maxp) {
+    if (eager_runs == 0 || graph_flags != flags || graph_maxp != maxp || graph_lmaxp != lmaxp) {
         drop_graph();
         graph_flags = flags;
         graph_maxp = maxp;
+        graph_lmaxp = lmaxp;
         eager_runs = 1;
         enqueue_pass(flags);
         return;
@@ -729,7 +735,7 @@ void Engine::enqueue_pass(uint32_t flags) {
     stage_end(ST_NET);
     if (flags & NNP_FORCE) {
         stage_begin(ST_FORCE);
-        const size_t lds = force_lds(shape, !mixed, pb.plan.generic, maxp, pb.plan.ldg);
+        const size_t lds = force_lds(shape, !mixed, pb.plan.generic, lmaxp, pb.plan.ldg);
         if (lds > 65536) throw unsupported_error("too many neighbours per atom for the force kernel's LDS");
         const int mom = mom_flag(lds, !mixed, 2) ? 1 : 0;
         d_X.reserve((size_t)ntot * maxp * 4 * (mixed ? 4 : 8));
@@ -737,13 +743,13 @@ void Engine::enqueue_pass(uint32_t flags) {
         if (mixed)
             HIPCHK(launch_force<float>(shape, pb.plan.generic, (const DevPlan*)d_plan.p, (const double*)d_rfe.p, (const double*)d_rfr.p,
                                        (const int*)d_type.p, (const int*)d_centres.p, ntot, (const int*)d_cnt.p,
-                                       (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp,
+                                       (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp, lmaxp,
                                        (const float*)d_dEdG.p, pb.plan.ldg, (float*)d_X.p, mom, pb.plan.T, stream));
         else
             HIPCHK(launch_force<double>(shape, pb.plan.generic, (const DevPlan*)d_plan.p, (const double*)d_rfe.p,
                                         (const double*)d_rfr.p, (const int*)d_type.p, (const int*)d_centres.p, ntot,
                                         (const int*)d_cnt.p, (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp,
-                                        (const double*)d_dEdG.p, pb.plan.ldg, (double*)d_X.p, mom, pb.plan.T,
+                                        lmaxp, (const double*)d_dEdG.p, pb.plan.ldg, (double*)d_X.p, mom, pb.plan.T,
                                         stream));
         stage_end(ST_FORCE);
         stage_begin(ST_ASSEMBLE);
@@ -768,10 +774,13 @@ void Engine::resident_fetch(uint32_t flags, nnp_result* res) {
     HIPCHK(hipMemcpyAsync(&mc, d_maxcnt.p, 4, hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
     profile_collect();
-    if (mc > maxp) {  // the rows overflowed: rebuild them larger and redo the pass
-        maxp = (mc + mc / 8 + 8 + 31) / 32 * 32;
+    if (mc > lmaxp) {  // the rows (or their LDS staging) overflowed: enlarge and redo the pass
+        if (mc > maxp) {
+            maxp = (mc + mc / 8 + 8 + 31) / 32 * 32;
+            build_neighbors(flags & NNP_ALL_IMAGES);
+        }
+        lmaxp = std::min(maxp, (mc + mc / 16 + 4 + 7) / 8 * 8);
         drop_graph();
-        build_neighbors(flags & NNP_ALL_IMAGES);
         enqueue_pass(flags);
         HIPCHK(hipStreamSynchronize(stream));
         profile_collect();

@@ -122,6 +122,7 @@ class Engine {
     hipGraphExec_t graph_exec = nullptr;
     uint32_t graph_flags = 0;
     int graph_maxp = 0;
+    int graph_lmaxp = 0;
     int eager_runs = 0;
     void drop_graph();
     void enqueue_pass(uint32_t flags);
@@ -134,7 +135,8 @@ class Engine {
     PlanBuffers pb;
     int shape = 0;
     hipStream_t stream = nullptr;
-    int maxp = 0;
+    int maxp = 0;   // neighbour row stride (entries per centre, multiple of 32)
+    int lmaxp = 0;  // entries the per-centre kernels stage in LDS (multiple of 8, <= maxp)
     int max_entries = 0;
     int ntot = 0, nstruc = 0, nbins = 0;
     bool all_images_ = false;

@@ -139,8 +139,9 @@ struct DevPlan {
     // moment path per centre species: every angular block has integer zeta in [0, 4], |lambda| <= 1,
     // and all blocks share one exponent table, alpha and cutoff (see symm.hip)
     int mom_ok[kMaxSpecies];
-    // the model uses a radial function other than GAUSSIAN, an angular factor other than GAUSS
-    // or an activation other than tanh / linear: the kernels' generic instantiations run it
+    // the model uses a radial function other than GAUSSIAN, an angular factor other than GAUSS,
+    // a non-integer zeta or an activation other than tanh / linear: the kernels' generic
+    // instantiations run it
     int generic;
     int nG[kMaxSpecies];
     int nR[kMaxSpecies];

@@ -71,33 +71,45 @@ __device__ __forceinline__ void monomials(T ux, T uy, T uz, T* m) {
     for (int q = 0; q < kMomK; ++q) m[q] = px[kMono[q][0]] * py[kMono[q][1]] * pz[kMono[q][2]];
 }
 
-// LDS of the moment path, sizes in elements of T
+// LDS of the moment path, sizes in elements of T.  Descriptors: Mom [S][NE][K] | D [S][NE] |
+// S table [blocks][NE][5] (the monomial table lives in chunks in the kernel's reduction buffer).
+// Forces: mono [K][lmaxp+1] | Mom | D | w table [blocks][NE][5] | L, Lp [S][NE][K] | Lx, Ly, Lz
+// [S][NE][20].
 template <int NE>
-__host__ __device__ constexpr int mom_lds_elems(int maxp, int ntypes_max) {
-    // mono [K][maxp+1] | Mom [S][NE][K] | D [S][NE] | tab [blocks][NE][5] | L,Lp [S][NE][K] | Lx,Ly,Lz [S][NE][20]
-    return kMomK * (maxp + 1) + ntypes_max * NE * kMomK + ntypes_max * NE +
-           (ntypes_max * (ntypes_max + 1) / 2) * NE * 5 + 2 * ntypes_max * NE * kMomK + 3 * ntypes_max * NE * kMomK3;
+__host__ __device__ constexpr int mom_symm_elems(int ntypes_max) {
+    return ntypes_max * NE * kMomK + ntypes_max * NE + (ntypes_max * (ntypes_max + 1) / 2) * NE * 5;
+}
+template <int NE>
+__host__ __device__ constexpr int mom_lds_elems(int lmaxp, int ntypes_max) {
+    return kMomK * (lmaxp + 1) + mom_symm_elems<NE>(ntypes_max) + 2 * ntypes_max * NE * kMomK +
+           3 * ntypes_max * NE * kMomK3;
 }
 
-// Monomial table of the staged entries and the per-species moments M and diagonals D.
-// Rf holds R_e(p) for every entry p (radial_tables over [0, P)).
+// Per-species moments M and diagonals D of the staged entries (Rf holds R_e(p) for every entry,
+// radial_tables over [0, P)).  The monomial table mono [K][ch + 1] holds `ch` entries at a time.
 template <typename T, int NE>
-__device__ __forceinline__ void build_moments(const T* su, const T* Rf, int P, int maxp, int ntypes, const int* seg,
+__device__ __forceinline__ void build_moments(const T* su, const T* Rf, int P, int ch, int ntypes, const int* seg,
                                               T* mono, T* Mom, T* D, int lane) {
-    const int ms = maxp + 1;
-    for (int p = lane; p < P; p += kWave) {
-        T m[kMomK];
-        monomials<T>(su[4 * p], su[4 * p + 1], su[4 * p + 2], m);
-#pragma unroll
-        for (int q = 0; q < kMomK; ++q) mono[q * ms + p] = m[q];
-    }
-    __syncthreads();
+    const int ms = ch + 1;
     const int items = ntypes * NE * kMomK;
-    for (int it = lane; it < items; it += kWave) {
-        const int q = it % kMomK, e = (it / kMomK) % NE, s = it / (kMomK * NE);
-        T acc = T(0);
-        for (int p = seg[s]; p < seg[s + 1]; ++p) acc += mono[q * ms + p] * Rf[p * NE + e];
-        Mom[it] = acc;  // [s][e][q]
+    for (int it = lane; it < items; it += kWave) Mom[it] = T(0);
+    for (int c0 = 0; c0 < P; c0 += ch) {
+        const int c1 = min(P, c0 + ch);
+        __syncthreads();
+        for (int p = c0 + lane; p < c1; p += kWave) {
+            T m[kMomK];
+            monomials<T>(su[4 * p], su[4 * p + 1], su[4 * p + 2], m);
+#pragma unroll
+            for (int q = 0; q < kMomK; ++q) mono[q * ms + (p - c0)] = m[q];
+        }
+        __syncthreads();
+        for (int it = lane; it < items; it += kWave) {
+            const int q = it % kMomK, e = (it / kMomK) % NE, s = it / (kMomK * NE);
+            const int lo = max(seg[s], c0), hi = min(seg[s + 1], c1);
+            T acc = Mom[it];
+            for (int p = lo; p < hi; ++p) acc += mono[q * ms + (p - c0)] * Rf[p * NE + e];
+            Mom[it] = acc;  // [s][e][q]
+        }
     }
     for (int it = lane; it < ntypes * NE; it += kWave) {
         const int e = it % NE, s = it / NE;
@@ -207,7 +219,7 @@ __device__ __forceinline__ void force_angular_moments(const DevPlan* __restrict_
     }
     __syncthreads();
     // entries: X_j -= dE/dr_j (angular part)
-    const int ms = maxp + 1;
+    const int ms = maxp + 1;  // the monomial table holds every entry
     for (int p = lane; p < P; p += kWave) {
         int s = 0;
         while (s + 1 < ntypes && p >= seg[s + 1]) ++s;
@@ -241,9 +253,9 @@ __device__ __forceinline__ void force_angular_moments(const DevPlan* __restrict_
             gy += a1 * uy + a2 * hy;
             gz += a1 * uz + a2 * hz;
         }
-        Xs[4 * p] -= gx;
-        Xs[4 * p + 1] -= gy;
-        Xs[4 * p + 2] -= gz;
+        Xs[3 * p] -= gx;
+        Xs[3 * p + 1] -= gy;
+        Xs[3 * p + 2] -= gz;
     }
 }
 

@@ -68,27 +68,57 @@ __device__ __forceinline__ T ipow(T x, int n) {
     return n > 0 ? yy : T(1) / yy;
 }
 
+// x^n for a wave-uniform n >= 0: straight-line products for the usual small exponents (the
+// same products repeated squaring forms), the loop beyond.
+template <typename T>
+__device__ __forceinline__ T ipow_u(T x, int n) {
+    switch (n) {
+        case 0: return T(1);
+        case 1: return x;
+        case 2: return x * x;
+        case 3: return (x * x) * x;
+        case 4: {
+            const T y = x * x;
+            return y * y;
+        }
+        case 5: {
+            const T y = x * x;
+            return x * (y * y);
+        }
+        case 7: {
+            const T y = x * x;
+            return (x * y) * (y * y);
+        }
+        case 8: {
+            const T y = x * x, z = y * y;
+            return z * z;
+        }
+        default: return ipow(x, n);
+    }
+}
+
 template <typename T>
 __device__ __forceinline__ T rpow(T x, T e) {
     return pow(x, e);
 }
 
-// |(1 + lambda cos)/2|^zeta (and the derivative factor zeta lambda |.|^(zeta-1) / 2)
-template <typename T>
+// |(1 + lambda cos)/2|^zeta (and the derivative factor zeta lambda |.|^(zeta-1) / 2).  The
+// non-generic kernels only see integer zeta (zint >= 0): the general pow stays out of them.
+template <typename T, bool GEN>
 __device__ __forceinline__ void angular_term(T cs, T lam, T zeta, int zint, T& A, T& dA) {
     const T acw = fabs(T(0.5) * (T(1) + lam * cs));
     T p1;
-    if (zint >= 1) p1 = ipow(acw, zint - 1);
-    else if (zint == 0) p1 = T(0);  // zeta = 0: A = 1, dA = 0
+    if (zint >= 1) p1 = ipow_u(acw, zint - 1);
+    else if (!GEN || zint == 0) p1 = T(0);  // zeta = 0: A = 1, dA = 0
     else p1 = rpow(acw, zeta - T(1));
     A = (zint == 0) ? T(1) : p1 * acw;
     dA = T(0.5) * zeta * lam * p1;
 }
 
-template <typename T>
+template <typename T, bool GEN>
 __device__ __forceinline__ T angular_value(T cs, T lam, T zeta, int zint) {
     const T acw = fabs(T(0.5) * (T(1) + lam * cs));
-    return zint >= 0 ? ipow(acw, zint) : rpow(acw, zeta);
+    return (!GEN || zint >= 0) ? ipow_u(acw, zint) : rpow(acw, zeta);
 }
 
 // Row index of flat pair q in the strict upper triangle of an n x n matrix (row-major).
@@ -277,13 +307,14 @@ __device__ __forceinline__ void symm_body(const DevPlan* __restrict__ plan, cons
 
     // angular blocks: moment expansion when the centre species allows it
     if (mom && plan->mom_ok[ti]) {
-        T* mono = mom;
-        T* Mom = mono + kMomK * (maxp + 1);
+        // the monomial table goes through the reduction buffer `red` in chunks of `ch` entries
+        constexpr int ch = kWave * NE * NC / kMomK - 1;
+        T* Mom = mom;
         T* D = Mom + ntypes * NE * kMomK;
         T* S = D + ntypes * NE;
         radial_tables<T, NE, GEN>(plan->ang[ti][0], su, 0, P, Rf, nullptr, lane);
         __syncthreads();
-        build_moments<T, NE>(su, Rf, P, maxp, ntypes, sg.start, mono, Mom, D, lane);
+        build_moments<T, NE>(su, Rf, P, ch, ntypes, sg.start, red, Mom, D, lane);
         symm_angular_moments<T, NE>(plan, ti, ntypes, Mom, D, S, Grow, lane);
         __syncthreads();
         return;
@@ -345,7 +376,7 @@ __device__ __forceinline__ void symm_body(const DevPlan* __restrict__ plan, cons
                         const T cs = ujx * su[4 * pk] + ujy * su[4 * pk + 1] + ujz * su[4 * pk + 2];
                         T Ac[NC];
 #pragma unroll
-                        for (int c = 0; c < NC; ++c) Ac[c] = angular_value<T>(cs, lam[c], zet[c], zin[c]);
+                        for (int c = 0; c < NC; ++c) Ac[c] = angular_value<T, GEN>(cs, lam[c], zet[c], zin[c]);
                         const T fcjk = nsb ? fcj * nsx[2 * pk] : T(0);
 #pragma unroll
                         for (int e = 0; e < NE; ++e) {
@@ -417,9 +448,9 @@ __device__ __forceinline__ void force_body(const DevPlan* __restrict__ plan, con
             }
         }
         s *= T(R.norm);
-        Xs[4 * p + 0] = -s * su[4 * p];
-        Xs[4 * p + 1] = -s * su[4 * p + 1];
-        Xs[4 * p + 2] = -s * su[4 * p + 2];
+        Xs[3 * p + 0] = -s * su[4 * p];
+        Xs[3 * p + 1] = -s * su[4 * p + 1];
+        Xs[3 * p + 2] = -s * su[4 * p + 2];
     }
     __syncthreads();
 
@@ -435,7 +466,7 @@ __device__ __forceinline__ void force_body(const DevPlan* __restrict__ plan, con
         T* Lz = Ly + ntypes * NE * kMomK3;
         radial_tables<T, NE, GEN>(plan->ang[ti][0], su, 0, P, Rf, Rd, lane);
         __syncthreads();
-        build_moments<T, NE>(su, Rf, P, maxp, ntypes, sg.start, mono, Mom, D, lane);
+        build_moments<T, NE>(su, Rf, P, maxp, ntypes, sg.start, mono, Mom, D, lane);  // whole table
         force_angular_moments<T, NE>(plan, ti, ntypes, P, maxp, su, Rf, Rd, W, sg.start, mono, Mom, wtab, Lam, Lamp,
                                      Lx, Ly, Lz, Xs, lane);
         __syncthreads();
@@ -464,8 +495,9 @@ __device__ __forceinline__ void force_body(const DevPlan* __restrict__ plan, con
 #pragma unroll
                 for (int c = 0; c < NC; ++c) {
                     const int f = A.fmap[e * kMaxC + c];
-                    w[e][c] = f >= 0 ? dEdGi[f] * T(A.norm) : T(0);
+                    w[e][c] = f >= 0 ? dEdGi[f] : T(0);  // the block norm scales each row's sum
                 }
+            const T bnorm = T(A.norm);
             T lam[NC], zet[NC];
             int zin[NC];
 #pragma unroll
@@ -537,7 +569,7 @@ __device__ __forceinline__ void force_body(const DevPlan* __restrict__ plan, con
                                     tq += w[e][cc] * pd[e];
                                 }
                                 T Av, dAv;
-                                angular_term<T>(cs, lam[cc], zet[cc], zin[cc], Av, dAv);
+                                angular_term<T, GEN>(cs, lam[cc], zet[cc], zin[cc], Av, dAv);
                                 dEdcos += dAv * tp;
                                 dEdd += Av * tq;
                             }
@@ -546,16 +578,19 @@ __device__ __forceinline__ void force_body(const DevPlan* __restrict__ plan, con
                             xy -= dEdd * ujy + g * (uky - cs * ujy);
                             xz -= dEdd * ujz + g * (ukz - cs * ujz);
                         }
+                        xx *= bnorm;
+                        xy *= bnorm;
+                        xz *= bnorm;
                         if (whole) {
-                            Xs[4 * pj] += xx;
-                            Xs[4 * pj + 1] += xy;
-                            Xs[4 * pj + 2] += xz;
+                            Xs[3 * pj] += xx;
+                            Xs[3 * pj + 1] += xy;
+                            Xs[3 * pj + 2] += xz;
                         } else {
                             const int sl = (q == q0) ? 0 : 1;
                             slot_row[sl * kWave + lane] = pj;
-                            slots[(sl * kWave + lane) * 4 + 0] = xx;
-                            slots[(sl * kWave + lane) * 4 + 1] = xy;
-                            slots[(sl * kWave + lane) * 4 + 2] = xz;
+                            slots[(sl * kWave + lane) * 3 + 0] = xx;
+                            slots[(sl * kWave + lane) * 3 + 1] = xy;
+                            slots[(sl * kWave + lane) * 3 + 2] = xz;
                         }
                         q += n;
                         ++r;
@@ -572,9 +607,9 @@ __device__ __forceinline__ void force_body(const DevPlan* __restrict__ plan, con
 #pragma unroll
                         for (int sl = 0; sl < 2; ++sl) {
                             if (slot_row[sl * kWave + l] == pj) {
-                                Xs[4 * pj] += slots[(sl * kWave + l) * 4 + 0];
-                                Xs[4 * pj + 1] += slots[(sl * kWave + l) * 4 + 1];
-                                Xs[4 * pj + 2] += slots[(sl * kWave + l) * 4 + 2];
+                                Xs[3 * pj] += slots[(sl * kWave + l) * 3 + 0];
+                                Xs[3 * pj + 1] += slots[(sl * kWave + l) * 3 + 1];
+                                Xs[3 * pj + 2] += slots[(sl * kWave + l) * 3 + 2];
                             }
                         }
                     }
@@ -587,9 +622,10 @@ __device__ __forceinline__ void force_body(const DevPlan* __restrict__ plan, con
 
 // LDS sizes (bytes) of the kernels for a given maxp.
 // Each kernel's LDS: its own arrays, then (moment path) the moment region of mom_lds_elems.
-// GEN: the kernels for models beyond GAUSSIAN / GAUSS / tanh (other radial functions, angular
-// distance factors and activations); they add the per-entry cutoff table nsx [maxp][2] that the
-// non-separable angular factors use.  GEN = false keeps the common case's register budget.
+// GEN: the kernels for models beyond GAUSSIAN / GAUSS / tanh with integer zeta (other radial
+// functions, angular distance factors, activations, non-integer zeta); they add the per-entry
+// cutoff table nsx [maxp][2] that the non-separable angular factors use.  GEN = false keeps the
+// common case's register budget.
 template <typename T, int NE, int NC, bool GEN>
 __host__ __device__ constexpr size_t symm_lds_bytes(int maxp, int ldg) {
     return sizeof(T) * (size_t)(4 * maxp + NE * maxp + kWave * NE * NC) + sizeof(T) * (size_t)((ldg + 1) & ~1) +
@@ -597,12 +633,13 @@ __host__ __device__ constexpr size_t symm_lds_bytes(int maxp, int ldg) {
 }
 template <typename T, int NE, int NC, bool GEN>
 __host__ __device__ constexpr size_t force_lds_bytes(int maxp, int ldg) {
-    return sizeof(T) * (size_t)(4 * maxp + 2 * NE * maxp + 4 * maxp + 8 * kWave + ((ldg + 1) & ~1)) +
+    return sizeof(T) * (size_t)(4 * maxp + 2 * NE * maxp + 3 * maxp + 6 * kWave + ((ldg + 1) & ~1)) +
            sizeof(int) * (size_t)(maxp + ((2 * kWave + 3) & ~3)) + (GEN ? sizeof(T) * 2 * (size_t)maxp : 0);
 }
+// bits: 1 descriptors, 2 forces (the fused kernel's region serves both phases)
 template <typename T, int NE>
-__host__ __device__ constexpr size_t mom_lds_bytes(int maxp, int ntypes) {
-    return sizeof(T) * (size_t)mom_lds_elems<NE>(maxp, ntypes);
+__host__ __device__ constexpr size_t mom_lds_bytes(int lmaxp, int ntypes, int bits = 3) {
+    return sizeof(T) * (size_t)((bits & 2) ? mom_lds_elems<NE>(lmaxp, ntypes) : mom_symm_elems<NE>(ntypes));
 }
 
 // ------------------------------------------------------------------ descriptors
@@ -610,66 +647,66 @@ template <typename T, int NE, int NC, bool GEN>
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <= 16 ? 3 : 1, 8))) void k_symm(
     const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta, const double* __restrict__ rfun_rs,
     const int* __restrict__ atom_type, const int* __restrict__ centres, const int* __restrict__ cnt,
-    const int* __restrict__ nbr_meta, const double4* __restrict__ nbr_r, int maxp, T* __restrict__ G, int ldg,
-    int use_mom) {
+    const int* __restrict__ nbr_meta, const double4* __restrict__ nbr_r, int maxp, int lmaxp, T* __restrict__ G,
+    int ldg, int use_mom) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     T* su = (T*)lds_raw;
-    T* Rf = su + 4 * maxp;
-    T* red = Rf + NE * maxp;
+    T* Rf = su + 4 * lmaxp;
+    T* red = Rf + NE * lmaxp;
     T* Grow = red + kWave * NE * NC;
     T* nsx = Grow + ((ldg + 1) & ~1);
-    T* mom = use_mom ? nsx + (GEN ? 2 * maxp : 0) : nullptr;
+    T* mom = use_mom ? nsx + (GEN ? 2 * lmaxp : 0) : nullptr;
     const int i = centres[blockIdx.x];
     const int lane = threadIdx.x;
     const int ti = atom_type[i];
-    const int P = min(cnt[i], maxp);
+    const int P = min(cnt[i], lmaxp);
     const int ntypes = plan->T;
     __shared__ int seg_lds[kMaxSpecies + 1];
     Segs sg{seg_lds};
     stage_entries<T>(i, P, maxp, ntypes, nbr_meta, nbr_r, su, nullptr, sg, lane);
     for (int k = lane; k < ldg; k += kWave) Grow[k] = T(0);
     __syncthreads();
-    symm_body<T, NE, NC, GEN>(plan, rfun_eta, rfun_rs, ti, ntypes, P, maxp, su, Rf, red, Grow, sg, nsx, mom, lane);
+    symm_body<T, NE, NC, GEN>(plan, rfun_eta, rfun_rs, ti, ntypes, P, lmaxp, su, Rf, red, Grow, sg, nsx, mom, lane);
     __syncthreads();
     for (int k = lane; k < ldg; k += kWave) G[(int64_t)i * ldg + k] = Grow[k];
 }
 
 // ------------------------------------------------------------------ force terms
 template <typename T, int NE, int NC, bool GEN>
-__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <= 16 ? 3 : 1, 8))) void k_force(
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <= 16 ? (GEN ? 3 : 4) : 1, 8))) void k_force(
     const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta, const double* __restrict__ rfun_rs,
     const int* __restrict__ atom_type, const int* __restrict__ centres, const int* __restrict__ cnt,
-    const int* __restrict__ nbr_meta, const double4* __restrict__ nbr_r, int maxp, const T* __restrict__ dEdG,
-    int ldg, T* __restrict__ Xout, int use_mom) {
+    const int* __restrict__ nbr_meta, const double4* __restrict__ nbr_r, int maxp, int lmaxp,
+    const T* __restrict__ dEdG, int ldg, T* __restrict__ Xout, int use_mom) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     T* su = (T*)lds_raw;
-    T* Rf = su + 4 * maxp;
-    T* Rd = Rf + NE * maxp;
-    T* Xs = Rd + NE * maxp;           // [maxp][4]
-    T* slots = Xs + 4 * maxp;         // [2][kWave][4]
-    T* W = slots + 8 * kWave;         // dE/dG row
+    T* Rf = su + 4 * lmaxp;
+    T* Rd = Rf + NE * lmaxp;
+    T* Xs = Rd + NE * lmaxp;          // [lmaxp][3]
+    T* slots = Xs + 3 * lmaxp;        // [2][kWave][3]
+    T* W = slots + 6 * kWave;         // dE/dG row
     int* orig = (int*)(W + ((ldg + 1) & ~1));
-    int* slot_row = orig + maxp;      // [2][kWave]
+    int* slot_row = orig + lmaxp;     // [2][kWave]
     T* nsx = (T*)(slot_row + ((2 * kWave + 3) & ~3));
-    T* mom = use_mom ? nsx + (GEN ? 2 * maxp : 0) : nullptr;
+    T* mom = use_mom ? nsx + (GEN ? 2 * lmaxp : 0) : nullptr;
     const int i = centres[blockIdx.x];
     const int lane = threadIdx.x;
     const int ti = atom_type[i];
-    const int P = min(cnt[i], maxp);
+    const int P = min(cnt[i], lmaxp);
     const int ntypes = plan->T;
     __shared__ int seg_lds[kMaxSpecies + 1];
     Segs sg{seg_lds};
     stage_entries<T>(i, P, maxp, ntypes, nbr_meta, nbr_r, su, orig, sg, lane);
     for (int k = lane; k < ldg; k += kWave) W[k] = dEdG[(int64_t)i * ldg + k];
     __syncthreads();
-    force_body<T, NE, NC, GEN>(plan, rfun_eta, rfun_rs, ti, ntypes, P, maxp, su, Rf, Rd, Xs, slots, slot_row, W,
+    force_body<T, NE, NC, GEN>(plan, rfun_eta, rfun_rs, ti, ntypes, P, lmaxp, su, Rf, Rd, Xs, slots, slot_row, W,
                               dEdG + (int64_t)i * ldg, sg, nsx, mom, lane);
     const int64_t row = (int64_t)i * maxp;
     for (int p = lane; p < P; p += kWave) {
         const int o = orig[p];
-        Xout[4 * (row + o) + 0] = Xs[4 * p];
-        Xout[4 * (row + o) + 1] = Xs[4 * p + 1];
-        Xout[4 * (row + o) + 2] = Xs[4 * p + 2];
+        Xout[4 * (row + o) + 0] = Xs[3 * p];
+        Xout[4 * (row + o) + 1] = Xs[3 * p + 1];
+        Xout[4 * (row + o) + 2] = Xs[3 * p + 2];
         Xout[4 * (row + o) + 3] = T(0);
     }
 }
@@ -746,7 +783,7 @@ __device__ __forceinline__ double net_body(const DevNet& N, const T* __restrict_
 
 template <typename T, int NE, int NC, bool GEN>
 __host__ __device__ constexpr size_t fused_lds_bytes(int maxp, int ldg, int maxw, int dersum) {
-    return sizeof(T) * (size_t)(4 * maxp + 2 * NE * maxp + 4 * maxp + 8 * kWave + kWave * NE * NC +
+    return sizeof(T) * (size_t)(4 * maxp + 2 * NE * maxp + 3 * maxp + 6 * kWave + kWave * NE * NC +
                                 2 * ((ldg + 1) & ~1) + 2 * maxw + dersum) +
            sizeof(int) * (size_t)(maxp + ((2 * kWave + 3) & ~3)) + (GEN ? sizeof(T) * 2 * (size_t)maxp : 0);
 }
@@ -760,24 +797,24 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
     const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta, const double* __restrict__ rfun_rs,
     const T* __restrict__ Wt, const int* __restrict__ atom_type, const int* __restrict__ order, int ntot,
     int* __restrict__ work, const int* __restrict__ cnt, const int* __restrict__ nbr_meta,
-    const double4* __restrict__ nbr_r, int maxp, int ldg, int maxw, int dersum, int vac, int do_force,
+    const double4* __restrict__ nbr_r, int maxp, int lmaxp, int ldg, int maxw, int dersum, int vac, int do_force,
     double* __restrict__ eatom, T* __restrict__ Xout, int use_mom) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     T* su = (T*)lds_raw;
-    T* Rf = su + 4 * maxp;
-    T* Rd = Rf + NE * maxp;
-    T* Xs = Rd + NE * maxp;
-    T* slots = Xs + 4 * maxp;
-    T* red = slots + 8 * kWave;
+    T* Rf = su + 4 * lmaxp;
+    T* Rd = Rf + NE * lmaxp;
+    T* Xs = Rd + NE * lmaxp;
+    T* slots = Xs + 3 * lmaxp;
+    T* red = slots + 6 * kWave;
     T* Grow = red + kWave * NE * NC;
     T* dErow = Grow + ((ldg + 1) & ~1);
     T* act0 = dErow + ((ldg + 1) & ~1);
     T* act1 = act0 + maxw;
     T* der = act1 + maxw;
     int* orig = (int*)(der + dersum);
-    int* slot_row = orig + maxp;
+    int* slot_row = orig + lmaxp;
     T* nsx = (T*)(slot_row + ((2 * kWave + 3) & ~3));
-    T* mom = use_mom ? nsx + (GEN ? 2 * maxp : 0) : nullptr;
+    T* mom = use_mom ? nsx + (GEN ? 2 * lmaxp : 0) : nullptr;
     __shared__ int seg_lds[kMaxSpecies + 1];
     __shared__ int next_atom;
     Segs sg{seg_lds};
@@ -791,24 +828,24 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
         if (idx >= ntot) break;
         const int i = order[idx];
         const int ti = atom_type[i];
-        const int P = min(cnt[i], maxp);
+        const int P = min(cnt[i], lmaxp);
         stage_entries<T>(i, P, maxp, ntypes, nbr_meta, nbr_r, su, orig, sg, lane);
         for (int k = lane; k < ldg; k += kWave) Grow[k] = T(0);
         __syncthreads();
-        symm_body<T, NE, NC, GEN>(plan, rfun_eta, rfun_rs, ti, ntypes, P, maxp, su, Rf, red, Grow, sg, nsx,
+        symm_body<T, NE, NC, GEN>(plan, rfun_eta, rfun_rs, ti, ntypes, P, lmaxp, su, Rf, red, Grow, sg, nsx,
                                   (use_mom & 1) ? mom : nullptr, lane);
         __syncthreads();
         const double E = net_body<T, GEN>(plan->net[ti], Wt, Grow, act0, act1, der, dErow, maxw, vac, lane);
         if (lane == 0) eatom[i] = E;
         if (do_force) {
-            force_body<T, NE, NC, GEN>(plan, rfun_eta, rfun_rs, ti, ntypes, P, maxp, su, Rf, Rd, Xs, slots,
+            force_body<T, NE, NC, GEN>(plan, rfun_eta, rfun_rs, ti, ntypes, P, lmaxp, su, Rf, Rd, Xs, slots,
                                        slot_row, dErow, dErow, sg, nsx, (use_mom & 2) ? mom : nullptr, lane);
             const int64_t row = (int64_t)i * maxp;
             for (int p = lane; p < P; p += kWave) {
                 const int o = orig[p];
-                Xout[4 * (row + o) + 0] = Xs[4 * p];
-                Xout[4 * (row + o) + 1] = Xs[4 * p + 1];
-                Xout[4 * (row + o) + 2] = Xs[4 * p + 2];
+                Xout[4 * (row + o) + 0] = Xs[3 * p];
+                Xout[4 * (row + o) + 1] = Xs[3 * p + 1];
+                Xout[4 * (row + o) + 2] = Xs[3 * p + 2];
                 Xout[4 * (row + o) + 3] = T(0);
             }
         }
@@ -819,20 +856,20 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
 template <typename T, int NE, int NC, bool GEN>
 static hipError_t launch_fused_t(int nblocks, const DevPlan* plan, const double* rfe, const double* rfr, const T* Wt,
                                  const int* atom_type, const int* order, int ntot, int* work, const int* cnt,
-                                 const int* meta, const double4* r, int maxp, int ldg, int maxw, int dersum, int vac,
+                                 const int* meta, const double4* r, int maxp, int lmaxp, int ldg, int maxw, int dersum, int vac,
                                  int do_force, double* eatom, T* X, int use_mom, int ntypes, hipStream_t st) {
-    const size_t lds = fused_lds_bytes<T, NE, NC, GEN>(maxp, ldg, maxw, dersum) +
-                       (use_mom ? mom_lds_bytes<T, NE>(maxp, ntypes) : 0);
+    const size_t lds = fused_lds_bytes<T, NE, NC, GEN>(lmaxp, ldg, maxw, dersum) +
+                       (use_mom ? mom_lds_bytes<T, NE>(lmaxp, ntypes, use_mom) : 0);
     if (lds > 65536) return hipErrorInvalidConfiguration;
     hipLaunchKernelGGL((k_fused<T, NE, NC, GEN>), dim3(nblocks), dim3(kWave), lds, st, plan, rfe, rfr, Wt, atom_type,
-                       order, ntot, work, cnt, meta, r, maxp, ldg, maxw, dersum, vac, do_force, eatom, X, use_mom);
+                       order, ntot, work, cnt, meta, r, maxp, lmaxp, ldg, maxw, dersum, vac, do_force, eatom, X, use_mom);
     return hipGetLastError();
 }
 
-size_t mom_lds(int shape, bool fp64, int maxp, int ntypes) {
-    const int ne = shape == 0 ? 2 : (shape == 1 ? 4 : 8);
-    return (fp64 ? 8 : 4) * (size_t)(ne == 2 ? mom_lds_elems<2>(maxp, ntypes)
-                                             : (ne == 4 ? mom_lds_elems<4>(maxp, ntypes) : mom_lds_elems<8>(maxp, ntypes)));
+size_t mom_lds(int shape, bool fp64, int lmaxp, int ntypes, int bits) {
+    if (shape == 0) return fp64 ? mom_lds_bytes<double, 2>(lmaxp, ntypes, bits) : mom_lds_bytes<float, 2>(lmaxp, ntypes, bits);
+    if (shape == 1) return fp64 ? mom_lds_bytes<double, 4>(lmaxp, ntypes, bits) : mom_lds_bytes<float, 4>(lmaxp, ntypes, bits);
+    return fp64 ? mom_lds_bytes<double, 8>(lmaxp, ntypes, bits) : mom_lds_bytes<float, 8>(lmaxp, ntypes, bits);
 }
 
 template <typename T, bool GEN>
@@ -852,37 +889,37 @@ size_t fused_lds(int shape, bool fp64, bool ns, int maxp, int ldg, int maxw, int
 template <typename T, bool GEN>
 static hipError_t launch_fused_s(int shape, int nblocks, const DevPlan* plan, const double* rfe, const double* rfr,
                                  const T* Wt, const int* atom_type, const int* order, int ntot, int* work,
-                                 const int* cnt, const int* meta, const double4* r, int maxp, int ldg, int maxw,
-                                 int dersum, int vac, int do_force, double* eatom, T* X, int use_mom, int ntypes,
-                                 hipStream_t st) {
+                                 const int* cnt, const int* meta, const double4* r, int maxp, int lmaxp, int ldg,
+                                 int maxw, int dersum, int vac, int do_force, double* eatom, T* X, int use_mom,
+                                 int ntypes, hipStream_t st) {
     if (shape == 0)
         return launch_fused_t<T, 2, 4, GEN>(nblocks, plan, rfe, rfr, Wt, atom_type, order, ntot, work, cnt, meta, r,
-                                           maxp, ldg, maxw, dersum, vac, do_force, eatom, X, use_mom, ntypes, st);
+                                           maxp, lmaxp, ldg, maxw, dersum, vac, do_force, eatom, X, use_mom, ntypes, st);
     if (shape == 1)
         return launch_fused_t<T, 4, 4, GEN>(nblocks, plan, rfe, rfr, Wt, atom_type, order, ntot, work, cnt, meta, r,
-                                           maxp, ldg, maxw, dersum, vac, do_force, eatom, X, use_mom, ntypes, st);
+                                           maxp, lmaxp, ldg, maxw, dersum, vac, do_force, eatom, X, use_mom, ntypes, st);
     return launch_fused_t<T, 8, 8, GEN>(nblocks, plan, rfe, rfr, Wt, atom_type, order, ntot, work, cnt, meta, r, maxp,
-                                       ldg, maxw, dersum, vac, do_force, eatom, X, use_mom, ntypes, st);
+                                       lmaxp, ldg, maxw, dersum, vac, do_force, eatom, X, use_mom, ntypes, st);
 }
 
 template <typename T>
 hipError_t launch_fused(int shape, int ns, int nblocks, const DevPlan* plan, const double* rfe, const double* rfr,
                         const T* Wt, const int* atom_type, const int* order, int ntot, int* work, const int* cnt,
-                        const int* meta, const double4* r, int maxp, int ldg, int maxw, int dersum, int vac,
+                        const int* meta, const double4* r, int maxp, int lmaxp, int ldg, int maxw, int dersum, int vac,
                         int do_force, double* eatom, T* X, int use_mom, int ntypes, hipStream_t st) {
     if (ns)
         return launch_fused_s<T, true>(shape, nblocks, plan, rfe, rfr, Wt, atom_type, order, ntot, work, cnt, meta, r,
-                                       maxp, ldg, maxw, dersum, vac, do_force, eatom, X, use_mom, ntypes, st);
+                                       maxp, lmaxp, ldg, maxw, dersum, vac, do_force, eatom, X, use_mom, ntypes, st);
     return launch_fused_s<T, false>(shape, nblocks, plan, rfe, rfr, Wt, atom_type, order, ntot, work, cnt, meta, r,
-                                    maxp, ldg, maxw, dersum, vac, do_force, eatom, X, use_mom, ntypes, st);
+                                    maxp, lmaxp, ldg, maxw, dersum, vac, do_force, eatom, X, use_mom, ntypes, st);
 }
 
 template hipError_t launch_fused<double>(int, int, int, const DevPlan*, const double*, const double*, const double*,
                                          const int*, const int*, int, int*, const int*, const int*, const double4*, int,
-                                         int, int, int, int, int, double*, double*, int, int, hipStream_t);
+                                         int, int, int, int, int, int, double*, double*, int, int, hipStream_t);
 template hipError_t launch_fused<float>(int, int, int, const DevPlan*, const double*, const double*, const float*,
                                         const int*, const int*, int, int*, const int*, const int*, const double4*, int,
-                                        int, int, int, int, int, double*, float*, int, int, hipStream_t);
+                                        int, int, int, int, int, int, double*, float*, int, int, hipStream_t);
 
 // F_a = sum_s X[a,s] - sum_{entries (j -> a) that carry a reaction} X[j -> a], in FP64.
 // One wavefront per atom, lanes over its entries.  The mirrored entry (j -> a, image
@@ -950,22 +987,23 @@ __global__ __launch_bounds__(kWave) void k_assemble(const DevStruc* __restrict__
 template <typename T, int NE, int NC, bool GEN>
 static hipError_t launch_symm_t(const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
                                 const int* centres, int ncentres, const int* cnt, const int* meta, const double4* r,
-                                int maxp, T* G, int ldg, int use_mom, int ntypes, hipStream_t st) {
-    const size_t lds = symm_lds_bytes<T, NE, NC, GEN>(maxp, ldg) + (use_mom ? mom_lds_bytes<T, NE>(maxp, ntypes) : 0);
+                                int maxp, int lmaxp, T* G, int ldg, int use_mom, int ntypes, hipStream_t st) {
+    const size_t lds = symm_lds_bytes<T, NE, NC, GEN>(lmaxp, ldg) + (use_mom ? mom_lds_bytes<T, NE>(lmaxp, ntypes, 1) : 0);
     if (lds > 65536) return hipErrorInvalidConfiguration;
     hipLaunchKernelGGL((k_symm<T, NE, NC, GEN>), dim3(ncentres), dim3(kWave), lds, st, plan, rfe, rfr, atom_type,
-                       centres, cnt, meta, r, maxp, G, ldg, use_mom);
+                       centres, cnt, meta, r, maxp, lmaxp, G, ldg, use_mom);
     return hipGetLastError();
 }
 
 template <typename T, int NE, int NC, bool GEN>
 static hipError_t launch_force_t(const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
                                  const int* centres, int ncentres, const int* cnt, const int* meta, const double4* r,
-                                 int maxp, const T* dEdG, int ldg, T* X, int use_mom, int ntypes, hipStream_t st) {
-    const size_t lds = force_lds_bytes<T, NE, NC, GEN>(maxp, ldg) + (use_mom ? mom_lds_bytes<T, NE>(maxp, ntypes) : 0);
+                                 int maxp, int lmaxp, const T* dEdG, int ldg, T* X, int use_mom, int ntypes,
+                                 hipStream_t st) {
+    const size_t lds = force_lds_bytes<T, NE, NC, GEN>(lmaxp, ldg) + (use_mom ? mom_lds_bytes<T, NE>(lmaxp, ntypes, 2) : 0);
     if (lds > 65536) return hipErrorInvalidConfiguration;
     hipLaunchKernelGGL((k_force<T, NE, NC, GEN>), dim3(ncentres), dim3(kWave), lds, st, plan, rfe, rfr, atom_type,
-                       centres, cnt, meta, r, maxp, dEdG, ldg, X, use_mom);
+                       centres, cnt, meta, r, maxp, lmaxp, dEdG, ldg, X, use_mom);
     return hipGetLastError();
 }
 
@@ -1002,52 +1040,54 @@ size_t force_lds(int shape, bool fp64, bool ns, int maxp, int ldg) {
 template <typename T, bool GEN>
 static hipError_t launch_symm_s(int shape, const DevPlan* plan, const double* rfe, const double* rfr,
                                 const int* atom_type, const int* centres, int ncentres, const int* cnt, const int* meta,
-                                const double4* r, int maxp, T* G, int ldg, int use_mom, int ntypes, hipStream_t st) {
+                                const double4* r, int maxp, int lmaxp, T* G, int ldg, int use_mom, int ntypes,
+                                hipStream_t st) {
     if (shape == 0)
-        return launch_symm_t<T, 2, 4, GEN>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, G, ldg,
+        return launch_symm_t<T, 2, 4, GEN>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp, G, ldg,
                                           use_mom, ntypes, st);
     if (shape == 1)
-        return launch_symm_t<T, 4, 4, GEN>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, G, ldg,
+        return launch_symm_t<T, 4, 4, GEN>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp, G, ldg,
                                           use_mom, ntypes, st);
-    return launch_symm_t<T, 8, 8, GEN>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, G, ldg, use_mom,
+    return launch_symm_t<T, 8, 8, GEN>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp, G, ldg, use_mom,
                                       ntypes, st);
 }
 
 template <typename T>
 hipError_t launch_symm(int shape, int ns, const DevPlan* plan, const double* rfe, const double* rfr,
                        const int* atom_type, const int* centres, int ncentres, const int* cnt, const int* meta,
-                       const double4* r, int maxp, T* G, int ldg, int use_mom, int ntypes, hipStream_t st) {
+                       const double4* r, int maxp, int lmaxp, T* G, int ldg, int use_mom, int ntypes,
+                                hipStream_t st) {
     if (ns)
-        return launch_symm_s<T, true>(shape, plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, G, ldg,
+        return launch_symm_s<T, true>(shape, plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp, G, ldg,
                                       use_mom, ntypes, st);
-    return launch_symm_s<T, false>(shape, plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, G, ldg,
+    return launch_symm_s<T, false>(shape, plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp, G, ldg,
                                    use_mom, ntypes, st);
 }
 
 template <typename T, bool GEN>
 static hipError_t launch_force_s(int shape, const DevPlan* plan, const double* rfe, const double* rfr,
                                  const int* atom_type, const int* centres, int ncentres, const int* cnt,
-                                 const int* meta, const double4* r, int maxp, const T* dEdG, int ldg, T* X,
+                                 const int* meta, const double4* r, int maxp, int lmaxp, const T* dEdG, int ldg, T* X,
                                  int use_mom, int ntypes, hipStream_t st) {
     if (shape == 0)
-        return launch_force_t<T, 2, 4, GEN>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, dEdG, ldg,
+        return launch_force_t<T, 2, 4, GEN>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp, dEdG, ldg,
                                            X, use_mom, ntypes, st);
     if (shape == 1)
-        return launch_force_t<T, 4, 4, GEN>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, dEdG, ldg,
+        return launch_force_t<T, 4, 4, GEN>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp, dEdG, ldg,
                                            X, use_mom, ntypes, st);
-    return launch_force_t<T, 8, 8, GEN>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, dEdG, ldg, X,
+    return launch_force_t<T, 8, 8, GEN>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp, dEdG, ldg, X,
                                        use_mom, ntypes, st);
 }
 
 template <typename T>
 hipError_t launch_force(int shape, int ns, const DevPlan* plan, const double* rfe, const double* rfr,
                         const int* atom_type, const int* centres, int ncentres, const int* cnt, const int* meta,
-                        const double4* r, int maxp, const T* dEdG, int ldg, T* X, int use_mom, int ntypes,
+                        const double4* r, int maxp, int lmaxp, const T* dEdG, int ldg, T* X, int use_mom, int ntypes,
                         hipStream_t st) {
     if (ns)
-        return launch_force_s<T, true>(shape, plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, dEdG,
+        return launch_force_s<T, true>(shape, plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp, dEdG,
                                        ldg, X, use_mom, ntypes, st);
-    return launch_force_s<T, false>(shape, plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, dEdG, ldg,
+    return launch_force_s<T, false>(shape, plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp, dEdG, ldg,
                                     X, use_mom, ntypes, st);
 }
 
@@ -1060,17 +1100,17 @@ hipError_t launch_assemble(const DevStruc* S, const int* atom_struc, const int* 
 }
 
 template hipError_t launch_symm<double>(int, int, const DevPlan*, const double*, const double*, const int*, const int*,
-                                        int, const int*, const int*, const double4*, int, double*, int, int, int,
+                                        int, const int*, const int*, const double4*, int, int, double*, int, int, int,
                                         hipStream_t);
 template hipError_t launch_symm<float>(int, int, const DevPlan*, const double*, const double*, const int*, const int*,
-                                       int, const int*, const int*, const double4*, int, float*, int, int, int,
+                                       int, const int*, const int*, const double4*, int, int, float*, int, int, int,
                                        hipStream_t);
 template hipError_t launch_force<double>(int, int, const DevPlan*, const double*, const double*, const int*,
-                                         const int*, int, const int*, const int*, const double4*, int, const double*,
-                                         int, double*, int, int, hipStream_t);
+                                         const int*, int, const int*, const int*, const double4*, int, int,
+                                         const double*, int, double*, int, int, hipStream_t);
 template hipError_t launch_force<float>(int, int, const DevPlan*, const double*, const double*, const int*, const int*,
-                                        int, const int*, const int*, const double4*, int, const float*, int, float*,
-                                        int, int, hipStream_t);
+                                        int, const int*, const int*, const double4*, int, int, const float*, int,
+                                        float*, int, int, hipStream_t);
 template hipError_t launch_assemble<double>(const DevStruc*, const int*, const int*, const int*, const int*,
                                             const double*, int, int, int, double*, hipStream_t);
 template hipError_t launch_assemble<float>(const DevStruc*, const int*, const int*, const int*, const int*, const float*,
