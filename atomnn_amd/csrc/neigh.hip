@@ -159,7 +159,6 @@ __global__ __launch_bounds__(kWave) void k_neigh_allpairs(const DevStruc* __rest
     }
     if (lane == 0) {
         cnt[i] = written;
-        atomicMax(maxcnt, written);
     }
 }
 
@@ -312,7 +311,23 @@ __global__ __launch_bounds__(kWave) void k_neigh_cells(const DevStruc* __restric
     }
     if (lane == 0) {
         cnt[i] = kept;
-        atomicMax(maxcnt, kept);
+    }
+}
+
+// Largest neighbour count of the pass (the row-overflow check).  A few hundred blocks each
+// fold a strided slice and issue one atomicMax: the per-centre kernels do not contend for
+// one address.
+__global__ __launch_bounds__(256) void k_count_max(const int* __restrict__ cnt, int n, int* __restrict__ maxcnt) {
+    int m = 0;
+    for (int t = blockIdx.x * 256 + threadIdx.x; t < n; t += gridDim.x * 256) m = max(m, cnt[t]);
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, kWave));
+    __shared__ int wm[256 / kWave];
+    if ((threadIdx.x & (kWave - 1)) == 0) wm[threadIdx.x / kWave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 256 / kWave; ++w) m = max(m, wm[w]);
+        atomicMax(maxcnt, m);
     }
 }
 
@@ -384,6 +399,9 @@ hipError_t launch_neighbors(const NeighborArgs& a, hipStream_t st) {
                            a.pair_centres, a.rc2, a.maxp, a.cnt, a.nbr_j, a.nbr_meta, a.nbr_r, a.maxcnt);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
+    if (a.ntot > 0)
+        hipLaunchKernelGGL(k_count_max, dim3(std::min((a.ntot + 255) / 256, 256)), dim3(256), 0, st, a.cnt, a.ntot,
+                           a.maxcnt);
     return hipGetLastError();
 }
 

@@ -244,6 +244,30 @@ __device__ __forceinline__ void ns_tables(const DevAngular& A, const T* su, int 
     }
 }
 
+// One entry's force term (x, y, z, 0) as a single 32-byte (FP64) or 16-byte (FP32) store.
+template <typename T>
+__device__ __forceinline__ void store_x4(T* dst, T x, T y, T z);
+template <>
+__device__ __forceinline__ void store_x4<double>(double* dst, double x, double y, double z) {
+    *reinterpret_cast<double4*>(dst) = make_double4(x, y, z, 0.0);
+}
+template <>
+__device__ __forceinline__ void store_x4<float>(float* dst, float x, float y, float z) {
+    *reinterpret_cast<float4*>(dst) = make_float4(x, y, z, 0.f);
+}
+
+template <typename T>
+__device__ __forceinline__ double4 load_x4(const T* src);
+template <>
+__device__ __forceinline__ double4 load_x4<double>(const double* src) {
+    return *reinterpret_cast<const double4*>(src);
+}
+template <>
+__device__ __forceinline__ double4 load_x4<float>(const float* src) {
+    const float4 v = *reinterpret_cast<const float4*>(src);
+    return make_double4(v.x, v.y, v.z, 0.0);
+}
+
 // ------------------------------------------------------------------ per-atom bodies
 // Descriptors of one staged centre into Grow (LDS, zeroed by the caller).
 template <typename T, int NE, int NC, bool GEN>
@@ -704,10 +728,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
     const int64_t row = (int64_t)i * maxp;
     for (int p = lane; p < P; p += kWave) {
         const int o = orig[p];
-        Xout[4 * (row + o) + 0] = Xs[3 * p];
-        Xout[4 * (row + o) + 1] = Xs[3 * p + 1];
-        Xout[4 * (row + o) + 2] = Xs[3 * p + 2];
-        Xout[4 * (row + o) + 3] = T(0);
+        store_x4<T>(Xout + 4 * (row + o), Xs[3 * p], Xs[3 * p + 1], Xs[3 * p + 2]);
     }
 }
 
@@ -788,8 +809,8 @@ __host__ __device__ constexpr size_t fused_lds_bytes(int maxp, int ldg, int maxw
            sizeof(int) * (size_t)(maxp + ((2 * kWave + 3) & ~3)) + (GEN ? sizeof(T) * 2 * (size_t)maxp : 0);
 }
 
-// Persistent fused pass: each wavefront takes atoms from a work counter until the batch is
-// done (every wave reaches the exit), and for each atom stages its entries once, sums the
+// Persistent fused pass: each wavefront takes chunks of atoms from a work counter until the
+// batch is done (every wave reaches the exit), and for each atom stages its entries once, sums the
 // descriptors, runs its network in-wave and computes its per-entry force terms.  The order of
 // atoms (species-sorted) keeps a species' weights hot in the caches.
 template <typename T, int NE, int NC, bool GEN>
@@ -798,7 +819,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
     const T* __restrict__ Wt, const int* __restrict__ atom_type, const int* __restrict__ order, int ntot,
     int* __restrict__ work, const int* __restrict__ cnt, const int* __restrict__ nbr_meta,
     const double4* __restrict__ nbr_r, int maxp, int lmaxp, int ldg, int maxw, int dersum, int vac, int do_force,
-    double* __restrict__ eatom, T* __restrict__ Xout, int use_mom) {
+    double* __restrict__ eatom, T* __restrict__ Xout, int use_mom, int chunk) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     T* su = (T*)lds_raw;
     T* Rf = su + 4 * lmaxp;
@@ -820,11 +841,17 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
     Segs sg{seg_lds};
     const int lane = threadIdx.x;
     const int ntypes = plan->T;
+    // each block starts on its own chunk of `chunk` atoms, then takes chunks from the counter
+    int base = blockIdx.x * chunk, k = 0;
     for (;;) {
-        if (lane == 0) next_atom = atomicAdd(work, 1);
-        __syncthreads();
-        const int idx = next_atom;
-        __syncthreads();
+        if (k == chunk) {
+            if (lane == 0) next_atom = (int)gridDim.x * chunk + atomicAdd(work, chunk);
+            __syncthreads();
+            base = next_atom;
+            __syncthreads();
+            k = 0;
+        }
+        const int idx = base + k++;
         if (idx >= ntot) break;
         const int i = order[idx];
         const int ti = atom_type[i];
@@ -843,10 +870,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
             const int64_t row = (int64_t)i * maxp;
             for (int p = lane; p < P; p += kWave) {
                 const int o = orig[p];
-                Xout[4 * (row + o) + 0] = Xs[3 * p];
-                Xout[4 * (row + o) + 1] = Xs[3 * p + 1];
-                Xout[4 * (row + o) + 2] = Xs[3 * p + 2];
-                Xout[4 * (row + o) + 3] = T(0);
+                store_x4<T>(Xout + 4 * (row + o), Xs[3 * p], Xs[3 * p + 1], Xs[3 * p + 2]);
             }
         }
         __syncthreads();
@@ -862,7 +886,8 @@ static hipError_t launch_fused_t(int nblocks, const DevPlan* plan, const double*
                        (use_mom ? mom_lds_bytes<T, NE>(lmaxp, ntypes, use_mom) : 0);
     if (lds > 65536) return hipErrorInvalidConfiguration;
     hipLaunchKernelGGL((k_fused<T, NE, NC, GEN>), dim3(nblocks), dim3(kWave), lds, st, plan, rfe, rfr, Wt, atom_type,
-                       order, ntot, work, cnt, meta, r, maxp, lmaxp, ldg, maxw, dersum, vac, do_force, eatom, X, use_mom);
+                       order, ntot, work, cnt, meta, r, maxp, lmaxp, ldg, maxw, dersum, vac, do_force, eatom, X, use_mom,
+                       std::max(1, std::min(8, ntot / (8 * std::max(nblocks, 1)))));
     return hipGetLastError();
 }
 
@@ -947,9 +972,12 @@ __global__ __launch_bounds__(kWave) void k_assemble(const DevStruc* __restrict__
     double fx = 0, fy = 0, fz = 0;
     for (int s = lane; s < P; s += kWave) {
         const int64_t e = row + s;
-        fx += (double)X[4 * e];
-        fy += (double)X[4 * e + 1];
-        fz += (double)X[4 * e + 2];
+        {
+            const double4 v = load_x4<T>(X + 4 * e);
+            fx += v.x;
+            fy += v.y;
+            fz += v.z;
+        }
         const int meta = nbr_meta[e];
         if ((meta & 1) || all_images) {
             const int img = meta >> 8;
@@ -966,10 +994,10 @@ __global__ __launch_bounds__(kWave) void k_assemble(const DevStruc* __restrict__
                 else hi = mid;
             }
             if (lo < Pj && (int64_t)nbr_j[rowj + lo] * st.nimg + (nbr_meta[rowj + lo] >> 8) == want) {
-                const int64_t q = rowj + lo;
-                fx -= (double)X[4 * q];
-                fy -= (double)X[4 * q + 1];
-                fz -= (double)X[4 * q + 2];
+                const double4 v = load_x4<T>(X + 4 * (rowj + lo));
+                fx -= v.x;
+                fy -= v.y;
+                fz -= v.z;
             }
         }
     }
