@@ -104,13 +104,32 @@ __device__ __forceinline__ T rpow(T x, T e) {
 
 // |(1 + lambda cos)/2|^zeta (and the derivative factor zeta lambda |.|^(zeta-1) / 2).  The
 // non-generic kernels only see integer zeta (zint >= 0): the general pow stays out of them.
+// x^n for a wave-uniform 0 <= n <= 15 without branches: selects on the bits of n over the
+// repeated-squaring powers.
+template <typename T>
+__device__ __forceinline__ T ipow_sel(T x, int n) {
+    const T x2 = x * x, x4 = x2 * x2;
+    T p = (n & 1) ? x : T(1);
+    p = (n & 2) ? p * x2 : p;
+    p = (n & 4) ? p * x4 : p;
+    p = (n & 8) ? p * (x4 * x4) : p;
+    return p;
+}
+
+// |(1 + lambda cos)/2|^zeta and the derivative factor zeta lambda |.|^(zeta-1) / 2.  The
+// non-generic kernels see integer 0 <= zeta <= 16 only (larger or fractional zeta takes the
+// generic instantiation), so they evaluate the terms without branches.
 template <typename T, bool GEN>
 __device__ __forceinline__ void angular_term(T cs, T lam, T zeta, int zint, T& A, T& dA) {
     const T acw = fabs(T(0.5) * (T(1) + lam * cs));
     T p1;
-    if (zint >= 1) p1 = ipow_u(acw, zint - 1);
-    else if (!GEN || zint == 0) p1 = T(0);  // zeta = 0: A = 1, dA = 0
-    else p1 = rpow(acw, zeta - T(1));
+    if (!GEN) {
+        p1 = zint > 0 ? ipow_sel(acw, zint - 1) : T(0);  // zeta = 0: A = 1, dA = 0
+    } else {
+        if (zint >= 1) p1 = ipow_u(acw, zint - 1);
+        else if (zint == 0) p1 = T(0);
+        else p1 = rpow(acw, zeta - T(1));
+    }
     A = (zint == 0) ? T(1) : p1 * acw;
     dA = T(0.5) * zeta * lam * p1;
 }
@@ -118,7 +137,8 @@ __device__ __forceinline__ void angular_term(T cs, T lam, T zeta, int zint, T& A
 template <typename T, bool GEN>
 __device__ __forceinline__ T angular_value(T cs, T lam, T zeta, int zint) {
     const T acw = fabs(T(0.5) * (T(1) + lam * cs));
-    return (!GEN || zint >= 0) ? ipow_u(acw, zint) : rpow(acw, zeta);
+    if (!GEN) return ipow_sel(acw, zint);
+    return zint >= 0 ? ipow_u(acw, zint) : rpow(acw, zeta);
 }
 
 // Row index of flat pair q in the strict upper triangle of an n x n matrix (row-major).
@@ -697,7 +717,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
 
 // ------------------------------------------------------------------ force terms
 template <typename T, int NE, int NC, bool GEN>
-__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <= 16 ? (GEN ? 3 : 4) : 1, 8))) void k_force(
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <= 16 ? 3 : 1, 8))) void k_force(
     const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta, const double* __restrict__ rfun_rs,
     const int* __restrict__ atom_type, const int* __restrict__ centres, const int* __restrict__ cnt,
     const int* __restrict__ nbr_meta, const double4* __restrict__ nbr_r, int maxp, int lmaxp,
