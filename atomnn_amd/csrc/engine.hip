@@ -104,6 +104,25 @@ void DevMem::reserve(size_t bytes) {
     }
     cap = want;
 }
+void HostPinned::reserve(size_t bytes) {
+    if (bytes <= cap) return;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = std::max<size_t>(bytes, 4096);
+    want = (want + 65535) & ~size_t(65535);
+    if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
+        p = nullptr;
+        throw std::bad_alloc();
+    }
+    cap = want;
+}
+void HostPinned::release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+}
+
 void DevMem::release() {
     if (p) (void)hipFree(p);
     p = nullptr;
@@ -239,9 +258,8 @@ Engine::Engine(const Engine& o, int) : host(o.host), pb(o.pb), shape(o.shape) {
 
 size_t Engine::workspace_bytes() const {
     size_t b = 0;
-    for (const DevMem* m : {&d_pos, &d_type, &d_astruc, &d_strucs, &d_centres, &d_cnt, &d_nbrj, &d_meta, &d_nbrr,
-                            &d_maxcnt, &d_abin, &d_bcount, &d_bstart, &d_bfill, &d_batoms, &d_G, &d_dEdG, &d_X,
-                            &d_eatom, &d_F, &d_spatoms, &d_E, &d_sstart})
+    for (const DevMem* m : {&d_up, &d_cnt, &d_nbrj, &d_meta, &d_nbrr, &d_maxcnt, &d_abin, &d_bcount, &d_bstart,
+                            &d_bfill, &d_batoms, &d_G, &d_dEdG, &d_X, &d_eatom, &d_F, &d_E, &d_sstart})
         b += m->cap;
     return b;
 }
@@ -257,10 +275,11 @@ Engine::~Engine() {
     d_work.release();
     if (owns_model)
         for (DevMem* m : {&d_plan, &d_rfe, &d_rfr, &d_w64, &d_w32}) m->release();
-    for (DevMem* m : {&d_pos, &d_type, &d_astruc, &d_strucs, &d_centres,
-                      &d_cnt, &d_nbrj, &d_meta, &d_nbrr, &d_maxcnt, &d_abin, &d_bcount, &d_bstart,
-                      &d_bfill, &d_batoms, &d_G, &d_dEdG, &d_X, &d_eatom, &d_F, &d_spatoms})
+    for (DevMem* m : {&d_up, &d_cnt, &d_nbrj, &d_meta, &d_nbrr, &d_maxcnt, &d_abin, &d_bcount, &d_bstart,
+                      &d_bfill, &d_batoms, &d_G, &d_dEdG, &d_X, &d_eatom, &d_F})
         m->release();
+    h_up.release();
+    h_down.release();
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -284,6 +303,7 @@ static int estimate_maxp(const std::vector<nnp_structure>& ss, double rc) {
 }
 
 void Engine::upload(const std::vector<nnp_structure>& ss) {
+    HIPCHK(hipStreamSynchronize(stream));  // the staging buffer may still feed an earlier copy
     resident_valid = false;
     drop_graph();
     eager_runs = 0;
@@ -333,19 +353,29 @@ void Engine::upload(const std::vector<nnp_structure>& ss) {
     }
     h_centres = h_cell_centres;
     h_centres.insert(h_centres.end(), h_pair_centres.begin(), h_pair_centres.end());
-    d_pos.reserve(h_pos.size() * 8);
-    d_type.reserve((size_t)ntot * 4);
-    d_astruc.reserve((size_t)ntot * 4);
-    d_strucs.reserve(h_strucs.size() * sizeof(DevStruc));
-    d_centres.reserve((size_t)ntot * 4);
-    d_spatoms.reserve((size_t)ntot * 4);
-    HIPCHK(hipMemcpyAsync(d_pos.p, h_pos.data(), h_pos.size() * 8, hipMemcpyHostToDevice, stream));
-    HIPCHK(hipMemcpyAsync(d_type.p, h_type.data(), (size_t)ntot * 4, hipMemcpyHostToDevice, stream));
-    HIPCHK(hipMemcpyAsync(d_astruc.p, h_astruc.data(), (size_t)ntot * 4, hipMemcpyHostToDevice, stream));
-    HIPCHK(hipMemcpyAsync(d_strucs.p, h_strucs.data(), h_strucs.size() * sizeof(DevStruc), hipMemcpyHostToDevice,
-                          stream));
-    HIPCHK(hipMemcpyAsync(d_centres.p, h_centres.data(), (size_t)ntot * 4, hipMemcpyHostToDevice, stream));
-    HIPCHK(hipMemcpyAsync(d_spatoms.p, h_spatoms.data(), (size_t)ntot * 4, hipMemcpyHostToDevice, stream));
+    // one pinned staging buffer, one copy: positions | species | structure of each atom |
+    // structures | centres (cell-list first) | species-sorted atoms, each 256-byte aligned
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t o_pos = 0, o_type = o_pos + al(h_pos.size() * 8), o_ast = o_type + al((size_t)ntot * 4),
+                 o_st = o_ast + al((size_t)ntot * 4), o_cen = o_st + al(h_strucs.size() * sizeof(DevStruc)),
+                 o_spa = o_cen + al((size_t)ntot * 4), total = o_spa + al((size_t)ntot * 4);
+    h_up.reserve(total);
+    d_up.reserve(total);
+    char* hs = (char*)h_up.p;
+    std::memcpy(hs + o_pos, h_pos.data(), h_pos.size() * 8);
+    std::memcpy(hs + o_type, h_type.data(), (size_t)ntot * 4);
+    std::memcpy(hs + o_ast, h_astruc.data(), (size_t)ntot * 4);
+    std::memcpy(hs + o_st, h_strucs.data(), h_strucs.size() * sizeof(DevStruc));
+    std::memcpy(hs + o_cen, h_centres.data(), (size_t)ntot * 4);
+    std::memcpy(hs + o_spa, h_spatoms.data(), (size_t)ntot * 4);
+    char* ds = (char*)d_up.p;
+    d_pos.p = ds + o_pos;
+    d_type.p = ds + o_type;
+    d_astruc.p = ds + o_ast;
+    d_strucs.p = ds + o_st;
+    d_centres.p = ds + o_cen;
+    d_spatoms.p = ds + o_spa;
+    HIPCHK(hipMemcpyAsync(d_up.p, h_up.p, total, hipMemcpyHostToDevice, stream));
     if (maxp == 0) maxp = estimate_maxp(ss, host.rc);
 }
 
@@ -393,6 +423,10 @@ void Engine::build_neighbors(bool all_images) {
             a.bin_atoms = (int*)d_batoms.p;
         }
         HIPCHK(launch_neighbors(a, stream));
+        if (defer_count) {  // checked at the download instead (serving lanes)
+            lmaxp = maxp;
+            return;
+        }
         int mc = 0;
         HIPCHK(hipMemcpyAsync(&mc, d_maxcnt.p, 4, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
@@ -769,6 +803,33 @@ void Engine::enqueue_pass(uint32_t flags) {
     stage_end(ST_ENERGY);
 }
 
+// Staged download: [maxcnt | E per structure | E_i | F] into pinned memory, one sync.
+int Engine::download(bool energies, bool forces) {
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t o_e = 256, o_ea = o_e + (energies ? al((size_t)nstruc * 8) : 0), o_f = o_ea + al((size_t)ntot * 8),
+                 total = o_f + (forces ? al((size_t)ntot * 24) : 0);
+    h_down.reserve(total);
+    char* hd = (char*)h_down.p;
+    HIPCHK(hipMemcpyAsync(hd, d_maxcnt.p, 4, hipMemcpyDeviceToHost, stream));
+    if (energies) HIPCHK(hipMemcpyAsync(hd + o_e, d_E.p, (size_t)nstruc * 8, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(hd + o_ea, d_eatom.p, (size_t)ntot * 8, hipMemcpyDeviceToHost, stream));
+    if (forces) HIPCHK(hipMemcpyAsync(hd + o_f, d_F.p, (size_t)ntot * 24, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    int mc;
+    std::memcpy(&mc, hd, 4);
+    if (energies) {
+        h_E.resize(nstruc);
+        std::memcpy(h_E.data(), hd + o_e, (size_t)nstruc * 8);
+    }
+    h_eatom.resize(ntot);
+    std::memcpy(h_eatom.data(), hd + o_ea, (size_t)ntot * 8);
+    if (forces) {
+        h_F.resize(3 * (size_t)ntot);
+        std::memcpy(h_F.data(), hd + o_f, (size_t)ntot * 24);
+    }
+    return mc;
+}
+
 void Engine::resident_fetch(uint32_t flags, nnp_result* res) {
     int mc = 0;
     HIPCHK(hipMemcpyAsync(&mc, d_maxcnt.p, 4, hipMemcpyDeviceToHost, stream));
@@ -785,15 +846,7 @@ void Engine::resident_fetch(uint32_t flags, nnp_result* res) {
         HIPCHK(hipStreamSynchronize(stream));
         profile_collect();
     }
-    h_E.resize(nstruc);
-    HIPCHK(hipMemcpyAsync(h_E.data(), d_E.p, (size_t)nstruc * 8, hipMemcpyDeviceToHost, stream));
-    h_eatom.resize(ntot);
-    HIPCHK(hipMemcpyAsync(h_eatom.data(), d_eatom.p, (size_t)ntot * 8, hipMemcpyDeviceToHost, stream));
-    if (flags & NNP_FORCE) {
-        h_F.resize(3 * (size_t)ntot);
-        HIPCHK(hipMemcpyAsync(h_F.data(), d_F.p, (size_t)ntot * 24, hipMemcpyDeviceToHost, stream));
-    }
-    HIPCHK(hipStreamSynchronize(stream));
+    (void)download(true, flags & NNP_FORCE);
     int a0 = 0;
     for (int k = 0; k < nstruc; ++k) {
         const int n = h_natoms[k];
@@ -805,13 +858,18 @@ void Engine::resident_fetch(uint32_t flags, nnp_result* res) {
 }
 
 void Engine::fetch(const std::vector<nnp_structure>& ss, uint32_t flags, nnp_result* res) {
-    h_eatom.resize(ntot);
-    HIPCHK(hipMemcpyAsync(h_eatom.data(), d_eatom.p, (size_t)ntot * 8, hipMemcpyDeviceToHost, stream));
-    if (flags & NNP_FORCE) {
-        h_F.resize(3 * (size_t)ntot);
-        HIPCHK(hipMemcpyAsync(h_F.data(), d_F.p, (size_t)ntot * 24, hipMemcpyDeviceToHost, stream));
+    const int mc = download(false, flags & NNP_FORCE);
+    if (defer_count && mc > maxp) {  // rows overflowed: redo the batch with the count read back
+        defer_count = false;
+        try {
+            run(ss, flags);
+        } catch (...) {
+            defer_count = true;
+            throw;
+        }
+        defer_count = true;
+        (void)download(false, flags & NNP_FORCE);
     }
-    HIPCHK(hipStreamSynchronize(stream));
     int a0 = 0;
     for (size_t k = 0; k < ss.size(); ++k) {
         const int n = ss[k].natoms;

@@ -24,6 +24,20 @@ struct DevMem {
     void release();
 };
 
+// A non-owning device pointer into a larger allocation.
+struct DevView {
+    void* p = nullptr;
+};
+
+// Grow-only page-locked host buffer (hipHostMalloc) for single-copy staging of uploads and
+// downloads.
+struct HostPinned {
+    void* p = nullptr;
+    size_t cap = 0;
+    void reserve(size_t bytes);
+    void release();
+};
+
 struct NeighborArgs {
     const DevStruc* S;
     const int* atom_struc;
@@ -111,6 +125,9 @@ class Engine {
         }
     };
     bool may_tune = true;
+    // defer the neighbour-count check of run() to the download in fetch() (one host sync per
+    // batch instead of two; LDS rows then span the full row stride)
+    bool defer_count = false;
     std::vector<std::pair<TuneKey, int>> tune_cache;  // key -> blocks per CU
     std::vector<std::pair<int, double>> last_tune;    // candidates and their times (ms) of the last tuning
     int fused_blocks(bool mixed, uint32_t flags);
@@ -147,9 +164,16 @@ class Engine {
     // bytes of device memory held by the batch buffers
     size_t workspace_bytes() const;
     // batch
-    DevMem d_pos, d_type, d_astruc, d_strucs, d_centres, d_cnt, d_nbrj, d_meta, d_nbrr, d_maxcnt;
+    // the per-batch inputs are one allocation filled by one copy from pinned staging
+    DevMem d_up;
+    HostPinned h_up, h_down;
+    DevView d_pos, d_type, d_astruc, d_strucs, d_centres, d_spatoms;
+    DevMem d_cnt, d_nbrj, d_meta, d_nbrr, d_maxcnt;
     DevMem d_abin, d_bcount, d_bstart, d_bfill, d_batoms;
-    DevMem d_G, d_dEdG, d_X, d_eatom, d_F, d_spatoms;
+    DevMem d_G, d_dEdG, d_X, d_eatom, d_F;
+    // copy E (per structure, optional), E_i, F (optional) and the neighbour maximum to the host
+    // in one staged download; returns the neighbour maximum
+    int download(bool energies, bool forces);
 
     std::vector<double> h_pos, h_eatom, h_F;
     std::vector<int> h_type, h_astruc, h_cell_centres, h_pair_centres, h_centres, h_spatoms, sp_off;

@@ -58,6 +58,7 @@ class Server {
         for (int k = 0; k < cfg_.lanes; ++k) {
             lanes_.emplace_back(new Engine(owner_, k));
             lanes_.back()->may_tune = false;
+            lanes_.back()->defer_count = true;
         }
         for (int k = 0; k < cfg_.lanes; ++k) workers_.emplace_back(&Server::worker, this, k);
     }
