@@ -348,8 +348,9 @@ PlanBuffers build_plan(const HostModel& m) {
                 d.lambda[c] = c < d.nc ? (double)C[c].second : 1.0;
                 const double z = d.zeta[c];
                 d.zint[c] = (z >= 0 && z <= 64 && std::floor(z) == z) ? (int)z : -1;
-                // fractional or zeta > 16: the generic kernels (general pow / power loop)
-                if (c < d.nc && (d.zint[c] < 0 || d.zint[c] > 16)) P.generic = 1;
+                // the fast kernels sum the angular terms of a block as one polynomial in cos of
+                // degree <= 4: fractional zeta, zeta > 4 or |lambda| > 1 take the generic kernels
+                if (c < d.nc && (d.zint[c] < 0 || d.zint[c] > 4 || std::fabs(d.lambda[c]) > 1.0)) P.generic = 1;
             }
             if (d.ne > P.ne_max) P.ne_max = d.ne;
             if (d.nc > P.nc_max) P.nc_max = d.nc;

@@ -104,6 +104,11 @@ __device__ __forceinline__ T rpow(T x, T e) {
 
 // |(1 + lambda cos)/2|^zeta (and the derivative factor zeta lambda |.|^(zeta-1) / 2).  The
 // non-generic kernels only see integer zeta (zint >= 0): the general pow stays out of them.
+// C(z, k) for 0 <= k <= z <= 4
+__device__ __forceinline__ int c_binom4(int z, int k) {
+    return z == 4 ? (k == 0 || k == 4 ? 1 : (k == 2 ? 6 : 4)) : (z == 3 ? (k == 0 || k == 3 ? 1 : 3) : (z == 2 && k == 1 ? 2 : 1));
+}
+
 // x^n for a wave-uniform 0 <= n <= 15 without branches: selects on the bits of n over the
 // repeated-squaring powers.
 template <typename T>
@@ -531,24 +536,50 @@ __device__ __forceinline__ void force_body(const DevPlan* __restrict__ plan, con
                 radial_tables<T, NE, GEN>(A, su, sa, sa + na, Rf, Rd, lane);
                 if (!same) radial_tables<T, NE, GEN>(A, su, sb, sb + nb, Rf, Rd, lane);
             }
-            // block weights W_ec = norm dE/dG[goff + fmap(e,c)]: uniform, read with scalar loads
-            T w[NE][NC];
+            // block weights W_ec = dE/dG[goff + fmap(e,c)] (uniform; the block norm scales each
+            // row's sum).  Fast kernels (integer 0 <= zeta <= 4, |lambda| <= 1 in every block):
+            // for each exponent e the angular sum  S_e(cos) = sum_c W_ec ((1 + lambda_c cos)/2)^zeta_c
+            // is a polynomial of degree <= 4, P[e][p] = sum_c W_ec 2^-zeta C(zeta,p) lambda^p, and
+            // a pair needs S_e and S_e' by Horner's rule.  Generic kernels: the W_ec and the terms.
             const T* dEdGi = Wang + A.goff;
+            const T bnorm = T(A.norm);
+            T w[GEN ? NE : 1][GEN ? NC : 1];
+            T lam[GEN ? NC : 1], zet[GEN ? NC : 1];
+            int zin[GEN ? NC : 1];
+            T P[GEN ? 1 : NE][5];
+            if constexpr (GEN) {
 #pragma unroll
-            for (int e = 0; e < NE; ++e)
+                for (int e = 0; e < NE; ++e)
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) {
+                        const int f = A.fmap[e * kMaxC + c];
+                        w[e][c] = f >= 0 ? dEdGi[f] : T(0);
+                    }
 #pragma unroll
                 for (int c = 0; c < NC; ++c) {
-                    const int f = A.fmap[e * kMaxC + c];
-                    w[e][c] = f >= 0 ? dEdGi[f] : T(0);  // the block norm scales each row's sum
+                    lam[c] = T(A.lambda[c]);
+                    zet[c] = T(A.zeta[c]);
+                    zin[c] = A.zint[c];
                 }
-            const T bnorm = T(A.norm);
-            T lam[NC], zet[NC];
-            int zin[NC];
+            } else {
 #pragma unroll
-            for (int c = 0; c < NC; ++c) {
-                lam[c] = T(A.lambda[c]);
-                zet[c] = T(A.zeta[c]);
-                zin[c] = A.zint[c];
+                for (int e = 0; e < NE; ++e) {
+#pragma unroll
+                    for (int k = 0; k < 5; ++k) P[e][k] = T(0);
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) {
+                        const int f = A.fmap[e * kMaxC + c];
+                        if (f < 0) continue;
+                        const int z = A.zint[c];
+                        const T lc = T(A.lambda[c]);
+                        T coef = dEdGi[f] * T(1.0 / (double)(1 << z));
+#pragma unroll
+                        for (int k = 0; k < 5; ++k) {
+                            if (k <= z) P[e][k] += coef * T(c_binom4(z, k));
+                            coef *= lc;
+                        }
+                    }
+                }
             }
             __syncthreads();
             // ordered pairs: rows in one segment, columns in the other (or the same, minus the diagonal)
@@ -604,18 +635,32 @@ __device__ __forceinline__ void force_body(const DevPlan* __restrict__ plan, con
                                 }
                             }
                             T dEdcos = T(0), dEdd = T(0);
+                            if constexpr (GEN) {
 #pragma unroll
-                            for (int cc = 0; cc < NC; ++cc) {
-                                T tp = T(0), tq = T(0);
+                                for (int cc = 0; cc < NC; ++cc) {
+                                    T tp = T(0), tq = T(0);
+#pragma unroll
+                                    for (int e = 0; e < NE; ++e) {
+                                        tp += w[e][cc] * pf[e];
+                                        tq += w[e][cc] * pd[e];
+                                    }
+                                    T Av, dAv;
+                                    angular_term<T, GEN>(cs, lam[cc], zet[cc], zin[cc], Av, dAv);
+                                    dEdcos += dAv * tp;
+                                    dEdd += Av * tq;
+                                }
+                            } else {
 #pragma unroll
                                 for (int e = 0; e < NE; ++e) {
-                                    tp += w[e][cc] * pf[e];
-                                    tq += w[e][cc] * pd[e];
+                                    T b = P[e][4], d = T(0);
+#pragma unroll
+                                    for (int k = 3; k >= 0; --k) {
+                                        d = d * cs + b;
+                                        b = b * cs + P[e][k];
+                                    }
+                                    dEdd += b * pd[e];
+                                    dEdcos += d * pf[e];
                                 }
-                                T Av, dAv;
-                                angular_term<T, GEN>(cs, lam[cc], zet[cc], zin[cc], Av, dAv);
-                                dEdcos += dAv * tp;
-                                dEdd += Av * tq;
                             }
                             const T g = dEdcos * dji;
                             xx -= dEdd * ujx + g * (ukx - cs * ujx);

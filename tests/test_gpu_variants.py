@@ -1,4 +1,5 @@
 """The engine's basis and activation variants against the oracle (FP64, <= 1e-10 relative):
+integer zeta beyond the fast kernels' polynomial range (8, 16) and zeta = 0,
 radial TANH / SOFTPLUS / LOGCOSH / SWISH / MISH, angular IPOWP / SECHP (separable) and
 IPOWS / SECHS (per-pair factor), and the 17 activations, through the separate kernels and the
 fused kernel.  One mixed-precision case checks the FP32 forms against the FP64 oracle.
@@ -25,17 +26,17 @@ def rel_f(F, Fo):
     return np.linalg.norm(F - Fo, axis=1).max() / np.linalg.norm(Fo, axis=1).max()
 
 
-def variant_potential(tmp_path, phi_r="GAUSSIAN", phi_a="GAUSS", neuron="TANH", seed=0):
+def variant_potential(tmp_path, phi_r="GAUSSIAN", phi_a="GAUSS", neuron="TANH", seed=0, zetas=(1.0, 4.0)):
     """Writes the potential (preconditioned on the oracle's descriptors of a calibration cell)
     and returns its path."""
     rng = np.random.default_rng(seed)
-    pot = synth.make_potential(["A", "B"], 6.0, 6, [0.005, 0.05], [1.0, 4.0], [1, -1], [16, 16], rng)
+    pot = synth.make_potential(["A", "B"], 6.0, 6, [0.005, 0.05], list(zetas), [1, -1], [16, 16], rng)
     # radial functions centred across the cutoff so each variant sees both signs of its argument
     for key in pot.radial:
         pot.radial[key] = [(rs, eta) for rs, eta in zip(np.linspace(0.5, 5.5, 6), np.linspace(0.3, 2.0, 6))]
     pot.phi_r, pot.phi_a, pot.neuron = phi_r, phi_a, neuron
     pot.outw[0], pot.outb[1] = 0.9, 0.05
-    path = str(tmp_path / f"v_{phi_r}_{phi_a}_{neuron}.ann")
+    path = str(tmp_path / f"v_{phi_r}_{phi_a}_{neuron}_{'_'.join(str(z) for z in zetas)}.ann")
     pot.write(path)
     cell = synth.cubic_cell(128, 0.05)
     s = synth.Structure(synth.species_exact(128, [0.5, 0.5], rng), synth.random_positions(128, cell, 2.0, seed + 1),
@@ -56,13 +57,14 @@ def structures():
     return synth.batch512(3) + [big]
 
 
-CASES = ([dict(phi_r=p) for p in ("TANH", "SOFTPLUS", "LOGCOSH", "SWISH", "MISH")] +
+CASES = ([dict(zetas=(2.0, 8.0)), dict(zetas=(3.0, 16.0)), dict(zetas=(0.0, 2.0))] +
+         [dict(phi_r=p) for p in ("TANH", "SOFTPLUS", "LOGCOSH", "SWISH", "MISH")] +
          [dict(phi_a=p) for p in ("IPOWP", "SECHP", "IPOWS", "SECHS")] +
          [dict(neuron=n) for n in NEURONS if n != "TANH"])
 
 
 def case_id(c):
-    return "-".join(f"{k}={v}" for k, v in c.items())
+    return "-".join(f"{k}={v}" for k, v in c.items()).replace(" ", "")
 
 
 @pytest.mark.parametrize("case", CASES, ids=[case_id(c) for c in CASES])
