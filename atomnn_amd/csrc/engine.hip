@@ -308,6 +308,7 @@ void Engine::upload(const std::vector<nnp_structure>& ss) {
     drop_graph();
     eager_runs = 0;
     nstruc = (int)ss.size();
+    cell_natoms_max = 0;
     h_natoms.clear();
     for (const auto& s : ss) h_natoms.push_back(s.natoms);
     ntot = 0;
@@ -329,6 +330,7 @@ void Engine::upload(const std::vector<nnp_structure>& ss) {
         if (d.mode == 1) {
             d.bin0 = bins;
             bins += d.nbin[0] * d.nbin[1] * d.nbin[2];
+            cell_natoms_max = std::max(cell_natoms_max, s.natoms);
         }
         h_strucs.push_back(d);
         std::memcpy(&h_pos[3 * (size_t)a0], s.pos, 3 * sizeof(double) * s.natoms);
@@ -409,6 +411,7 @@ void Engine::build_neighbors(bool all_images) {
         a.pair_centres = (const int*)d_centres.p + h_cell_centres.size();
         a.npair = (int)h_pair_centres.size();
         a.nbins = nbins;
+        a.cell_natoms_max = cell_natoms_max;
         a.all_images = all_images ? 1 : 0;
         if (nbins > 0) {
             d_abin.reserve((size_t)ntot * 4);
@@ -729,6 +732,7 @@ void Engine::enqueue_pass(uint32_t flags) {
     a.pair_centres = (const int*)d_centres.p + h_cell_centres.size();
     a.npair = (int)h_pair_centres.size();
     a.nbins = nbins;
+    a.cell_natoms_max = cell_natoms_max;
     a.all_images = (flags & NNP_ALL_IMAGES) ? 1 : 0;
     a.atom_bin = (int*)d_abin.p;
     a.bin_count = (int*)d_bcount.p;

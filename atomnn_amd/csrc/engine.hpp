@@ -63,6 +63,7 @@ struct NeighborArgs {
     int* bin_fill = nullptr;
     int* bin_atoms = nullptr;
     int all_images;
+    int cell_natoms_max = 0;  // atoms of the largest cell-list structure (LDS bitmap size)
 };
 
 DevStruc make_struc(const nnp_structure& s, double rc, int atom0);
@@ -156,6 +157,7 @@ class Engine {
     int lmaxp = 0;  // entries the per-centre kernels stage in LDS (multiple of 8, <= maxp)
     int max_entries = 0;
     int ntot = 0, nstruc = 0, nbins = 0;
+    int cell_natoms_max = 0;
     bool all_images_ = false;
 
     // model (owned by the first engine; workspaces alias it)

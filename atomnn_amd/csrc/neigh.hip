@@ -15,8 +15,9 @@
 //    small cells (any image shell > 0) and small structures.
 //  * cell list: for structures whose image shells are all 0 (every cell height > 2 rc) the
 //    entry of j is exactly diff(r_i, r_j) when within rc; candidates come from the 27
-//    neighbouring fractional bins as one flat list, and each wavefront sorts its kept
-//    candidates by j (bitonic sort in LDS) before writing them, so the order is the
+//    neighbouring fractional bins as one flat list; each wavefront marks its kept candidates
+//    in an LDS bitmap over the structure's atoms and reads them out in index order (very
+//    large structures: a bitonic sort of the kept candidates instead), so the order is the
 //    reference's.
 // The partner of an entry (the mirrored pair that carries the Newton reaction) is found
 // by the force-assembly kernel.
@@ -89,6 +90,19 @@ __device__ __forceinline__ void put_entry(int64_t slot, int j, int img, int typ,
     nbr_j[slot] = j;
     nbr_meta[slot] = (img << 8) | (typ << 1) | (mn ? 1 : 0);
     nbr_r[slot] = make_double4(x, y, z, d);
+}
+
+// The last of the nb (<= 32) neighbouring bins whose first flat candidate is <= c: a binary
+// search over the lanes' bin starts in five shuffles, executed by every lane.
+__device__ __forceinline__ int owner_bin(int bfirst, int nb, int c) {
+    int lo = 0;
+#pragma unroll
+    for (int step = 16; step > 0; step >>= 1) {
+        const int mid = lo + step;
+        const int fq = __shfl(bfirst, mid < nb ? mid : nb - 1, kWave);
+        if (mid < nb && fq <= c) lo = mid;
+    }
+    return lo;
 }
 
 // ------------------------------------------------------------------ all-pairs search
@@ -242,12 +256,7 @@ __global__ __launch_bounds__(kWave) void k_neigh_cells(const DevStruc* __restric
     int kept = 0;
     for (int c0 = 0; c0 < ncand; c0 += kWave) {
         const int c = c0 + lane;
-        // owning bin: the last bin whose first candidate is <= c
-        int ob = 0;
-        for (int q = 1; q < nb; ++q) {
-            const int fq = __shfl(bfirst, q, kWave);
-            if (fq <= c) ob = q;
-        }
+        const int ob = owner_bin(bfirst, nb, c);
         const int ostart = __shfl(bstart, ob, kWave), ofirst = __shfl(bfirst, ob, kWave);
         bool keep = false;
         double rx = 0, ry = 0, rz = 0, d2 = 0;
@@ -312,6 +321,85 @@ __global__ __launch_bounds__(kWave) void k_neigh_cells(const DevStruc* __restric
     if (lane == 0) {
         cnt[i] = kept;
     }
+}
+
+// Cell list without a sort: kept neighbours are marked in an LDS bitmap over the structure's
+// atoms (one bit each, set with an LDS atomic OR, so the result does not depend on the order
+// the candidates are visited in), then read out in ascending index order, which is the
+// reference's order; each kept entry's displacement is recomputed by the same Cell::diff.
+// LDS: one 32-bit word per 32 atoms of the largest cell-list structure of the batch.
+__global__ __launch_bounds__(kWave) void k_neigh_cells_bits(
+    const DevStruc* __restrict__ S, const int* __restrict__ atom_struc, const double* __restrict__ pos,
+    const int* __restrict__ type, const int* __restrict__ centres, const int* __restrict__ atom_bin,
+    const int* __restrict__ bin_start, const int* __restrict__ bin_count, const int* __restrict__ bin_atoms,
+    double rc2, int maxp, int* __restrict__ cnt, int* __restrict__ nbr_j, int* __restrict__ nbr_meta,
+    double4* __restrict__ nbr_r) {
+    extern __shared__ unsigned int lds_bits[];
+    const int i = centres[blockIdx.x];
+    const DevStruc& s = S[atom_struc[i]];
+    const int lane = threadIdx.x;
+    const int nwords = (s.natoms + 31) >> 5;
+    for (int w = lane; w < nwords; w += kWave) lds_bits[w] = 0u;
+    __syncthreads();
+    const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+    const int myb = atom_bin[i] - s.bin0;
+    const int bz = myb % s.nbin[2], by = (myb / s.nbin[2]) % s.nbin[1], bx = myb / (s.nbin[2] * s.nbin[1]);
+    const int n0 = s.nbin[0] < 3 ? s.nbin[0] : 3, n1 = s.nbin[1] < 3 ? s.nbin[1] : 3, n2 = s.nbin[2] < 3 ? s.nbin[2] : 3;
+    const int nb = n0 * n1 * n2;  // <= 27
+    int bstart = 0, bcount = 0;
+    if (lane < nb) {
+        const int o2 = lane % n2, o1 = (lane / n2) % n1, o0 = lane / (n2 * n1);
+        const int cx = s.nbin[0] < 3 ? o0 : (bx + o0 - 1 + s.nbin[0]) % s.nbin[0];
+        const int cy = s.nbin[1] < 3 ? o1 : (by + o1 - 1 + s.nbin[1]) % s.nbin[1];
+        const int cz = s.nbin[2] < 3 ? o2 : (bz + o2 - 1 + s.nbin[2]) % s.nbin[2];
+        const int b = s.bin0 + (cx * s.nbin[1] + cy) * s.nbin[2] + cz;
+        bstart = bin_start[b];
+        bcount = bin_count[b];
+    }
+    int ncand;
+    const int bfirst = wave_excl_scan(bcount, lane, ncand);
+    for (int c0 = 0; c0 < ncand; c0 += kWave) {
+        const int c = c0 + lane;
+        const int ob = owner_bin(bfirst, nb, c);
+        const int ostart = __shfl(bstart, ob, kWave), ofirst = __shfl(bfirst, ob, kWave);
+        if (c < ncand) {
+            const int gj = bin_atoms[ostart + (c - ofirst)];
+            if (gj != i) {
+                double rx, ry, rz;
+                cell_diff(s, xi - pos[3 * gj], yi - pos[3 * gj + 1], zi - pos[3 * gj + 2], rx, ry, rz);
+                if (rx * rx + ry * ry + rz * rz <= rc2) {
+                    const int lj = gj - s.atom0;
+                    atomicOr(&lds_bits[lj >> 5], 1u << (lj & 31));
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // lanes own contiguous word ranges; a wave prefix sum of their popcounts gives the slots
+    const int per = (nwords + kWave - 1) / kWave;
+    const int w0 = min(lane * per, nwords), w1 = min(w0 + per, nwords);
+    int mine = 0;
+    for (int w = w0; w < w1; ++w) mine += __popc(lds_bits[w]);
+    int kept;
+    int at = wave_excl_scan(mine, lane, kept);
+    const int64_t row = (int64_t)i * maxp;
+    for (int w = w0; w < w1; ++w) {
+        unsigned int m = lds_bits[w];
+        while (m) {
+            const int bit = __ffs(m) - 1;
+            m &= m - 1;
+            if (at < maxp) {
+                const int lj = (w << 5) + bit, gj = s.atom0 + lj;
+                double rx, ry, rz;
+                cell_diff(s, xi - pos[3 * gj], yi - pos[3 * gj + 1], zi - pos[3 * gj + 2], rx, ry, rz);
+                const double d = sqrt(rx * rx + ry * ry + rz * rz);
+                // one image per j: it is the minimum image when below ||R||_F (neighbor.cpp:132-140)
+                put_entry(row + at, lj, 0, type[gj], d < s.fro, rx, ry, rz, d, nbr_j, nbr_meta, nbr_r);
+            }
+            ++at;
+        }
+    }
+    if (lane == 0) cnt[i] = kept;
 }
 
 // Largest neighbour count of the pass (the row-overflow check).  A few hundred blocks each
@@ -386,12 +474,19 @@ hipError_t launch_neighbors(const NeighborArgs& a, hipStream_t st) {
         hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, a.bin_count, a.bin_start, a.nbins);
         hipLaunchKernelGGL(k_fill_bins, dim3((a.ntot + 255) / 256), dim3(256), 0, st, a.atom_bin, a.ntot, a.bin_start,
                            a.bin_fill, a.bin_atoms);
-        int n2cap = 1;
-        while (n2cap < a.maxp) n2cap <<= 1;
-        const size_t lds = (size_t)a.maxp * 32 + (size_t)a.maxp * 4 + (size_t)n2cap * 8;
-        hipLaunchKernelGGL(k_neigh_cells, dim3(a.ncell), dim3(kWave), lds, st, a.S, a.atom_struc, a.pos, a.type,
-                           a.cell_centres, a.atom_bin, a.bin_start, a.bin_count, a.bin_atoms, a.rc2, a.maxp, n2cap,
-                           a.cnt, a.nbr_j, a.nbr_meta, a.nbr_r, a.maxcnt);
+        const size_t bits_lds = (size_t)((a.cell_natoms_max + 31) / 32) * 4;
+        if (bits_lds <= 32768) {
+            hipLaunchKernelGGL(k_neigh_cells_bits, dim3(a.ncell), dim3(kWave), bits_lds, st, a.S, a.atom_struc, a.pos,
+                               a.type, a.cell_centres, a.atom_bin, a.bin_start, a.bin_count, a.bin_atoms, a.rc2,
+                               a.maxp, a.cnt, a.nbr_j, a.nbr_meta, a.nbr_r);
+        } else {  // very large structures: sort the kept candidates instead
+            int n2cap = 1;
+            while (n2cap < a.maxp) n2cap <<= 1;
+            const size_t lds = (size_t)a.maxp * 32 + (size_t)a.maxp * 4 + (size_t)n2cap * 8;
+            hipLaunchKernelGGL(k_neigh_cells, dim3(a.ncell), dim3(kWave), lds, st, a.S, a.atom_struc, a.pos, a.type,
+                               a.cell_centres, a.atom_bin, a.bin_start, a.bin_count, a.bin_atoms, a.rc2, a.maxp, n2cap,
+                               a.cnt, a.nbr_j, a.nbr_meta, a.nbr_r, a.maxcnt);
+        }
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     if (a.npair > 0) {
