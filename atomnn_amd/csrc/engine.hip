@@ -458,7 +458,12 @@ template <typename T>
 void Engine::descriptors() {
     const size_t lds = symm_lds(shape, sizeof(T) == 8, pb.plan.generic, lmaxp, pb.plan.ldg);
     if (lds > 65536) throw unsupported_error("too many neighbours per atom for the descriptor kernel's LDS");
-    const int mom = mom_flag(lds, sizeof(T) == 8, 1) ? 1 : 0;
+    int mom = mom_flag(lds, sizeof(T) == 8, 1) ? 1 : 0;
+    if (mom) {  // 2: every species takes the moments (smaller reduction buffer)
+        bool all = true;
+        for (int t = 0; t < pb.plan.T; ++t) all = all && pb.plan.mom_ok[t];
+        if (all) mom = 2;
+    }
     d_G.reserve((size_t)ntot * pb.plan.ldg * sizeof(T));
     HIPCHK(launch_symm<T>(shape, pb.plan.generic, (const DevPlan*)d_plan.p, (const double*)d_rfe.p, (const double*)d_rfr.p,
                           (const int*)d_type.p, (const int*)d_centres.p, ntot, (const int*)d_cnt.p,

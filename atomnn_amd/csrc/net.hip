@@ -11,7 +11,7 @@
 //                                                    DODZ::grad, nn.cpp:1620-1628
 // E_i = out_i (+ E_vac of the species with NNP_VACUUM_ENERGY, nnh.cpp:129).
 //
-// One wavefront handles a tile of 16 atoms.  FP64 form: v_mfma_f64_16x16x4f64, 16x16
+// A block of kNetWaves wavefronts handles a tile of 16 atoms.  FP64 form: v_mfma_f64_16x16x4f64, 16x16
 // output tiles, K in steps of 4; A lane l holds A[l%16][l/16], B lane l holds B[l/16][l%16],
 // and result register v of lane l is D[4v + l/16][l%16].  Mixed form: v_mfma_f32_16x16x32_bf16
 // with the activations and weights rounded to BF16 and FP32 accumulation; tanh and the
@@ -70,6 +70,11 @@ __device__ __forceinline__ d4_t tile_mm_f64(const double* A, int lda, const doub
     return acc;
 }
 
+// Waves per 16-atom tile: they share the tile's activations in LDS and split each layer's
+// 16-column output tiles between them (the layer widths, 32-256, give 2-16 tiles).
+constexpr int kNetWaves = 4;
+constexpr int kNetThreads = kNetWaves * kWave;
+
 struct NetTiles {
     int tile_start[kMaxSpecies + 1];  // first 16-atom tile of each species
     int atom_start[kMaxSpecies + 1];  // first atom of each species in the species-sorted list
@@ -81,7 +86,7 @@ __device__ __forceinline__ int tile_species(const NetTiles& nt, int ntypes, int 
     return s;
 }
 
-__global__ __launch_bounds__(kWave) void k_net_f64(const DevPlan* __restrict__ plan, const double* __restrict__ Wb,
+__global__ __launch_bounds__(kNetWaves * kWave) void k_net_f64(const DevPlan* __restrict__ plan, const double* __restrict__ Wb,
                                                    NetTiles nt, const int* __restrict__ spatoms,
                                                    const double* __restrict__ G, int ldg, int vac,
                                                    double* __restrict__ eatom, double* __restrict__ dEdG) {
@@ -90,7 +95,7 @@ __global__ __launch_bounds__(kWave) void k_net_f64(const DevPlan* __restrict__ p
     const DevNet& N = plan->net[s];
     const int* atoms = spatoms + nt.atom_start[s];
     const int M = nt.atom_start[s + 1] - nt.atom_start[s];
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x, wv = tid / kWave, lane = tid % kWave;
     const int r0 = (blockIdx.x - nt.tile_start[s]) * 16;
     const int maxw = net_maxw(N);
     double* cur = shd;
@@ -98,7 +103,7 @@ __global__ __launch_bounds__(kWave) void k_net_f64(const DevPlan* __restrict__ p
     double* der = shd + 32 * maxw;
     const int li = lane & 15, lk = lane >> 4;
     // scaled input
-    for (int idx = lane; idx < 16 * N.nin_pad; idx += kWave) {
+    for (int idx = tid; idx < 16 * N.nin_pad; idx += kNetThreads) {
         const int r = idx / N.nin_pad, k = idx % N.nin_pad;
         const int row = r0 + r;
         double v = 0.0;
@@ -116,7 +121,7 @@ __global__ __launch_bounds__(kWave) void k_net_f64(const DevPlan* __restrict__ p
         const double* W = Wb + N.woff[l];
         const double* b = Wb + N.boff[l];
         const int act = (l == N.L - 1) ? NEU_LINEAR : N.act;
-        for (int jt = 0; jt < out_pad / 16; ++jt) {
+        for (int jt = wv; jt < out_pad / 16; jt += kNetWaves) {
             const d4_t acc = tile_mm_f64<true>(cur, maxw, W, in_pad, in_pad, jt, li, lk);
             const int col = jt * 16 + li;
 #pragma unroll
@@ -137,14 +142,14 @@ __global__ __launch_bounds__(kWave) void k_net_f64(const DevPlan* __restrict__ p
         doff += 16 * out_pad;
     }
     // energies
-    if (lane < 16 && r0 + lane < M) {
-        const int at = atoms[r0 + lane];
-        eatom[at] = N.outb + cur[lane * maxw] * N.outw + (vac ? N.evac : 0.0);
+    if (tid < 16 && r0 + tid < M) {
+        const int at = atoms[r0 + tid];
+        eatom[at] = N.outb + cur[tid * maxw] * N.outw + (vac ? N.evac : 0.0);
     }
     // backward: delta_L = outw .* d_L (single output)
     const int last = N.wpad[N.L - 1];
     doff -= 16 * last;
-    for (int idx = lane; idx < 16 * last; idx += kWave) {
+    for (int idx = tid; idx < 16 * last; idx += kNetThreads) {
         const int r = idx / last, o = idx % last;
         nxt[r * maxw + o] = (o == 0) ? N.outw * der[doff + r * last + o] : 0.0;
     }
@@ -158,7 +163,7 @@ __global__ __launch_bounds__(kWave) void k_net_f64(const DevPlan* __restrict__ p
         const int K = N.wpad[l], J = N.wpad[l - 1];
         const double* W = Wb + N.woff[l];  // wpad[l] x wpad[l-1], row-major
         doff -= 16 * J;
-        for (int jt = 0; jt < J / 16; ++jt) {
+        for (int jt = wv; jt < J / 16; jt += kNetWaves) {
             const d4_t acc = tile_mm_f64<false>(cur, maxw, W, J, K, jt, li, lk);
             const int col = jt * 16 + li;
 #pragma unroll
@@ -176,7 +181,7 @@ __global__ __launch_bounds__(kWave) void k_net_f64(const DevPlan* __restrict__ p
     {
         const int K = N.wpad[0], J = N.nin_pad;
         const double* W = Wb + N.woff[0];
-        for (int jt = 0; jt < J / 16; ++jt) {
+        for (int jt = wv; jt < J / 16; jt += kNetWaves) {
             const d4_t acc = tile_mm_f64<false>(cur, maxw, W, J, K, jt, li, lk);
             const int col = jt * 16 + li;
 #pragma unroll
@@ -193,7 +198,7 @@ __global__ __launch_bounds__(kWave) void k_net_f64(const DevPlan* __restrict__ p
 // holds B[8(l/16) .. +7][l%16], result register v of lane l is D[4(l/16) + v][l%16].
 __device__ __forceinline__ __bf16 to_bf16(float x) { return (__bf16)x; }
 
-__global__ __launch_bounds__(kWave) void k_net_mixed(const DevPlan* __restrict__ plan, const float* __restrict__ Wf,
+__global__ __launch_bounds__(kNetWaves * kWave) void k_net_mixed(const DevPlan* __restrict__ plan, const float* __restrict__ Wf,
                                                      NetTiles nt, const int* __restrict__ spatoms,
                                                      const float* __restrict__ G, int ldg, int vac,
                                                      double* __restrict__ eatom, float* __restrict__ dEdG) {
@@ -202,7 +207,7 @@ __global__ __launch_bounds__(kWave) void k_net_mixed(const DevPlan* __restrict__
     const DevNet& N = plan->net[s];
     const int* atoms = spatoms + nt.atom_start[s];
     const int M = nt.atom_start[s + 1] - nt.atom_start[s];
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x, wv = tid / kWave, lane = tid % kWave;
     const int r0 = (blockIdx.x - nt.tile_start[s]) * 16;
     int maxw = N.nin_pad;
     for (int l = 0; l < N.L; ++l) maxw = N.wpad[l] > maxw ? N.wpad[l] : maxw;
@@ -210,7 +215,7 @@ __global__ __launch_bounds__(kWave) void k_net_mixed(const DevPlan* __restrict__
     float* nxt = shf + 16 * maxw;
     float* der = shf + 32 * maxw;
     const int li = lane & 15, lk = lane >> 4;
-    for (int idx = lane; idx < 16 * N.nin_pad; idx += kWave) {
+    for (int idx = tid; idx < 16 * N.nin_pad; idx += kNetThreads) {
         const int r = idx / N.nin_pad, k = idx % N.nin_pad;
         const int row = r0 + r;
         float v = 0.f;
@@ -228,7 +233,7 @@ __global__ __launch_bounds__(kWave) void k_net_mixed(const DevPlan* __restrict__
         const float* W = Wf + N.woff[l];
         const float* b = Wf + N.boff[l];
         const int act = (l == N.L - 1) ? NEU_LINEAR : N.act;
-        for (int jt = 0; jt < out_pad / 16; ++jt) {
+        for (int jt = wv; jt < out_pad / 16; jt += kNetWaves) {
             f4_t acc = {0.f, 0.f, 0.f, 0.f};
             for (int k0 = 0; k0 < in_pad; k0 += 32) {
                 bf8_t a, bv;
@@ -259,13 +264,13 @@ __global__ __launch_bounds__(kWave) void k_net_mixed(const DevPlan* __restrict__
         in_pad = out_pad;
         doff += 16 * out_pad;
     }
-    if (lane < 16 && r0 + lane < M) {
-        const int at = atoms[r0 + lane];
-        eatom[at] = (double)N.outb + (double)cur[lane * maxw] * N.outw + (vac ? N.evac : 0.0);
+    if (tid < 16 && r0 + tid < M) {
+        const int at = atoms[r0 + tid];
+        eatom[at] = (double)N.outb + (double)cur[tid * maxw] * N.outw + (vac ? N.evac : 0.0);
     }
     const int last = N.wpad[N.L - 1];
     doff -= 16 * last;
-    for (int idx = lane; idx < 16 * last; idx += kWave) {
+    for (int idx = tid; idx < 16 * last; idx += kNetThreads) {
         const int r = idx / last, o = idx % last;
         nxt[r * maxw + o] = (o == 0) ? (float)N.outw * der[doff + r * last + o] : 0.f;
     }
@@ -279,7 +284,7 @@ __global__ __launch_bounds__(kWave) void k_net_mixed(const DevPlan* __restrict__
         const int K = N.wpad[l], J = (l == 0) ? N.nin_pad : N.wpad[l - 1];
         const float* W = Wf + N.woff[l];
         if (l > 0) doff -= 16 * J;
-        for (int jt = 0; jt < J / 16; ++jt) {
+        for (int jt = wv; jt < J / 16; jt += kNetWaves) {
             f4_t acc = {0.f, 0.f, 0.f, 0.f};
             for (int k0 = 0; k0 < K; k0 += 32) {
                 bf8_t a, bv;
@@ -339,10 +344,10 @@ hipError_t launch_net(bool fp64, const DevPlan* plan, const DevPlan& hplan, cons
     size_t lds = 0;
     for (int s = 0; s < hplan.T; ++s) lds = std::max(lds, net_lds_bytes(hplan.net[s], fp64));
     if (fp64)
-        hipLaunchKernelGGL(k_net_f64, dim3(ntiles), dim3(kWave), lds, st, plan, (const double*)W, nt, spatoms,
+        hipLaunchKernelGGL(k_net_f64, dim3(ntiles), dim3(kNetThreads), lds, st, plan, (const double*)W, nt, spatoms,
                            (const double*)G, ldg, vac, eatom, (double*)dEdG);
     else
-        hipLaunchKernelGGL(k_net_mixed, dim3(ntiles), dim3(kWave), lds, st, plan, (const float*)W, nt, spatoms,
+        hipLaunchKernelGGL(k_net_mixed, dim3(ntiles), dim3(kNetThreads), lds, st, plan, (const float*)W, nt, spatoms,
                            (const float*)G, ldg, vac, eatom, (float*)dEdG);
     return hipGetLastError();
 }

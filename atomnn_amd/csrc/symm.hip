@@ -298,8 +298,8 @@ __device__ __forceinline__ double4 load_x4<float>(const float* src) {
 template <typename T, int NE, int NC, bool GEN>
 __device__ __forceinline__ void symm_body(const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta,
                                           const double* __restrict__ rfun_rs, int ti, int ntypes, int P, int maxp,
-                                          const T* su, T* Rf, T* red, T* Grow, const Segs& sg, T* nsx, T* mom,
-                                          int lane) {
+                                          const T* su, T* Rf, T* red, int red_elems, T* Grow, const Segs& sg,
+                                          T* nsx, T* mom, int lane) {
     // radial: cutoff once per neighbour, then (function, part) items over 64 lanes; the parts of
     // a function are summed in a fixed order
     {
@@ -357,7 +357,7 @@ __device__ __forceinline__ void symm_body(const DevPlan* __restrict__ plan, cons
     // angular blocks: moment expansion when the centre species allows it
     if (mom && plan->mom_ok[ti]) {
         // the monomial table goes through the reduction buffer `red` in chunks of `ch` entries
-        constexpr int ch = kWave * NE * NC / kMomK - 1;
+        const int ch = red_elems / kMomK - 1;
         T* Mom = mom;
         T* D = Mom + ntypes * NE * kMomK;
         T* S = D + ntypes * NE;
@@ -716,8 +716,8 @@ __device__ __forceinline__ void force_body(const DevPlan* __restrict__ plan, con
 // cutoff table nsx [maxp][2] that the non-separable angular factors use.  GEN = false keeps the
 // common case's register budget.
 template <typename T, int NE, int NC, bool GEN>
-__host__ __device__ constexpr size_t symm_lds_bytes(int maxp, int ldg) {
-    return sizeof(T) * (size_t)(4 * maxp + NE * maxp + kWave * NE * NC) + sizeof(T) * (size_t)((ldg + 1) & ~1) +
+__host__ __device__ constexpr size_t symm_lds_bytes(int maxp, int ldg, int red_elems = kWave * NE * NC) {
+    return sizeof(T) * (size_t)(4 * maxp + NE * maxp + red_elems) + sizeof(T) * (size_t)((ldg + 1) & ~1) +
            (GEN ? sizeof(T) * 2 * (size_t)maxp : 0);
 }
 template <typename T, int NE, int NC, bool GEN>
@@ -737,12 +737,12 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
     const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta, const double* __restrict__ rfun_rs,
     const int* __restrict__ atom_type, const int* __restrict__ centres, const int* __restrict__ cnt,
     const int* __restrict__ nbr_meta, const double4* __restrict__ nbr_r, int maxp, int lmaxp, T* __restrict__ G,
-    int ldg, int use_mom) {
+    int ldg, int use_mom, int red_elems) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     T* su = (T*)lds_raw;
     T* Rf = su + 4 * lmaxp;
-    T* red = Rf + NE * lmaxp;
-    T* Grow = red + kWave * NE * NC;
+    T* red = Rf + NE * lmaxp;  // red_elems: kWave NE NC, or less when every species takes the moments
+    T* Grow = red + red_elems;
     T* nsx = Grow + ((ldg + 1) & ~1);
     T* mom = use_mom ? nsx + (GEN ? 2 * lmaxp : 0) : nullptr;
     const int i = centres[blockIdx.x];
@@ -755,7 +755,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
     stage_entries<T>(i, P, maxp, ntypes, nbr_meta, nbr_r, su, nullptr, sg, lane);
     for (int k = lane; k < ldg; k += kWave) Grow[k] = T(0);
     __syncthreads();
-    symm_body<T, NE, NC, GEN>(plan, rfun_eta, rfun_rs, ti, ntypes, P, lmaxp, su, Rf, red, Grow, sg, nsx, mom, lane);
+    symm_body<T, NE, NC, GEN>(plan, rfun_eta, rfun_rs, ti, ntypes, P, lmaxp, su, Rf, red, red_elems, Grow, sg, nsx,
+                              mom, lane);
     __syncthreads();
     for (int k = lane; k < ldg; k += kWave) G[(int64_t)i * ldg + k] = Grow[k];
 }
@@ -924,7 +925,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
         stage_entries<T>(i, P, maxp, ntypes, nbr_meta, nbr_r, su, orig, sg, lane);
         for (int k = lane; k < ldg; k += kWave) Grow[k] = T(0);
         __syncthreads();
-        symm_body<T, NE, NC, GEN>(plan, rfun_eta, rfun_rs, ti, ntypes, P, lmaxp, su, Rf, red, Grow, sg, nsx,
+        symm_body<T, NE, NC, GEN>(plan, rfun_eta, rfun_rs, ti, ntypes, P, lmaxp, su, Rf, red, kWave * NE * NC, Grow,
+                                  sg, nsx,
                                   (use_mom & 1) ? mom : nullptr, lane);
         __syncthreads();
         const double E = net_body<T, GEN>(plan->net[ti], Wt, Grow, act0, act1, der, dErow, maxw, vac, lane);
@@ -1081,10 +1083,13 @@ template <typename T, int NE, int NC, bool GEN>
 static hipError_t launch_symm_t(const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
                                 const int* centres, int ncentres, const int* cnt, const int* meta, const double4* r,
                                 int maxp, int lmaxp, T* G, int ldg, int use_mom, int ntypes, hipStream_t st) {
-    const size_t lds = symm_lds_bytes<T, NE, NC, GEN>(lmaxp, ldg) + (use_mom ? mom_lds_bytes<T, NE>(lmaxp, ntypes, 1) : 0);
+    // use_mom 2: every species takes the moments, the reduction buffer only holds monomial chunks
+    const int red = use_mom == 2 ? 16 * kMomK : kWave * NE * NC;
+    const size_t lds =
+        symm_lds_bytes<T, NE, NC, GEN>(lmaxp, ldg, red) + (use_mom ? mom_lds_bytes<T, NE>(lmaxp, ntypes, 1) : 0);
     if (lds > 65536) return hipErrorInvalidConfiguration;
     hipLaunchKernelGGL((k_symm<T, NE, NC, GEN>), dim3(ncentres), dim3(kWave), lds, st, plan, rfe, rfr, atom_type,
-                       centres, cnt, meta, r, maxp, lmaxp, G, ldg, use_mom);
+                       centres, cnt, meta, r, maxp, lmaxp, G, ldg, use_mom, red);
     return hipGetLastError();
 }
 
