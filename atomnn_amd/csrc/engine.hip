@@ -78,16 +78,6 @@ static thread_local std::string g_err;
 void set_error(const std::string& s) { g_err = s; }
 const char* last_error() { return g_err.c_str(); }
 
-struct hip_failure : std::runtime_error {
-    using std::runtime_error::runtime_error;
-};
-
-#define HIPCHK(expr)                                                                                          \
-    do {                                                                                                      \
-        hipError_t e_ = (expr);                                                                               \
-        if (e_ != hipSuccess)                                                                                 \
-            throw hip_failure(std::string("HIP error in ") + #expr + ": " + hipGetErrorString(e_));            \
-    } while (0)
 
 // ------------------------------------------------------------------ device buffers
 void DevMem::reserve(size_t bytes) {
@@ -458,12 +448,9 @@ template <typename T>
 void Engine::descriptors() {
     const size_t lds = symm_lds(shape, sizeof(T) == 8, pb.plan.generic, lmaxp, pb.plan.ldg);
     if (lds > 65536) throw unsupported_error("too many neighbours per atom for the descriptor kernel's LDS");
-    int mom = mom_flag(lds, sizeof(T) == 8, 1) ? 1 : 0;
-    if (mom) {  // 2: every species takes the moments (smaller reduction buffer)
-        bool all = true;
-        for (int t = 0; t < pb.plan.T; ++t) all = all && pb.plan.mom_ok[t];
-        if (all) mom = 2;
-    }
+    // (mom = 2 would shrink the reduction buffer when every species takes the moments; measured
+    // slower on the 4096-atom cell: more monomial chunks and barriers outweigh the occupancy)
+    const int mom = mom_flag(lds, sizeof(T) == 8, 1) ? 1 : 0;
     d_G.reserve((size_t)ntot * pb.plan.ldg * sizeof(T));
     HIPCHK(launch_symm<T>(shape, pb.plan.generic, (const DevPlan*)d_plan.p, (const double*)d_rfe.p, (const double*)d_rfr.p,
                           (const int*)d_type.p, (const int*)d_centres.p, ntot, (const int*)d_cnt.p,

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -15,6 +16,17 @@ namespace atomnn {
 
 void set_error(const std::string& s);
 const char* last_error();
+
+struct hip_failure : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define HIPCHK(expr)                                                                                          \
+    do {                                                                                                      \
+        hipError_t e_ = (expr);                                                                               \
+        if (e_ != hipSuccess)                                                                                 \
+            throw hip_failure(std::string("HIP error in ") + #expr + ": " + hipGetErrorString(e_));            \
+    } while (0)
 
 // A device allocation that only grows.
 struct DevMem {

@@ -87,6 +87,11 @@ def lib():
         L.nnp_profile_read.argtypes = [vp, vp, vp, ctypes.c_int]
         L.nnp_info.argtypes = [vp, vp, ctypes.c_int]
         L.nnp_set_option.argtypes = [vp, ctypes.c_char_p, i64]
+        L.nnp_md_create.argtypes = [vp, ctypes.POINTER(_Struc), vp, vp, dp, ctypes.c_uint32, ctypes.POINTER(vp)]
+        L.nnp_md_step.argtypes = [vp, ctypes.c_int]
+        L.nnp_md_state.argtypes = [vp, vp, vp, vp, vp]
+        L.nnp_md_destroy.argtypes = [vp]
+        L.nnp_md_destroy.restype = None
         _lib = L
     return _lib
 
@@ -243,3 +248,42 @@ class Resident:
         calls = np.zeros(len(STAGES), dtype=np.int64)
         _check(lib().nnp_profile_read(self.m._h, ms.ctypes.data, calls.ctypes.data, len(STAGES)))
         return ms, calls
+
+
+class MD:
+    """Velocity-Verlet molecular dynamics on the GPU (METAL units: A, eV, amu, fs), the step of
+    the reference's Verlet::compute (torch/integrator.cpp:170-188).  masses: one per species."""
+
+    def __init__(self, model: NNP, s: Structure, masses: Sequence[float], vel=None, dt: float = 1.0,
+                 flags: int = NNP_FORCE):
+        self.n = s.natoms
+        pin = _Pinned(s)
+        m = np.ascontiguousarray(masses, dtype=np.float64)
+        v = None if vel is None else np.ascontiguousarray(vel, dtype=np.float64).reshape(self.n, 3)
+        h = ctypes.c_void_p()
+        _check(lib().nnp_md_create(model._h, ctypes.byref(pin.c_struct), m.ctypes.data,
+                                   None if v is None else v.ctypes.data, float(dt), flags, ctypes.byref(h)))
+        self._h = h
+
+    def step(self, nsteps: int = 1) -> None:
+        _check(lib().nnp_md_step(self._h, int(nsteps)))
+
+    def state(self) -> dict:
+        pos = np.zeros((self.n, 3))
+        vel = np.zeros((self.n, 3))
+        frc = np.zeros((self.n, 3))
+        e = np.zeros(4)
+        _check(lib().nnp_md_state(self._h, pos.ctypes.data, vel.ctypes.data, frc.ctypes.data, e.ctypes.data))
+        return {"pos": pos, "vel": vel, "force": frc, "epot": float(e[0]), "ekin": float(e[1]),
+                "temperature": float(e[2]), "steps": int(e[3])}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().nnp_md_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

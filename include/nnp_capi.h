@@ -158,11 +158,31 @@ int nnp_profile_read(nnp_model* m, double* ms, int64_t* calls, int nstage);
  *                   (default);
  *               1 = one persistent fused descriptor + network + force kernel (the batched form;
  *                   networks evaluated in-wave, so results differ from pipeline 0 in the last bits)
- *   "fused_blocks_per_cu"  persistent grid of the fused kernel; 0 = chosen by the autotuner */
+ *   "fused_blocks_per_cu"  persistent grid of the fused kernel; 0 = chosen by the autotuner
+ *   "moments"   bit 0: descriptor angular sums by the moment expansion (default on), bit 1: the
+ *               angular force terms by it (default off); models outside its domain ignore it */
 int nnp_set_option(nnp_model* m, const char* key, int64_t value);
 /* Engine state: maxp, max entries, atoms, structures, kernel shape, NE, NC, descriptor width,
- * pipeline, fused blocks per CU (tuned), compute units. */
+ * pipeline, fused blocks per CU (tuned), compute units, moment bits of the last pass. */
 int nnp_info(nnp_model* m, int64_t* out, int nout);
+
+/* ---- molecular dynamics on the device-resident structure ----
+ * Replaces the reference's MD step, Verlet::compute (src/torch/integrator.cpp:170-188: half
+ * kick, drift, Cell::returnToCell, Engine::compute -> Pot::compute (src/torch/engine.cpp:
+ * 107-119, pot_nnpe.cpp:175-295), half kick, kinetic energy and temperature).  METAL units:
+ * positions A, velocities A/fs, masses amu (one per species), dt fs, energies eV.
+ * flags as for nnp_compute (NNP_FORCE implied; NNP_ALL_IMAGES gives true -dE/dr forces, which
+ * conserve energy also in cells shorter than 2 rc).  The model handle stays usable: the MD
+ * state has its own workspace.  nnp_md_step advances nsteps steps on the device (each step
+ * one replayed graph; the host waits once per call).  nnp_md_state copies positions,
+ * velocities and forces (any may be NULL) and energies[4] = {potential, kinetic (eV),
+ * temperature (K), steps taken}. */
+typedef struct nnp_md nnp_md;
+int nnp_md_create(nnp_model* m, const nnp_structure* s, const double* masses, const double* vel, double dt,
+                  uint32_t flags, nnp_md** out);
+int nnp_md_step(nnp_md* md, int nsteps);
+int nnp_md_state(nnp_md* md, double* pos, double* vel, double* force, double* energies);
+void nnp_md_destroy(nnp_md* md);
 
 #ifdef __cplusplus
 }
