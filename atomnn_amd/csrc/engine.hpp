@@ -122,9 +122,8 @@ class Engine {
     // moments: bit 0 = descriptor angular sums, bit 1 = angular force terms by the moment
     // expansion, where the model qualifies (integer zeta <= 4, |lambda| <= 1, one shared
     // exponent table and separable distance factor) and the LDS fits; clear bits use the
-    // triplet loops.  Default: descriptors only (the triplet force loop is the faster one at
-    // 45-70 neighbours per atom).
-    int moments = 1;
+    // triplet loops.  Default: both (O(P) per centre against the triplet loops' O(P^2)).
+    int moments = 3;
     int last_mom = 0;  // the moment bits the last pass used
     int mom_flag(size_t base_lds, bool fp64, int bits);
     int num_cus = 0;

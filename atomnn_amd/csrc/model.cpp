@@ -357,19 +357,20 @@ PlanBuffers build_plan(const HostModel& m) {
         }
         // moment expansion eligibility of this centre species
         {
-            bool ok = true;
+            bool shared = true, ok = true;
             const DevAngular& A0 = P.ang[i][0];
-            for (int b = 0; b < nblk && ok; ++b) {
+            for (int b = 0; b < nblk && shared; ++b) {
                 const DevAngular& A = P.ang[i][b];
                 if (A.ne != A0.ne || A.alpha != A0.alpha || A.cut != A0.cut || A.rc != A0.rc || A.phi != A0.phi ||
                     !angular_separable(A.phi))
-                    ok = false;
-                for (int e = 0; e < A.ne && ok; ++e)
-                    if (A.ietap[e] != A0.ietap[e]) ok = false;
-                for (int c = 0; c < A.nc && ok; ++c)
+                    shared = false;
+                for (int e = 0; e < A.ne && shared; ++e)
+                    if (A.ietap[e] != A0.ietap[e]) shared = false;
+                for (int c = 0; c < A.nc; ++c)
                     if (A.zint[c] < 0 || A.zint[c] > 4 || std::fabs(A.lambda[c]) > 1.0) ok = false;
             }
-            P.mom_ok[i] = ok ? 1 : 0;
+            P.ang_shared[i] = shared ? 1 : 0;
+            P.mom_ok[i] = (shared && ok) ? 1 : 0;
         }
         // network: weights padded (outputs to 16, inputs to 16) and stored row-major
         const NetSpec& net = s.net;

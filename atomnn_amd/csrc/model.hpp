@@ -139,6 +139,9 @@ struct DevPlan {
     // moment path per centre species: every angular block has integer zeta in [0, 4], |lambda| <= 1,
     // and all blocks share one exponent table, alpha and cutoff (see symm.hip)
     int mom_ok[kMaxSpecies];
+    // every angular block of the centre species shares one exponent table, alpha and cutoff and
+    // has a separable distance factor: the entry tables R_e(p), R'_e(p) are computed once
+    int ang_shared[kMaxSpecies];
     // the model uses a radial function other than GAUSSIAN, an angular factor other than GAUSS,
     // a non-integer zeta or an activation other than tanh / linear: the kernels' generic
     // instantiations run it

@@ -10,114 +10,173 @@
 // it is (S_aa - D_a)/2 with the diagonal D_a(e) = sum_{j in a} R_e(j)^2 (cos_jj = 1).
 //
 // Forces: with w_{ab,e,p} = sum_c dE/dG_f norm 2^-zeta_c C(zeta_c,p) lambda_c^p (x 1/2 for a == b)
-// the adjoint moments are  L[s][e][a] = (p!/a!) (sum_{b != s} w_{sb,e,p} M_b + 2 w_{ss,e,p} M_s)
-// and, for entry j of species s,
-//   dE/dr_j = sum_e [ (sum_a L u^a - 2 Delta_s(e) R_e) R'_e u_j + R_e (sum_a L grad_u u^a
-//                     - (sum_a |a| L u^a) u_j) / d_j ],   Delta_s(e) = sum_p w_{ss,e,p},
-// using grad_r u^a = (grad_u u^a - |a| u^a u) / d (Euler's theorem for degree-|a| monomials).
+// the adjoint moments are  L[s][a][e] = (p!/a!) (sum_{b != s} w_{sb,e,p} M_b + 2 w_{ss,e,p} M_s)
+// and, for entry j of species s, with the polynomial  Q_e(u) = sum_a L[s][a][e] u^a  and its
+// gradient h_e = grad_u Q_e (u taken as three free variables),
+//   dE/dr_j = sum_e [ (Q_e(u_j) - 2 Delta_s(e) R_e) R'_e u_j + R_e (h_e - (u_j . h_e) u_j) / d_j ],
+//   Delta_s(e) = sum_p w_{ss,e,p},
+// using grad_r u^a = (grad_u u^a - |a| u^a u) / d and Euler's theorem sum_a |a| L u^a = u . h.
+//
+// GPU mapping (one wavefront per centre):
+//  * moments: lane q < 35 owns monomial q and sums R_e(p) u_p^q over the species' entries in a
+//    fixed order, reading a per-entry power table (1, u, u^2, u^3, u^4 per axis, 15 values)
+//    from LDS: three broadcast-row reads and two products per entry instead of a stored
+//    35-column monomial table;
+//  * forces: lane p owns entry p, keeps the 20 monomials of degree <= 3 in registers and walks
+//    the 35 monomials unrolled at compile time, so each adjoint row L[s][q][0..NE) is read once
+//    and feeds the value and all three gradient components.
 #pragma once
+
+#include <utility>
 
 #include "device.hpp"
 #include "model.hpp"
 
 namespace atomnn {
 
-constexpr int kMomZ = 4;   // highest monomial degree
-constexpr int kMomK = 35;  // monomials of degree <= 4 in three variables
-constexpr int kMomK3 = 20; // monomials of degree <= 3
+constexpr int kMomZ = 4;    // highest monomial degree
+constexpr int kMomK = 35;   // monomials of degree <= 4 in three variables
+constexpr int kMomK3 = 20;  // monomials of degree <= 3
+constexpr int kMomPw = 15;  // per-entry power table: (1, x, x^2, x^3, x^4) per axis
+constexpr int kMomCh = 32;  // entries per power-table chunk in the force path
 
-// Exponents (ax, ay, az) of the monomials in degree order; the first 20 have degree <= 3.
-__constant__ static const signed char kMono[kMomK][3] = {
-    {0, 0, 0},                                                                                // degree 0
-    {1, 0, 0}, {0, 1, 0}, {0, 0, 1},                                                          // 1
-    {2, 0, 0}, {1, 1, 0}, {1, 0, 1}, {0, 2, 0}, {0, 1, 1}, {0, 0, 2},                         // 2
-    {3, 0, 0}, {2, 1, 0}, {2, 0, 1}, {1, 2, 0}, {1, 1, 1}, {1, 0, 2}, {0, 3, 0}, {0, 2, 1},   // 3
-    {0, 1, 2}, {0, 0, 3},
-    {4, 0, 0}, {3, 1, 0}, {3, 0, 1}, {2, 2, 0}, {2, 1, 1}, {2, 0, 2}, {1, 3, 0}, {1, 2, 1},   // 4
-    {1, 1, 2}, {1, 0, 3}, {0, 4, 0}, {0, 3, 1}, {0, 2, 2}, {0, 1, 3}, {0, 0, 4}};
+// Monomials in degree order (x exponent descending, then y): the first 20 have degree <= 3.
+struct MomTab {
+    signed char a[kMomK][3];    // exponents
+    signed char low[kMomK][3];  // index of the monomial with a_c lowered by one, or -1
+    unsigned char mult[kMomK];  // multinomial coefficient |a|! / (ax! ay! az!)
+    unsigned char deg[kMomK];
+};
 
-__device__ __forceinline__ int mono_deg(int q) { return kMono[q][0] + kMono[q][1] + kMono[q][2]; }
-
-// multinomial coefficient p! / (ax! ay! az!)
-__device__ __forceinline__ int mono_mult(int q) {
+constexpr MomTab make_mom_tab() {
+    MomTab t{};
+    int q = 0;
+    for (int d = 0; d <= kMomZ; ++d)
+        for (int ax = d; ax >= 0; --ax)
+            for (int ay = d - ax; ay >= 0; --ay) {
+                t.a[q][0] = (signed char)ax;
+                t.a[q][1] = (signed char)ay;
+                t.a[q][2] = (signed char)(d - ax - ay);
+                t.deg[q] = (unsigned char)d;
+                ++q;
+            }
     const int f[5] = {1, 1, 2, 6, 24};
-    return f[mono_deg(q)] / (f[kMono[q][0]] * f[kMono[q][1]] * f[kMono[q][2]]);
-}
-
-// index of the monomial with exponents (ax, ay, az), degree <= 4
-__device__ __forceinline__ int mono_index(int ax, int ay, int az) {
-    for (int q = 0; q < kMomK; ++q)
-        if (kMono[q][0] == ax && kMono[q][1] == ay && kMono[q][2] == az) return q;
-    return -1;
-}
-
-__device__ __forceinline__ int binom4(int z, int p) {
-    const int c[5][5] = {{1, 0, 0, 0, 0}, {1, 1, 0, 0, 0}, {1, 2, 1, 0, 0}, {1, 3, 3, 1, 0}, {1, 4, 6, 4, 1}};
-    return c[z][p];
-}
-
-// u^a for all 35 monomials of one unit vector
-template <typename T>
-__device__ __forceinline__ void monomials(T ux, T uy, T uz, T* m) {
-    T px[5], py[5], pz[5];
-    px[0] = py[0] = pz[0] = T(1);
-#pragma unroll
-    for (int k = 1; k < 5; ++k) {
-        px[k] = px[k - 1] * ux;
-        py[k] = py[k - 1] * uy;
-        pz[k] = pz[k - 1] * uz;
+    for (int r = 0; r < kMomK; ++r) {
+        t.mult[r] = (unsigned char)(f[t.deg[r]] / (f[t.a[r][0]] * f[t.a[r][1]] * f[t.a[r][2]]));
+        for (int c = 0; c < 3; ++c) {
+            t.low[r][c] = -1;
+            if (t.a[r][c] == 0) continue;
+            for (int s = 0; s < kMomK; ++s) {
+                bool same = true;
+                for (int k = 0; k < 3; ++k)
+                    if (t.a[s][k] != t.a[r][k] - (k == c ? 1 : 0)) same = false;
+                if (same) t.low[r][c] = (signed char)s;
+            }
+        }
     }
-#pragma unroll
-    for (int q = 0; q < kMomK; ++q) m[q] = px[kMono[q][0]] * py[kMono[q][1]] * pz[kMono[q][2]];
+    return t;
 }
 
-// LDS of the moment path, sizes in elements of T.  Descriptors: Mom [S][NE][K] | D [S][NE] |
-// S table [blocks][NE][5] (the monomial table lives in chunks in the kernel's reduction buffer).
-// Forces: mono [K][lmaxp+1] | Mom | D | w table [blocks][NE][5] | L, Lp [S][NE][K] | Lx, Ly, Lz
-// [S][NE][20].
+// compile-time copy (folded in unrolled code) and device copy (lane-varying lookups)
+constexpr MomTab kMomTabC = make_mom_tab();
+__constant__ const MomTab kMomTab = make_mom_tab();
+
+constexpr int mom_first_dir(int q) {
+    return kMomTabC.a[q][0] > 0 ? 0 : (kMomTabC.a[q][1] > 0 ? 1 : 2);
+}
+
+// LDS of the moment path, in elements of T.
+//   descriptors: Mom [S][NE][K] | D [S][NE] | S table [blocks][NE][5]  (the power table goes
+//                through the kernel's reduction buffer in chunks)
+//   forces:      Mom | D (Delta) | w table [blocks][NE][5] | U
+//                U = power table [kMomCh][15], then the adjoint table L [S][K][NE]
 template <int NE>
 __host__ __device__ constexpr int mom_symm_elems(int ntypes_max) {
     return ntypes_max * NE * kMomK + ntypes_max * NE + (ntypes_max * (ntypes_max + 1) / 2) * NE * 5;
 }
 template <int NE>
-__host__ __device__ constexpr int mom_lds_elems(int lmaxp, int ntypes_max) {
-    return kMomK * (lmaxp + 1) + mom_symm_elems<NE>(ntypes_max) + 2 * ntypes_max * NE * kMomK +
-           3 * ntypes_max * NE * kMomK3;
+__host__ __device__ constexpr int mom_u_elems(int ntypes_max) {
+    return kMomCh * kMomPw > ntypes_max * kMomK * NE ? kMomCh * kMomPw : ntypes_max * kMomK * NE;
+}
+template <int NE>
+__host__ __device__ constexpr int mom_lds_elems(int ntypes_max) {
+    return mom_symm_elems<NE>(ntypes_max) + mom_u_elems<NE>(ntypes_max);
 }
 
-// Per-species moments M and diagonals D of the staged entries (Rf holds R_e(p) for every entry,
-// radial_tables over [0, P)).  The monomial table mono [K][ch + 1] holds `ch` entries at a time.
+// Power table rows for the entries [c0, c1): pw[(p - c0) * 15 + 5 c + k] = u_c^k.
+template <typename T>
+__device__ __forceinline__ void power_rows(const T* su, int c0, int c1, T* pw, int lane) {
+    for (int p = c0 + lane; p < c1; p += kWave) {
+        T* row = pw + (p - c0) * kMomPw;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const T x = su[4 * p + c], x2 = x * x;
+            row[5 * c + 0] = T(1);
+            row[5 * c + 1] = x;
+            row[5 * c + 2] = x2;
+            row[5 * c + 3] = x2 * x;
+            row[5 * c + 4] = x2 * x2;
+        }
+    }
+}
+
+// Per-species moments Mom[s][e][q] of the staged entries (Rf holds R_e(p) for every entry) and,
+// when D is given, the diagonals D[s][e] = sum_{p in s} R_e(p)^2.  The power table pw holds
+// `ch` entries at a time.  Every sum runs over the entries in index order.
 template <typename T, int NE>
 __device__ __forceinline__ void build_moments(const T* su, const T* Rf, int P, int ch, int ntypes, const int* seg,
-                                              T* mono, T* Mom, T* D, int lane) {
-    const int ms = ch + 1;
-    const int items = ntypes * NE * kMomK;
-    for (int it = lane; it < items; it += kWave) Mom[it] = T(0);
-    for (int c0 = 0; c0 < P; c0 += ch) {
-        const int c1 = min(P, c0 + ch);
-        __syncthreads();
-        for (int p = c0 + lane; p < c1; p += kWave) {
-            T m[kMomK];
-            monomials<T>(su[4 * p], su[4 * p + 1], su[4 * p + 2], m);
-#pragma unroll
-            for (int q = 0; q < kMomK; ++q) mono[q * ms + (p - c0)] = m[q];
-        }
-        __syncthreads();
-        for (int it = lane; it < items; it += kWave) {
-            const int q = it % kMomK, e = (it / kMomK) % NE, s = it / (kMomK * NE);
-            const int lo = max(seg[s], c0), hi = min(seg[s + 1], c1);
-            T acc = Mom[it];
-            for (int p = lo; p < hi; ++p) acc += mono[q * ms + (p - c0)] * Rf[p * NE + e];
-            Mom[it] = acc;  // [s][e][q]
-        }
+                                              T* pw, T* Mom, T* D, int lane) {
+    ch = max(ch, 1);
+    const int q = lane;
+    int ox = 0, oy = 5, oz = 10;
+    if (q < kMomK) {
+        ox = kMomTab.a[q][0];
+        oy = 5 + kMomTab.a[q][1];
+        oz = 10 + kMomTab.a[q][2];
     }
-    for (int it = lane; it < ntypes * NE; it += kWave) {
-        const int e = it % NE, s = it / NE;
-        T acc = T(0);
-        for (int p = seg[s]; p < seg[s + 1]; ++p) acc += Rf[p * NE + e] * Rf[p * NE + e];
-        D[it] = acc;  // [s][e]
+    for (int c0 = 0; c0 < P || c0 == 0; c0 += ch) {
+        const int c1 = min(P, c0 + ch);
+        __syncthreads();  // the previous chunk's readers are done
+        power_rows<T>(su, c0, c1, pw, lane);
+        __syncthreads();
+        if (q < kMomK) {
+            for (int s = 0; s < ntypes; ++s) {
+                const int lo = max(seg[s], c0), hi = min(seg[s + 1], c1);
+                T a[NE];
+#pragma unroll
+                for (int e = 0; e < NE; ++e) a[e] = c0 == 0 ? T(0) : Mom[(s * NE + e) * kMomK + q];
+                for (int p = lo; p < hi; ++p) {
+                    const T* row = pw + (p - c0) * kMomPw;
+                    const T m = row[ox] * row[oy] * row[oz];
+#pragma unroll
+                    for (int e = 0; e < NE; ++e) a[e] += Rf[p * NE + e] * m;
+                }
+#pragma unroll
+                for (int e = 0; e < NE; ++e) Mom[(s * NE + e) * kMomK + q] = a[e];
+            }
+        }
+        if (c1 >= P) break;
+    }
+    if (D) {
+        for (int it = lane; it < ntypes * NE; it += kWave) {
+            const int e = it % NE, s = it / NE;
+            T a = T(0);
+            for (int p = seg[s]; p < seg[s + 1]; ++p) a += Rf[p * NE + e] * Rf[p * NE + e];
+            D[it] = a;  // [s][e]
+        }
     }
     __syncthreads();
+}
+
+__device__ __forceinline__ int binom4(int z, int p) {
+    return z == 4 ? (p == 0 || p == 4 ? 1 : (p == 2 ? 6 : 4)) : (z == 3 ? (p == 0 || p == 3 ? 1 : 3) : (z == 2 && p == 1 ? 2 : (p <= z ? 1 : 0)));
+}
+
+__device__ __forceinline__ void blk_species(int blk, int& a, int& b) {
+    int bb = 0;
+    while ((bb + 1) * (bb + 2) / 2 <= blk) ++bb;
+    b = bb;
+    a = blk - bb * (bb + 1) / 2;
 }
 
 // Angular descriptors of centre species ti from the moments.
@@ -128,15 +187,13 @@ __device__ __forceinline__ void symm_angular_moments(const DevPlan* __restrict__
     // S[blk][e][p] = sum_{|a| = p} (p!/a!) M_a[e][a] M_b[e][a]
     for (int it = lane; it < nblk * NE * (kMomZ + 1); it += kWave) {
         const int pd = it % (kMomZ + 1), e = (it / (kMomZ + 1)) % NE, blk = it / ((kMomZ + 1) * NE);
-        int bb = 0;
-        while ((bb + 1) * (bb + 2) / 2 <= blk) ++bb;
-        const int a = blk - bb * (bb + 1) / 2;
+        int a, bb;
+        blk_species(blk, a, bb);
         const T* Ma = Mom + (a * NE + e) * kMomK;
         const T* Mb = Mom + (bb * NE + e) * kMomK;
         T acc = T(0);
-        const int q0 = pd == 0 ? 0 : (pd == 1 ? 1 : (pd == 2 ? 4 : (pd == 3 ? 10 : 20)));
-        const int q1 = pd == 0 ? 1 : (pd == 1 ? 4 : (pd == 2 ? 10 : (pd == 3 ? 20 : 35)));
-        for (int q = q0; q < q1; ++q) acc += T(mono_mult(q)) * Ma[q] * Mb[q];
+        const int q0 = pd * (pd + 1) * (pd + 2) / 6, q1 = (pd + 1) * (pd + 2) * (pd + 3) / 6;
+        for (int q = q0; q < q1; ++q) acc += T(kMomTab.mult[q]) * Ma[q] * Mb[q];
         S[it] = acc;
     }
     __syncthreads();
@@ -147,9 +204,8 @@ __device__ __forceinline__ void symm_angular_moments(const DevPlan* __restrict__
         if (e >= A.ne || c >= A.nc) continue;
         const int f = A.fmap[e * kMaxC + c];
         if (f < 0) continue;
-        int bb = 0;
-        while ((bb + 1) * (bb + 2) / 2 <= blk) ++bb;
-        const int a = blk - bb * (bb + 1) / 2;
+        int a, bb;
+        blk_species(blk, a, bb);
         const int z = A.zint[c];
         const T lam = T(A.lambda[c]);
         T acc = T(0), lp = T(1);
@@ -163,20 +219,17 @@ __device__ __forceinline__ void symm_angular_moments(const DevPlan* __restrict__
     }
 }
 
-// Adjoint moments from dE/dG (row W), then the angular force term of every entry into Xs.
+// Adjoint tables from dE/dG (row W): w table, Delta[s][e] (into D) and L[s][q][e] (into U, which
+// held the power table: the caller's barrier after the moments separates the two uses).
 template <typename T, int NE>
-__device__ __forceinline__ void force_angular_moments(const DevPlan* __restrict__ plan, int ti, int ntypes, int P,
-                                                      int maxp, const T* su, const T* Rf, const T* Rd, const T* W,
-                                                      const int* seg, const T* mono, const T* Mom, T* wtab, T* Lam,
-                                                      T* Lamp, T* Lx, T* Ly, T* Lz, T* Xs, int lane) {
+__device__ __forceinline__ void adjoint_moments(const DevPlan* __restrict__ plan, int ti, int ntypes, const T* W,
+                                                const T* Mom, T* Delta, T* wtab, T* L, int lane) {
     const int nblk = ntypes * (ntypes + 1) / 2;
-    // w_{blk,e,p}
     for (int it = lane; it < nblk * NE * (kMomZ + 1); it += kWave) {
         const int pd = it % (kMomZ + 1), e = (it / (kMomZ + 1)) % NE, blk = it / ((kMomZ + 1) * NE);
         const DevAngular& A = plan->ang[ti][blk];
-        int bb = 0;
-        while ((bb + 1) * (bb + 2) / 2 <= blk) ++bb;
-        const int a = blk - bb * (bb + 1) / 2;
+        int a, bb;
+        blk_species(blk, a, bb);
         T acc = T(0);
         if (e < A.ne) {
             for (int c = 0; c < A.nc; ++c) {
@@ -184,7 +237,7 @@ __device__ __forceinline__ void force_angular_moments(const DevPlan* __restrict_
                 const int z = A.zint[c];
                 if (f < 0 || pd > z) continue;
                 T lp = T(1);
-                for (int q = 0; q < pd; ++q) lp *= T(A.lambda[c]);
+                for (int k = 0; k < pd; ++k) lp *= T(A.lambda[c]);
                 acc += W[A.goff + f] * T(binom4(z, pd)) * lp * T(1.0 / (double)(1 << z));
             }
             acc *= T(A.norm);
@@ -193,65 +246,110 @@ __device__ __forceinline__ void force_angular_moments(const DevPlan* __restrict_
         wtab[it] = acc;
     }
     __syncthreads();
-    // L[s][e][q]
-    for (int it = lane; it < ntypes * NE * kMomK; it += kWave) {
-        const int q = it % kMomK, e = (it / kMomK) % NE, s = it / (kMomK * NE);
-        const int pd = mono_deg(q);
+    for (int it = lane; it < ntypes * kMomK * NE; it += kWave) {
+        const int e = it % NE, q = (it / NE) % kMomK, s = it / (NE * kMomK);
+        const int pd = kMomTab.deg[q];
         T acc = T(0);
         for (int b = 0; b < ntypes; ++b) {
             const int blk = b >= s ? b * (b + 1) / 2 + s : s * (s + 1) / 2 + b;
             const T wv = wtab[(blk * NE + e) * (kMomZ + 1) + pd];
             acc += (b == s ? T(2) : T(1)) * wv * Mom[(b * NE + e) * kMomK + q];
         }
-        acc *= T(mono_mult(q));
-        Lam[it] = acc;
-        Lamp[it] = T(pd) * acc;
+        L[it] = T(kMomTab.mult[q]) * acc;  // [s][q][e]
+    }
+    for (int it = lane; it < ntypes * NE; it += kWave) {
+        const int e = it % NE, s = it / NE;
+        const int blk = s * (s + 1) / 2 + s;
+        T acc = T(0);
+#pragma unroll
+        for (int pd = 0; pd <= kMomZ; ++pd) acc += wtab[(blk * NE + e) * (kMomZ + 1) + pd];
+        Delta[it] = acc;
     }
     __syncthreads();
-    // gradient coefficients: L_x[b] = (b_x + 1) L[b + e_x] for the 20 monomials b of degree <= 3
-    for (int it = lane; it < ntypes * NE * kMomK3; it += kWave) {
-        const int q = it % kMomK3, se = it / kMomK3;
-        const int ax = kMono[q][0], ay = kMono[q][1], az = kMono[q][2];
-        const T* L = Lam + se * kMomK;
-        Lx[it] = T(ax + 1) * L[mono_index(ax + 1, ay, az)];
-        Ly[it] = T(ay + 1) * L[mono_index(ax, ay + 1, az)];
-        Lz[it] = T(az + 1) * L[mono_index(ax, ay, az + 1)];
+}
+
+// Q_e(u), and the gradient h_e, accumulated over monomial Q (compile-time index).
+template <typename T, int NE, int Q, int C>
+__device__ __forceinline__ void mom_grad(const T* m, const T (&Lq)[NE], T (&h)[3][NE]) {
+    constexpr int ac = kMomTabC.a[Q][C];
+    if constexpr (ac > 0) {
+        constexpr int lo = kMomTabC.low[Q][C];
+        const T t = ac == 1 ? m[lo] : T(ac) * m[lo];
+#pragma unroll
+        for (int e = 0; e < NE; ++e) h[C][e] += Lq[e] * t;
     }
-    __syncthreads();
-    // entries: X_j -= dE/dr_j (angular part)
-    const int ms = maxp + 1;  // the monomial table holds every entry
+}
+
+template <typename T, int NE, int Q>
+__device__ __forceinline__ void mom_term(const T* Ls, const T* m, const T (&u)[3], T (&g0)[NE], T (&h)[3][NE]) {
+    T Lq[NE];
+#pragma unroll
+    for (int e = 0; e < NE; ++e) Lq[e] = Ls[Q * NE + e];
+    T mq;
+    if constexpr (Q < kMomK3) {
+        mq = m[Q];
+    } else {
+        constexpr int c = mom_first_dir(Q);
+        constexpr int lo = kMomTabC.low[Q][c];
+        mq = m[lo] * u[c];
+    }
+#pragma unroll
+    for (int e = 0; e < NE; ++e) g0[e] += Lq[e] * mq;
+    mom_grad<T, NE, Q, 0>(m, Lq, h);
+    mom_grad<T, NE, Q, 1>(m, Lq, h);
+    mom_grad<T, NE, Q, 2>(m, Lq, h);
+}
+
+template <typename T, int NE, int... Qs>
+__device__ __forceinline__ void mom_terms(const T* Ls, const T* m, const T (&u)[3], T (&g0)[NE], T (&h)[3][NE],
+                                          std::integer_sequence<int, Qs...>) {
+    (mom_term<T, NE, Qs>(Ls, m, u, g0, h), ...);
+}
+
+// monomial Q (1 <= Q < 20) as the product of a lower one and one coordinate
+template <typename T, int Q>
+__device__ __forceinline__ void mom_mono(T* m, const T (&u)[3]) {
+    if constexpr (Q > 0) {
+        constexpr int c = mom_first_dir(Q);
+        constexpr int lo = kMomTabC.low[Q][c];
+        m[Q] = m[lo] * u[c];
+    } else {
+        m[0] = T(1);
+    }
+}
+
+template <typename T, int... Qs>
+__device__ __forceinline__ void mom_monos(T* m, const T (&u)[3], std::integer_sequence<int, Qs...>) {
+    (mom_mono<T, Qs>(m, u), ...);
+}
+
+// Angular force term of every entry from the adjoint tables, subtracted from Xs.
+template <typename T, int NE>
+__device__ __forceinline__ void force_from_moments(int ntypes, int P, const T* su, const T* Rf, const T* Rd,
+                                                   const int* seg, const T* L, const T* Delta, T* Xs, int lane) {
     for (int p = lane; p < P; p += kWave) {
         int s = 0;
         while (s + 1 < ntypes && p >= seg[s + 1]) ++s;
-        const T ux = su[4 * p], uy = su[4 * p + 1], uz = su[4 * p + 2], di = T(1) / su[4 * p + 3];
+        const T u[3] = {su[4 * p], su[4 * p + 1], su[4 * p + 2]};
+        const T di = T(1) / su[4 * p + 3];
+        // monomials of degree <= 3, each the product of a lower one and one coordinate
+        T m[kMomK3];
+        mom_monos<T>(m, u, std::make_integer_sequence<int, kMomK3>{});
+        T g0[NE], h[3][NE];
+#pragma unroll
+        for (int e = 0; e < NE; ++e) g0[e] = h[0][e] = h[1][e] = h[2][e] = T(0);
+        mom_terms<T, NE>(L + s * kMomK * NE, m, u, g0, h, std::make_integer_sequence<int, kMomK>{});
         T gx = T(0), gy = T(0), gz = T(0);
 #pragma unroll
         for (int e = 0; e < NE; ++e) {
-            const int se = s * NE + e;
-            const T* L = Lam + se * kMomK;
-            const T* Lp = Lamp + se * kMomK;
-            T g0 = T(0), g1 = T(0), hx = T(0), hy = T(0), hz = T(0);
-            for (int q = 0; q < kMomK; ++q) {
-                const T m = mono[q * ms + p];
-                g0 += L[q] * m;
-                g1 += Lp[q] * m;
-                if (q < kMomK3) {
-                    hx += Lx[se * kMomK3 + q] * m;
-                    hy += Ly[se * kMomK3 + q] * m;
-                    hz += Lz[se * kMomK3 + q] * m;
-                }
-            }
-            // Delta_s(e) = sum_p w_{ss,e,p}
-            const int blk = s * (s + 1) / 2 + s;
-            T delta = T(0);
-            for (int pd = 0; pd <= kMomZ; ++pd) delta += wtab[(blk * NE + e) * (kMomZ + 1) + pd];
             const T r = Rf[p * NE + e], rd = Rd[p * NE + e];
-            const T dEdR = g0 - T(2) * delta * r;
-            const T a1 = dEdR * rd - r * g1 * di;
+            const T gu = u[0] * h[0][e] + u[1] * h[1][e] + u[2] * h[2][e];  // sum_q |q| L_q u^q
+            const T dEdR = g0[e] - T(2) * Delta[s * NE + e] * r;
+            const T a1 = dEdR * rd - r * gu * di;
             const T a2 = r * di;
-            gx += a1 * ux + a2 * hx;
-            gy += a1 * uy + a2 * hy;
-            gz += a1 * uz + a2 * hz;
+            gx += a1 * u[0] + a2 * h[0][e];
+            gy += a1 * u[1] + a2 * h[1][e];
+            gz += a1 * u[2] + a2 * h[2][e];
         }
         Xs[3 * p] -= gx;
         Xs[3 * p + 1] -= gy;

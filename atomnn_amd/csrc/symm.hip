@@ -35,16 +35,29 @@
 
 namespace atomnn {
 
+// sin(pi x), cos(pi x): the reduction by pi is exact, so no large-argument path is needed
+template <typename T>
+__device__ __forceinline__ void sincos_pi(T x, T& s, T& c);
+template <>
+__device__ __forceinline__ void sincos_pi<double>(double x, double& s, double& c) {
+    sincospi(x, &s, &c);
+}
+template <>
+__device__ __forceinline__ void sincos_pi<float>(float x, float& s, float& c) {
+    sincospif(x, &s, &c);
+}
+
 template <typename T>
 __device__ __forceinline__ void cutoff_fg(int kind, T d, T rc, T rci, T pirci, T& f, T& g) {
-    if (kind == CUT_COS) {  // basis.cpp:86-88, 106-108
+    if (kind == CUT_COS) {  // basis.cpp:86-88, 106-108: cos(pi d / rc)
         if (d > rc) {
             f = T(0);
             g = T(0);
         } else {
-            const T a = d * pirci;
-            f = T(0.5) * (cos(a) + T(1));
-            g = T(-0.5) * pirci * sin(a);
+            T sn, cs;
+            sincos_pi<T>(d * rci, sn, cs);
+            f = T(0.5) * (cs + T(1));
+            g = T(-0.5) * pirci * sn;
         }
     } else if (kind == CUT_TANH) {  // basis.cpp:89-92, 109-114
         const T t = tanh(T(1) - d * rci);
@@ -215,6 +228,33 @@ __device__ __forceinline__ void stage_entries(int i, int P, int maxp, int ntypes
     }
 }
 
+// Radial exponent factors of one entry at distance d for an angular block with a separable
+// distance factor, given the block's cutoff value fc and derivative gc: R_e = fc h_e,
+// R'_e = dR_e/dd (when Rd is given).
+template <typename T, int NE, bool GEN>
+__device__ __forceinline__ void ang_sep_entry(const DevAngular& A, T d, T fc, T gc, T* Rf, T* Rd) {
+    const T dp = ipow(d, A.alpha);
+    const T ddp = T(A.alpha) * dp / d;  // d(d^alpha)/dd
+    if (!GEN || A.phi == ANG_GAUSS) {
+#pragma unroll
+        for (int e = 0; e < NE; ++e) {
+            const T ie = T(A.ietap[e]);
+            const T ex = exp(-ie * dp);
+            Rf[e] = fc * ex;
+            if (Rd) Rd[e] = (gc - ie * ddp * fc) * ex;
+        }
+    } else {  // IPOWP, SECHP: leg factor h(ietap d^alpha)
+#pragma unroll
+        for (int e = 0; e < NE; ++e) {
+            const T ie = T(A.ietap[e]);
+            T h, dh;
+            ang_leg<T>(A.phi, ie * dp, h, dh);
+            Rf[e] = fc * h;
+            if (Rd) Rd[e] = gc * h + fc * dh * ie * ddp;
+        }
+    }
+}
+
 // Radial exponent tables of one angular block for the slots [lo, hi): Rf (and Rd).
 template <typename T, int NE, bool GEN>
 __device__ __forceinline__ void radial_tables(const DevAngular& A, const T* su, int lo, int hi, T* Rf, T* Rd, int lane) {
@@ -223,26 +263,7 @@ __device__ __forceinline__ void radial_tables(const DevAngular& A, const T* su, 
         const T d = su[4 * p + 3];
         T fc, gc;
         cutoff_fg<T>(A.cut, d, rc, rci, pirci, fc, gc);
-        const T dp = ipow(d, A.alpha);
-        const T ddp = T(A.alpha) * dp / d;  // d(d^alpha)/dd
-        if (!GEN || A.phi == ANG_GAUSS) {
-#pragma unroll
-            for (int e = 0; e < NE; ++e) {
-                const T ie = T(A.ietap[e]);
-                const T ex = exp(-ie * dp);
-                Rf[p * NE + e] = fc * ex;
-                if (Rd) Rd[p * NE + e] = (gc - ie * ddp * fc) * ex;
-            }
-        } else {  // IPOWP, SECHP: leg factor h(ietap d^alpha)
-#pragma unroll
-            for (int e = 0; e < NE; ++e) {
-                const T ie = T(A.ietap[e]);
-                T h, dh;
-                ang_leg<T>(A.phi, ie * dp, h, dh);
-                Rf[p * NE + e] = fc * h;
-                if (Rd) Rd[p * NE + e] = gc * h + fc * dh * ie * ddp;
-            }
-        }
+        ang_sep_entry<T, NE, GEN>(A, d, fc, gc, Rf + p * NE, Rd ? Rd + p * NE : nullptr);
     }
 }
 
@@ -356,8 +377,8 @@ __device__ __forceinline__ void symm_body(const DevPlan* __restrict__ plan, cons
 
     // angular blocks: moment expansion when the centre species allows it
     if (mom && plan->mom_ok[ti]) {
-        // the monomial table goes through the reduction buffer `red` in chunks of `ch` entries
-        const int ch = red_elems / kMomK - 1;
+        // the power table goes through the reduction buffer `red` in chunks of `ch` entries
+        const int ch = red_elems / kMomPw;
         T* Mom = mom;
         T* D = Mom + ntypes * NE * kMomK;
         T* S = D + ntypes * NE;
@@ -369,6 +390,8 @@ __device__ __forceinline__ void symm_body(const DevPlan* __restrict__ plan, cons
         return;
     }
     // angular blocks in LMat order, pair by pair
+    const bool shared = plan->ang_shared[ti];
+    if (shared) radial_tables<T, NE, GEN>(plan->ang[ti][0], su, 0, P, Rf, nullptr, lane);
     for (int bb = 0; bb < ntypes; ++bb) {
         for (int a = 0; a <= bb; ++a) {
             const DevAngular& A = plan->ang[ti][bb * (bb + 1) / 2 + a];
@@ -381,7 +404,7 @@ __device__ __forceinline__ void symm_body(const DevPlan* __restrict__ plan, cons
             if (nsb) {
                 ns_tables<T, NE>(A, su, sa, sa + na, Rf, nullptr, nsx, lane);
                 if (!tri) ns_tables<T, NE>(A, su, sb, sb + nb, Rf, nullptr, nsx, lane);
-            } else {
+            } else if (!shared) {
                 radial_tables<T, NE, GEN>(A, su, sa, sa + na, Rf, nullptr, lane);
                 if (!tri) radial_tables<T, NE, GEN>(A, su, sb, sb + nb, Rf, nullptr, lane);
             }
@@ -472,7 +495,11 @@ __device__ __forceinline__ void force_body(const DevPlan* __restrict__ plan, con
                                            const double* __restrict__ rfun_rs, int ti, int ntypes, int P, int maxp,
                                            const T* su, T* Rf, T* Rd, T* Xs, T* slots, int* slot_row, const T* W,
                                            const T* Wang, const Segs& sg, T* nsx, T* mom, int lane) {
-    // radial term, one lane per entry: X_j = -dE/dd_j u_j
+    // radial term, one lane per entry: X_j = -dE/dd_j u_j; with one exponent table for every
+    // angular block, the entry's angular factors R_e, R'_e too (the cutoff is shared when the
+    // angular basis uses the radial one's)
+    const bool shared = plan->ang_shared[ti];
+    const DevAngular& A0 = plan->ang[ti][0];
     for (int p = lane; p < P; p += kWave) {
         int b = 0;
         while (b + 1 < ntypes && p >= sg.start[b + 1]) ++b;
@@ -480,6 +507,11 @@ __device__ __forceinline__ void force_body(const DevPlan* __restrict__ plan, con
         const T d = su[4 * p + 3];
         T fc, gc;
         cutoff_fg<T>(R.cut, d, T(R.rc), T(R.rci), T(R.pirci), fc, gc);
+        if (shared) {
+            T fa = fc, ga = gc;
+            if (A0.cut != R.cut || A0.rc != R.rc) cutoff_fg<T>(A0.cut, d, T(A0.rc), T(A0.rci), T(A0.pirci), fa, ga);
+            ang_sep_entry<T, NE, GEN>(A0, d, fa, ga, Rf + p * NE, Rd + p * NE);
+        }
         T s = T(0);
         if (!GEN || R.phi == RAD_GAUSSIAN) {
             for (int f = 0; f < R.n; ++f) {
@@ -504,20 +536,14 @@ __device__ __forceinline__ void force_body(const DevPlan* __restrict__ plan, con
     __syncthreads();
 
     if (mom && plan->mom_ok[ti]) {
-        T* mono = mom;
-        T* Mom = mono + kMomK * (maxp + 1);
-        T* D = Mom + ntypes * NE * kMomK;
-        T* wtab = D + ntypes * NE;
-        T* Lam = wtab + (ntypes * (ntypes + 1) / 2) * NE * (kMomZ + 1);
-        T* Lamp = Lam + ntypes * NE * kMomK;
-        T* Lx = Lamp + ntypes * NE * kMomK;
-        T* Ly = Lx + ntypes * NE * kMomK3;
-        T* Lz = Ly + ntypes * NE * kMomK3;
-        radial_tables<T, NE, GEN>(plan->ang[ti][0], su, 0, P, Rf, Rd, lane);
-        __syncthreads();
-        build_moments<T, NE>(su, Rf, P, maxp, ntypes, sg.start, mono, Mom, D, lane);  // whole table
-        force_angular_moments<T, NE>(plan, ti, ntypes, P, maxp, su, Rf, Rd, W, sg.start, mono, Mom, wtab, Lam, Lamp,
-                                     Lx, Ly, Lz, Xs, lane);
+        // mom_ok implies shared tables: Rf, Rd are complete
+        T* Mom = mom;
+        T* Delta = Mom + ntypes * NE * kMomK;
+        T* wtab = Delta + ntypes * NE;
+        T* U = wtab + (ntypes * (ntypes + 1) / 2) * NE * (kMomZ + 1);
+        build_moments<T, NE>(su, Rf, P, kMomCh, ntypes, sg.start, U, Mom, nullptr, lane);
+        adjoint_moments<T, NE>(plan, ti, ntypes, W, Mom, Delta, wtab, U, lane);
+        force_from_moments<T, NE>(ntypes, P, su, Rf, Rd, sg.start, U, Delta, Xs, lane);
         __syncthreads();
         return;
     }
@@ -532,7 +558,7 @@ __device__ __forceinline__ void force_body(const DevPlan* __restrict__ plan, con
             if (nsb) {
                 ns_tables<T, NE>(A, su, sa, sa + na, Rf, Rd, nsx, lane);
                 if (!same) ns_tables<T, NE>(A, su, sb, sb + nb, Rf, Rd, nsx, lane);
-            } else {
+            } else if (!shared) {
                 radial_tables<T, NE, GEN>(A, su, sa, sa + na, Rf, Rd, lane);
                 if (!same) radial_tables<T, NE, GEN>(A, su, sb, sb + nb, Rf, Rd, lane);
             }
@@ -728,12 +754,13 @@ __host__ __device__ constexpr size_t force_lds_bytes(int maxp, int ldg) {
 // bits: 1 descriptors, 2 forces (the fused kernel's region serves both phases)
 template <typename T, int NE>
 __host__ __device__ constexpr size_t mom_lds_bytes(int lmaxp, int ntypes, int bits = 3) {
-    return sizeof(T) * (size_t)((bits & 2) ? mom_lds_elems<NE>(lmaxp, ntypes) : mom_symm_elems<NE>(ntypes));
+    (void)lmaxp;
+    return sizeof(T) * (size_t)((bits & 2) ? mom_lds_elems<NE>(ntypes) : mom_symm_elems<NE>(ntypes));
 }
 
 // ------------------------------------------------------------------ descriptors
 template <typename T, int NE, int NC, bool GEN>
-__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <= 16 ? 3 : 1, 8))) void k_symm(
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <= 16 ? (GEN ? 2 : 3) : 1, 8))) void k_symm(
     const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta, const double* __restrict__ rfun_rs,
     const int* __restrict__ atom_type, const int* __restrict__ centres, const int* __restrict__ cnt,
     const int* __restrict__ nbr_meta, const double4* __restrict__ nbr_r, int maxp, int lmaxp, T* __restrict__ G,
@@ -763,7 +790,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
 
 // ------------------------------------------------------------------ force terms
 template <typename T, int NE, int NC, bool GEN>
-__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <= 16 ? 3 : 1, 8))) void k_force(
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <= 16 ? (GEN ? 2 : 3) : 1, 8))) void k_force(
     const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta, const double* __restrict__ rfun_rs,
     const int* __restrict__ atom_type, const int* __restrict__ centres, const int* __restrict__ cnt,
     const int* __restrict__ nbr_meta, const double4* __restrict__ nbr_r, int maxp, int lmaxp,
@@ -880,7 +907,7 @@ __host__ __device__ constexpr size_t fused_lds_bytes(int maxp, int ldg, int maxw
 // descriptors, runs its network in-wave and computes its per-entry force terms.  The order of
 // atoms (species-sorted) keeps a species' weights hot in the caches.
 template <typename T, int NE, int NC, bool GEN>
-__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <= 16 ? (sizeof(T) == 8 ? 2 : 3) : 1, 8))) void k_fused(
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <= 16 ? (sizeof(T) == 8 ? (GEN ? 1 : 2) : 3) : 1, 8))) void k_fused(
     const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta, const double* __restrict__ rfun_rs,
     const T* __restrict__ Wt, const int* __restrict__ atom_type, const int* __restrict__ order, int ntot,
     int* __restrict__ work, const int* __restrict__ cnt, const int* __restrict__ nbr_meta,

@@ -73,10 +73,19 @@ def work_counts(model: NNP, strucs):
     return P, T
 
 
-def force_ops(P, T, nR, nA):
-    """Algorithmic ops of the force pass: c_p + 13 per (pair, radial fn), c_t + 28 per (triplet, angular fn),
-    with c_p = 30 and c_t = 75 (the per-pair and per-triplet geometry and assembly counts)."""
+def force_ops_triplets(P, T, nR, nA):
+    """Algorithmic ops of the triplet force pass: c_p + 13 per (pair, radial fn), c_t + 28 per (triplet,
+    angular fn), with c_p = 30 and c_t = 75 (the per-pair and per-triplet geometry and assembly counts)."""
     return P * (30 + 13 * nR) + T * (75 + 28 * nA)
+
+
+def force_ops_moments(P, N, S, nR, NE):
+    """Algorithmic ops of the moment force pass (35 monomials of degree <= 4, NE radial exponents, S
+    species).  Per entry: 30 + 13 nR (cutoff, radial functions), 13 NE (angular radial factors),
+    35 (2 + 2 NE) (moments: monomial and one FMA per exponent), 64 + 216 NE (monomials, the adjoint
+    polynomial's value and gradient per exponent: 35 + 60 FMAs, and the force assembly); per centre
+    35 NE S (3 S + 1) (adjoint moments)."""
+    return P * (30 + 13 * nR + 13 * NE + 35 * (2 + 2 * NE) + 64 + 216 * NE) + N * 35 * NE * S * (3 * S + 1)
 
 
 def traffic_from_profile(path):
@@ -140,8 +149,9 @@ def main():
     value = natoms * args.steps / sec
     stage = profiled_steps(R, args.steps)
     P, T = work_counts(model, strucs)
-    nR, nA = 16, 16
-    ops = force_ops(P, T, nR, nA)
+    nR, nA, NE, S = 16, 16, 4, 2
+    mom_forces = bool(model.info()["moments"] & 2)
+    ops = force_ops_moments(P, natoms, S, nR, NE) if mom_forces else force_ops_triplets(P, T, nR, nA)
     t_force = stage[3] * 1e-3
     achieved = ops / t_force / 1e12
     traffic = traffic_from_profile(os.path.join(ROOT, "profiles", "k_force_plain4096_traffic.txt"))
@@ -168,7 +178,8 @@ def main():
         "stage_ms": {k: round(float(v), 5) for k, v in
                      zip(["neighbors", "descriptors", "networks", "force_terms", "force_assembly", "energy"], stage)},
         "roofline": {
-            "kernel": "k_force<double,4,4> (per-entry force terms)",
+            "kernel": "k_force<double,4,4> (per-entry force terms, " +
+                      ("moment expansion)" if mom_forces else "triplet loops)"),
             "bound": "valu",
             "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS,

@@ -159,8 +159,10 @@ int nnp_profile_read(nnp_model* m, double* ms, int64_t* calls, int nstage);
  *               1 = one persistent fused descriptor + network + force kernel (the batched form;
  *                   networks evaluated in-wave, so results differ from pipeline 0 in the last bits)
  *   "fused_blocks_per_cu"  persistent grid of the fused kernel; 0 = chosen by the autotuner
- *   "moments"   bit 0: descriptor angular sums by the moment expansion (default on), bit 1: the
- *               angular force terms by it (default off); models outside its domain ignore it */
+ *   "moments"   bit 0: descriptor angular sums by the moment expansion, bit 1: the angular
+ *               force terms by it (default 3, both); clear bits take the triplet loops; models
+ *               outside its domain (integer zeta <= 4, |lambda| <= 1, one exponent table per
+ *               centre species) take the triplet loops whatever the bits */
 int nnp_set_option(nnp_model* m, const char* key, int64_t value);
 /* Engine state: maxp, max entries, atoms, structures, kernel shape, NE, NC, descriptor width,
  * pipeline, fused blocks per CU (tuned), compute units, moment bits of the last pass. */
