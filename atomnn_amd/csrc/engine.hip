@@ -40,11 +40,17 @@ template <typename T>
 hipError_t launch_force(int shape, int ns, const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
                         const int* centres, int ncentres, const int* cnt, const int* meta, const double4* r, int maxp,
                         int lmaxp, const T* dEdG, int ldg, T* X, int use_mom, int ntypes, hipStream_t st);
+size_t force_mom_lds(int shape, bool fp64, int lmaxp, int ldg, int ntypes);
+template <typename T>
+hipError_t launch_force_mom(int shape, const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
+                            const int* centres, int ncentres, const int* cnt, const int* meta, const double4* r,
+                            int maxp, int lmaxp, const T* dEdG, int ldg, T* X, int ntypes, hipStream_t st);
 template <typename T>
 hipError_t launch_assemble(const DevStruc* S, const int* atom_struc, const int* cnt, const int* nbr_j, const int* meta,
                            const T* X, int maxp, int ntot, int all_images, double* F, hipStream_t st);
 hipError_t launch_net(bool fp64, const DevPlan* plan, const DevPlan& hplan, const void* W, const int* sp_off,
-                      const int* spatoms, const void* G, int ldg, int vac, double* eatom, void* dEdG, hipStream_t st);
+                      const int* spatoms, const void* G, int ldg, int vac, double* eatom, void* dEdG, int bf16_terms,
+                      hipStream_t st);
 hipError_t launch_neighbors(const NeighborArgs& a, hipStream_t st);
 size_t fused_lds(int shape, bool fp64, bool ns, int lmaxp, int ldg, int maxw, int dersum);
 size_t mom_lds(int shape, bool fp64, int lmaxp, int ntypes, int bits);
@@ -236,6 +242,8 @@ Engine::Engine(const Engine& o, int) : host(o.host), pb(o.pb), shape(o.shape) {
     pipeline = o.pipeline;
     fused_blocks_per_cu = o.fused_blocks_per_cu;
     moments = o.moments;
+    force_waves = o.force_waves;
+    net_bf16_terms = o.net_bf16_terms;
     num_cus = o.num_cus;
     tune_cache = o.tune_cache;
     d_plan = o.d_plan;
@@ -462,23 +470,46 @@ void Engine::networks(bool mixed, bool vac) {
     d_eatom.reserve((size_t)ntot * 8);
     d_dEdG.reserve((size_t)ntot * pb.plan.ldg * (mixed ? 4 : 8));
     HIPCHK(launch_net(!mixed, (const DevPlan*)d_plan.p, pb.plan, mixed ? d_w32.p : d_w64.p, sp_off.data(),
-                      (const int*)d_spatoms.p, d_G.p, pb.plan.ldg, vac ? 1 : 0, (double*)d_eatom.p, d_dEdG.p, stream));
+                      (const int*)d_spatoms.p, d_G.p, pb.plan.ldg, vac ? 1 : 0, (double*)d_eatom.p, d_dEdG.p,
+                      net_bf16_terms, stream));
 }
 
 template <typename T>
-void Engine::forces() {
-    const size_t lds = force_lds(shape, sizeof(T) == 8, pb.plan.generic, lmaxp, pb.plan.ldg);
-    if (lds > 65536) throw unsupported_error("too many neighbours per atom for the force kernel's LDS");
-    const int mom = mom_flag(lds, sizeof(T) == 8, 2) ? 1 : 0;
+void Engine::force_terms() {
+    const bool fp64 = sizeof(T) == 8;
     d_X.reserve((size_t)ntot * maxp * 4 * sizeof(T));
-    d_F.reserve((size_t)ntot * 3 * 8);
+    bool all = true;
+    for (int t = 0; t < pb.plan.T; ++t) all = all && pb.plan.mom_ok[t];
+    if (force_waves > 1 && all && !pb.plan.generic && (moments & 2) &&
+        force_mom_lds(shape, fp64, lmaxp, pb.plan.ldg, pb.plan.T) <= 65536) {
+        last_mom |= 2;
+        HIPCHK(launch_force_mom<T>(shape, (const DevPlan*)d_plan.p, (const double*)d_rfe.p, (const double*)d_rfr.p,
+                                   (const int*)d_type.p, (const int*)d_centres.p, ntot, (const int*)d_cnt.p,
+                                   (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp, lmaxp, (const T*)d_dEdG.p,
+                                   pb.plan.ldg, (T*)d_X.p, pb.plan.T, stream));
+        return;
+    }
+    const size_t lds = force_lds(shape, fp64, pb.plan.generic, lmaxp, pb.plan.ldg);
+    if (lds > 65536) throw unsupported_error("too many neighbours per atom for the force kernel's LDS");
+    const int mom = mom_flag(lds, fp64, 2) ? 1 : 0;
     HIPCHK(launch_force<T>(shape, pb.plan.generic, (const DevPlan*)d_plan.p, (const double*)d_rfe.p, (const double*)d_rfr.p,
                            (const int*)d_type.p, (const int*)d_centres.p, ntot, (const int*)d_cnt.p,
                            (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp, lmaxp, (const T*)d_dEdG.p, pb.plan.ldg,
                            (T*)d_X.p, mom, pb.plan.T, stream));
+}
+
+template <typename T>
+void Engine::assemble() {
+    d_F.reserve((size_t)ntot * 3 * 8);
     HIPCHK(launch_assemble<T>((const DevStruc*)d_strucs.p, (const int*)d_astruc.p, (const int*)d_cnt.p,
                               (const int*)d_nbrj.p, (const int*)d_meta.p, (const T*)d_X.p, maxp, ntot,
                               all_images_ ? 1 : 0, (double*)d_F.p, stream));
+}
+
+template <typename T>
+void Engine::forces() {
+    force_terms<T>();
+    assemble<T>();
 }
 
 void Engine::run(const std::vector<nnp_structure>& ss, uint32_t flags) {
@@ -765,33 +796,12 @@ void Engine::enqueue_pass(uint32_t flags) {
     stage_end(ST_NET);
     if (flags & NNP_FORCE) {
         stage_begin(ST_FORCE);
-        const size_t lds = force_lds(shape, !mixed, pb.plan.generic, lmaxp, pb.plan.ldg);
-        if (lds > 65536) throw unsupported_error("too many neighbours per atom for the force kernel's LDS");
-        const int mom = mom_flag(lds, !mixed, 2) ? 1 : 0;
-        d_X.reserve((size_t)ntot * maxp * 4 * (mixed ? 4 : 8));
-        d_F.reserve((size_t)ntot * 3 * 8);
-        if (mixed)
-            HIPCHK(launch_force<float>(shape, pb.plan.generic, (const DevPlan*)d_plan.p, (const double*)d_rfe.p, (const double*)d_rfr.p,
-                                       (const int*)d_type.p, (const int*)d_centres.p, ntot, (const int*)d_cnt.p,
-                                       (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp, lmaxp,
-                                       (const float*)d_dEdG.p, pb.plan.ldg, (float*)d_X.p, mom, pb.plan.T, stream));
-        else
-            HIPCHK(launch_force<double>(shape, pb.plan.generic, (const DevPlan*)d_plan.p, (const double*)d_rfe.p,
-                                        (const double*)d_rfr.p, (const int*)d_type.p, (const int*)d_centres.p, ntot,
-                                        (const int*)d_cnt.p, (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp,
-                                        lmaxp, (const double*)d_dEdG.p, pb.plan.ldg, (double*)d_X.p, mom, pb.plan.T,
-                                        stream));
+        if (mixed) force_terms<float>();
+        else force_terms<double>();
         stage_end(ST_FORCE);
         stage_begin(ST_ASSEMBLE);
-        const int ai = (flags & NNP_ALL_IMAGES) ? 1 : 0;
-        if (mixed)
-            HIPCHK(launch_assemble<float>((const DevStruc*)d_strucs.p, (const int*)d_astruc.p, (const int*)d_cnt.p,
-                                          (const int*)d_nbrj.p, (const int*)d_meta.p, (const float*)d_X.p, maxp, ntot,
-                                          ai, (double*)d_F.p, stream));
-        else
-            HIPCHK(launch_assemble<double>((const DevStruc*)d_strucs.p, (const int*)d_astruc.p, (const int*)d_cnt.p,
-                                           (const int*)d_nbrj.p, (const int*)d_meta.p, (const double*)d_X.p, maxp,
-                                           ntot, ai, (double*)d_F.p, stream));
+        if (mixed) assemble<float>();
+        else assemble<double>();
         stage_end(ST_ASSEMBLE);
     }
     stage_begin(ST_ENERGY);
@@ -1107,6 +1117,18 @@ int nnp_set_option(nnp_model* m, const char* key, int64_t value) {
             HIPCHK(hipStreamSynchronize(E.stream));
             E.moments = (int)value;
             E.tune_cache.clear();
+            E.drop_graph();
+            E.eager_runs = 0;
+        } else if (k == "net_bf16_terms") {
+            if (value != 1 && value != 3) throw std::invalid_argument("nnp_set_option: net_bf16_terms is 1 or 3");
+            HIPCHK(hipStreamSynchronize(E.stream));
+            E.net_bf16_terms = (int)value;
+            E.drop_graph();
+            E.eager_runs = 0;
+        } else if (k == "force_waves") {
+            if (value != 1 && value != 2) throw std::invalid_argument("nnp_set_option: force_waves is 1 or 2");
+            HIPCHK(hipStreamSynchronize(E.stream));
+            E.force_waves = (int)value;
             E.drop_graph();
             E.eager_runs = 0;
         } else if (k == "fused_blocks_per_cu") {

@@ -96,6 +96,10 @@ class Engine {
     void networks(bool mixed, bool vac);
     template <typename T>
     void forces();
+    template <typename T>
+    void force_terms();
+    template <typename T>
+    void assemble();
     void run(const std::vector<nnp_structure>& ss, uint32_t flags);
     void fetch(const std::vector<nnp_structure>& ss, uint32_t flags, nnp_result* res);
 
@@ -124,6 +128,13 @@ class Engine {
     // exponent table and separable distance factor) and the LDS fits; clear bits use the
     // triplet loops.  Default: both (O(P) per centre against the triplet loops' O(P^2)).
     int moments = 3;
+    // wavefronts per centre of the moment-path force kernel (when every species takes the
+    // moments on a fast-path model): 1 = the general kernel, 2 = split exponents, radial
+    // functions and moment entries between two wavefronts
+    int force_waves = 1;
+    // mixed precision networks: BF16 MFMA products with operands split into hi + lo BF16 parts
+    // (3 products, about 16 mantissa bits per operand) or single BF16 products (1)
+    int net_bf16_terms = 3;
     int last_mom = 0;  // the moment bits the last pass used
     int mom_flag(size_t base_lds, bool fp64, int bits);
     int num_cus = 0;

@@ -145,6 +145,7 @@ __device__ __forceinline__ void build_moments(const T* su, const T* Rf, int P, i
                 T a[NE];
 #pragma unroll
                 for (int e = 0; e < NE; ++e) a[e] = c0 == 0 ? T(0) : Mom[(s * NE + e) * kMomK + q];
+#pragma unroll 4
                 for (int p = lo; p < hi; ++p) {
                     const T* row = pw + (p - c0) * kMomPw;
                     const T m = row[ox] * row[oy] * row[oz];
@@ -219,13 +220,16 @@ __device__ __forceinline__ void symm_angular_moments(const DevPlan* __restrict__
     }
 }
 
-// Adjoint tables from dE/dG (row W): w table, Delta[s][e] (into D) and L[s][q][e] (into U, which
-// held the power table: the caller's barrier after the moments separates the two uses).
+// Adjoint tables from dE/dG (row W): w table, Delta[s][e] (into Delta) and L[s][q][e] (into L,
+// which held the power table: the caller's barrier after the moments separates the two uses).
+// The moments are the sum of `nparts` partial tables `pstride` elements apart, added in order.
+// Threads tid of nthreads share the items; the barriers are block-wide.
 template <typename T, int NE>
 __device__ __forceinline__ void adjoint_moments(const DevPlan* __restrict__ plan, int ti, int ntypes, const T* W,
-                                                const T* Mom, T* Delta, T* wtab, T* L, int lane) {
+                                                const T* Mom, int nparts, int pstride, T* Delta, T* wtab, T* L,
+                                                int tid, int nthreads) {
     const int nblk = ntypes * (ntypes + 1) / 2;
-    for (int it = lane; it < nblk * NE * (kMomZ + 1); it += kWave) {
+    for (int it = tid; it < nblk * NE * (kMomZ + 1); it += nthreads) {
         const int pd = it % (kMomZ + 1), e = (it / (kMomZ + 1)) % NE, blk = it / ((kMomZ + 1) * NE);
         const DevAngular& A = plan->ang[ti][blk];
         int a, bb;
@@ -246,18 +250,21 @@ __device__ __forceinline__ void adjoint_moments(const DevPlan* __restrict__ plan
         wtab[it] = acc;
     }
     __syncthreads();
-    for (int it = lane; it < ntypes * kMomK * NE; it += kWave) {
+    for (int it = tid; it < ntypes * kMomK * NE; it += nthreads) {
         const int e = it % NE, q = (it / NE) % kMomK, s = it / (NE * kMomK);
         const int pd = kMomTab.deg[q];
         T acc = T(0);
         for (int b = 0; b < ntypes; ++b) {
             const int blk = b >= s ? b * (b + 1) / 2 + s : s * (s + 1) / 2 + b;
             const T wv = wtab[(blk * NE + e) * (kMomZ + 1) + pd];
-            acc += (b == s ? T(2) : T(1)) * wv * Mom[(b * NE + e) * kMomK + q];
+            const int mi = (b * NE + e) * kMomK + q;
+            T mb = Mom[mi];
+            for (int k = 1; k < nparts; ++k) mb += Mom[k * pstride + mi];
+            acc += (b == s ? T(2) : T(1)) * wv * mb;
         }
         L[it] = T(kMomTab.mult[q]) * acc;  // [s][q][e]
     }
-    for (int it = lane; it < ntypes * NE; it += kWave) {
+    for (int it = tid; it < ntypes * NE; it += nthreads) {
         const int e = it % NE, s = it / NE;
         const int blk = s * (s + 1) / 2 + s;
         T acc = T(0);
@@ -268,23 +275,24 @@ __device__ __forceinline__ void adjoint_moments(const DevPlan* __restrict__ plan
     __syncthreads();
 }
 
-// Q_e(u), and the gradient h_e, accumulated over monomial Q (compile-time index).
-template <typename T, int NE, int Q, int C>
-__device__ __forceinline__ void mom_grad(const T* m, const T (&Lq)[NE], T (&h)[3][NE]) {
+// Q_e(u) and the gradient h_e accumulated over monomial Q (compile-time index) for NK exponents
+// e = e0 + ES k: Lb points at L[s][0][e0], RS = NE is the row stride of the L table.
+template <typename T, int NK, int Q, int C>
+__device__ __forceinline__ void mom_grad(const T* m, const T (&Lq)[NK], T (&h)[3][NK]) {
     constexpr int ac = kMomTabC.a[Q][C];
     if constexpr (ac > 0) {
         constexpr int lo = kMomTabC.low[Q][C];
         const T t = ac == 1 ? m[lo] : T(ac) * m[lo];
 #pragma unroll
-        for (int e = 0; e < NE; ++e) h[C][e] += Lq[e] * t;
+        for (int k = 0; k < NK; ++k) h[C][k] += Lq[k] * t;
     }
 }
 
-template <typename T, int NE, int Q>
-__device__ __forceinline__ void mom_term(const T* Ls, const T* m, const T (&u)[3], T (&g0)[NE], T (&h)[3][NE]) {
-    T Lq[NE];
+template <typename T, int NK, int RS, int ES, int Q>
+__device__ __forceinline__ void mom_term(const T* Lb, const T* m, const T (&u)[3], T (&g0)[NK], T (&h)[3][NK]) {
+    T Lq[NK];
 #pragma unroll
-    for (int e = 0; e < NE; ++e) Lq[e] = Ls[Q * NE + e];
+    for (int k = 0; k < NK; ++k) Lq[k] = Lb[Q * RS + ES * k];
     T mq;
     if constexpr (Q < kMomK3) {
         mq = m[Q];
@@ -294,16 +302,16 @@ __device__ __forceinline__ void mom_term(const T* Ls, const T* m, const T (&u)[3
         mq = m[lo] * u[c];
     }
 #pragma unroll
-    for (int e = 0; e < NE; ++e) g0[e] += Lq[e] * mq;
-    mom_grad<T, NE, Q, 0>(m, Lq, h);
-    mom_grad<T, NE, Q, 1>(m, Lq, h);
-    mom_grad<T, NE, Q, 2>(m, Lq, h);
+    for (int k = 0; k < NK; ++k) g0[k] += Lq[k] * mq;
+    mom_grad<T, NK, Q, 0>(m, Lq, h);
+    mom_grad<T, NK, Q, 1>(m, Lq, h);
+    mom_grad<T, NK, Q, 2>(m, Lq, h);
 }
 
-template <typename T, int NE, int... Qs>
-__device__ __forceinline__ void mom_terms(const T* Ls, const T* m, const T (&u)[3], T (&g0)[NE], T (&h)[3][NE],
+template <typename T, int NK, int RS, int ES, int... Qs>
+__device__ __forceinline__ void mom_terms(const T* Lb, const T* m, const T (&u)[3], T (&g0)[NK], T (&h)[3][NK],
                                           std::integer_sequence<int, Qs...>) {
-    (mom_term<T, NE, Qs>(Ls, m, u, g0, h), ...);
+    (mom_term<T, NK, RS, ES, Qs>(Lb, m, u, g0, h), ...);
 }
 
 // monomial Q (1 <= Q < 20) as the product of a lower one and one coordinate
@@ -338,7 +346,7 @@ __device__ __forceinline__ void force_from_moments(int ntypes, int P, const T* s
         T g0[NE], h[3][NE];
 #pragma unroll
         for (int e = 0; e < NE; ++e) g0[e] = h[0][e] = h[1][e] = h[2][e] = T(0);
-        mom_terms<T, NE>(L + s * kMomK * NE, m, u, g0, h, std::make_integer_sequence<int, kMomK>{});
+        mom_terms<T, NE, NE, 1>(L + s * kMomK * NE, m, u, g0, h, std::make_integer_sequence<int, kMomK>{});
         T gx = T(0), gy = T(0), gz = T(0);
 #pragma unroll
         for (int e = 0; e < NE; ++e) {

@@ -177,17 +177,11 @@ __global__ __launch_bounds__(kWave) void k_neigh_allpairs(const DevStruc* __rest
 }
 
 // ------------------------------------------------------------------ cell list
-// Bin index of each atom (fractional coordinates wrapped into [0,1)).
-__global__ void k_bin_atoms(const DevStruc* __restrict__ S, const int* __restrict__ atom_struc,
-                            const double* __restrict__ pos, int natoms_total, int* __restrict__ atom_bin,
-                            int* __restrict__ bin_count) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= natoms_total) return;
+// Bin of atom i (fractional coordinates wrapped into [0,1)), or -1 outside the cell list.
+__device__ __forceinline__ int bin_of(const DevStruc* __restrict__ S, const int* __restrict__ atom_struc,
+                                      const double* __restrict__ pos, int i) {
     const DevStruc& s = S[atom_struc[i]];
-    if (s.mode != 1) {
-        atom_bin[i] = -1;
-        return;
-    }
+    if (s.mode != 1) return -1;
     const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
     int b[3];
 #pragma unroll
@@ -198,9 +192,80 @@ __global__ void k_bin_atoms(const DevStruc* __restrict__ S, const int* __restric
         q = q < 0 ? 0 : (q >= s.nbin[r] ? s.nbin[r] - 1 : q);
         b[r] = q;
     }
-    const int bin = s.bin0 + (b[0] * s.nbin[1] + b[1]) * s.nbin[2] + b[2];
+    return s.bin0 + (b[0] * s.nbin[1] + b[1]) * s.nbin[2] + b[2];
+}
+
+__global__ void k_bin_atoms(const DevStruc* __restrict__ S, const int* __restrict__ atom_struc,
+                            const double* __restrict__ pos, int natoms_total, int* __restrict__ atom_bin,
+                            int* __restrict__ bin_count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= natoms_total) return;
+    const int bin = bin_of(S, atom_struc, pos, i);
     atom_bin[i] = bin;
-    atomicAdd(&bin_count[bin], 1);
+    if (bin >= 0) atomicAdd(&bin_count[bin], 1);
+}
+
+// Small batches: the counter reset, binning, bin scan and scatter of the four kernels below in
+// one workgroup with the bin counters in LDS (one launch instead of four at a few thousand atoms).
+constexpr int kBinsSmallAtoms = 32768;
+constexpr int kBinsSmallBins = 4096;
+__global__ __launch_bounds__(1024) void k_bins_small(const DevStruc* __restrict__ S, const int* __restrict__ atom_struc,
+                                                     const double* __restrict__ pos, int ntot, int nbins,
+                                                     int* __restrict__ maxcnt, int* __restrict__ work,
+                                                     int* __restrict__ atom_bin, int* __restrict__ bin_count,
+                                                     int* __restrict__ bin_start, int* __restrict__ bin_atoms) {
+    __shared__ int cnt_l[kBinsSmallBins];
+    __shared__ int start_l[kBinsSmallBins];
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    if (t == 0) {
+        *maxcnt = 0;
+        if (work) *work = 0;
+    }
+    for (int b = t; b < nbins; b += 1024) cnt_l[b] = 0;
+    __syncthreads();
+    for (int i = t; i < ntot; i += 1024) {
+        const int bin = bin_of(S, atom_struc, pos, i);
+        atom_bin[i] = bin;
+        if (bin >= 0) atomicAdd(&cnt_l[bin], 1);
+    }
+    __syncthreads();
+    // exclusive scan: four consecutive bins per thread, then a scan of the 1024 thread sums
+    int loc[4], sum = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int b = 4 * t + k;
+        loc[k] = sum;
+        sum += b < nbins ? cnt_l[b] : 0;
+    }
+    part[t] = sum;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const int y = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += y;
+        __syncthreads();
+    }
+    const int excl = part[t] - sum;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int b = 4 * t + k;
+        if (b < nbins) {
+            const int st = excl + loc[k];
+            bin_start[b] = st;
+            bin_count[b] = cnt_l[b];
+            start_l[b] = st;
+        }
+    }
+    __syncthreads();
+    for (int b = t; b < nbins; b += 1024) cnt_l[b] = 0;  // now the fill counters
+    __syncthreads();
+    for (int i = t; i < ntot; i += 1024) {
+        const int b = atom_bin[i];  // written above by this thread
+        if (b < 0) continue;
+        const int slot = atomicAdd(&cnt_l[b], 1);
+        bin_atoms[start_l[b] + slot] = i;
+    }
 }
 
 // Scatter atoms into their bins.  Order inside a bin does not matter: candidates are sorted.
@@ -465,15 +530,23 @@ namespace atomnn {
 
 hipError_t launch_neighbors(const NeighborArgs& a, hipStream_t st) {
     hipError_t e;
-    const int nz = a.ncell > 0 ? a.nbins : 0;
-    hipLaunchKernelGGL(k_zero, dim3((nz + 255) / 256 + 1), dim3(256), 0, st, a.maxcnt, a.work, a.bin_count, a.bin_fill,
-                       nz);
+    const bool small = a.ncell > 0 && a.ntot <= kBinsSmallAtoms && a.nbins <= kBinsSmallBins;
+    if (small) {
+        hipLaunchKernelGGL(k_bins_small, dim3(1), dim3(1024), 0, st, a.S, a.atom_struc, a.pos, a.ntot, a.nbins,
+                           a.maxcnt, a.work, a.atom_bin, a.bin_count, a.bin_start, a.bin_atoms);
+    } else {
+        const int nz = a.ncell > 0 ? a.nbins : 0;
+        hipLaunchKernelGGL(k_zero, dim3((nz + 255) / 256 + 1), dim3(256), 0, st, a.maxcnt, a.work, a.bin_count,
+                           a.bin_fill, nz);
+    }
     if (a.ncell > 0) {
-        hipLaunchKernelGGL(k_bin_atoms, dim3((a.ntot + 255) / 256), dim3(256), 0, st, a.S, a.atom_struc, a.pos, a.ntot,
-                           a.atom_bin, a.bin_count);
-        hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, a.bin_count, a.bin_start, a.nbins);
-        hipLaunchKernelGGL(k_fill_bins, dim3((a.ntot + 255) / 256), dim3(256), 0, st, a.atom_bin, a.ntot, a.bin_start,
-                           a.bin_fill, a.bin_atoms);
+        if (!small) {
+            hipLaunchKernelGGL(k_bin_atoms, dim3((a.ntot + 255) / 256), dim3(256), 0, st, a.S, a.atom_struc, a.pos,
+                               a.ntot, a.atom_bin, a.bin_count);
+            hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, a.bin_count, a.bin_start, a.nbins);
+            hipLaunchKernelGGL(k_fill_bins, dim3((a.ntot + 255) / 256), dim3(256), 0, st, a.atom_bin, a.ntot,
+                               a.bin_start, a.bin_fill, a.bin_atoms);
+        }
         const size_t bits_lds = (size_t)((a.cell_natoms_max + 31) / 32) * 4;
         if (bits_lds <= 32768) {
             hipLaunchKernelGGL(k_neigh_cells_bits, dim3(a.ncell), dim3(kWave), bits_lds, st, a.S, a.atom_struc, a.pos,

@@ -198,6 +198,32 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_f64(const DevPlan* __
 // holds B[8(l/16) .. +7][l%16], result register v of lane l is D[4(l/16) + v][l%16].
 __device__ __forceinline__ __bf16 to_bf16(float x) { return (__bf16)x; }
 
+// Split-BF16 products: x = hi + lo with hi = bf16(x), lo = bf16(x - hi) (the subtraction is exact),
+// and A B ~ Ah Bh + Ah Bl + Al Bh, three BF16 MFMAs with FP32 accumulation: about 16 mantissa
+// bits per operand instead of 8 (the dropped Al Bl term is 2^-16 relative).  TERMS = 1 is the
+// single BF16 product.
+template <int TERMS>
+__device__ __forceinline__ f4_t mfma_split(const float (&a)[8], const float (&b)[8], f4_t acc) {
+    bf8_t ah, bh;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        ah[q] = to_bf16(a[q]);
+        bh[q] = to_bf16(b[q]);
+    }
+    if constexpr (TERMS == 3) {
+        bf8_t al, bl;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            al[q] = to_bf16(a[q] - (float)ah[q]);
+            bl[q] = to_bf16(b[q] - (float)bh[q]);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc, 0, 0, 0);
+    }
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc, 0, 0, 0);
+}
+
+template <int TERMS>
 __global__ __launch_bounds__(kNetWaves * kWave) void k_net_mixed(const DevPlan* __restrict__ plan, const float* __restrict__ Wf,
                                                      NetTiles nt, const int* __restrict__ spatoms,
                                                      const float* __restrict__ G, int ldg, int vac,
@@ -236,15 +262,15 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_mixed(const DevPlan* 
         for (int jt = wv; jt < out_pad / 16; jt += kNetWaves) {
             f4_t acc = {0.f, 0.f, 0.f, 0.f};
             for (int k0 = 0; k0 < in_pad; k0 += 32) {
-                bf8_t a, bv;
+                float a[8], bv[8];
 #pragma unroll
                 for (int q = 0; q < 8; ++q) {
                     const int k = k0 + 8 * lk + q;
                     const bool ok = k < in_pad;
-                    a[q] = to_bf16(ok ? cur[li * maxw + k] : 0.f);
-                    bv[q] = to_bf16(ok ? W[(int64_t)(jt * 16 + li) * in_pad + k] : 0.f);
+                    a[q] = ok ? cur[li * maxw + k] : 0.f;
+                    bv[q] = ok ? W[(int64_t)(jt * 16 + li) * in_pad + k] : 0.f;
                 }
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc, 0, 0, 0);
+                acc = mfma_split<TERMS>(a, bv, acc);
             }
             const int col = jt * 16 + li;
 #pragma unroll
@@ -287,15 +313,15 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_mixed(const DevPlan* 
         for (int jt = wv; jt < J / 16; jt += kNetWaves) {
             f4_t acc = {0.f, 0.f, 0.f, 0.f};
             for (int k0 = 0; k0 < K; k0 += 32) {
-                bf8_t a, bv;
+                float a[8], bv[8];
 #pragma unroll
                 for (int q = 0; q < 8; ++q) {
                     const int k = k0 + 8 * lk + q;
                     const bool ok = k < K;
-                    a[q] = to_bf16(ok ? cur[li * maxw + k] : 0.f);
-                    bv[q] = to_bf16(ok ? W[(int64_t)k * J + jt * 16 + li] : 0.f);
+                    a[q] = ok ? cur[li * maxw + k] : 0.f;
+                    bv[q] = ok ? W[(int64_t)k * J + jt * 16 + li] : 0.f;
                 }
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc, 0, 0, 0);
+                acc = mfma_split<TERMS>(a, bv, acc);
             }
             const int col = jt * 16 + li;
 #pragma unroll
@@ -337,7 +363,8 @@ static NetTiles make_tiles(const int* sp_off, int ntypes, int& ntiles) {
 }
 
 hipError_t launch_net(bool fp64, const DevPlan* plan, const DevPlan& hplan, const void* W, const int* sp_off,
-                      const int* spatoms, const void* G, int ldg, int vac, double* eatom, void* dEdG, hipStream_t st) {
+                      const int* spatoms, const void* G, int ldg, int vac, double* eatom, void* dEdG, int bf16_terms,
+                      hipStream_t st) {
     int ntiles = 0;
     const NetTiles nt = make_tiles(sp_off, hplan.T, ntiles);
     if (ntiles == 0) return hipSuccess;
@@ -346,8 +373,11 @@ hipError_t launch_net(bool fp64, const DevPlan* plan, const DevPlan& hplan, cons
     if (fp64)
         hipLaunchKernelGGL(k_net_f64, dim3(ntiles), dim3(kNetThreads), lds, st, plan, (const double*)W, nt, spatoms,
                            (const double*)G, ldg, vac, eatom, (double*)dEdG);
+    else if (bf16_terms == 1)
+        hipLaunchKernelGGL(k_net_mixed<1>, dim3(ntiles), dim3(kNetThreads), lds, st, plan, (const float*)W, nt, spatoms,
+                           (const float*)G, ldg, vac, eatom, (float*)dEdG);
     else
-        hipLaunchKernelGGL(k_net_mixed, dim3(ntiles), dim3(kNetThreads), lds, st, plan, (const float*)W, nt, spatoms,
+        hipLaunchKernelGGL(k_net_mixed<3>, dim3(ntiles), dim3(kNetThreads), lds, st, plan, (const float*)W, nt, spatoms,
                            (const float*)G, ldg, vac, eatom, (float*)dEdG);
     return hipGetLastError();
 }

@@ -542,7 +542,7 @@ __device__ __forceinline__ void force_body(const DevPlan* __restrict__ plan, con
         T* wtab = Delta + ntypes * NE;
         T* U = wtab + (ntypes * (ntypes + 1) / 2) * NE * (kMomZ + 1);
         build_moments<T, NE>(su, Rf, P, kMomCh, ntypes, sg.start, U, Mom, nullptr, lane);
-        adjoint_moments<T, NE>(plan, ti, ntypes, W, Mom, Delta, wtab, U, lane);
+        adjoint_moments<T, NE>(plan, ti, ntypes, W, Mom, 1, 0, Delta, wtab, U, lane, kWave);
         force_from_moments<T, NE>(ntypes, P, su, Rf, Rd, sg.start, U, Delta, Xs, lane);
         __syncthreads();
         return;
@@ -895,6 +895,204 @@ __device__ __forceinline__ double net_body(const DevNet& N, const T* __restrict_
     return E;
 }
 
+// ------------------------------------------------------------------ force terms, NW waves per centre
+// Moment-path force terms when every species takes the moments (fast-path models).  NW
+// wavefronts share one centre: wave w owns the exponents e = w NE/NW ... (w + 1) NE/NW - 1 and the radial
+// functions f = w, w + NW, ... of every entry, and a contiguous 1/NW of the entries in the moment
+// sums; partial moments and partial force terms meet in LDS and are added in wave order.  Per
+// centre this divides the latency chain by about NW at the same LDS, so twice the wavefronts
+// are resident.
+constexpr int kMomChW = 8;  // power-table rows per wave and chunk
+
+template <int NE, int NW>
+__host__ __device__ constexpr int force_mom_mp_elems(int ntypes) {
+    return NW * ntypes * NE * kMomK > (NW - 1) * kWave * 3 ? NW * ntypes * NE * kMomK : (NW - 1) * kWave * 3;
+}
+template <int NE, int NW>
+__host__ __device__ constexpr int force_mom_u_elems(int ntypes) {
+    return NW * kMomChW * kMomPw > ntypes * kMomK * NE ? NW * kMomChW * kMomPw : ntypes * kMomK * NE;
+}
+template <typename T, int NE, int NW>
+__host__ __device__ constexpr size_t force_mom_lds_bytes(int lmaxp, int ldg, int ntypes) {
+    return sizeof(T) * (size_t)(4 * lmaxp + NE * lmaxp + ((ldg + 1) & ~1) + force_mom_mp_elems<NE, NW>(ntypes) +
+                                ntypes * NE + (ntypes * (ntypes + 1) / 2) * NE * (kMomZ + 1) +
+                                force_mom_u_elems<NE, NW>(ntypes)) +
+           sizeof(int) * (size_t)lmaxp;
+}
+
+template <typename T, int NE, int NW>
+__global__ __launch_bounds__(kWave * NW) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 4 : 8, 8))) void k_force_mom(
+    const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta, const double* __restrict__ rfun_rs,
+    const int* __restrict__ atom_type, const int* __restrict__ centres, const int* __restrict__ cnt,
+    const int* __restrict__ nbr_meta, const double4* __restrict__ nbr_r, int maxp, int lmaxp,
+    const T* __restrict__ dEdG, int ldg, T* __restrict__ Xout) {
+    constexpr int NK = NE / NW;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int ntypes = plan->T;
+    const int nblk = ntypes * (ntypes + 1) / 2;
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), w = tid / kWave;
+    T* su = (T*)lds_raw;
+    T* Rf = su + 4 * lmaxp;
+    T* W = Rf + NE * lmaxp;
+    T* Mp = W + ((ldg + 1) & ~1);  // partial moments [NW][S][NE][K]; then the force exchange [NW-1][kWave][3]
+    T* Delta = Mp + force_mom_mp_elems<NE, NW>(ntypes);
+    T* wtab = Delta + ntypes * NE;
+    T* U = wtab + nblk * NE * (kMomZ + 1);  // power rows [NW][kMomChW][15]; then L [S][K][NE]
+    int* orig = (int*)(U + force_mom_u_elems<NE, NW>(ntypes));
+    __shared__ int seg_lds[kMaxSpecies + 1];
+    Segs sg{seg_lds};
+    const int i = centres[blockIdx.x];
+    const int ti = atom_type[i];
+    const int P = min(cnt[i], lmaxp);
+    if (w == 0) {
+        stage_entries<T>(i, P, maxp, ntypes, nbr_meta, nbr_r, su, orig, sg, lane);
+    } else {
+        for (int k = tid - kWave; k < ldg; k += kWave * (NW - 1)) W[k] = dEdG[(int64_t)i * ldg + k];
+    }
+    __syncthreads();
+
+    // angular radial factors R_e(p) of this wave's exponents (one table for every block)
+    const DevAngular& A0 = plan->ang[ti][0];
+    for (int p = lane; p < P; p += kWave) {
+        const T d = su[4 * p + 3];
+        T fc, gc;
+        cutoff_fg<T>(A0.cut, d, T(A0.rc), T(A0.rci), T(A0.pirci), fc, gc);
+        const T dp = ipow(d, A0.alpha);
+#pragma unroll
+        for (int k = 0; k < NK; ++k) {
+            const int e = w * NK + k;
+            Rf[p * NE + e] = fc * exp(-T(A0.ietap[e]) * dp);
+        }
+    }
+    __syncthreads();
+
+    // partial moments of this wave's entries [lo, hi): lane q < 35 owns monomial q
+    {
+        const int q = lane;
+        int ox = 0, oy = 5, oz = 10;
+        if (q < kMomK) {
+            ox = kMomTab.a[q][0];
+            oy = 5 + kMomTab.a[q][1];
+            oz = 10 + kMomTab.a[q][2];
+        }
+        T* Mw = Mp + w * ntypes * NE * kMomK;
+        T* pw = U + w * kMomChW * kMomPw;
+        if (q < kMomK)
+            for (int it = 0; it < ntypes * NE; ++it) Mw[it * kMomK + q] = T(0);
+        const int per = (P + NW - 1) / NW;
+        const int lo = min(P, w * per), hi = min(P, lo + per);
+        const int nch = (per + kMomChW - 1) / kMomChW;  // the same for every wave
+        for (int c = 0; c < nch; ++c) {
+            const int c0 = min(hi, lo + c * kMomChW), c1 = min(hi, c0 + kMomChW);
+            __syncthreads();  // the previous chunk's readers are done
+            power_rows<T>(su, c0, c1, pw, lane);
+            __syncthreads();
+            if (q < kMomK) {
+                for (int s = 0; s < ntypes; ++s) {
+                    const int a0 = max(sg.start[s], c0), a1 = min(sg.start[s + 1], c1);
+                    if (a0 >= a1) continue;
+                    T a[NE];
+#pragma unroll
+                    for (int e = 0; e < NE; ++e) a[e] = Mw[(s * NE + e) * kMomK + q];
+#pragma unroll 4
+                    for (int p = a0; p < a1; ++p) {
+                        const T* row = pw + (p - c0) * kMomPw;
+                        const T m = row[ox] * row[oy] * row[oz];
+#pragma unroll
+                        for (int e = 0; e < NE; ++e) a[e] += Rf[p * NE + e] * m;
+                    }
+#pragma unroll
+                    for (int e = 0; e < NE; ++e) Mw[(s * NE + e) * kMomK + q] = a[e];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    adjoint_moments<T, NE>(plan, ti, ntypes, W, Mp, NW, ntypes * NE * kMomK, Delta, wtab, U, tid, kWave * NW);
+
+    // per entry: this wave's radial functions and exponents; X = -(sum of the waves' dE/dr)
+    T* Xx = Mp;  // the partial moments are no longer needed
+    const int64_t row = (int64_t)i * maxp;
+    for (int j0 = 0; j0 < P; j0 += kWave) {
+        const int p = j0 + lane;
+        T gx = T(0), gy = T(0), gz = T(0);
+        if (p < P) {
+            int s = 0;
+            while (s + 1 < ntypes && p >= sg.start[s + 1]) ++s;
+            const T u[3] = {su[4 * p], su[4 * p + 1], su[4 * p + 2]};
+            const T d = su[4 * p + 3], di = T(1) / d;
+            // radial term dE/dd u over the functions f = w, w + NW, ...
+            const DevRadial& R = plan->rad[ti][s];
+            T fc, gc;
+            cutoff_fg<T>(R.cut, d, T(R.rc), T(R.rci), T(R.pirci), fc, gc);
+            T sr = T(0);
+            for (int f = w; f < R.n; f += NW) {
+                const T eta = T(rfun_eta[R.fbase + f]), rs = T(rfun_rs[R.fbase + f]);
+                const T x = d - rs;
+                sr += W[R.goff + f] * exp(-eta * x * x) * (T(-2) * eta * x * fc + gc);
+            }
+            sr *= T(R.norm);
+            gx = sr * u[0];
+            gy = sr * u[1];
+            gz = sr * u[2];
+            // angular term of this wave's exponents e = w NK + k
+            T fa = fc, ga = gc;
+            if (A0.cut != R.cut || A0.rc != R.rc) cutoff_fg<T>(A0.cut, d, T(A0.rc), T(A0.rci), T(A0.pirci), fa, ga);
+            const T dp = ipow(d, A0.alpha);
+            const T ddp = T(A0.alpha) * dp * di;
+            T m[kMomK3];
+            mom_monos<T>(m, u, std::make_integer_sequence<int, kMomK3>{});
+            T g0[NK], h[3][NK];
+#pragma unroll
+            for (int k = 0; k < NK; ++k) g0[k] = h[0][k] = h[1][k] = h[2][k] = T(0);
+            mom_terms<T, NK, NE, 1>(U + s * kMomK * NE + w * NK, m, u, g0, h, std::make_integer_sequence<int, kMomK>{});
+#pragma unroll
+            for (int k = 0; k < NK; ++k) {
+                const int e = w * NK + k;
+                const T ie = T(A0.ietap[e]);
+                const T ex = exp(-ie * dp);
+                const T r = fa * ex, rd = (ga - ie * ddp * fa) * ex;
+                const T gu = u[0] * h[0][k] + u[1] * h[1][k] + u[2] * h[2][k];
+                const T dEdR = g0[k] - T(2) * Delta[s * NE + e] * r;
+                const T a1 = dEdR * rd - r * gu * di;
+                const T a2 = r * di;
+                gx += a1 * u[0] + a2 * h[0][k];
+                gy += a1 * u[1] + a2 * h[1][k];
+                gz += a1 * u[2] + a2 * h[2][k];
+            }
+        }
+        if (w > 0) {
+            T* x = Xx + ((w - 1) * kWave + lane) * 3;
+            x[0] = gx;
+            x[1] = gy;
+            x[2] = gz;
+        }
+        __syncthreads();
+        if (w == 0 && p < P) {
+            for (int v = 1; v < NW; ++v) {
+                const T* x = Xx + ((v - 1) * kWave + lane) * 3;
+                gx += x[0];
+                gy += x[1];
+                gz += x[2];
+            }
+            store_x4<T>(Xout + 4 * (row + orig[p]), -gx, -gy, -gz);
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T, int NE, int NW>
+static hipError_t launch_force_mom_t(const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
+                                     const int* centres, int ncentres, const int* cnt, const int* meta,
+                                     const double4* r, int maxp, int lmaxp, const T* dEdG, int ldg, T* X, int ntypes,
+                                     hipStream_t st) {
+    const size_t lds = force_mom_lds_bytes<T, NE, NW>(lmaxp, ldg, ntypes);
+    if (lds > 65536) return hipErrorInvalidConfiguration;
+    hipLaunchKernelGGL((k_force_mom<T, NE, NW>), dim3(ncentres), dim3(kWave * NW), lds, st, plan, rfe, rfr,
+                       atom_type, centres, cnt, meta, r, maxp, lmaxp, dEdG, ldg, X);
+    return hipGetLastError();
+}
+
 template <typename T, int NE, int NC, bool GEN>
 __host__ __device__ constexpr size_t fused_lds_bytes(int maxp, int ldg, int maxw, int dersum) {
     return sizeof(T) * (size_t)(4 * maxp + 2 * NE * maxp + 3 * maxp + 6 * kWave + kWave * NE * NC +
@@ -1043,8 +1241,8 @@ template hipError_t launch_fused<float>(int, int, int, const DevPlan*, const dou
 // F_a = sum_s X[a,s] - sum_{entries (j -> a) that carry a reaction} X[j -> a], in FP64.
 // One wavefront per atom, lanes over its entries.  The mirrored entry (j -> a, image
 // nimg-1-n) of each entry (a -> j, image n) has the same distance, so it carries the min flag
-// exactly when (a -> j) does; it is found by binary search in j's row, which is sorted by
-// (neighbour, image).  With all_images every entry carries a reaction.  The lane partials are
+// exactly when (a -> j) does; it is found by an 8-ary search in j's row, which is sorted by
+// (neighbour, image): independent probes per level keep the dependent load chain short.  With all_images every entry carries a reaction.  The lane partials are
 // summed in a fixed tree order (deterministic, no atomics).
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -1080,14 +1278,31 @@ __global__ __launch_bounds__(kWave) void k_assemble(const DevStruc* __restrict__
             const int64_t rowj = (int64_t)gj * maxp;
             const int Pj = min(cnt[gj], maxp);
             const int64_t want = (int64_t)la * st.nimg + want_img;
+            const bool multi = st.nimg > 1;  // one image per j: the key is the neighbour index
+            auto key = [&](int idx) -> int64_t {
+                const int64_t kj = nbr_j[rowj + idx];
+                return multi ? kj * st.nimg + (nbr_meta[rowj + idx] >> 8) : kj;
+            };
+            // 8-ary search: 7 independent probes per level narrow [lo, hi) to one of 8 parts,
+            // then the last <= 16 keys are counted; the first index with key >= want is lo
             int lo = 0, hi = Pj;
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                const int64_t key = (int64_t)nbr_j[rowj + mid] * st.nimg + (nbr_meta[rowj + mid] >> 8);
-                if (key < want) lo = mid + 1;
-                else hi = mid;
+            while (hi - lo > 16) {
+                const int len = hi - lo;
+                int c = 0;
+#pragma unroll
+                for (int m = 1; m < 8; ++m) c += key(lo + (len * m) / 8) < want ? 1 : 0;
+                const int nlo = c == 0 ? lo : lo + (len * c) / 8 + 1;
+                const int nhi = c == 7 ? hi : lo + (len * (c + 1)) / 8;
+                lo = nlo;
+                hi = nhi;
             }
-            if (lo < Pj && (int64_t)nbr_j[rowj + lo] * st.nimg + (nbr_meta[rowj + lo] >> 8) == want) {
+            {
+                int c = 0;
+#pragma unroll
+                for (int m = 0; m < 16; ++m) c += (lo + m < hi && key(lo + m) < want) ? 1 : 0;
+                lo += c;
+            }
+            if (lo < Pj && key(lo) == want) {
                 const double4 v = load_x4<T>(X + 4 * (rowj + lo));
                 fx -= v.x;
                 fy -= v.y;
@@ -1215,6 +1430,34 @@ hipError_t launch_force(int shape, int ns, const DevPlan* plan, const double* rf
     return launch_force_s<T, false>(shape, plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp, dEdG, ldg,
                                     X, use_mom, ntypes, st);
 }
+
+size_t force_mom_lds(int shape, bool fp64, int lmaxp, int ldg, int ntypes) {
+    if (shape == 0) return fp64 ? force_mom_lds_bytes<double, 2, 2>(lmaxp, ldg, ntypes)
+                                : force_mom_lds_bytes<float, 2, 2>(lmaxp, ldg, ntypes);
+    if (shape == 1) return fp64 ? force_mom_lds_bytes<double, 4, 2>(lmaxp, ldg, ntypes)
+                                : force_mom_lds_bytes<float, 4, 2>(lmaxp, ldg, ntypes);
+    return fp64 ? force_mom_lds_bytes<double, 8, 2>(lmaxp, ldg, ntypes) : force_mom_lds_bytes<float, 8, 2>(lmaxp, ldg, ntypes);
+}
+
+template <typename T>
+hipError_t launch_force_mom(int shape, const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
+                            const int* centres, int ncentres, const int* cnt, const int* meta, const double4* r,
+                            int maxp, int lmaxp, const T* dEdG, int ldg, T* X, int ntypes, hipStream_t st) {
+    if (shape == 0)
+        return launch_force_mom_t<T, 2, 2>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp, dEdG,
+                                           ldg, X, ntypes, st);
+    if (shape == 1)
+        return launch_force_mom_t<T, 4, 2>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp, dEdG,
+                                           ldg, X, ntypes, st);
+    return launch_force_mom_t<T, 8, 2>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp, dEdG, ldg,
+                                       X, ntypes, st);
+}
+template hipError_t launch_force_mom<double>(int, const DevPlan*, const double*, const double*, const int*, const int*,
+                                             int, const int*, const int*, const double4*, int, int, const double*, int,
+                                             double*, int, hipStream_t);
+template hipError_t launch_force_mom<float>(int, const DevPlan*, const double*, const double*, const int*, const int*,
+                                            int, const int*, const int*, const double4*, int, int, const float*, int,
+                                            float*, int, hipStream_t);
 
 template <typename T>
 hipError_t launch_assemble(const DevStruc* S, const int* atom_struc, const int* cnt, const int* nbr_j, const int* meta,

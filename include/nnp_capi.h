@@ -162,7 +162,12 @@ int nnp_profile_read(nnp_model* m, double* ms, int64_t* calls, int nstage);
  *   "moments"   bit 0: descriptor angular sums by the moment expansion, bit 1: the angular
  *               force terms by it (default 3, both); clear bits take the triplet loops; models
  *               outside its domain (integer zeta <= 4, |lambda| <= 1, one exponent table per
- *               centre species) take the triplet loops whatever the bits */
+ *               centre species) take the triplet loops whatever the bits
+ *   "force_waves"  1 = one wavefront per centre in the force-term kernel (default), 2 = two
+ *               wavefronts split a centre's exponents, radial functions and moment entries
+ *               (moment path on fast-path models only)
+ *   "net_bf16_terms"  mixed-precision networks: 3 = BF16 MFMA products of split operands
+ *               (hi + lo, about 16 mantissa bits; default), 1 = single BF16 products */
 int nnp_set_option(nnp_model* m, const char* key, int64_t value);
 /* Engine state: maxp, max entries, atoms, structures, kernel shape, NE, NC, descriptor width,
  * pipeline, fused blocks per CU (tuned), compute units, moment bits of the last pass. */

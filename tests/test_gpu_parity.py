@@ -161,6 +161,31 @@ def test_mixed_precision_accuracy(models):
     assert f_rms <= 20e-3
 
 
+def test_mixed_precision_accuracy_serving_pool(potentials):
+    """Mixed form on the serving request distribution (16-20000 atoms, three species in random
+    proportions): every structure within 1 meV/atom and 20 meV/A RMS of the FP64 form (itself
+    parity-tested against the oracle).  The networks' BF16 products take split operands
+    (hi + lo); single BF16 products exceed 1 meV/atom on some of these structures."""
+    pool = [p for p in synth.serve_pool(64) if p.natoms <= 6000][:24]
+    m = NNP.read(potentials["serve20k"])
+    try:
+        ref = m.compute_batch(pool, NNP_FORCE | NNP_VACUUM_ENERGY)
+        errs = {}
+        for terms in (1, 3):
+            m.set_option("net_bf16_terms", terms)
+            got = m.compute_batch(pool, NNP_FORCE | NNP_VACUUM_ENERGY | NNP_PREC_MIXED)
+            e_err = max(abs(a[0] - b[0]) / s.natoms for a, b, s in zip(got, ref, pool))
+            f_rms = max(float(np.sqrt(np.mean(np.sum((a[2] - b[2]) ** 2, axis=1)))) for a, b in zip(got, ref))
+            errs[terms] = (e_err, f_rms)
+            print(f"bf16 terms {terms}: max dE/atom = {e_err * 1e3:.4f} meV, max F rms = {f_rms * 1e3:.4f} meV/A")
+    finally:
+        m.close()
+    e3, f3 = errs[3]
+    assert e3 <= 1e-3
+    assert f3 <= 20e-3
+    assert e3 <= errs[1][0]
+
+
 def test_moment_path_engagement(models, potentials, request):
     """Qualifying models use the moment expansion exactly when it is on; a model with
     non-integer zeta never does and still matches the oracle."""
