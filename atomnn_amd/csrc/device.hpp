@@ -33,6 +33,8 @@ struct DevStruc {
     double R[9];     // R[r*3+c]; columns are the lattice vectors
     double RInv[9];
     double Rv[9];    // norm-sorted lattice vectors, Rv[q*3+r]
+    double Gv[9];    // their dual vectors, Gv[q*3+r] (Gv_q . Rv_p = delta_qp)
+    double reach[3]; // rc |Gv_q| (+ a rounding margin): images further off along q are out of range
     double dMax2;    // (half the shortest lattice vector)^2
     double fro;      // Frobenius norm of R
 };

@@ -40,11 +40,12 @@ template <typename T>
 hipError_t launch_force(int shape, int ns, const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
                         const int* centres, int ncentres, const int* cnt, const int* meta, const double4* r, int maxp,
                         int lmaxp, const T* dEdG, int ldg, T* X, int use_mom, int ntypes, hipStream_t st);
-size_t force_mom_lds(int shape, bool fp64, int lmaxp, int ldg, int ntypes);
+size_t force_mom_lds(int shape, bool fp64, int waves, int lmaxp, int ldg, int ntypes);
 template <typename T>
-hipError_t launch_force_mom(int shape, const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
-                            const int* centres, int ncentres, const int* cnt, const int* meta, const double4* r,
-                            int maxp, int lmaxp, const T* dEdG, int ldg, T* X, int ntypes, hipStream_t st);
+hipError_t launch_force_mom(int shape, int waves, const DevPlan* plan, const double* rfe, const double* rfr,
+                            const int* atom_type, const int* centres, int ncentres, const int* cnt, const int* meta,
+                            const double4* r, int maxp, int lmaxp, const T* dEdG, int ldg, T* X, int ntypes,
+                            hipStream_t st);
 template <typename T>
 hipError_t launch_assemble(const DevStruc* S, const int* atom_struc, const int* cnt, const int* nbr_j, const int* meta,
                            const T* X, int maxp, int ntot, int all_images, double* F, hipStream_t st);
@@ -200,6 +201,17 @@ DevStruc make_struc(const nnp_structure& s, double rc, int atom0) {
     d.shell[2] = (int)std::floor(2.0 * rc / std::fabs((Rv[2][0] * x01[0] + Rv[2][1] * x01[1] + Rv[2][2] * x01[2]) /
                                                       (nrm(Rv[0]) * nrm(Rv[1]))));
     d.nimg = (2 * d.shell[0] + 1) * (2 * d.shell[1] + 1) * (2 * d.shell[2] + 1);
+    {
+        double x12[3], x20[3];
+        cross(Rv[1], Rv[2], x12);
+        cross(Rv[2], Rv[0], x20);
+        const double V = Rv[0][0] * x12[0] + Rv[0][1] * x12[1] + Rv[0][2] * x12[2];
+        const double* g[3] = {x12, x20, x01};
+        for (int q = 0; q < 3; ++q) {
+            for (int r = 0; r < 3; ++r) d.Gv[3 * q + r] = g[q][r] / V;
+            d.reach[q] = rc * nrm(&d.Gv[3 * q]) * (1.0 + 1e-12) + 1e-9;
+        }
+    }
     // cell list when every shell is 0 and the perpendicular heights hold at least one rc bin
     if (d.shell[0] == 0 && d.shell[1] == 0 && d.shell[2] == 0 && s.natoms >= 256) {
         const double vol = std::fabs(det);
@@ -456,9 +468,11 @@ template <typename T>
 void Engine::descriptors() {
     const size_t lds = symm_lds(shape, sizeof(T) == 8, pb.plan.generic, lmaxp, pb.plan.ldg);
     if (lds > 65536) throw unsupported_error("too many neighbours per atom for the descriptor kernel's LDS");
-    // (mom = 2 would shrink the reduction buffer when every species takes the moments; measured
-    // slower on the 4096-atom cell: more monomial chunks and barriers outweigh the occupancy)
-    const int mom = mom_flag(lds, sizeof(T) == 8, 1) ? 1 : 0;
+    // mom = 2 when every species takes the moments: no triplet blocks, so the reduction buffer
+    // only holds power-table chunks and the kernel's LDS drops to about 9 KB (FP64)
+    bool all = true;
+    for (int t = 0; t < pb.plan.T; ++t) all = all && pb.plan.mom_ok[t];
+    const int mom = mom_flag(lds, sizeof(T) == 8, 1) ? (all ? 2 : 1) : 0;
     d_G.reserve((size_t)ntot * pb.plan.ldg * sizeof(T));
     HIPCHK(launch_symm<T>(shape, pb.plan.generic, (const DevPlan*)d_plan.p, (const double*)d_rfe.p, (const double*)d_rfr.p,
                           (const int*)d_type.p, (const int*)d_centres.p, ntot, (const int*)d_cnt.p,
@@ -480,10 +494,10 @@ void Engine::force_terms() {
     d_X.reserve((size_t)ntot * maxp * 4 * sizeof(T));
     bool all = true;
     for (int t = 0; t < pb.plan.T; ++t) all = all && pb.plan.mom_ok[t];
-    if (force_waves > 1 && all && !pb.plan.generic && (moments & 2) &&
-        force_mom_lds(shape, fp64, lmaxp, pb.plan.ldg, pb.plan.T) <= 65536) {
+    if (all && !pb.plan.generic && (moments & 2) &&
+        force_mom_lds(shape, fp64, force_waves, lmaxp, pb.plan.ldg, pb.plan.T) <= 65536) {
         last_mom |= 2;
-        HIPCHK(launch_force_mom<T>(shape, (const DevPlan*)d_plan.p, (const double*)d_rfe.p, (const double*)d_rfr.p,
+        HIPCHK(launch_force_mom<T>(shape, force_waves, (const DevPlan*)d_plan.p, (const double*)d_rfe.p, (const double*)d_rfr.p,
                                    (const int*)d_type.p, (const int*)d_centres.p, ntot, (const int*)d_cnt.p,
                                    (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp, lmaxp, (const T*)d_dEdG.p,
                                    pb.plan.ldg, (T*)d_X.p, pb.plan.T, stream));

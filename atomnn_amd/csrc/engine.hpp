@@ -128,9 +128,9 @@ class Engine {
     // exponent table and separable distance factor) and the LDS fits; clear bits use the
     // triplet loops.  Default: both (O(P) per centre against the triplet loops' O(P^2)).
     int moments = 3;
-    // wavefronts per centre of the moment-path force kernel (when every species takes the
-    // moments on a fast-path model): 1 = the general kernel, 2 = split exponents, radial
-    // functions and moment entries between two wavefronts
+    // wavefronts per centre of the moment-only force kernel (every species takes the moments on
+    // a fast-path model): 1, or 2 = exponents, radial functions and moment entries split between
+    // two wavefronts.  Other models take the general force kernel.
     int force_waves = 1;
     // mixed precision networks: BF16 MFMA products with operands split into hi + lo BF16 parts
     // (3 products, about 16 mantissa bits per operand) or single BF16 products (1)

@@ -105,6 +105,23 @@ __device__ __forceinline__ int owner_bin(int bfirst, int nb, int c) {
     return lo;
 }
 
+// Image index range that can hold r - R_n within rc along each norm-sorted lattice vector: with
+// f_q = Gv_q . r, |f_q - n_q| = |(r - R_n) . Gv_q| <= rc |Gv_q|.  The ranges are sub-ranges of the
+// image table, walked in its order, so the kept images and their order are unchanged.
+struct ImgRange {
+    int lo[3], hi[3];
+};
+__device__ __forceinline__ ImgRange img_range(const DevStruc& s, double bx, double by, double bz) {
+    ImgRange g;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const double f = s.Gv[3 * q] * bx + s.Gv[3 * q + 1] * by + s.Gv[3 * q + 2] * bz;
+        g.lo[q] = max(-s.shell[q], (int)ceil(f - s.reach[q]));
+        g.hi[q] = min(s.shell[q], (int)floor(f + s.reach[q]));
+    }
+    return g;
+}
+
 // ------------------------------------------------------------------ all-pairs search
 __global__ __launch_bounds__(kWave) void k_neigh_allpairs(const DevStruc* __restrict__ S, const int* __restrict__ atom_struc,
                                                           const double* __restrict__ pos, const int* __restrict__ type,
@@ -138,36 +155,47 @@ __global__ __launch_bounds__(kWave) void k_neigh_allpairs(const DevStruc* __rest
                 by = dy;
                 bz = dz;
             }
-            for (int n = 0; n < s.nimg; ++n) {
-                double ix = 0, iy = 0, iz = 0;
-                if (s.periodic) image_vec(s, n, ix, iy, iz);
-                const double rx = bx - ix, ry = by - iy, rz = bz - iz;
-                const double d2 = rx * rx + ry * ry + rz * rz;
-                const bool keep = (j != li) ? (d2 <= rc2) : (1e-6 < d2 && d2 <= rc2);
-                if (keep) {
-                    ++mine;
-                    const double d = sqrt(d2);
-                    if (j != li && s.periodic && d < mind) {
-                        mind = d;
-                        minn = n;
+        }
+        // images of this j that can be within rc (open structures: the single image 0)
+        ImgRange g{{0, 0, 0}, {-1, -1, -1}};
+        if (valid) g = s.periodic ? img_range(s, bx, by, bz) : ImgRange{{0, 0, 0}, {0, 0, 0}};
+        const int w1 = 2 * s.shell[1] + 1, w2 = 2 * s.shell[2] + 1;
+        for (int a = g.lo[0]; a <= g.hi[0]; ++a)
+            for (int b = g.lo[1]; b <= g.hi[1]; ++b)
+                for (int c = g.lo[2]; c <= g.hi[2]; ++c) {
+                    const int n = ((a + s.shell[0]) * w1 + (b + s.shell[1])) * w2 + (c + s.shell[2]);
+                    double ix = 0, iy = 0, iz = 0;
+                    if (s.periodic) image_vec(s, n, ix, iy, iz);
+                    const double rx = bx - ix, ry = by - iy, rz = bz - iz;
+                    const double d2 = rx * rx + ry * ry + rz * rz;
+                    const bool keep = (j != li) ? (d2 <= rc2) : (1e-6 < d2 && d2 <= rc2);
+                    if (keep) {
+                        ++mine;
+                        const double d = sqrt(d2);
+                        if (j != li && s.periodic && d < mind) {
+                            mind = d;
+                            minn = n;
+                        }
                     }
                 }
-            }
-        }
         int tot;
         int at = written + wave_excl_scan(mine, lane, tot);
         if (mine > 0) {
-            for (int n = 0; n < s.nimg; ++n) {
-                double ix = 0, iy = 0, iz = 0;
-                if (s.periodic) image_vec(s, n, ix, iy, iz);
-                const double rx = bx - ix, ry = by - iy, rz = bz - iz;
-                const double d2 = rx * rx + ry * ry + rz * rz;
-                const bool keep = (j != li) ? (d2 <= rc2) : (1e-6 < d2 && d2 <= rc2);
-                if (keep) {
-                    if (at < maxp) put_entry(row + at, j, n, tj, n == minn, rx, ry, rz, sqrt(d2), nbr_j, nbr_meta, nbr_r);
-                    ++at;
-                }
-            }
+            for (int a = g.lo[0]; a <= g.hi[0]; ++a)
+                for (int b = g.lo[1]; b <= g.hi[1]; ++b)
+                    for (int c = g.lo[2]; c <= g.hi[2]; ++c) {
+                        const int n = ((a + s.shell[0]) * w1 + (b + s.shell[1])) * w2 + (c + s.shell[2]);
+                        double ix = 0, iy = 0, iz = 0;
+                        if (s.periodic) image_vec(s, n, ix, iy, iz);
+                        const double rx = bx - ix, ry = by - iy, rz = bz - iz;
+                        const double d2 = rx * rx + ry * ry + rz * rz;
+                        const bool keep = (j != li) ? (d2 <= rc2) : (1e-6 < d2 && d2 <= rc2);
+                        if (keep) {
+                            if (at < maxp)
+                                put_entry(row + at, j, n, tj, n == minn, rx, ry, rz, sqrt(d2), nbr_j, nbr_meta, nbr_r);
+                            ++at;
+                        }
+                    }
         }
         written += tot;
     }

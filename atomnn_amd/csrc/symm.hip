@@ -902,7 +902,11 @@ __device__ __forceinline__ double net_body(const DevNet& N, const T* __restrict_
 // sums; partial moments and partial force terms meet in LDS and are added in wave order.  Per
 // centre this divides the latency chain by about NW at the same LDS, so twice the wavefronts
 // are resident.
-constexpr int kMomChW = 8;  // power-table rows per wave and chunk
+// power-table rows per wave and chunk (the L table, ntypes x 35 x NE, sets the region's size)
+template <int NW>
+__host__ __device__ constexpr int mom_chw() {
+    return NW == 1 ? 16 : 8;
+}
 
 template <int NE, int NW>
 __host__ __device__ constexpr int force_mom_mp_elems(int ntypes) {
@@ -910,7 +914,7 @@ __host__ __device__ constexpr int force_mom_mp_elems(int ntypes) {
 }
 template <int NE, int NW>
 __host__ __device__ constexpr int force_mom_u_elems(int ntypes) {
-    return NW * kMomChW * kMomPw > ntypes * kMomK * NE ? NW * kMomChW * kMomPw : ntypes * kMomK * NE;
+    return NW * mom_chw<NW>() * kMomPw > ntypes * kMomK * NE ? NW * mom_chw<NW>() * kMomPw : ntypes * kMomK * NE;
 }
 template <typename T, int NE, int NW>
 __host__ __device__ constexpr size_t force_mom_lds_bytes(int lmaxp, int ldg, int ntypes) {
@@ -921,7 +925,7 @@ __host__ __device__ constexpr size_t force_mom_lds_bytes(int lmaxp, int ldg, int
 }
 
 template <typename T, int NE, int NW>
-__global__ __launch_bounds__(kWave * NW) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 4 : 8, 8))) void k_force_mom(
+__global__ __launch_bounds__(kWave * NW) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 4 : 6, 8))) void k_force_mom(
     const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta, const double* __restrict__ rfun_rs,
     const int* __restrict__ atom_type, const int* __restrict__ centres, const int* __restrict__ cnt,
     const int* __restrict__ nbr_meta, const double4* __restrict__ nbr_r, int maxp, int lmaxp,
@@ -944,10 +948,10 @@ __global__ __launch_bounds__(kWave * NW) __attribute__((amdgpu_waves_per_eu(size
     const int i = centres[blockIdx.x];
     const int ti = atom_type[i];
     const int P = min(cnt[i], lmaxp);
-    if (w == 0) {
-        stage_entries<T>(i, P, maxp, ntypes, nbr_meta, nbr_r, su, orig, sg, lane);
-    } else {
-        for (int k = tid - kWave; k < ldg; k += kWave * (NW - 1)) W[k] = dEdG[(int64_t)i * ldg + k];
+    if (w == 0) stage_entries<T>(i, P, maxp, ntypes, nbr_meta, nbr_r, su, orig, sg, lane);
+    if (NW == 1 || w > 0) {
+        const int t0 = NW == 1 ? tid : tid - kWave, nt = NW == 1 ? kWave : kWave * (NW - 1);
+        for (int k = t0; k < ldg; k += nt) W[k] = dEdG[(int64_t)i * ldg + k];
     }
     __syncthreads();
 
@@ -976,6 +980,7 @@ __global__ __launch_bounds__(kWave * NW) __attribute__((amdgpu_waves_per_eu(size
             oz = 10 + kMomTab.a[q][2];
         }
         T* Mw = Mp + w * ntypes * NE * kMomK;
+        constexpr int kMomChW = mom_chw<NW>();
         T* pw = U + w * kMomChW * kMomPw;
         if (q < kMomK)
             for (int it = 0; it < ntypes * NE; ++it) Mw[it * kMomK + q] = T(0);
@@ -994,7 +999,7 @@ __global__ __launch_bounds__(kWave * NW) __attribute__((amdgpu_waves_per_eu(size
                     T a[NE];
 #pragma unroll
                     for (int e = 0; e < NE; ++e) a[e] = Mw[(s * NE + e) * kMomK + q];
-#pragma unroll 4
+#pragma unroll 2
                     for (int p = a0; p < a1; ++p) {
                         const T* row = pw + (p - c0) * kMomPw;
                         const T m = row[ox] * row[oy] * row[oz];
@@ -1325,8 +1330,9 @@ template <typename T, int NE, int NC, bool GEN>
 static hipError_t launch_symm_t(const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
                                 const int* centres, int ncentres, const int* cnt, const int* meta, const double4* r,
                                 int maxp, int lmaxp, T* G, int ldg, int use_mom, int ntypes, hipStream_t st) {
-    // use_mom 2: every species takes the moments, the reduction buffer only holds monomial chunks
-    const int red = use_mom == 2 ? 16 * kMomK : kWave * NE * NC;
+    // use_mom 2: every species takes the moments, the reduction buffer only holds power-table
+    // chunks of 16 entries (and the radial partial sums, kWave values)
+    const int red = use_mom == 2 ? 16 * kMomPw : kWave * NE * NC;
     const size_t lds =
         symm_lds_bytes<T, NE, NC, GEN>(lmaxp, ldg, red) + (use_mom ? mom_lds_bytes<T, NE>(lmaxp, ntypes, 1) : 0);
     if (lds > 65536) return hipErrorInvalidConfiguration;
@@ -1431,33 +1437,50 @@ hipError_t launch_force(int shape, int ns, const DevPlan* plan, const double* rf
                                     X, use_mom, ntypes, st);
 }
 
-size_t force_mom_lds(int shape, bool fp64, int lmaxp, int ldg, int ntypes) {
-    if (shape == 0) return fp64 ? force_mom_lds_bytes<double, 2, 2>(lmaxp, ldg, ntypes)
-                                : force_mom_lds_bytes<float, 2, 2>(lmaxp, ldg, ntypes);
-    if (shape == 1) return fp64 ? force_mom_lds_bytes<double, 4, 2>(lmaxp, ldg, ntypes)
-                                : force_mom_lds_bytes<float, 4, 2>(lmaxp, ldg, ntypes);
-    return fp64 ? force_mom_lds_bytes<double, 8, 2>(lmaxp, ldg, ntypes) : force_mom_lds_bytes<float, 8, 2>(lmaxp, ldg, ntypes);
+template <typename T, int NW>
+static size_t force_mom_lds_w(int shape, int lmaxp, int ldg, int ntypes) {
+    if (shape == 0) return force_mom_lds_bytes<T, 2, NW>(lmaxp, ldg, ntypes);
+    if (shape == 1) return force_mom_lds_bytes<T, 4, NW>(lmaxp, ldg, ntypes);
+    return force_mom_lds_bytes<T, 8, NW>(lmaxp, ldg, ntypes);
+}
+size_t force_mom_lds(int shape, bool fp64, int waves, int lmaxp, int ldg, int ntypes) {
+    if (waves == 1)
+        return fp64 ? force_mom_lds_w<double, 1>(shape, lmaxp, ldg, ntypes) : force_mom_lds_w<float, 1>(shape, lmaxp, ldg, ntypes);
+    return fp64 ? force_mom_lds_w<double, 2>(shape, lmaxp, ldg, ntypes) : force_mom_lds_w<float, 2>(shape, lmaxp, ldg, ntypes);
+}
+
+template <typename T, int NW>
+static hipError_t launch_force_mom_w(int shape, const DevPlan* plan, const double* rfe, const double* rfr,
+                                     const int* atom_type, const int* centres, int ncentres, const int* cnt,
+                                     const int* meta, const double4* r, int maxp, int lmaxp, const T* dEdG, int ldg,
+                                     T* X, int ntypes, hipStream_t st) {
+    if (shape == 0)
+        return launch_force_mom_t<T, 2, NW>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp,
+                                            dEdG, ldg, X, ntypes, st);
+    if (shape == 1)
+        return launch_force_mom_t<T, 4, NW>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp,
+                                            dEdG, ldg, X, ntypes, st);
+    return launch_force_mom_t<T, 8, NW>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp, dEdG,
+                                        ldg, X, ntypes, st);
 }
 
 template <typename T>
-hipError_t launch_force_mom(int shape, const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
-                            const int* centres, int ncentres, const int* cnt, const int* meta, const double4* r,
-                            int maxp, int lmaxp, const T* dEdG, int ldg, T* X, int ntypes, hipStream_t st) {
-    if (shape == 0)
-        return launch_force_mom_t<T, 2, 2>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp, dEdG,
-                                           ldg, X, ntypes, st);
-    if (shape == 1)
-        return launch_force_mom_t<T, 4, 2>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp, dEdG,
-                                           ldg, X, ntypes, st);
-    return launch_force_mom_t<T, 8, 2>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp, dEdG, ldg,
-                                       X, ntypes, st);
+hipError_t launch_force_mom(int shape, int waves, const DevPlan* plan, const double* rfe, const double* rfr,
+                            const int* atom_type, const int* centres, int ncentres, const int* cnt, const int* meta,
+                            const double4* r, int maxp, int lmaxp, const T* dEdG, int ldg, T* X, int ntypes,
+                            hipStream_t st) {
+    if (waves == 1)
+        return launch_force_mom_w<T, 1>(shape, plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp,
+                                        dEdG, ldg, X, ntypes, st);
+    return launch_force_mom_w<T, 2>(shape, plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp, dEdG,
+                                    ldg, X, ntypes, st);
 }
-template hipError_t launch_force_mom<double>(int, const DevPlan*, const double*, const double*, const int*, const int*,
-                                             int, const int*, const int*, const double4*, int, int, const double*, int,
-                                             double*, int, hipStream_t);
-template hipError_t launch_force_mom<float>(int, const DevPlan*, const double*, const double*, const int*, const int*,
-                                            int, const int*, const int*, const double4*, int, int, const float*, int,
-                                            float*, int, hipStream_t);
+template hipError_t launch_force_mom<double>(int, int, const DevPlan*, const double*, const double*, const int*,
+                                             const int*, int, const int*, const int*, const double4*, int, int,
+                                             const double*, int, double*, int, hipStream_t);
+template hipError_t launch_force_mom<float>(int, int, const DevPlan*, const double*, const double*, const int*,
+                                            const int*, int, const int*, const int*, const double4*, int, int,
+                                            const float*, int, float*, int, hipStream_t);
 
 template <typename T>
 hipError_t launch_assemble(const DevStruc* S, const int* atom_struc, const int* cnt, const int* nbr_j, const int* meta,

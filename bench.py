@@ -178,8 +178,8 @@ def main():
         "stage_ms": {k: round(float(v), 5) for k, v in
                      zip(["neighbors", "descriptors", "networks", "force_terms", "force_assembly", "energy"], stage)},
         "roofline": {
-            "kernel": "k_force<double,4,4> (per-entry force terms, " +
-                      ("moment expansion)" if mom_forces else "triplet loops)"),
+            "kernel": ("k_force_mom<double,4,1> (per-entry force terms, moment expansion)" if mom_forces
+                       else "k_force<double,4,4> (per-entry force terms, triplet loops)"),
             "bound": "valu",
             "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS,
