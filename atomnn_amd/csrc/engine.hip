@@ -51,7 +51,7 @@ hipError_t launch_assemble(const DevStruc* S, const int* atom_struc, const int* 
                            const T* X, int maxp, int ntot, int all_images, double* F, hipStream_t st);
 hipError_t launch_net(bool fp64, const DevPlan* plan, const DevPlan& hplan, const void* W, const int* sp_off,
                       const int* spatoms, const void* G, int ldg, int vac, double* eatom, void* dEdG, int bf16_terms,
-                      hipStream_t st);
+                      int wl_tiles_max, hipStream_t st);
 hipError_t launch_neighbors(const NeighborArgs& a, hipStream_t st);
 size_t fused_lds(int shape, bool fp64, bool ns, int lmaxp, int ldg, int maxw, int dersum);
 size_t mom_lds(int shape, bool fp64, int lmaxp, int ntypes, int bits);
@@ -62,10 +62,27 @@ hipError_t launch_fused(int shape, int ns, int nblocks, const DevPlan* plan, con
                         double* eatom, T* X, int use_mom, int ntypes, hipStream_t st);
 
 // Total energy per structure: one block per structure, fixed-order tree sum of E_i (FP64).
+// With cnt given, the blocks past the structures fold the pass's largest neighbour count into
+// *maxcnt (the row-overflow check of resident passes, which need it only after the pass).
 __global__ __launch_bounds__(256) void k_energy(const double* __restrict__ eatom, const int* __restrict__ sstart,
-                                                double* __restrict__ E) {
+                                                double* __restrict__ E, int nstruc, const int* __restrict__ cnt,
+                                                int ntot, int* __restrict__ maxcnt) {
     __shared__ double part[256];
     const int s = blockIdx.x, t = threadIdx.x;
+    if (cnt && s >= nstruc) {  // the count blocks: strided slices, one atomicMax each
+        __shared__ int pm[256];
+        const int nb = (int)gridDim.x - nstruc, b = s - nstruc;
+        int m = 0;
+        for (int i = b * 256 + t; i < ntot; i += nb * 256) m = max(m, cnt[i]);
+        pm[t] = m;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if (t < o) pm[t] = max(pm[t], pm[t + o]);
+            __syncthreads();
+        }
+        if (t == 0) atomicMax(maxcnt, pm[0]);
+        return;
+    }
     const int a0 = sstart[s], a1 = sstart[s + 1];
     const int n = a1 - a0, chunk = (n + 255) / 256;
     double acc = 0.0;
@@ -485,7 +502,7 @@ void Engine::networks(bool mixed, bool vac) {
     d_dEdG.reserve((size_t)ntot * pb.plan.ldg * (mixed ? 4 : 8));
     HIPCHK(launch_net(!mixed, (const DevPlan*)d_plan.p, pb.plan, mixed ? d_w32.p : d_w64.p, sp_off.data(),
                       (const int*)d_spatoms.p, d_G.p, pb.plan.ldg, vac ? 1 : 0, (double*)d_eatom.p, d_dEdG.p,
-                      net_bf16_terms, stream));
+                      net_bf16_terms, 2 * num_cus, stream));
 }
 
 template <typename T>
@@ -675,10 +692,12 @@ void Engine::profile_collect() {
     }
 }
 
-void Engine::energies() {
+void Engine::energies(bool count_max) {
     d_E.reserve((size_t)nstruc * 8);
-    hipLaunchKernelGGL(k_energy, dim3(nstruc), dim3(256), 0, stream, (const double*)d_eatom.p,
-                       (const int*)d_sstart.p, (double*)d_E.p);
+    const int ncount = count_max ? std::min(256, (ntot + 4095) / 4096) : 0;
+    hipLaunchKernelGGL(k_energy, dim3(nstruc + ncount), dim3(256), 0, stream, (const double*)d_eatom.p,
+                       (const int*)d_sstart.p, (double*)d_E.p, nstruc, count_max ? (const int*)d_cnt.p : nullptr,
+                       ntot, (int*)d_maxcnt.p);
     HIPCHK(hipGetLastError());
 }
 
@@ -776,6 +795,7 @@ void Engine::enqueue_pass(uint32_t flags) {
     a.bin_start = (int*)d_bstart.p;
     a.bin_fill = (int*)d_bfill.p;
     a.bin_atoms = (int*)d_batoms.p;
+    a.count_max = false;  // folded into the energy launch at the end of the pass
     stage_begin(ST_NEIGH);
     HIPCHK(launch_neighbors(a, stream));
     stage_end(ST_NEIGH);
@@ -797,7 +817,7 @@ void Engine::enqueue_pass(uint32_t flags) {
             stage_end(ST_ASSEMBLE);
         }
         stage_begin(ST_ENERGY);
-        energies();
+        energies(true);
         stage_end(ST_ENERGY);
         return;
     }
@@ -819,7 +839,7 @@ void Engine::enqueue_pass(uint32_t flags) {
         stage_end(ST_ASSEMBLE);
     }
     stage_begin(ST_ENERGY);
-    energies();
+    energies(true);
     stage_end(ST_ENERGY);
 }
 

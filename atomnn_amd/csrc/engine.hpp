@@ -76,6 +76,7 @@ struct NeighborArgs {
     int* bin_atoms = nullptr;
     int all_images;
     int cell_natoms_max = 0;  // atoms of the largest cell-list structure (LDS bitmap size)
+    bool count_max = true;    // fold the largest neighbour count into maxcnt after the search
 };
 
 DevStruc make_struc(const nnp_structure& s, double rc, int atom0);
@@ -107,7 +108,7 @@ class Engine {
     void resident_load(const std::vector<nnp_structure>& ss, uint32_t flags);
     void resident_run(uint32_t flags);
     void resident_fetch(uint32_t flags, nnp_result* res);
-    void energies();
+    void energies(bool count_max = false);
 
     // stage timing with HIP events on the engine's stream
     enum Stage { ST_NEIGH = 0, ST_SYMM, ST_NET, ST_FORCE, ST_ASSEMBLE, ST_ENERGY, ST_COUNT };

@@ -413,6 +413,26 @@ PlanBuffers build_plan(const HostModel& m) {
             in_real = out_real;
             in_pad = out_pad;
         }
+        // contiguous copy of the layer matrices and biases (one vector copy into LDS)
+        align();
+        dn.pack_off = (int64_t)W.size();
+        for (int l = 0; l < dn.L; ++l) {
+            dn.pw_off[l] = (int)((int64_t)W.size() - dn.pack_off);
+            const int ip = l == 0 ? dn.nin_pad : dn.wpad[l - 1];
+            for (int q = 0; q < dn.wpad[l] * ip; ++q) {
+                const double v = W[dn.woff[l] + q];
+                W.push_back(v);
+            }
+        }
+        for (int l = 0; l < dn.L; ++l) {
+            dn.pb_off[l] = (int)((int64_t)W.size() - dn.pack_off);
+            for (int o = 0; o < dn.wpad[l]; ++o) {
+                const double v = W[dn.boff[l] + o];
+                W.push_back(v);
+            }
+        }
+        if (W.size() % 2) W.push_back(0.0);
+        dn.pack_len = (int)((int64_t)W.size() - dn.pack_off);
     }
     if (pb.rfun_eta.empty()) {
         pb.rfun_eta.push_back(0.0);

@@ -127,6 +127,10 @@ struct DevNet {
     int64_t wtoff[kMaxLayers]; // the same weights transposed (in_pad x wpad)
     int64_t boff[kMaxLayers];
     int64_t inpw_off, inpb_off;
+    int64_t pack_off;          // contiguous copy [W_0 | W_1 | ... | b_0 | b_1 | ...] (row-major, padded)
+    int pack_len;              // its length in doubles (even)
+    int pw_off[kMaxLayers];    // offsets of W_l and b_l inside the copy
+    int pb_off[kMaxLayers];
     int act;                   // NeuronKind of the hidden layers
     int pad_;
     double outw, outb, evac;

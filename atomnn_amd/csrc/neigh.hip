@@ -252,6 +252,7 @@ __global__ __launch_bounds__(1024) void k_bins_small(const DevStruc* __restrict_
     }
     for (int b = t; b < nbins; b += 1024) cnt_l[b] = 0;
     __syncthreads();
+#pragma unroll 4
     for (int i = t; i < ntot; i += 1024) {
         const int bin = bin_of(S, atom_struc, pos, i);
         atom_bin[i] = bin;
@@ -288,6 +289,7 @@ __global__ __launch_bounds__(1024) void k_bins_small(const DevStruc* __restrict_
     __syncthreads();
     for (int b = t; b < nbins; b += 1024) cnt_l[b] = 0;  // now the fill counters
     __syncthreads();
+#pragma unroll 4
     for (int i = t; i < ntot; i += 1024) {
         const int b = atom_bin[i];  // written above by this thread
         if (b < 0) continue;
@@ -595,7 +597,7 @@ hipError_t launch_neighbors(const NeighborArgs& a, hipStream_t st) {
                            a.pair_centres, a.rc2, a.maxp, a.cnt, a.nbr_j, a.nbr_meta, a.nbr_r, a.maxcnt);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    if (a.ntot > 0)
+    if (a.ntot > 0 && a.count_max)
         hipLaunchKernelGGL(k_count_max, dim3(std::min((a.ntot + 255) / 256, 256)), dim3(256), 0, st, a.cnt, a.ntot,
                            a.maxcnt);
     return hipGetLastError();

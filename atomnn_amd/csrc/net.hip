@@ -23,10 +23,10 @@
 #include "device.hpp"
 #include "basis.hpp"
 #include "model.hpp"
+#include "netmm.hpp"
 
 namespace atomnn {
 
-typedef double d4_t __attribute__((ext_vector_type(4)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
 
@@ -40,36 +40,16 @@ size_t net_lds_bytes(const DevNet& n, bool fp64) {
     return (size_t)(fp64 ? 8 : 4) * (size_t)(2 * 16 * maxw + 16 * dsum);
 }
 
+// Elements of the LDS copy of the weights-in-LDS form: every layer's matrix and bias.
+static __host__ __device__ int net_weight_elems(const DevNet& n) { return n.pack_len; }
+
 static __device__ __forceinline__ int net_maxw(const DevNet& n) {
     int maxw = n.nin_pad;
     for (int l = 0; l < n.L; ++l) maxw = n.wpad[l] > maxw ? n.wpad[l] : maxw;
     return maxw;
 }
 
-// ------------------------------------------------------------------ FP64
-// Tile product D(16 x 16, column tile jt) = A(16 x K, LDS rows of stride lda) * B(K x 16).
-// Forward layers use B[k][j] = W[(16 jt + j) ldw + k] (W row-major out x in); backward
-// layers use B[k][j] = W[k ldw + 16 jt + j].  Eight K-steps of operands are loaded before
-// their eight MFMAs so the weight loads overlap.
-template <bool FWD>
-__device__ __forceinline__ d4_t tile_mm_f64(const double* A, int lda, const double* __restrict__ W, int ldw, int K,
-                                            int jt, int li, int lk) {
-    d4_t acc = {0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < K; k0 += 32) {
-        double a[8], b[8];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int k = k0 + 4 * q + lk;
-            const bool ok = k < K;
-            a[q] = ok ? A[li * lda + k] : 0.0;
-            b[q] = ok ? (FWD ? W[(int64_t)(jt * 16 + li) * ldw + k] : W[(int64_t)k * ldw + jt * 16 + li]) : 0.0;
-        }
-#pragma unroll
-        for (int q = 0; q < 8; ++q) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[q], b[q], acc, 0, 0, 0);
-    }
-    return acc;
-}
-
+// ------------------------------------------------------------------ FP64 (tile_mm_f64: netmm.hpp)
 // Waves per 16-atom tile: they share the tile's activations in LDS and split each layer's
 // 16-column output tiles between them (the layer widths, 32-256, give 2-16 tiles).
 constexpr int kNetWaves = 4;
@@ -86,6 +66,10 @@ __device__ __forceinline__ int tile_species(const NetTiles& nt, int ntypes, int 
     return s;
 }
 
+// WL: the species' weight matrices are copied into LDS once per tile (all loads in flight
+// together), so the layers read weights from LDS instead of waiting on global loads layer by
+// layer (small batches, where the network kernel is a latency chain of one tile per CU).
+template <bool WL>
 __global__ __launch_bounds__(kNetWaves * kWave) void k_net_f64(const DevPlan* __restrict__ plan, const double* __restrict__ Wb,
                                                    NetTiles nt, const int* __restrict__ spatoms,
                                                    const double* __restrict__ G, int ldg, int vac,
@@ -102,6 +86,27 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_f64(const DevPlan* __
     double* nxt = shd + 16 * maxw;
     double* der = shd + 32 * maxw;
     const int li = lane & 15, lk = lane >> 4;
+    double* wl = nullptr;
+    if constexpr (WL) {
+        int dsum = 0;
+        for (int l = 0; l < N.L; ++l) dsum += N.wpad[l];
+        wl = der + 16 * dsum;
+        // one flat 16-byte copy of [W_0 | W_1 | ... | b_0 | ...]: all loads in flight together
+        const double2* src = reinterpret_cast<const double2*>(Wb + N.pack_off);
+        double2* dst = reinterpret_cast<double2*>(wl);
+        const int n2 = N.pack_len / 2;
+#pragma unroll 8
+        for (int idx = tid; idx < n2; idx += kNetThreads) dst[idx] = src[idx];
+    }
+    // layer l's weights and biases: LDS copy or global
+    auto wmat = [&](int l) -> const double* {
+        if constexpr (WL) return wl + N.pw_off[l];
+        else return Wb + N.woff[l];
+    };
+    auto bvec = [&](int l) -> const double* {
+        if constexpr (WL) return wl + N.pb_off[l];
+        else return Wb + N.boff[l];
+    };
     // scaled input
     for (int idx = tid; idx < 16 * N.nin_pad; idx += kNetThreads) {
         const int r = idx / N.nin_pad, k = idx % N.nin_pad;
@@ -118,8 +123,8 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_f64(const DevPlan* __
     int doff = 0;
     for (int l = 0; l < N.L; ++l) {
         const int out_pad = N.wpad[l];
-        const double* W = Wb + N.woff[l];
-        const double* b = Wb + N.boff[l];
+        const double* W = wmat(l);
+        const double* b = bvec(l);
         const int act = (l == N.L - 1) ? NEU_LINEAR : N.act;
         for (int jt = wv; jt < out_pad / 16; jt += kNetWaves) {
             const d4_t acc = tile_mm_f64<true>(cur, maxw, W, in_pad, in_pad, jt, li, lk);
@@ -161,7 +166,7 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_f64(const DevPlan* __
     }
     for (int l = N.L - 1; l >= 1; --l) {
         const int K = N.wpad[l], J = N.wpad[l - 1];
-        const double* W = Wb + N.woff[l];  // wpad[l] x wpad[l-1], row-major
+        const double* W = wmat(l);  // wpad[l] x wpad[l-1], row-major
         doff -= 16 * J;
         for (int jt = wv; jt < J / 16; jt += kNetWaves) {
             const d4_t acc = tile_mm_f64<false>(cur, maxw, W, J, K, jt, li, lk);
@@ -180,7 +185,7 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_f64(const DevPlan* __
     // dE/dG = (delta_0 W_0) .* inpw
     {
         const int K = N.wpad[0], J = N.nin_pad;
-        const double* W = Wb + N.woff[0];
+        const double* W = wmat(0);
         for (int jt = wv; jt < J / 16; jt += kNetWaves) {
             const d4_t acc = tile_mm_f64<false>(cur, maxw, W, J, K, jt, li, lk);
             const int col = jt * 16 + li;
@@ -364,15 +369,22 @@ static NetTiles make_tiles(const int* sp_off, int ntypes, int& ntiles) {
 
 hipError_t launch_net(bool fp64, const DevPlan* plan, const DevPlan& hplan, const void* W, const int* sp_off,
                       const int* spatoms, const void* G, int ldg, int vac, double* eatom, void* dEdG, int bf16_terms,
-                      hipStream_t st) {
+                      int wl_tiles_max, hipStream_t st) {
     int ntiles = 0;
     const NetTiles nt = make_tiles(sp_off, hplan.T, ntiles);
     if (ntiles == 0) return hipSuccess;
-    size_t lds = 0;
-    for (int s = 0; s < hplan.T; ++s) lds = std::max(lds, net_lds_bytes(hplan.net[s], fp64));
-    if (fp64)
-        hipLaunchKernelGGL(k_net_f64, dim3(ntiles), dim3(kNetThreads), lds, st, plan, (const double*)W, nt, spatoms,
-                           (const double*)G, ldg, vac, eatom, (double*)dEdG);
+    size_t lds = 0, ldsw = 0;
+    for (int s = 0; s < hplan.T; ++s) {
+        lds = std::max(lds, net_lds_bytes(hplan.net[s], fp64));
+        ldsw = std::max(ldsw, net_lds_bytes(hplan.net[s], fp64) + 8 * (size_t)net_weight_elems(hplan.net[s]));
+    }
+    // weights in LDS: a few tiles per CU at most, and the copy fits beside the activations
+    if (fp64 && ntiles <= wl_tiles_max && ldsw <= 65536)
+        hipLaunchKernelGGL(k_net_f64<true>, dim3(ntiles), dim3(kNetThreads), ldsw, st, plan, (const double*)W, nt,
+                           spatoms, (const double*)G, ldg, vac, eatom, (double*)dEdG);
+    else if (fp64)
+        hipLaunchKernelGGL(k_net_f64<false>, dim3(ntiles), dim3(kNetThreads), lds, st, plan, (const double*)W, nt,
+                           spatoms, (const double*)G, ldg, vac, eatom, (double*)dEdG);
     else if (bf16_terms == 1)
         hipLaunchKernelGGL(k_net_mixed<1>, dim3(ntiles), dim3(kNetThreads), lds, st, plan, (const float*)W, nt, spatoms,
                            (const float*)G, ldg, vac, eatom, (float*)dEdG);

@@ -1,0 +1,565 @@
+// Energy-training inner loop on the device: the training part of NNPTE::error
+// (src/nnp/nnpte.cpp:869-927) with Cost::grad (src/ml/nn.cpp:1554-1589), for a training set
+// whose descriptors are computed once on the GPU (NNP::symm, nnpte.cpp:1976-1977).
+//
+// Per call (parameter vector x, batch of structures):
+//   1. forward pass of every batch atom (per-species 16-atom tiles, FP64 MFMA as the network
+//      kernels): out_i = outb + outw a_L (no type energy, as the reference's training energy);
+//   2. per structure b: E_b = sum_i out_i (fixed-order tree), dE_b = (E_b - E_ref) / N_b, the loss
+//      and dcdo_b = dC/dE_b / N_b of the loss function (MSE, MAE, HUBER, ASINH);
+//   3. forward again, then backward per tile: dcdz_L = dcdo outw .* dadz_L, dcdz_{l-1} =
+//      (W_l^T dcdz_l) .* dadz_{l-1}; the tile's gradient (bias: sum over its 16 atoms of dcdz_l;
+//      weights: dcdz_l^T a_{l-1} as 16 x 16 MFMA tiles with K = 16 atoms) goes to its own row of
+//      a partial buffer in the reference's parameter order (operator>>, nn.cpp:797-831: every
+//      layer's biases, then every weight matrix column-major);
+//   4. the partial rows of a species are summed in tile order.
+// Every sum has a fixed order, so results do not depend on timing; they match the CPU
+// restatement to rounding (different summation order).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <vector>
+
+#include "../../include/nnp_capi.h"
+#include "basis.hpp"
+#include "engine.hpp"
+#include "netmm.hpp"
+
+namespace atomnn {
+
+// Per-species network for training.  W_l (padded, row-major wpad_l x in_pad_l) and b_l (padded)
+// live in the training parameter buffer; inpw / inpb stay in the model's weight buffer.
+struct TrainNet {
+    int L, nin, nin_pad, act;
+    int nodes[kMaxLayers], wpad[kMaxLayers];
+    int64_t woff[kMaxLayers], boff[kMaxLayers];  // into the training parameter buffer
+    int rb[kMaxLayers], rw[kMaxLayers];          // offsets of b_l and w_l in the species' parameter vector
+    int nparams;
+    int64_t inpw_off, inpb_off;  // into the model's weight buffer
+    double outw, outb;
+};
+struct TrainPlan {
+    int T;
+    TrainNet net[kMaxSpecies];
+};
+// Species ranges of a batch: tiles, species-sorted batch atoms, partial-buffer rows.
+struct TrainTiles {
+    int tile_start[kMaxSpecies + 1];
+    int atom_start[kMaxSpecies + 1];
+    int64_t pbase[kMaxSpecies + 1];  // first partial element of the species' first tile
+};
+
+constexpr int kTrainWaves = 4;
+constexpr int kTrainThreads = kTrainWaves * kWave;
+
+__device__ __forceinline__ int train_species(const TrainTiles& tt, int ntypes, int tile) {
+    int s = 0;
+    while (s + 1 < ntypes && tile >= tt.tile_start[s + 1]) ++s;
+    return s;
+}
+
+// LDS: ins [16][nin_pad], then per layer A_l [16][wpad_l] and D_l [16][wpad_l] (dadz; dcdz in the
+// backward pass, in place).
+static size_t train_lds_bytes(const TrainNet& n) {
+    size_t e = 16 * (size_t)n.nin_pad;
+    for (int l = 0; l < n.L; ++l) e += 2 * 16 * (size_t)n.wpad[l];
+    return 8 * e;
+}
+
+// MODE 0: forward, out[slot] = outb + outw a_L.  MODE 1: forward and backward, the tile's
+// gradient row into `partial`.
+template <int MODE>
+__global__ __launch_bounds__(kTrainThreads) void k_train_tile(
+    const TrainPlan* __restrict__ tp, const double* __restrict__ Wm, const double* __restrict__ par, TrainTiles tt,
+    int tile0, const int* __restrict__ batom, const int* __restrict__ bslot, const int* __restrict__ bstruct,
+    const double* __restrict__ G, int ldg, double* __restrict__ out, const double* __restrict__ dcdo,
+    double* __restrict__ partial) {
+    extern __shared__ __align__(16) double tsh[];
+    const int tile = tile0 + blockIdx.x;
+    const int s = train_species(tt, tp->T, tile);
+    const TrainNet& N = tp->net[s];
+    const int a0 = tt.atom_start[s], M = tt.atom_start[s + 1] - a0;
+    const int r0 = (tile - tt.tile_start[s]) * 16;
+    const int tid = threadIdx.x, wv = tid / kWave, lane = tid % kWave, li = lane & 15, lk = lane >> 4;
+    double* ins = tsh;
+    double* A[kMaxLayers];
+    double* D[kMaxLayers];
+    {
+        double* p = ins + 16 * N.nin_pad;
+        for (int l = 0; l < N.L; ++l) {
+            A[l] = p;
+            D[l] = p + 16 * N.wpad[l];
+            p += 32 * N.wpad[l];
+        }
+    }
+    for (int idx = tid; idx < 16 * N.nin_pad; idx += kTrainThreads) {
+        const int r = idx / N.nin_pad, k = idx % N.nin_pad;
+        double v = 0.0;
+        if (r0 + r < M && k < N.nin) v = Wm[N.inpw_off + k] * (G[(int64_t)batom[a0 + r0 + r] * ldg + k] + Wm[N.inpb_off + k]);
+        ins[r * N.nin_pad + k] = v;
+    }
+    __syncthreads();
+    for (int l = 0; l < N.L; ++l) {
+        const double* prev = l == 0 ? ins : A[l - 1];
+        const int in_pad = l == 0 ? N.nin_pad : N.wpad[l - 1], wp = N.wpad[l];
+        const int act = l == N.L - 1 ? NEU_LINEAR : N.act;
+        for (int jt = wv; jt < wp / 16; jt += kTrainWaves) {
+            const d4_t acc = tile_mm_f64<true>(prev, in_pad, par + N.woff[l], in_pad, in_pad, jt, li, lk);
+            const int col = jt * 16 + li;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = 4 * v + lk;
+                double av, dv;
+                act_eval<double>(act, acc[v] + par[N.boff[l] + col], av, dv);
+                A[l][row * wp + col] = av;
+                D[l][row * wp + col] = dv;
+            }
+        }
+        __syncthreads();
+    }
+    const int L = N.L;
+    if constexpr (MODE == 0) {
+        if (tid < 16 && r0 + tid < M) out[bslot[a0 + r0 + tid]] = N.outb + A[L - 1][tid * N.wpad[L - 1]] * N.outw;
+        return;
+    } else {
+        // dcdz_L = dcdo outw .* dadz_L (single output unit; padded units and rows carry zero)
+        {
+            const int wp = N.wpad[L - 1];
+            for (int idx = tid; idx < 16 * wp; idx += kTrainThreads) {
+                const int r = idx / wp, n = idx % wp;
+                const double c = (r0 + r < M && n < N.nodes[L - 1]) ? dcdo[bstruct[a0 + r0 + r]] * N.outw : 0.0;
+                D[L - 1][idx] *= c;
+            }
+        }
+        __syncthreads();
+        double* prow = partial + tt.pbase[s] + (int64_t)(tile - tt.tile_start[s]) * N.nparams;
+        for (int l = L - 1; l >= 0; --l) {
+            const double* prev = l == 0 ? ins : A[l - 1];
+            const int wp = N.wpad[l], in_pad = l == 0 ? N.nin_pad : N.wpad[l - 1];
+            const int in_real = l == 0 ? N.nin : N.nodes[l - 1], no = N.nodes[l];
+            const double* dz = D[l];  // dcdz_l [16][wp]
+            // bias gradient: sum over the tile's 16 atoms
+            for (int n = tid; n < no; n += kTrainThreads) {
+                double g = 0.0;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) g += dz[r * wp + n];
+                prow[N.rb[l] + n] = g;
+            }
+            // weight gradient dcdz_l^T a_{l-1}: tiles (nt, mt) of 16 x 16, K = the 16 atoms
+            const int ntn = wp / 16, ntm = in_pad / 16;
+            for (int t = wv; t < ntn * ntm; t += kTrainWaves) {
+                const int nt = t / ntm, mt = t % ntm;
+                d4_t acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int r = 4 * q + lk;
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(dz[r * wp + nt * 16 + li], prev[r * in_pad + mt * 16 + li],
+                                                               acc, 0, 0, 0);
+                }
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int n = nt * 16 + 4 * v + lk, m = mt * 16 + li;
+                    if (n < no && m < in_real) prow[N.rw[l] + m * no + n] = acc[v];
+                }
+            }
+            // dcdz_{l-1} = (W_l^T dcdz_l) .* dadz_{l-1}, in place over dadz_{l-1}
+            if (l > 0) {
+                const int wq = N.wpad[l - 1];
+                for (int jt = wv; jt < wq / 16; jt += kTrainWaves) {
+                    const d4_t acc = tile_mm_f64<false>(dz, wp, par + N.woff[l], wq, wp, jt, li, lk);
+                    const int col = jt * 16 + li;
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int row = 4 * v + lk;
+                        D[l - 1][row * wq + col] *= acc[v];
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Per batch structure: E = sum of its atoms' outputs (fixed-order tree), dE = (E - E_ref) / N,
+// the loss and dcdo (nnpte.cpp:876-916).
+__global__ __launch_bounds__(256) void k_train_struct(const double* __restrict__ out, const int* __restrict__ bstart,
+                                                      const double* __restrict__ eref_b, int loss, double delta,
+                                                      double* __restrict__ dcdo, double* __restrict__ lossv,
+                                                      double* __restrict__ dE2) {
+    __shared__ double part[256];
+    const int b = blockIdx.x, t = threadIdx.x;
+    const int o0 = bstart[b], n = bstart[b + 1] - o0, chunk = (n + 255) / 256;
+    double acc = 0.0;
+    for (int i = o0 + t * chunk; i < o0 + min(n, (t + 1) * chunk); ++i) acc += out[i];
+    part[t] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (t < o) part[t] += part[t + o];
+        __syncthreads();
+    }
+    if (t == 0) {
+        const double norm = 1.0 / n;
+        const double dE = (part[0] - eref_b[b]) * norm;
+        double c = norm, L = 0.0;
+        const double deltai = 1.0 / delta, delta2 = delta * delta;
+        if (loss == NNP_LOSS_MSE) {
+            L = 0.5 * dE * dE;
+            c *= dE;
+        } else if (loss == NNP_LOSS_MAE) {
+            L = fabs(dE);
+            c *= (double)((0.0 < dE) - (dE < 0.0));
+        } else if (loss == NNP_LOSS_HUBER) {
+            const double arg = dE * deltai, sq = sqrt(1.0 + arg * arg);
+            L = delta2 * (sq - 1.0);
+            c *= dE / sq;
+        } else {
+            const double arg = dE * deltai, sq = sqrt(1.0 + arg * arg), lg = log(arg + sq);
+            L = delta2 * (1.0 - sq + arg * lg);
+            c *= lg * delta;
+        }
+        dcdo[b] = c;
+        lossv[b] = L;
+        dE2[b] = dE * dE;
+    }
+}
+
+// grad[goff_s + p] (+)= sum over the chunk's tiles of species s, in tile order.
+__global__ __launch_bounds__(256) void k_train_reduce(const TrainPlan* __restrict__ tp, TrainTiles tt, int tile0,
+                                                      int tile1, const double* __restrict__ partial,
+                                                      const int64_t* __restrict__ goff, double* __restrict__ grad,
+                                                      int accumulate) {
+    const int s = blockIdx.y;
+    const TrainNet& N = tp->net[s];
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= N.nparams) return;
+    const int t0 = max(tile0, tt.tile_start[s]), t1 = min(tile1, tt.tile_start[s + 1]);
+    double acc = accumulate ? grad[goff[s] + p] : 0.0;
+    for (int t = t0; t < t1; ++t) acc += partial[tt.pbase[s] + (int64_t)(t - tt.tile_start[s]) * N.nparams + p];
+    grad[goff[s] + p] = acc;
+}
+
+// ------------------------------------------------------------------ host side
+class TrainSet {
+  public:
+    TrainSet(Engine& owner, int n, const nnp_structure* ss, const double* energy) : owner_(owner) {
+        if (n <= 0 || !ss || !energy) throw std::invalid_argument("nnp_train_create: invalid argument");
+        const HostModel& hm = owner.host;
+        const DevPlan& P = owner.pb.plan;
+        T_ = hm.T;
+        ldg_ = P.ldg;
+        // network layout
+        std::memset(&plan_, 0, sizeof(plan_));
+        plan_.T = T_;
+        int64_t pofs = 0;
+        goff_.assign(T_ + 1, 0);
+        for (int t = 0; t < T_; ++t) {
+            const NetSpec& ns = hm.sp[t].net;
+            const DevNet& dn = P.net[t];
+            TrainNet& N = plan_.net[t];
+            N.L = dn.L;
+            N.nin = ns.nin;
+            N.nin_pad = dn.nin_pad;
+            N.act = dn.act;
+            N.inpw_off = dn.inpw_off;
+            N.inpb_off = dn.inpb_off;
+            N.outw = dn.outw;
+            N.outb = dn.outb;
+            if (ns.nodes.back() != 1) throw unsupported_error("nnp_train: networks with more than one output");
+            int rb = 0, nb = 0;
+            for (int l = 0; l < N.L; ++l) nb += ns.nodes[l];
+            int rw = nb;
+            for (int l = 0; l < N.L; ++l) {
+                const int in_pad = l == 0 ? N.nin_pad : dn.wpad[l - 1];
+                const int in_real = l == 0 ? ns.nin : ns.nodes[l - 1];
+                N.nodes[l] = ns.nodes[l];
+                N.wpad[l] = dn.wpad[l];
+                N.woff[l] = pofs;
+                pofs += (int64_t)dn.wpad[l] * in_pad;
+                N.boff[l] = pofs;
+                pofs += dn.wpad[l];
+                pofs = (pofs + 1) & ~int64_t(1);
+                N.rb[l] = rb;
+                rb += ns.nodes[l];
+                N.rw[l] = rw;
+                rw += ns.nodes[l] * in_real;
+            }
+            N.nparams = rw;
+            goff_[t + 1] = goff_[t] + N.nparams;
+        }
+        npar_dev_ = pofs;
+        hpar_.assign(npar_dev_, 0.0);
+        d_plan_.reserve(sizeof(TrainPlan));
+        HIPCHK(hipMemcpy(d_plan_.p, &plan_, sizeof(TrainPlan), hipMemcpyHostToDevice));
+        d_goff_.reserve(sizeof(int64_t) * (T_ + 1));
+        HIPCHK(hipMemcpy(d_goff_.p, goff_.data(), sizeof(int64_t) * (T_ + 1), hipMemcpyHostToDevice));
+        // descriptors of the training set, computed once in chunks by a workspace of the model
+        natoms_.resize(n);
+        aoff_.assign(n + 1, 0);
+        for (int k = 0; k < n; ++k) {
+            if (ss[k].natoms <= 0) throw std::invalid_argument("nnp_train_create: empty structure");
+            natoms_[k] = ss[k].natoms;
+            aoff_[k + 1] = aoff_[k] + ss[k].natoms;
+            for (int i = 0; i < ss[k].natoms; ++i) types_.push_back(ss[k].type[i]);
+        }
+        eref_.assign(energy, energy + n);
+        ntot_ = aoff_[n];
+        d_G_.reserve((size_t)ntot_ * ldg_ * 8);
+        Engine E(owner, 1000);
+        int k0 = 0;
+        while (k0 < n) {
+            int k1 = k0;
+            int64_t atoms = 0;
+            while (k1 < n && (k1 == k0 || atoms + natoms_[k1] <= (1 << 18))) atoms += natoms_[k1++];
+            std::vector<nnp_structure> chunk(ss + k0, ss + k1);
+            E.upload(chunk);
+            E.build_neighbors(false);
+            E.descriptors<double>();
+            HIPCHK(hipMemcpyAsync((double*)d_G_.p + (size_t)aoff_[k0] * ldg_, E.d_G.p, (size_t)atoms * ldg_ * 8,
+                                  hipMemcpyDeviceToDevice, E.stream));
+            HIPCHK(hipStreamSynchronize(E.stream));
+            k0 = k1;
+        }
+        HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    }
+    ~TrainSet() {
+        if (stream_) (void)hipStreamDestroy(stream_);
+    }
+
+    int64_t nparams(int t) const {
+        if (t < 0 || t >= T_) throw std::invalid_argument("nnp_train_nparams: species out of range");
+        return plan_.net[t].nparams;
+    }
+    int64_t total() const { return goff_[T_]; }
+
+    // the model's current parameters in the reference order (operator>>)
+    void get_params(double* x) const {
+        for (int t = 0; t < T_; ++t) {
+            const NetSpec& ns = owner_.host.sp[t].net;
+            const TrainNet& N = plan_.net[t];
+            double* y = x + goff_[t];
+            for (int l = 0; l < N.L; ++l)
+                for (int n = 0; n < N.nodes[l]; ++n) y[N.rb[l] + n] = ns.b[l][n];
+            for (int l = 0; l < N.L; ++l) {
+                const int in_real = l == 0 ? ns.nin : ns.nodes[l - 1];
+                for (int m = 0; m < in_real; ++m)
+                    for (int n = 0; n < N.nodes[l]; ++n) y[N.rw[l] + m * N.nodes[l] + n] = ns.W[l][(size_t)n * in_real + m];
+            }
+        }
+    }
+
+    void error(const double* x, int nbatch, const int* batch, int loss, double delta, double* err, double* grad) {
+        if (!x || !err || nbatch <= 0 || !batch) throw std::invalid_argument("nnp_train_error: invalid argument");
+        if (loss < NNP_LOSS_MSE || loss > NNP_LOSS_ASINH) throw std::invalid_argument("nnp_train_error: unknown loss");
+        if ((loss == NNP_LOSS_HUBER || loss == NNP_LOSS_ASINH) && !(delta > 0))
+            throw std::invalid_argument("nnp_train_error: delta must be positive");
+        const int nst = (int)natoms_.size();
+        for (int b = 0; b < nbatch; ++b)
+            if (batch[b] < 0 || batch[b] >= nst) throw std::invalid_argument("nnp_train_error: structure out of range");
+        // parameters: reference order -> padded row-major layers
+        for (int t = 0; t < T_; ++t) {
+            const TrainNet& N = plan_.net[t];
+            const double* y = x + goff_[t];
+            for (int l = 0; l < N.L; ++l) {
+                const int in_pad = l == 0 ? N.nin_pad : N.wpad[l - 1];
+                const int in_real = l == 0 ? N.nin : N.nodes[l - 1];
+                for (int n = 0; n < N.nodes[l]; ++n) {
+                    hpar_[N.boff[l] + n] = y[N.rb[l] + n];
+                    for (int m = 0; m < in_real; ++m) hpar_[N.woff[l] + (int64_t)n * in_pad + m] = y[N.rw[l] + m * N.nodes[l] + n];
+                }
+            }
+        }
+        // batch atoms: structure order (slots) and species order (tiles)
+        std::vector<int> bstart(nbatch + 1, 0);
+        for (int b = 0; b < nbatch; ++b) bstart[b + 1] = bstart[b] + natoms_[batch[b]];
+        const int nba = bstart[nbatch];
+        std::vector<std::vector<int>> atom_s(T_), slot_s(T_), struct_s(T_);
+        for (int b = 0; b < nbatch; ++b) {
+            const int st = batch[b];
+            for (int i = 0; i < natoms_[st]; ++i) {
+                const int g = aoff_[st] + i, t = types_[g];
+                if (t < 0 || t >= T_) throw std::invalid_argument("atom type out of range");
+                atom_s[t].push_back(g);
+                slot_s[t].push_back(bstart[b] + i);
+                struct_s[t].push_back(b);
+            }
+        }
+        TrainTiles tt;
+        std::memset(&tt, 0, sizeof(tt));
+        std::vector<int> batom, bslot, bstruct;
+        int tiles = 0;
+        int64_t pb = 0;
+        for (int t = 0; t <= kMaxSpecies; ++t) {
+            tt.tile_start[t] = tiles;
+            tt.atom_start[t] = (int)batom.size();
+            tt.pbase[t] = pb;
+            if (t < T_) {
+                batom.insert(batom.end(), atom_s[t].begin(), atom_s[t].end());
+                bslot.insert(bslot.end(), slot_s[t].begin(), slot_s[t].end());
+                bstruct.insert(bstruct.end(), struct_s[t].begin(), struct_s[t].end());
+                const int nt = ((int)atom_s[t].size() + 15) / 16;
+                tiles += nt;
+                pb += (int64_t)nt * plan_.net[t].nparams;
+            }
+        }
+        std::vector<double> eb(nbatch);
+        for (int b = 0; b < nbatch; ++b) eb[b] = eref_[batch[b]];
+        // device buffers (grow only)
+        d_par_.reserve(8 * (size_t)npar_dev_);
+        d_batom_.reserve(4 * (size_t)std::max(nba, 1));
+        d_bslot_.reserve(4 * (size_t)std::max(nba, 1));
+        d_bstruct_.reserve(4 * (size_t)std::max(nba, 1));
+        d_bstart_.reserve(4 * (size_t)(nbatch + 1));
+        d_eref_.reserve(8 * (size_t)nbatch);
+        d_out_.reserve(8 * (size_t)std::max(nba, 1));
+        d_dcdo_.reserve(8 * (size_t)nbatch);
+        d_loss_.reserve(8 * (size_t)nbatch);
+        d_dE2_.reserve(8 * (size_t)nbatch);
+        d_grad_.reserve(8 * (size_t)total());
+        HIPCHK(hipMemcpyAsync(d_par_.p, hpar_.data(), 8 * (size_t)npar_dev_, hipMemcpyHostToDevice, stream_));
+        HIPCHK(hipMemcpyAsync(d_batom_.p, batom.data(), 4 * (size_t)nba, hipMemcpyHostToDevice, stream_));
+        HIPCHK(hipMemcpyAsync(d_bslot_.p, bslot.data(), 4 * (size_t)nba, hipMemcpyHostToDevice, stream_));
+        HIPCHK(hipMemcpyAsync(d_bstruct_.p, bstruct.data(), 4 * (size_t)nba, hipMemcpyHostToDevice, stream_));
+        HIPCHK(hipMemcpyAsync(d_bstart_.p, bstart.data(), 4 * (size_t)(nbatch + 1), hipMemcpyHostToDevice, stream_));
+        HIPCHK(hipMemcpyAsync(d_eref_.p, eb.data(), 8 * (size_t)nbatch, hipMemcpyHostToDevice, stream_));
+        size_t lds = 0;
+        for (int t = 0; t < T_; ++t) lds = std::max(lds, train_lds_bytes(plan_.net[t]));
+        if (lds > 65536) throw unsupported_error("nnp_train: network too wide for the training kernel's LDS");
+        const double* Wm = (const double*)owner_.d_w64.p;
+        if (tiles > 0)
+            hipLaunchKernelGGL(k_train_tile<0>, dim3(tiles), dim3(kTrainThreads), lds, stream_, (const TrainPlan*)d_plan_.p,
+                               Wm, (const double*)d_par_.p, tt, 0, (const int*)d_batom_.p, (const int*)d_bslot_.p,
+                               (const int*)d_bstruct_.p, (const double*)d_G_.p, ldg_, (double*)d_out_.p, nullptr,
+                               nullptr);
+        hipLaunchKernelGGL(k_train_struct, dim3(nbatch), dim3(256), 0, stream_, (const double*)d_out_.p,
+                           (const int*)d_bstart_.p, (const double*)d_eref_.p, loss, delta, (double*)d_dcdo_.p,
+                           (double*)d_loss_.p, (double*)d_dE2_.p);
+        HIPCHK(hipGetLastError());
+        if (grad) {
+            // partial rows in chunks of tiles (at most ~128 MB of partial sums at a time)
+            int64_t maxnp = 1;
+            for (int t = 0; t < T_; ++t) maxnp = std::max<int64_t>(maxnp, plan_.net[t].nparams);
+            const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, ((int64_t)16 << 20) / maxnp));
+            d_partial_.reserve(8 * (size_t)std::min<int64_t>(pb, (int64_t)chunk * maxnp + maxnp));
+            for (int c0 = 0; c0 < tiles; c0 += chunk) {
+                const int c1 = std::min(tiles, c0 + chunk);
+                // partial rows of this chunk start at element 0: shift each species' base
+                TrainTiles tc = tt;
+                int64_t base = 0;
+                for (int t = 0; t < T_; ++t) {
+                    const int t0 = std::max(c0, tt.tile_start[t]), t1 = std::min(c1, tt.tile_start[t + 1]);
+                    // rows of tiles [t0, t1) of species t at base; the kernel indexes by (tile - tile_start)
+                    tc.pbase[t] = base - (int64_t)(t0 - tt.tile_start[t]) * plan_.net[t].nparams;
+                    if (t1 > t0) base += (int64_t)(t1 - t0) * plan_.net[t].nparams;
+                }
+                hipLaunchKernelGGL(k_train_tile<1>, dim3(c1 - c0), dim3(kTrainThreads), lds, stream_,
+                                   (const TrainPlan*)d_plan_.p, Wm, (const double*)d_par_.p, tc, c0,
+                                   (const int*)d_batom_.p, (const int*)d_bslot_.p, (const int*)d_bstruct_.p,
+                                   (const double*)d_G_.p, ldg_, nullptr, (const double*)d_dcdo_.p,
+                                   (double*)d_partial_.p);
+                hipLaunchKernelGGL(k_train_reduce, dim3((unsigned)((maxnp + 255) / 256), T_), dim3(256), 0, stream_,
+                                   (const TrainPlan*)d_plan_.p, tc, c0, c1, (const double*)d_partial_.p,
+                                   (const int64_t*)d_goff_.p, (double*)d_grad_.p, c0 > 0 ? 1 : 0);
+                HIPCHK(hipGetLastError());
+            }
+            if (tiles == 0) HIPCHK(hipMemsetAsync(d_grad_.p, 0, 8 * (size_t)total(), stream_));
+        }
+        std::vector<double> lv(nbatch), e2(nbatch);
+        HIPCHK(hipMemcpyAsync(lv.data(), d_loss_.p, 8 * (size_t)nbatch, hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipMemcpyAsync(e2.data(), d_dE2_.p, 8 * (size_t)nbatch, hipMemcpyDeviceToHost, stream_));
+        if (grad) HIPCHK(hipMemcpyAsync(grad, d_grad_.p, 8 * (size_t)total(), hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+        err[0] = err[1] = 0.0;
+        for (int b = 0; b < nbatch; ++b) {
+            err[0] += lv[b];
+            err[1] += e2[b];
+        }
+    }
+
+  private:
+    Engine& owner_;
+    int T_ = 0, ldg_ = 0;
+    int64_t ntot_ = 0, npar_dev_ = 0;
+    TrainPlan plan_;
+    std::vector<int64_t> goff_;
+    std::vector<int> natoms_, types_;
+    std::vector<int64_t> aoff_;
+    std::vector<double> eref_, hpar_;
+    hipStream_t stream_ = nullptr;
+    DevMem d_plan_, d_goff_, d_G_, d_par_, d_batom_, d_bslot_, d_bstruct_, d_bstart_, d_eref_, d_out_, d_dcdo_, d_loss_,
+        d_dE2_, d_grad_, d_partial_;
+};
+
+Engine& engine_of(nnp_model* m);
+
+}  // namespace atomnn
+
+using namespace atomnn;
+
+struct nnp_train {
+    std::unique_ptr<TrainSet> t;
+};
+
+template <typename F>
+static int train_guard(F&& f) {
+    try {
+        set_error("");
+        return f();
+    } catch (const unsupported_error& e) {
+        set_error(e.what());
+        return NNP_EUNSUPPORTED;
+    } catch (const hip_failure& e) {
+        set_error(e.what());
+        return NNP_EDEVICE;
+    } catch (const std::bad_alloc&) {
+        set_error("out of memory");
+        return NNP_ENOMEM;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return NNP_EINVAL;
+    }
+}
+
+extern "C" {
+
+int nnp_train_create(nnp_model* m, int nstruc, const nnp_structure* s, const double* energy, nnp_train** out) {
+    return train_guard([&]() {
+        if (!m || !out) throw std::invalid_argument("nnp_train_create: null argument");
+        *out = new nnp_train{std::make_unique<TrainSet>(engine_of(m), nstruc, s, energy)};
+        return NNP_OK;
+    });
+}
+
+int64_t nnp_train_nparams(nnp_train* t, int species) {
+    int64_t n = -1;
+    const int rc = train_guard([&]() {
+        if (!t) throw std::invalid_argument("nnp_train_nparams: null argument");
+        n = species < 0 ? t->t->total() : t->t->nparams(species);
+        return NNP_OK;
+    });
+    return rc < 0 ? rc : n;
+}
+
+int nnp_train_get_params(nnp_train* t, double* x) {
+    return train_guard([&]() {
+        if (!t || !x) throw std::invalid_argument("nnp_train_get_params: null argument");
+        t->t->get_params(x);
+        return NNP_OK;
+    });
+}
+
+int nnp_train_error(nnp_train* t, const double* x, int nbatch, const int* batch, int loss, double delta, double* err,
+                    double* grad) {
+    return train_guard([&]() {
+        if (!t) throw std::invalid_argument("nnp_train_error: null argument");
+        t->t->error(x, nbatch, batch, loss, delta, err, grad);
+        return NNP_OK;
+    });
+}
+
+void nnp_train_destroy(nnp_train* t) { delete t; }
+
+}  // extern "C"

@@ -91,6 +91,12 @@ def lib():
         L.nnp_md_step.argtypes = [vp, ctypes.c_int]
         L.nnp_md_state.argtypes = [vp, vp, vp, vp, vp]
         L.nnp_md_destroy.argtypes = [vp]
+        L.nnp_train_create.argtypes = [vp, ctypes.c_int, ctypes.POINTER(_Struc), vp, ctypes.POINTER(vp)]
+        L.nnp_train_nparams.restype = i64
+        L.nnp_train_nparams.argtypes = [vp, ctypes.c_int]
+        L.nnp_train_get_params.argtypes = [vp, vp]
+        L.nnp_train_error.argtypes = [vp, vp, ctypes.c_int, vp, ctypes.c_int, dp, vp, vp]
+        L.nnp_train_destroy.argtypes = [vp]
         L.nnp_md_destroy.restype = None
         _lib = L
     return _lib
@@ -280,6 +286,60 @@ class MD:
     def close(self) -> None:
         if getattr(self, "_h", None):
             lib().nnp_md_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+LOSS_MSE, LOSS_MAE, LOSS_HUBER, LOSS_ASINH = 0, 1, 2, 3
+
+
+class Train:
+    """The energy-training inner loop on the GPU: NNPTE::error's training part
+    (src/nnp/nnpte.cpp:869-927) with Cost::grad (src/ml/nn.cpp:1554-1589).  Descriptors of the
+    structures are computed once; error(x, batch) returns ([sum of losses, sum of dE^2], gradient)
+    for the parameter vector x (species in order, each in ANN operator>> order)."""
+
+    def __init__(self, model: NNP, structures: Sequence[Structure], energies):
+        pins = [_Pinned(s) for s in structures]
+        arr = (_Struc * len(structures))(*[p.c_struct for p in pins])
+        e = np.ascontiguousarray(energies, dtype=np.float64)
+        if e.shape[0] != len(structures):
+            raise ValueError("one reference energy per structure")
+        h = ctypes.c_void_p()
+        _check(lib().nnp_train_create(model._h, len(structures), arr, e.ctypes.data, ctypes.byref(h)))
+        self._h = h
+        self.ntypes = model.ntypes
+
+    def nparams(self, species: int = -1) -> int:
+        n = lib().nnp_train_nparams(self._h, int(species))
+        if n < 0:
+            _check(int(n))
+        return int(n)
+
+    def params(self) -> np.ndarray:
+        x = np.zeros(self.nparams())
+        _check(lib().nnp_train_get_params(self._h, x.ctypes.data))
+        return x
+
+    def error(self, x, batch, loss: int = LOSS_MSE, delta: float = 1.0, grad: bool = True):
+        xx = np.ascontiguousarray(x, dtype=np.float64)
+        if xx.shape[0] != self.nparams():
+            raise ValueError("parameter vector of the wrong length")
+        b = np.ascontiguousarray(batch, dtype=np.int32)
+        err = np.zeros(2)
+        g = np.zeros(xx.shape[0]) if grad else None
+        _check(lib().nnp_train_error(self._h, xx.ctypes.data, int(b.shape[0]), b.ctypes.data, int(loss), float(delta),
+                                     err.ctypes.data, g.ctypes.data if grad else None))
+        return err, g
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().nnp_train_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -173,6 +173,32 @@ int nnp_set_option(nnp_model* m, const char* key, int64_t value);
  * pipeline, fused blocks per CU (tuned), compute units, moment bits of the last pass. */
 int nnp_info(nnp_model* m, int64_t* out, int nout);
 
+/* ---- energy-training inner loop ----
+ * Replaces the training part of NNPTE::error (src/nnp/nnpte.cpp:869-927) for one process:
+ * for each structure b of a batch, E_b = sum of the atomic network outputs (ANN::fp, without
+ * the type energies), dE_b = (E_b - E_ref,b) / N_b, the loss and dC/dE of the loss function
+ * (nnpte.cpp:884-916), and per atom fpbp + Cost::grad (src/ml/nn.cpp:1554-1589) added into its
+ * species' gradient.  The descriptors of the training structures are computed once at creation
+ * (NNP::symm, nnpte.cpp:1976-1977).  Parameter vectors hold every species in species order, each
+ * in ANN operator>> order (src/ml/nn.cpp:797-831): the biases of every layer, then every weight
+ * matrix column-major. */
+#define NNP_LOSS_MSE 0   /* 0.5 dE^2 */
+#define NNP_LOSS_MAE 1   /* |dE| */
+#define NNP_LOSS_HUBER 2 /* delta^2 (sqrt(1 + (dE/delta)^2) - 1) */
+#define NNP_LOSS_ASINH 3 /* delta^2 (1 - sqrt(1 + a^2) + a asinh(a)), a = dE/delta */
+typedef struct nnp_train nnp_train;
+int nnp_train_create(nnp_model* m, int nstruc, const nnp_structure* s, const double* energy, nnp_train** out);
+/* parameters of one species' network, or of all species (species < 0); negative on error */
+int64_t nnp_train_nparams(nnp_train* t, int species);
+/* the model's current parameters (the x of the next call) */
+int nnp_train_get_params(nnp_train* t, double* x);
+/* err[0] = sum over the batch of the losses, err[1] = sum of dE^2 (error_[0] and error_[2] of
+ * NNPTE::error before its BATCH normalisation); grad (may be NULL) = d err[0] / d x.  The model
+ * used for evaluation (nnp_compute ...) keeps its own parameters. */
+int nnp_train_error(nnp_train* t, const double* x, int nbatch, const int* batch, int loss, double delta, double* err,
+                    double* grad);
+void nnp_train_destroy(nnp_train* t);
+
 /* ---- molecular dynamics on the device-resident structure ----
  * Replaces the reference's MD step, Verlet::compute (src/torch/integrator.cpp:170-188: half
  * kick, drift, Cell::returnToCell, Engine::compute -> Pot::compute (src/torch/engine.cpp:

@@ -728,6 +728,59 @@ struct ANN {
         }
         return outb[0] + a.back()[0] * outw[0];  // nn.cpp:1229-1230
     }
+    // Parameter vector in ANN operator>> / operator<< order (nn.cpp:797-831): the biases of
+    // every layer, then every weight matrix in Eigen's column-major order (w(l)(n,m) at
+    // m n_l + n).
+    int nparams() const {
+        int p = 0;
+        for (int l = 0; l < nlayer(); ++l) p += nodes[l] + nodes[l] * in_size(l);
+        return p;
+    }
+    void get_params(double* x) const {
+        int c = 0;
+        for (int l = 0; l < nlayer(); ++l)
+            for (int n = 0; n < nodes[l]; ++n) x[c++] = b[l][n];
+        for (int l = 0; l < nlayer(); ++l)
+            for (int m = 0; m < in_size(l); ++m)
+                for (int n = 0; n < nodes[l]; ++n) x[c++] = w[l][(size_t)n * in_size(l) + m];
+    }
+    void set_params(const double* x) {
+        int c = 0;
+        for (int l = 0; l < nlayer(); ++l)
+            for (int n = 0; n < nodes[l]; ++n) b[l][n] = x[c++];
+        for (int l = 0; l < nlayer(); ++l)
+            for (int m = 0; m < in_size(l); ++m)
+                for (int n = 0; n < nodes[l]; ++n) w[l][(size_t)n * in_size(l) + m] = x[c++];
+    }
+    // Cost::grad (nn.cpp:1554-1589) after fpbp on input `inp`: dcdz_L = dcdo outw .* dadz_L,
+    // dcdz_{l-1} = (w_l^T dcdz_l) .* dadz_{l-1}; gradient = the dcdz (biases), then
+    // dcdz_l[n] a_{l-1}[m] (weights, column-major; a_{-1} = ins).  Adds into g.
+    void cost_grad(const double* inp, const std::vector<std::vector<double>>& a,
+                   const std::vector<std::vector<double>>& d, double dcdo, double* g) const {
+        const int L = nlayer();
+        std::vector<std::vector<double>> dz(L);
+        dz[L - 1].assign(nodes[L - 1], 0.0);
+        for (int i = 0; i < nodes[L - 1]; ++i) dz[L - 1][i] = dcdo * outw[i] * d[L - 1][i];
+        for (int l = L - 1; l > 0; --l) {
+            const int no = nodes[l], ni = nodes[l - 1];
+            dz[l - 1].assign(ni, 0.0);
+            for (int k = 0; k < ni; ++k) {
+                double s = 0;
+                for (int o = 0; o < no; ++o) s += w[l][(size_t)o * ni + k] * dz[l][o];
+                dz[l - 1][k] = s * d[l - 1][k];
+            }
+        }
+        std::vector<double> ins(nin);
+        for (int i = 0; i < nin; ++i) ins[i] = inpw[i] * (inp[i] + inpb[i]);
+        int c = 0;
+        for (int l = 0; l < L; ++l)
+            for (int n = 0; n < nodes[l]; ++n) g[c++] += dz[l][n];
+        for (int l = 0; l < L; ++l) {
+            const double* prev = l == 0 ? ins.data() : a[l - 1].data();
+            for (int m = 0; m < in_size(l); ++m)
+                for (int n = 0; n < nodes[l]; ++n) g[c++] += dz[l][n] * prev[m];
+        }
+    }
     // DODZ::grad (nn.cpp:1620-1628), output row 0: dodi = d out / d inp
     void dodi(const std::vector<std::vector<double>>& d, double* g) const {
         const int L = nlayer();
@@ -1282,6 +1335,111 @@ double orc_bench(void* mp, int nstruc, const int* natoms, const double* cells, c
     } catch (std::exception& e) {
         g_err = e.what();
         return -1.0;
+    }
+}
+
+// ---------------------------------------------------------------- training inner loop
+// NNPTE::error, training part (nnpte.cpp:869-927): for each structure of the batch,
+// E = sum_i out_i (ANN::fp, no type energy), dE = (E - E_ref) / N, the loss and dcdo = dC/dE by
+// the loss function (MSE, MAE, HUBER, ASINH with delta, deltai = 1/delta, delta2 = delta^2;
+// nnpte.cpp:1195-1197), then per atom fpbp + Cost::grad(nn, dcdo) added to its species'
+// gradient.  err[0] = sum of losses, err[1] = sum of dE^2 (error_[0], error_[2] before the
+// BATCH normalisation, which is 1 on one process).  Descriptors are computed once at creation
+// (NNP::symm, nnpte.cpp:1976-1977).
+struct TrainSet {
+    Model m;  // a copy: the parameters change per call
+    std::vector<Structure> st;
+    std::vector<double> eref;
+};
+
+int orc_train_nparams(void* tp, int type) { return ((TrainSet*)tp)->m.nnh[type].nn.nparams(); }
+
+void* orc_train_create(void* mp, int nstruc, const int* natoms, const double* cells, const double* pos, const int* type,
+                       const double* eref) {
+    try {
+        auto* t = new TrainSet();
+        t->m = *(Model*)mp;
+        long long off = 0;
+        for (int s = 0; s < nstruc; ++s) {
+            Structure st;
+            const double* cell = cells ? cells + 9 * (size_t)s : nullptr;
+            bool open = true;
+            if (cell)
+                for (int q = 0; q < 9; ++q) open = open && cell[q] == 0.0;
+            load_structure(st, natoms[s], open ? nullptr : cell, pos + 3 * off, type + off, &t->m);
+            NeighborList nl;
+            neighbor_build(st.cell, st.n, st.pos.data(), st.type.data(), t->m.rc, nl);
+            nnp_symm(t->m, st, nl);
+            t->st.push_back(std::move(st));
+            t->eref.push_back(eref[s]);
+            off += natoms[s];
+        }
+        return t;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
+void orc_train_free(void* tp) { delete (TrainSet*)tp; }
+
+int orc_train_get_params(void* tp, double* x) {
+    TrainSet& t = *(TrainSet*)tp;
+    int c = 0;
+    for (int n = 0; n < t.m.ntypes; ++n) {
+        t.m.nnh[n].nn.get_params(x + c);
+        c += t.m.nnh[n].nn.nparams();
+    }
+    return c;
+}
+
+int orc_train_error(void* tp, const double* x, int nbatch, const int* batch, int loss, double delta, double* err,
+                    double* grad) {
+    try {
+        TrainSet& t = *(TrainSet*)tp;
+        const int T = t.m.ntypes;
+        std::vector<int> off(T + 1, 0);
+        for (int n = 0; n < T; ++n) off[n + 1] = off[n] + t.m.nnh[n].nn.nparams();
+        for (int n = 0; n < T; ++n) t.m.nnh[n].nn.set_params(x + off[n]);
+        if (grad)
+            for (int q = 0; q < off[T]; ++q) grad[q] = 0.0;
+        const double deltai = 1.0 / delta, delta2 = delta * delta;
+        err[0] = err[1] = 0.0;
+        std::vector<std::vector<double>> a, d;
+        for (int b = 0; b < nbatch; ++b) {
+            const Structure& s = t.st.at(batch[b]);
+            double energy = 0;
+            for (int i = 0; i < s.n; ++i) energy += t.m.nnh[s.type[i]].nn.fpbp(s.symm[i].data(), a, d);
+            const double norm = 1.0 / s.n;
+            const double dE = (energy - t.eref[batch[b]]) * norm;
+            double dcdo = norm;
+            switch (loss) {
+                case 0: err[0] += 0.5 * dE * dE; dcdo *= dE; break;                           // MSE
+                case 1: err[0] += std::fabs(dE); dcdo *= (double)((0.0 < dE) - (dE < 0.0)); break;  // MAE
+                case 2: {                                                                     // HUBER
+                    const double arg = dE * deltai, sq = std::sqrt(1.0 + arg * arg);
+                    err[0] += delta2 * (sq - 1.0);
+                    dcdo *= dE / sq;
+                } break;
+                case 3: {                                                                     // ASINH
+                    const double arg = dE * deltai, sq = std::sqrt(1.0 + arg * arg), lg = std::log(arg + sq);
+                    err[0] += delta2 * (1.0 - sq + arg * lg);
+                    dcdo *= lg * delta;
+                } break;
+                default: throw std::invalid_argument("orc_train_error: unknown loss");
+            }
+            err[1] += dE * dE;
+            if (grad)
+                for (int i = 0; i < s.n; ++i) {
+                    const ANN& nn = t.m.nnh[s.type[i]].nn;
+                    nn.fpbp(s.symm[i].data(), a, d);
+                    nn.cost_grad(s.symm[i].data(), a, d, dcdo, grad + off[s.type[i]]);
+                }
+        }
+        return 0;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return -1;
     }
 }
 

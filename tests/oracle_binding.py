@@ -47,6 +47,12 @@ def lib():
         L.orc_compute.argtypes = [vp, ip, vp, vp, vp, ip, vp, vp, vp]
         L.orc_bench.restype = dp
         L.orc_bench.argtypes = [vp, ip, vp, vp, vp, vp, ip, ip, vp]
+        L.orc_train_create.restype = vp
+        L.orc_train_create.argtypes = [vp, ip, vp, vp, vp, vp, vp]
+        L.orc_train_free.argtypes = [vp]
+        L.orc_train_nparams.argtypes = [vp, ip]
+        L.orc_train_get_params.argtypes = [vp, vp]
+        L.orc_train_error.argtypes = [vp, vp, ip, vp, ip, dp, vp, vp]
         _lib = L
     return _lib
 
@@ -136,6 +142,49 @@ class OracleModel:
         if sec < 0:
             raise ValueError(_err())
         return sec, en
+
+
+LOSS_MSE, LOSS_MAE, LOSS_HUBER, LOSS_ASINH = 0, 1, 2, 3
+
+
+class OracleTrain:
+    """NNPTE::error's training part on the CPU (descriptors computed once at creation)."""
+
+    def __init__(self, model: OracleModel, structures, energies):
+        nat = _arr([s.natoms for s in structures], np.int32)
+        cells = _arr(np.concatenate([np.zeros(9) if s.cell is None else np.asarray(s.cell, dtype=np.float64).reshape(9)
+                                     for s in structures]), np.float64)
+        pos = _arr(np.concatenate([s.pos for s in structures]), np.float64)
+        typ = _arr(np.concatenate([s.types for s in structures]), np.int32)
+        e = _arr(energies, np.float64)
+        self.ntypes = model.ntypes
+        self.h = lib().orc_train_create(model.h, len(structures), nat.ctypes.data, cells.ctypes.data, pos.ctypes.data,
+                                        typ.ctypes.data, e.ctypes.data)
+        if not self.h:
+            raise ValueError(_err())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_train_free(self.h)
+            self.h = None
+
+    def nparams(self, t: int) -> int:
+        return lib().orc_train_nparams(self.h, t)
+
+    def params(self) -> np.ndarray:
+        x = np.zeros(sum(self.nparams(t) for t in range(self.ntypes)))
+        lib().orc_train_get_params(self.h, x.ctypes.data)
+        return x
+
+    def error(self, x, batch, loss: int = LOSS_MSE, delta: float = 1.0, grad: bool = True):
+        xx = _arr(x, np.float64)
+        b = _arr(batch, np.int32)
+        err = np.zeros(2)
+        g = np.zeros(xx.shape[0]) if grad else None
+        if lib().orc_train_error(self.h, xx.ctypes.data, b.shape[0], b.ctypes.data, loss, delta, err.ctypes.data,
+                                 g.ctypes.data if grad else None) != 0:
+            raise ValueError(_err())
+        return err, g
 
 
 def neighbors(types, pos, cell, rc: float) -> dict:

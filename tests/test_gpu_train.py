@@ -1,0 +1,121 @@
+"""The energy-training inner loop on the GPU (nnp_train_*: NNPTE::error's training part,
+src/nnp/nnpte.cpp:869-927, with Cost::grad, src/ml/nn.cpp:1554-1589) against the CPU oracle's
+restatement (itself pinned by finite differences and closed forms in test_oracle_train.py):
+losses within 1e-10 relative, gradients within 1e-10 of their largest component, for every
+loss function; plus the properties the loop must have at sizes the oracle does not reach."""
+import numpy as np
+import pytest
+
+from atomnn_amd import synth
+from atomnn_amd.nnp import NNP, NNPError, Train, NNP_ATOM_ENERGY, LOSS_MSE, LOSS_MAE, LOSS_HUBER, LOSS_ASINH
+import oracle_binding as ob
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def training_set(seed=3):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k, n in enumerate((40, 17, 64, 9, 33, 120, 25, 58)):
+        cell = synth.strained_cell(n, 0.05, 0.08, rng) if k != 3 else None
+        box = cell if cell is not None else synth.cubic_cell(n, 0.05)
+        pos = synth.random_positions(n, box, 2.0, 100 + k)
+        out.append(synth.Structure(synth.species_exact(n, [0.5, 0.5], rng), pos, cell))
+    eref = np.array([rng.normal(-4.0, 0.5) * s.natoms for s in out])
+    return out, eref
+
+
+@pytest.fixture(scope="module")
+def pair(potentials):
+    strucs, eref = training_set()
+    m = NNP.read(potentials["plain4096"])
+    t = Train(m, strucs, eref)
+    o = ob.OracleTrain(ob.OracleModel(potentials["plain4096"]), strucs, eref)
+    yield m, t, o, strucs, eref
+    t.close()
+    m.close()
+
+
+def test_parameters_match_the_oracle(pair):
+    m, t, o, strucs, eref = pair
+    assert t.nparams() == o.params().size
+    assert t.nparams(0) == o.nparams(0) and t.nparams(1) == o.nparams(1)
+    assert np.array_equal(t.params(), o.params())
+
+
+@pytest.mark.parametrize("loss,delta", [(LOSS_MSE, 1.0), (LOSS_MAE, 1.0), (LOSS_HUBER, 0.02), (LOSS_ASINH, 0.02)])
+def test_error_and_gradient_match_the_oracle(pair, loss, delta):
+    m, t, o, strucs, eref = pair
+    rng = np.random.default_rng(loss + 11)
+    x = t.params() + rng.normal(0, 0.02, t.nparams())
+    for batch in ([0, 1, 2, 3, 4, 5, 6, 7], [5, 2, 7], [3]):
+        e, g = t.error(x, batch, loss, delta)
+        eo, go = o.error(x, batch, loss, delta)
+        assert abs(e[0] - eo[0]) <= TOL * abs(eo[0]), (batch, e, eo)
+        assert abs(e[1] - eo[1]) <= TOL * abs(eo[1])
+        assert np.abs(g - go).max() <= TOL * np.abs(go).max(), batch
+        # without the gradient: the same errors
+        e2, _ = t.error(x, batch, loss, delta, grad=False)
+        assert np.array_equal(e2, e)
+
+
+def test_gradient_is_additive_over_structures(pair):
+    m, t, o, strucs, eref = pair
+    x = t.params()
+    _, g = t.error(x, list(range(8)))
+    parts = sum(t.error(x, [b])[1] for b in range(8))
+    assert np.abs(g - parts).max() <= 1e-12 * np.abs(g).max()
+
+
+def test_training_leaves_the_model_unchanged_and_descends(pair):
+    m, t, o, strucs, eref = pair
+    s = strucs[0]
+    E0, Ea0, _ = m.compute(s, NNP_ATOM_ENERGY)
+    x = t.params()
+    batch = list(range(8))
+    l0, g = t.error(x, batch)
+    for _ in range(5):
+        x = x - 0.5 * g / np.linalg.norm(g) * 1e-2
+        l1, g = t.error(x, batch)
+    assert l1[0] < l0[0]
+    E1, Ea1, _ = m.compute(s, NNP_ATOM_ENERGY)
+    assert E1 == E0 and np.array_equal(Ea0, Ea1)
+
+
+def test_large_batch_gradient_in_chunks(potentials):
+    """32768 atoms of a three-species potential (84-64-64-1 networks): the gradient is summed in
+    chunks of tiles; it equals the sum of the per-structure gradients."""
+    rng = np.random.default_rng(19)
+    strucs = [s for s in synth.mixed64x4096(8)]
+    eref = np.array([rng.normal(-3.0, 0.2) * s.natoms for s in strucs])
+    m = NNP.read(potentials["mixed64x4096"])
+    try:
+        t = Train(m, strucs, eref)
+        x = t.params()
+        err, g = t.error(x, list(range(8)), LOSS_HUBER, 0.05)
+        parts_e = np.zeros(2)
+        parts_g = np.zeros_like(g)
+        for b in range(8):
+            e, gb = t.error(x, [b], LOSS_HUBER, 0.05)
+            parts_e += e
+            parts_g += gb
+        assert np.allclose(err, parts_e, rtol=1e-12)
+        assert np.abs(g - parts_g).max() <= 1e-11 * np.abs(g).max()
+        t.close()
+    finally:
+        m.close()
+
+
+def test_bad_arguments(pair):
+    m, t, o, strucs, eref = pair
+    x = t.params()
+    with pytest.raises(NNPError, match="out of range"):
+        t.error(x, [0, 8])
+    with pytest.raises(NNPError, match="unknown loss"):
+        t.error(x, [0], loss=7)
+    with pytest.raises(NNPError, match="delta"):
+        t.error(x, [0], LOSS_HUBER, 0.0)
+    with pytest.raises(ValueError):
+        t.error(x[:-1], [0])
