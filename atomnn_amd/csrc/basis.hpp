@@ -258,4 +258,76 @@ __device__ __forceinline__ void act_eval(int kind, T z, T& a, T& d) {
     }
 }
 
+// Second derivative a''(z) of the hidden-layer activations (AFFPBP2, ml/nn.cpp:149-766), the
+// reference's expressions where they are the derivative of its d(z); for SOFTSIGN (left unset
+// there), IQUAD, IFABS and SINSIG (set to 0 there) the true second derivative of its a(z).
+__device__ __forceinline__ double act_d2(int kind, double z) {
+    switch (kind) {
+        case NEU_TANH:
+        case NEU_TANHRE: {
+            if (kind == NEU_TANHRE && z > 0.0) return 0.0;
+            const double t = tanh(z);
+            return -2.0 * t * (1.0 - t * t);
+        }
+        case NEU_SIGMOID: {
+            const double e = exp(-fabs(z)), fr = 1.0 / (1.0 + e);
+            return (z >= 0.0 ? -e * (1.0 - e) : -e * (e - 1.0)) * fr * fr * fr;
+        }
+        case NEU_ISRU: {
+            const double r = 1.0 / sqrt(1.0 + z * z), r2 = r * r;
+            return -3.0 * z * r2 * r2 * r;
+        }
+        case NEU_ARCTAN: {
+            const double fr = 1.0 / (1.0 + z * z);
+            return -2.0 * z * (2.0 / 3.141592653589793) * fr * fr;
+        }
+        case NEU_SOFTSIGN: {
+            const double r = 1.0 / (1.0 + fabs(z));
+            return (z > 0.0 ? -2.0 : (z < 0.0 ? 2.0 : 0.0)) * r * r * r;
+        }
+        case NEU_SOFTPLUS: {
+            const double e = exp(-fabs(z)), fr = 1.0 / (1.0 + e);
+            return e * fr * fr;
+        }
+        case NEU_ELU: return z >= 0.0 ? 0.0 : exp(z);
+        case NEU_GELU:
+        case NEU_SWISH: {
+            const double X = kind == NEU_GELU ? 1.702 : 1.0;
+            const double e = exp(-fabs(X * z)), den = 1.0 / (1.0 + e), xz = X * z;
+            return z > 0.0 ? X * e * (2.0 - xz + e * (2.0 + xz)) * den * den * den
+                           : X * e * (2.0 + xz + e * (2.0 - xz)) * den * den * den;
+        }
+        case NEU_MISH: {
+            const double e = exp(-fabs(z));
+            if (z > 0.0) {
+                const double den = 1.0 / (2.0 * e * (e + 1.0) + 1.0);
+                return 4.0 * e * e * (e * (2.0 * e * (e * (z + 2.0) + z + 4.0) + 3.0 * (-z + 2.0)) - 2.0 * (z - 1.0)) * den *
+                       den * den;
+            }
+            const double den = 1.0 / (e * (e + 2.0) + 2.0);
+            return 4.0 * e * (2.0 * (z + 2.0) + e * (e * (-2.0 * e * (z - 1.0) - 3.0 * (z - 2.0)) + 2.0 * (z + 4.0))) * den *
+                   den * den;
+        }
+        case NEU_LOGCOSH: {
+            const double e = exp(-2.0 * fabs(z)), den = 1.0 / (e + 1.0);
+            return 2.0 * e * den * den;
+        }
+        case NEU_IQUAD: {
+            const double r = 1.0 / sqrt(1.0 + z * z), r2 = r * r;
+            return 0.5 * (2.0 - z * z) * r2 * r2 * r;
+        }
+        case NEU_IFABS: {
+            const double r = 1.0 / (1.0 + fabs(z));
+            return r * r * r;
+        }
+        case NEU_SINSIG: {
+            const double pi = 3.141592653589793;
+            const double e = exp(-z), s = 1.0 / (1.0 + e), s1 = e * s * s, s2 = s1 * (1.0 - 2.0 * s);
+            const double sv = sin(0.5 * pi * s), cv = cos(0.5 * pi * s);
+            return pi * s1 * cv + z * 0.5 * pi * (cv * s2 - 0.5 * pi * s1 * s1 * sv);
+        }
+        default: return 0.0;  // NEU_LINEAR, NEU_RELU
+    }
+}
+
 }  // namespace atomnn

@@ -825,6 +825,272 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
     }
 }
 
+// ------------------------------------------------------------------ descriptor tangents
+// Adjoint of the force pass with respect to dE/dG (force training).  Given gF[k] = dL/dF_k, the
+// loss depends on entry j of centre i through X_ij (F_i += X_ij, and F_j -= X_ij when the entry
+// carries the reaction), so phi_ij = dL/dX_ij = gF_i - [reaction] gF_j, and since
+// X_ij = -sum_g dE/dG_g dG_g/dr_ij:
+//   V_i = dL/d(dE_i/dG) = -sum_j phi_ij . dG_i/dr_ij,
+// the directional derivative of the centre's descriptors along the entry displacements phi_ij,
+// negated.  With d_j = |r_j|, u_j = r_j / d_j: d_j' = u_j . phi_j, u_j' = (phi_j - d_j' u_j) / d_j,
+// cos_jk' = u_k . w_j + u_j . w_k - cos_jk (s_j + s_k) with w = phi / d, s = d' / d.
+// Radial: G_f' = norm sum_j d_j' (phi_f fc)'(d_j).  Angular, per pair and (exponent e, term c):
+// (A_c R_e(j) R_e(k))' = A_c' cos' R_e(j) R_e(k) + A_c (R_e'(j) d_j' R_e(k) + R_e(j) R_e'(k) d_k').
+// One wavefront per centre, the same staging, tables and fixed-order lane reductions as k_symm.
+template <int NE, int NC, bool GEN>
+__host__ __device__ constexpr size_t jvp_lds_bytes(int maxp, int ldg) {
+    return 8 * (size_t)(8 * maxp + 2 * NE * maxp + kWave * NE * NC + ((ldg + 1) & ~1) + (GEN ? 2 * maxp : 0)) +
+           4 * (size_t)((maxp + 3) & ~3);
+}
+
+template <int NE, int NC, bool GEN>
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 8))) void k_symm_jvp(
+    const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta, const double* __restrict__ rfun_rs,
+    const int* __restrict__ atom_type, const DevStruc* __restrict__ S, const int* __restrict__ atom_struc,
+    const int* __restrict__ cnt, const int* __restrict__ nbr_j, const int* __restrict__ nbr_meta,
+    const double4* __restrict__ nbr_r, int maxp, int lmaxp, const double* __restrict__ gF, int all_images,
+    double* __restrict__ V, int ldg) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    double* su = (double*)lds_raw;
+    double* tg = su + 4 * lmaxp;  // (w_x, w_y, w_z, d')
+    double* Rf = tg + 4 * lmaxp;
+    double* Rd = Rf + NE * lmaxp;
+    double* red = Rd + NE * lmaxp;
+    double* Vrow = red + kWave * NE * NC;
+    double* nsx = Vrow + ((ldg + 1) & ~1);
+    int* orig = (int*)(nsx + (GEN ? 2 * lmaxp : 0));
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int ti = atom_type[i];
+    const int P = min(cnt[i], lmaxp);
+    const int ntypes = plan->T;
+    __shared__ int seg_lds[kMaxSpecies + 1];
+    Segs sg{seg_lds};
+    stage_entries<double>(i, P, maxp, ntypes, nbr_meta, nbr_r, su, orig, sg, lane);
+    for (int k = lane; k < ldg; k += kWave) Vrow[k] = 0.0;
+    __syncthreads();
+    {
+        const DevStruc& st = S[atom_struc[i]];
+        const double gx = gF[3 * (int64_t)i], gy = gF[3 * (int64_t)i + 1], gz = gF[3 * (int64_t)i + 2];
+        const int64_t row = (int64_t)i * maxp;
+        for (int p = lane; p < P; p += kWave) {
+            const int64_t e = row + orig[p];
+            double px = gx, py = gy, pz = gz;
+            if ((nbr_meta[e] & 1) || all_images) {
+                const int64_t gj = st.atom0 + nbr_j[e];
+                px -= gF[3 * gj];
+                py -= gF[3 * gj + 1];
+                pz -= gF[3 * gj + 2];
+            }
+            const double d = su[4 * p + 3], di = 1.0 / d;
+            tg[4 * p + 0] = px * di;
+            tg[4 * p + 1] = py * di;
+            tg[4 * p + 2] = pz * di;
+            tg[4 * p + 3] = su[4 * p] * px + su[4 * p + 1] * py + su[4 * p + 2] * pz;
+        }
+    }
+    __syncthreads();
+    // radial: cutoff value and slope once per entry, then (function, part) items over the lanes
+    {
+        double* fcr = Rf;
+        double* gcr = Rd;
+        double* part = red;
+        for (int b = 0; b < ntypes; ++b) {
+            const DevRadial& R = plan->rad[ti][b];
+            for (int p = sg.start[b] + lane; p < sg.start[b + 1]; p += kWave) {
+                double fc, gc;
+                cutoff_fg<double>(R.cut, su[4 * p + 3], R.rc, R.rci, R.pirci, fc, gc);
+                fcr[p] = fc;
+                gcr[p] = gc;
+            }
+        }
+        __syncthreads();
+        int nrt = 0;
+        for (int b = 0; b < ntypes; ++b) nrt += plan->rad[ti][b].n;
+        const int nparts = nrt > 0 ? max(1, kWave / nrt) : 1;
+        for (int f0 = 0; f0 < nrt; f0 += kWave / nparts) {
+            const int f = f0 + lane / nparts, prt = lane % nparts;
+            double acc = 0.0, norm = 0.0;
+            int goff = -1;
+            if (lane < (kWave / nparts) * nparts && f < nrt) {
+                int b = 0, fl = f;
+                while (fl >= plan->rad[ti][b].n) fl -= plan->rad[ti][b++].n;
+                const DevRadial& R = plan->rad[ti][b];
+                const double eta = rfun_eta[R.fbase + fl], rs = rfun_rs[R.fbase + fl];
+                for (int p = sg.start[b] + prt; p < sg.start[b + 1]; p += nparts) {
+                    const double d = su[4 * p + 3];
+                    double dv;
+                    if (!GEN || R.phi == RAD_GAUSSIAN) {
+                        const double x = d - rs;
+                        dv = exp(-eta * x * x) * (-2.0 * eta * x * fcr[p] + gcr[p]);
+                    } else {
+                        double v, dphi;
+                        rad_phi<double>(R.phi, -eta * (d - rs), v, dphi);
+                        dv = -eta * dphi * fcr[p] + v * gcr[p];
+                    }
+                    acc += tg[4 * p + 3] * dv;
+                }
+                goff = R.goff + fl;
+                norm = R.norm;
+            }
+            part[lane] = acc;
+            __syncthreads();
+            if (goff >= 0 && prt == 0) {
+                double s = 0.0;
+                for (int q = 0; q < nparts; ++q) s += part[lane + q];
+                Vrow[goff] = -s * norm;
+            }
+            __syncthreads();
+        }
+    }
+    // angular blocks in LMat order
+    const bool shared = plan->ang_shared[ti];
+    if (shared) radial_tables<double, NE, GEN>(plan->ang[ti][0], su, 0, P, Rf, Rd, lane);
+    for (int bb = 0; bb < ntypes; ++bb) {
+        for (int a = 0; a <= bb; ++a) {
+            const DevAngular& A = plan->ang[ti][bb * (bb + 1) / 2 + a];
+            const int sa = sg.start[a], na = sg.start[a + 1] - sa;
+            const int sb = sg.start[bb], nb = sg.start[bb + 1] - sb;
+            const bool tri = (a == bb);
+            const long long Q = tri ? (long long)na * (na - 1) / 2 : (long long)na * nb;
+            if (Q == 0) continue;
+            const bool nsb = GEN && !ang_separable(A.phi);
+            if (nsb) {
+                ns_tables<double, NE>(A, su, sa, sa + na, Rf, Rd, nsx, lane);
+                if (!tri) ns_tables<double, NE>(A, su, sb, sb + nb, Rf, Rd, nsx, lane);
+            } else if (!shared) {
+                radial_tables<double, NE, GEN>(A, su, sa, sa + na, Rf, Rd, lane);
+                if (!tri) radial_tables<double, NE, GEN>(A, su, sb, sb + nb, Rf, Rd, lane);
+            }
+            __syncthreads();
+            double lam[NC], zet[NC];
+            int zin[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                lam[c] = A.lambda[c];
+                zet[c] = A.zeta[c];
+                zin[c] = A.zint[c];
+            }
+            double acc[NE][NC];
+#pragma unroll
+            for (int e = 0; e < NE; ++e)
+#pragma unroll
+                for (int c = 0; c < NC; ++c) acc[e][c] = 0.0;
+            const long long L = (Q + kWave - 1) / kWave;
+            const long long q0 = (long long)lane * L, q1 = min(q0 + L, Q);
+            if (q0 < q1) {
+                int j, k;
+                if (tri) {
+                    j = tri_row(q0, na);
+                    k = j + 1 + (int)(q0 - (long long)j * (2LL * na - j - 1) / 2);
+                } else {
+                    j = (int)(q0 / nb);
+                    k = (int)(q0 % nb);
+                }
+                long long q = q0;
+                while (q < q1) {
+                    const int kend = tri ? na : nb;
+                    const int n = (int)min((long long)(kend - k), q1 - q);
+                    const int pj = sa + j;
+                    const double ujx = su[4 * pj], ujy = su[4 * pj + 1], ujz = su[4 * pj + 2];
+                    const double wjx = tg[4 * pj], wjy = tg[4 * pj + 1], wjz = tg[4 * pj + 2], ddj = tg[4 * pj + 3];
+                    const double sj = ddj / su[4 * pj + 3];
+                    double rfj[NE], rdj[NE];
+#pragma unroll
+                    for (int e = 0; e < NE; ++e) {
+                        rfj[e] = Rf[pj * NE + e];
+                        rdj[e] = Rd[pj * NE + e];
+                    }
+                    const double fcj = nsb ? nsx[2 * pj] : 0.0, gcj = nsb ? nsx[2 * pj + 1] : 0.0;
+                    for (int t = 0; t < n; ++t) {
+                        const int pk = sb + k + t;
+                        const double ukx = su[4 * pk], uky = su[4 * pk + 1], ukz = su[4 * pk + 2];
+                        const double ddk = tg[4 * pk + 3];
+                        const double cs = ujx * ukx + ujy * uky + ujz * ukz;
+                        const double cdot = ukx * wjx + uky * wjy + ukz * wjz + ujx * tg[4 * pk] + ujy * tg[4 * pk + 1] +
+                                            ujz * tg[4 * pk + 2] - cs * (sj + ddk / su[4 * pk + 3]);
+                        double pf[NE], pdt[NE];
+                        if (nsb) {
+                            const double fck = nsx[2 * pk], gck = nsx[2 * pk + 1];
+#pragma unroll
+                            for (int e = 0; e < NE; ++e) {
+                                double F, tt;
+                                ang_pair<double>(A.phi, rfj[e], Rf[pk * NE + e], F, tt);
+                                pf[e] = fcj * fck * F;
+                                pdt[e] = fck * F * (gcj - fcj * tt * rdj[e]) * ddj +
+                                         fcj * F * (gck - fck * tt * Rd[pk * NE + e]) * ddk;
+                            }
+                        } else {
+#pragma unroll
+                            for (int e = 0; e < NE; ++e) {
+                                const double rfk = Rf[pk * NE + e];
+                                pf[e] = rfj[e] * rfk;
+                                pdt[e] = rdj[e] * rfk * ddj + rfj[e] * Rd[pk * NE + e] * ddk;
+                            }
+                        }
+#pragma unroll
+                        for (int c = 0; c < NC; ++c) {
+                            double Av, dAv;
+                            angular_term<double, GEN>(cs, lam[c], zet[c], zin[c], Av, dAv);
+                            const double dc = dAv * cdot;
+#pragma unroll
+                            for (int e = 0; e < NE; ++e) acc[e][c] += dc * pf[e] + Av * pdt[e];
+                        }
+                    }
+                    q += n;
+                    ++j;
+                    k = tri ? j + 1 : 0;
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < NE; ++e)
+#pragma unroll
+                for (int c = 0; c < NC; ++c) red[(e * NC + c) * kWave + lane] = acc[e][c];
+            __syncthreads();
+            for (int t = lane; t < NE * NC; t += kWave) {
+                const int e = t / NC, c = t % NC;
+                const int f = A.fmap[e * kMaxC + c];
+                if (f >= 0) {
+                    double s = 0.0;
+                    for (int l = 0; l < kWave; ++l) s += red[t * kWave + l];
+                    Vrow[A.goff + f] = -s * A.norm;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int k = lane; k < ldg; k += kWave) V[(int64_t)i * ldg + k] = Vrow[k];
+}
+
+template <int NE, int NC, bool GEN>
+static hipError_t launch_jvp_t(const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
+                               const DevStruc* S, const int* atom_struc, int ntot, const int* cnt, const int* nbr_j,
+                               const int* meta, const double4* r, int maxp, int lmaxp, const double* gF, int all_images,
+                               double* V, int ldg, hipStream_t st) {
+    const size_t lds = jvp_lds_bytes<NE, NC, GEN>(lmaxp, ldg);
+    if (lds > 65536) return hipErrorInvalidConfiguration;
+    hipLaunchKernelGGL((k_symm_jvp<NE, NC, GEN>), dim3(ntot), dim3(kWave), lds, st, plan, rfe, rfr, atom_type, S,
+                       atom_struc, cnt, nbr_j, meta, r, maxp, lmaxp, gF, all_images, V, ldg);
+    return hipGetLastError();
+}
+
+size_t symm_jvp_lds(int shape, bool ns, int lmaxp, int ldg) {
+    if (shape == 0) return ns ? jvp_lds_bytes<2, 4, true>(lmaxp, ldg) : jvp_lds_bytes<2, 4, false>(lmaxp, ldg);
+    if (shape == 1) return ns ? jvp_lds_bytes<4, 4, true>(lmaxp, ldg) : jvp_lds_bytes<4, 4, false>(lmaxp, ldg);
+    return ns ? jvp_lds_bytes<8, 8, true>(lmaxp, ldg) : jvp_lds_bytes<8, 8, false>(lmaxp, ldg);
+}
+
+hipError_t launch_symm_jvp(int shape, int ns, const DevPlan* plan, const double* rfe, const double* rfr,
+                           const int* atom_type, const DevStruc* S, const int* atom_struc, int ntot, const int* cnt,
+                           const int* nbr_j, const int* meta, const double4* r, int maxp, int lmaxp, const double* gF,
+                           int all_images, double* V, int ldg, hipStream_t st) {
+#define JVP_ARGS plan, rfe, rfr, atom_type, S, atom_struc, ntot, cnt, nbr_j, meta, r, maxp, lmaxp, gF, all_images, V, ldg, st
+    if (shape == 0) return ns ? launch_jvp_t<2, 4, true>(JVP_ARGS) : launch_jvp_t<2, 4, false>(JVP_ARGS);
+    if (shape == 1) return ns ? launch_jvp_t<4, 4, true>(JVP_ARGS) : launch_jvp_t<4, 4, false>(JVP_ARGS);
+    return ns ? launch_jvp_t<8, 8, true>(JVP_ARGS) : launch_jvp_t<8, 8, false>(JVP_ARGS);
+#undef JVP_ARGS
+}
+
 // ------------------------------------------------------------------ fused persistent kernel
 // Network of one atom inside the wavefront (the batched form's fused launch): lanes own output
 // units; forward layers read the transposed weights (coalesced over lanes), backward layers the

@@ -96,6 +96,8 @@ def lib():
         L.nnp_train_nparams.argtypes = [vp, ctypes.c_int]
         L.nnp_train_get_params.argtypes = [vp, vp]
         L.nnp_train_error.argtypes = [vp, vp, ctypes.c_int, vp, ctypes.c_int, dp, vp, vp]
+        L.nnp_train_set_forces.argtypes = [vp, vp]
+        L.nnp_train_error_force.argtypes = [vp, vp, ctypes.c_int, vp, ctypes.c_int, dp, dp, ctypes.c_uint32, vp, vp]
         L.nnp_train_destroy.argtypes = [vp]
         L.nnp_md_destroy.restype = None
         _lib = L
@@ -335,6 +337,26 @@ class Train:
         g = np.zeros(xx.shape[0]) if grad else None
         _check(lib().nnp_train_error(self._h, xx.ctypes.data, int(b.shape[0]), b.ctypes.data, int(loss), float(delta),
                                      err.ctypes.data, g.ctypes.data if grad else None))
+        return err, g
+
+    def set_forces(self, forces) -> None:
+        """Reference forces of every structure (natoms x 3 each, in creation order)."""
+        f = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64).reshape(-1, 3) for x in forces]))
+        _check(lib().nnp_train_set_forces(self._h, f.ctypes.data))
+
+    def error_force(self, x, batch, loss: int = LOSS_MSE, delta: float = 1.0, beta: float = 1.0,
+                    all_images: bool = False, grad: bool = True):
+        """Energy + force loss (NNPTEF::error's training part, src/nnp/nnptef.cpp:1040-1293):
+        ([sum of losses, sum of dE^2, sum of |fdiff|^2], exact gradient)."""
+        xx = np.ascontiguousarray(x, dtype=np.float64)
+        if xx.shape[0] != self.nparams():
+            raise ValueError("parameter vector of the wrong length")
+        b = np.ascontiguousarray(batch, dtype=np.int32)
+        err = np.zeros(3)
+        g = np.zeros(xx.shape[0]) if grad else None
+        _check(lib().nnp_train_error_force(self._h, xx.ctypes.data, int(b.shape[0]), b.ctypes.data, int(loss),
+                                           float(delta), float(beta), NNP_ALL_IMAGES if all_images else 0,
+                                           err.ctypes.data, g.ctypes.data if grad else None))
         return err, g
 
     def close(self) -> None:

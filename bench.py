@@ -34,7 +34,7 @@ sys.path.insert(0, ROOT)
 
 from atomnn_amd import serve, synth  # noqa: E402
 from atomnn_amd import nnp as nnpmod  # noqa: E402
-from atomnn_amd.nnp import NNP, NNP_FORCE, NNP_PREC_MIXED, NNP_VACUUM_ENERGY, Resident  # noqa: E402
+from atomnn_amd.nnp import LOSS_MSE, NNP, NNP_FORCE, NNP_PREC_MIXED, NNP_VACUUM_ENERGY, Resident, Train  # noqa: E402
 
 METRIC = "E+F atoms/s, mixed vs plain FP64 at ≤1 meV/atom; p50/p99 request latency"
 FP64_PEAK_TFLOPS = 78.6
@@ -105,6 +105,64 @@ def traffic_from_profile(path):
         return None
 
 
+def train_line(tmp, args):
+    """Error + gradient of the energy loss over a resident training set (NNPTE::error's training
+    part with Cost::grad): the mixed64x4096 structures and potential, FP64 throughout.  Each call
+    uploads the parameter vector and the batch, runs the per-species networks forward and
+    backward, sums the gradient rows and downloads the loss and the gradient."""
+    tpath = os.path.join(tmp, "mixed64x4096.ann")
+    pot = synth.workload_potential("mixed64x4096")
+    pot.write(tpath)
+    ts = synth.mixed64x4096(args.train_structures)
+    tat = sum(s.natoms for s in ts)
+    rng = np.random.default_rng(7)
+    eref = np.array([rng.normal(-3.0, 0.2) * s.natoms for s in ts])
+    model_t = NNP.read(tpath)
+    tr = Train(model_t, ts, eref)
+    x = tr.params()
+    batch = list(range(len(ts)))
+    tr.error(x, batch, LOSS_MSE)
+    calls = max(3, args.steps // 2)
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        tr.error(x, batch, LOSS_MSE)
+    sec = (time.perf_counter() - t0) / calls
+    # per atom of species s: forward 2 n_l n_(l-1), back-propagation 2 n_l n_(l-1) (l >= 1), gradient rows
+    # 2 n_l n_(l-1) multiply-adds over every weight matrix
+    counts = np.bincount(np.concatenate([s.types for s in ts]), minlength=pot.ntypes)
+    flops = 0.0
+    for sp in range(pot.ntypes):
+        w = [W.size for W in pot.weights[sp]]
+        flops += counts[sp] * (4 * sum(w) + 2 * sum(w[1:]))
+    line = {
+        "workload": f"{len(ts)} mixed64x4096 structures ({tat} atoms, 3 species, 84-64-64-1 networks), MSE loss, "
+                    "error + gradient over the whole set per call, FP64",
+        "atoms_per_s": tat / sec,
+        "ms_per_call": sec * 1e3,
+        "nparams": tr.nparams(),
+        "network_tflops": flops / sec / 1e12,
+    }
+    tr.close()
+    model_t.close()
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_binding as ob
+
+        sample = ts[:2]
+        ot = ob.OracleTrain(ob.OracleModel(tpath), sample, eref[:2])
+        xo = ot.params()
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ot.error(xo, [0, 1], ob.LOSS_MSE)
+        sec_cpu = (time.perf_counter() - t0) / reps
+        nat_cpu = sum(s.natoms for s in sample)
+        line["cpu_baseline"] = {"value": nat_cpu / sec_cpu, "unit": "atoms/s", "cores": 1, "kind": "port",
+                                "sample": f"2 structures ({nat_cpu} atoms), error + gradient, single thread, "
+                                          f"{sec_cpu:.2f} s per call"}
+    return line
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,6 +173,8 @@ def main():
     ap.add_argument("--mixed-structures", type=int, default=64)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-train", action="store_true")
+    ap.add_argument("--train-structures", type=int, default=64)
     ap.add_argument("--serve-requests", type=int, default=20000)
     ap.add_argument("--serve-pool", type=int, default=1024)
     ap.add_argument("--serve-load", type=float, default=0.7, help="arrival rate as a fraction of measured capacity")
@@ -251,6 +311,8 @@ def main():
             },
         }
         model_b.close()
+    if not args.no_train:
+        out["training"] = train_line(tmp, args)
     if not args.no_serve:
         spath = os.path.join(tmp, "serve20k.ann")
         synth.workload_potential("serve20k").write(spath)

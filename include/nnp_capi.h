@@ -197,6 +197,20 @@ int nnp_train_get_params(nnp_train* t, double* x);
  * used for evaluation (nnp_compute ...) keeps its own parameters. */
 int nnp_train_error(nnp_train* t, const double* x, int nbatch, const int* batch, int loss, double delta, double* err,
                     double* grad);
+/* ---- force training (the training part of NNPTEF::error, src/nnp/nnptef.cpp:1040-1293) ----
+ * Reference forces of every training atom: 3 doubles per atom, structures in the order given to
+ * nnp_train_create. */
+int nnp_train_set_forces(nnp_train* t, const double* forces);
+/* Per batch structure the energy loss of nnp_train_error plus beta/2 |fdiff_i|^2 per atom,
+ * fdiff = (F_i - F_ref,i) / N (nnptef.cpp:1163-1166), with F the forces of the networks at x
+ * (reactions on minimum images only, the reference rule, or on every image with NNP_ALL_IMAGES in
+ * flags).  err[0] = sum of the losses, err[1] = sum of dE^2, err[2] = sum of |fdiff|^2;
+ * grad (may be NULL) = d err[0] / d x, the exact derivative.  The reference's gradient
+ * (nnptef.cpp:1138-1293, built on D2ODZDI::grad, src/ml/nn.cpp:1799-1890, and BasisA::forcep,
+ * src/nnp/basis_angular.cpp:642-761) keeps only the centre atom's share of each force and does
+ * not compile as shipped. */
+int nnp_train_error_force(nnp_train* t, const double* x, int nbatch, const int* batch, int loss, double delta,
+                          double beta, uint32_t flags, double* err, double* grad);
 void nnp_train_destroy(nnp_train* t);
 
 /* ---- molecular dynamics on the device-resident structure ----

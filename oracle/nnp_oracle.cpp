@@ -688,6 +688,85 @@ static void activate(Neuron n, double z, double& a, double& d) {
     }
 }
 
+// Second derivative of the hidden-layer activations (AFFPBP2, ml/nn.cpp:149-766), as the
+// reference writes them where they are the derivative of its AFFPBP d(z).  Where its AFFPBP2
+// leaves d2 unset (SOFTSIGN, 289-297) or sets it to 0 (IQUAD 686-694, IFABS 720-735, SINSIG
+// 754-765) the true second derivative of its a(z) is used instead.
+static double activate_d2(Neuron n, double z) {
+    switch (n) {
+        case NEURON_LINEAR: case NEURON_RELU: return 0.0;                 // 149-154, 321-334
+        case NEURON_SIGMOID: {                                            // 184-201
+            const double e = std::exp(-std::fabs(z)), fr = 1.0 / (1.0 + e);
+            return (z >= 0 ? -e * (1.0 - e) : -e * (e - 1.0)) * fr * fr * fr;
+        }
+        case NEURON_TANH: {                                               // 217-225
+            const double t = std::tanh(z);
+            return -2.0 * t * (1.0 - t * t);
+        }
+        case NEURON_ISRU: {                                               // 241-250
+            const double r = 1.0 / std::sqrt(1.0 + z * z);
+            return -3.0 * z * r * r * r * r * r;
+        }
+        case NEURON_ARCTAN: {                                             // 265-273
+            const double fr = 1.0 / (1.0 + z * z);
+            return -2.0 * z * (2.0 / PI) * fr * fr;
+        }
+        case NEURON_SOFTSIGN: {  // d/dz (1 + |z|)^-2
+            const double r = 1.0 / (1.0 + std::fabs(z));
+            return (z > 0 ? -2.0 : (z < 0 ? 2.0 : 0.0)) * r * r * r;
+        }
+        case NEURON_SOFTPLUS: {                                           // 366-385
+            const double e = std::exp(-std::fabs(z)), fr = 1.0 / (1.0 + e);
+            return e * fr * fr;
+        }
+        case NEURON_ELU: return z >= 0.0 ? 0.0 : std::exp(z);             // 410-424
+        case NEURON_GELU:                                                 // 471-497
+        case NEURON_SWISH: {                                              // 526-543
+            const double X = (n == NEURON_GELU) ? 1.702 : 1.0;
+            if (z > 0.0) {
+                const double e = std::exp(-X * z), den = 1.0 / (1.0 + e);
+                return X * e * (2.0 - X * z + e * (2.0 + X * z)) * den * den * den;
+            }
+            const double e = std::exp(X * z), den = 1.0 / (1.0 + e);
+            return X * e * (2.0 + X * z + e * (2.0 - X * z)) * den * den * den;
+        }
+        case NEURON_MISH:                                                 // 573-590
+            if (z > 0.0) {
+                const double e = std::exp(-z), den = 1.0 / (2.0 * e * (e + 1.0) + 1.0);
+                return 4.0 * e * e * (e * (2.0 * e * (e * (z + 2.0) + z + 4.0) + 3.0 * (-z + 2.0)) - 2.0 * (z - 1.0)) * den *
+                       den * den;
+            } else {
+                const double e = std::exp(z), den = 1.0 / (e * (e + 2.0) + 2.0);
+                return 4.0 * e * (2.0 * (z + 2.0) + e * (e * (-2.0 * e * (z - 1.0) - 3.0 * (z - 2.0)) + 2.0 * (z + 4.0))) *
+                       den * den * den;
+            }
+        case NEURON_TANHRE:                                               // 614-627
+            if (z > 0) return 0.0;
+            else {
+                const double t = std::tanh(z);
+                return -2.0 * t * (1.0 - t * t);
+            }
+        case NEURON_LOGCOSH: {                                            // 653-670
+            const double e = std::exp(-2.0 * std::fabs(z)), den = 1.0 / (e + 1.0);
+            return 2.0 * e * den * den;
+        }
+        case NEURON_IQUAD: {  // d/dz of 0.5 (1 + z (z^2 + 2) (1 + z^2)^-3/2) = (2 - z^2) (1 + z^2)^-5/2 / 2
+            const double r = 1.0 / std::sqrt(1.0 + z * z);
+            return 0.5 * (2.0 - z * z) * r * r * r * r * r;
+        }
+        case NEURON_IFABS: {  // both branches: (1 + |z|)^-3
+            const double r = 1.0 / (1.0 + std::fabs(z));
+            return r * r * r;
+        }
+        case NEURON_SINSIG: {  // a = z sin(pi s / 2), s = 1 / (1 + e^-z), s' = s (1 - s), s'' = s' (1 - 2 s)
+            const double e = std::exp(-z), s = 1.0 / (1.0 + e), s1 = e * s * s, s2 = s1 * (1.0 - 2.0 * s);
+            const double sv = std::sin(0.5 * PI * s), cv = std::cos(0.5 * PI * s);
+            return PI * s1 * cv + z * 0.5 * PI * (cv * s2 - 0.5 * PI * s1 * s1 * sv);
+        }
+        default: throw std::invalid_argument("ANN: invalid neuron");
+    }
+}
+
 // ml/nn.cpp:969-1138 (resize), 1191-1233 (fp/fpbp), 1620-1628 (DODZ::grad)
 struct ANN {
     int nin = 0;
@@ -779,6 +858,91 @@ struct ANN {
             const double* prev = l == 0 ? ins.data() : a[l - 1].data();
             for (int m = 0; m < in_size(l); ++m)
                 for (int n = 0; n < nodes[l]; ++n) g[c++] += dz[l][n] * prev[m];
+        }
+    }
+    // fpbp with second derivatives (fpbp2 / AFFPBP2, nn.cpp:149-766): a, d = a', e = a'' per layer
+    double fpbp2(const double* inp, std::vector<std::vector<double>>& a, std::vector<std::vector<double>>& d,
+                 std::vector<std::vector<double>>& e) const {
+        const double out = fpbp(inp, a, d);
+        std::vector<double> ins(nin);
+        for (int i = 0; i < nin; ++i) ins[i] = inpw[i] * (inp[i] + inpb[i]);
+        e.assign(nlayer(), {});
+        const double* prev = ins.data();
+        for (int l = 0; l < nlayer(); ++l) {
+            const int no = nodes[l], ni = in_size(l);
+            e[l].assign(no, 0.0);
+            if (l < nlayer() - 1)
+                for (int o = 0; o < no; ++o) {
+                    double z = b[l][o];
+                    for (int k = 0; k < ni; ++k) z += w[l][(size_t)o * ni + k] * prev[k];
+                    e[l][o] = activate_d2(neuron, z);
+                }
+            prev = a[l].data();
+        }
+        return out;
+    }
+    // Gradient, with respect to the parameters, of dcdo * out + (dout/dinp . v) for one atom
+    // (the energy term of the loss and the term through which the forces depend on the network:
+    // the forces are linear in dout/dinp, and v is the loss's derivative with respect to it).
+    // dout/dinp . v is the network's tangent output for the input tangent inpw .* v:
+    //   t_-1 = inpw .* v,  zt_l = w_l t_(l-1),  t_l = a'_l .* zt_l,  tangent out = outw t_L;
+    // reverse pass over the primal and tangent graphs (p: adjoint of t, q: adjoint of a):
+    //   pz_l = p_l .* a'_l,  qz_l = q_l .* a'_l + p_l .* a''_l .* zt_l,
+    //   grad b_l += qz_l,  grad w_l(n, m) += pz_l[n] t_(l-1)[m] + qz_l[n] a_(l-1)[m],
+    //   p_(l-1) = w_l^T pz_l,  q_(l-1) = w_l^T qz_l,  with p_L = outw, q_L = dcdo outw.
+    // This is the product of the reference's D2ODZDI (nn.cpp:1799-1890, d2out/dparam dinp) with
+    // v, plus Cost::grad (nn.cpp:1554-1589).  Adds into g (parameter order of get_params).
+    void force_grad(const double* inp, const double* v, const std::vector<std::vector<double>>& a,
+                    const std::vector<std::vector<double>>& d, const std::vector<std::vector<double>>& e, double dcdo,
+                    double* g) const {
+        const int L = nlayer();
+        std::vector<double> ins(nin), tin(nin);
+        for (int i = 0; i < nin; ++i) {
+            ins[i] = inpw[i] * (inp[i] + inpb[i]);
+            tin[i] = inpw[i] * v[i];
+        }
+        std::vector<std::vector<double>> zt(L), t(L);
+        const double* tp = tin.data();
+        for (int l = 0; l < L; ++l) {
+            const int no = nodes[l], ni = in_size(l);
+            zt[l].assign(no, 0.0);
+            t[l].assign(no, 0.0);
+            for (int o = 0; o < no; ++o) {
+                double s = 0;
+                for (int k = 0; k < ni; ++k) s += w[l][(size_t)o * ni + k] * tp[k];
+                zt[l][o] = s;
+                t[l][o] = d[l][o] * s;
+            }
+            tp = t[l].data();
+        }
+        std::vector<double> p(nodes[L - 1], 0.0), q(nodes[L - 1], 0.0);
+        p[0] = outw[0];
+        q[0] = dcdo * outw[0];
+        std::vector<int> rb(L), rw(L);
+        int c = 0;
+        for (int l = 0; l < L; ++l) rb[l] = c, c += nodes[l];
+        for (int l = 0; l < L; ++l) rw[l] = c, c += nodes[l] * in_size(l);
+        for (int l = L - 1; l >= 0; --l) {
+            const int no = nodes[l], ni = in_size(l);
+            std::vector<double> pz(no), qz(no);
+            for (int o = 0; o < no; ++o) {
+                pz[o] = p[o] * d[l][o];
+                qz[o] = q[o] * d[l][o] + p[o] * e[l][o] * zt[l][o];
+                g[rb[l] + o] += qz[o];
+            }
+            const double* tprev = l == 0 ? tin.data() : t[l - 1].data();
+            const double* aprev = l == 0 ? ins.data() : a[l - 1].data();
+            for (int m = 0; m < ni; ++m)
+                for (int o = 0; o < no; ++o) g[rw[l] + m * no + o] += pz[o] * tprev[m] + qz[o] * aprev[m];
+            if (l > 0) {
+                p.assign(ni, 0.0);
+                q.assign(ni, 0.0);
+                for (int m = 0; m < ni; ++m)
+                    for (int o = 0; o < no; ++o) {
+                        p[m] += w[l][(size_t)o * ni + m] * pz[o];
+                        q[m] += w[l][(size_t)o * ni + m] * qz[o];
+                    }
+            }
         }
     }
     // DODZ::grad (nn.cpp:1620-1628), output row 0: dodi = d out / d inp
@@ -1134,6 +1298,72 @@ static void nnp_energy_force(const Model& m, Structure& s, const NeighborList& n
     }
 }
 
+// Adjoint of the force loop above with respect to dE/dG: given gF[k] = dL/dF_k for every atom,
+// v[i][g] = dL/d(dE_i/dG_g).  The forces are linear in dE/dG, so each basis function's share of a
+// pair or triplet force is the force() of a one-hot dE/dG (the reference's per-function force
+// amplitudes, BasisA::forcep, basis_angular.cpp:642-761); v collects gF . (that share) over the
+// same updates nnp.cpp:441-478 makes, reactions included.
+static void force_adjoint(const Model& m, const Structure& s, const NeighborList& nl, bool all_images,
+                          const std::vector<V3>& gF, std::vector<std::vector<double>>& v) {
+    v.assign(s.n, {});
+    std::vector<double> oh;
+    for (int i = 0; i < s.n; ++i) {
+        const NNH& h = m.nnh[s.type[i]];
+        v[i].assign(h.nInput, 0.0);
+        const auto& L = nl.neigh[i];
+        for (size_t j = 0; j < L.size(); ++j) {
+            const int JJ = L[j].type, jj = L[j].index;
+            const bool jmin = all_images ? true : L[j].min;
+            const V3& rIJ = L[j].r;
+            const double dIJ = L[j].dr, dIJi = 1.0 / dIJ;
+            {
+                const BasisR& br = h.basisR[JJ];
+                oh.assign(br.size(), 0.0);
+                const double gr = dot(gF[i], rIJ) - (jmin ? dot(gF[jj], rIJ) : 0.0);
+                for (int q = 0; q < br.size(); ++q) {
+                    oh[q] = 1.0;
+                    v[i][h.offsetR[JJ] + q] += br.force(dIJ, oh.data()) * dIJi * gr;
+                    oh[q] = 0.0;
+                }
+            }
+            for (size_t k = j + 1; k < L.size(); ++k) {
+                const int KK = L[k].type, kk = L[k].index;
+                const bool kmin = all_images ? true : L[k].min;
+                const V3& rIK = L[k].r;
+                const double dIK = L[k].dr, dIKi = 1.0 / dIK;
+                const V3 rJK = sub(rIK, rIJ);
+                const double dJK = nrm(rJK), dJKi = 1.0 / dJK;
+                const int blk = lmat(JJ, KK);
+                const int off = h.nInputR + h.offsetA[blk];
+                const double cs = dot(rIJ, rIK) * dIJi * dIKi;
+                const double dd[3] = {dIJ, dIK, dJK};
+                const BasisA& ba = h.basisA[blk];
+                oh.assign(ba.size(), 0.0);
+                for (int q = 0; q < ba.size(); ++q) {
+                    oh[q] = 1.0;
+                    double phi = 0, eta[3] = {0, 0, 0};
+                    ba.force(phi, eta, cs, dd, oh.data());
+                    oh[q] = 0.0;
+                    const V3 fi = add(scl((phi * (dIKi - cs * dIJi) + eta[0]) * dIJi, rIJ),
+                                      scl((phi * (dIJi - cs * dIKi) + eta[1]) * dIKi, rIK));
+                    double acc = dot(gF[i], fi);
+                    if (jmin) {
+                        const V3 fj = add(add(scl((-phi * cs * dIJi + eta[0]) * dIJi, rIJ), scl(phi * dIJi * dIKi, rIK)),
+                                          scl(eta[2] * dJKi, rJK));
+                        acc -= dot(gF[jj], fj);
+                    }
+                    if (kmin) {
+                        const V3 fk = sub(add(scl((-phi * cs * dIKi + eta[1]) * dIKi, rIK), scl(phi * dIKi * dIJi, rIJ)),
+                                          scl(eta[2] * dJKi, rJK));
+                        acc -= dot(gF[kk], fk);
+                    }
+                    v[i][off + q] += acc;
+                }
+            }
+        }
+    }
+}
+
 static void load_structure(Structure& s, int n, const double* cell, const double* pos, const int* type, const Model* m) {
     if (n <= 0) throw std::invalid_argument("natoms must be positive");
     s.n = n;
@@ -1152,6 +1382,27 @@ static thread_local std::string g_err;
 }  // namespace orc
 
 using namespace orc;
+
+// Loss of one structure's per-atom energy error dE (nnpte.cpp:876-916): adds the loss to L and
+// returns dC/d(dE).
+static double energy_loss(int loss, double dE, double delta, double& L) {
+    const double deltai = 1.0 / delta, delta2 = delta * delta;
+    switch (loss) {
+        case 0: L += 0.5 * dE * dE; return dE;                                         // MSE
+        case 1: L += std::fabs(dE); return (double)((0.0 < dE) - (dE < 0.0));          // MAE
+        case 2: {                                                                      // HUBER
+            const double arg = dE * deltai, sq = std::sqrt(1.0 + arg * arg);
+            L += delta2 * (sq - 1.0);
+            return dE / sq;
+        }
+        case 3: {                                                                      // ASINH
+            const double arg = dE * deltai, sq = std::sqrt(1.0 + arg * arg), lg = std::log(arg + sq);
+            L += delta2 * (1.0 - sq + arg * lg);
+            return lg * delta;
+        }
+        default: throw std::invalid_argument("orc_train_error: unknown loss");
+    }
+}
 
 // ---------------------------------------------------------------- C entry points (ctypes)
 extern "C" {
@@ -1349,7 +1600,9 @@ double orc_bench(void* mp, int nstruc, const int* natoms, const double* cells, c
 struct TrainSet {
     Model m;  // a copy: the parameters change per call
     std::vector<Structure> st;
+    std::vector<NeighborList> nl;
     std::vector<double> eref;
+    std::vector<std::vector<V3>> fref;  // reference forces (force training)
 };
 
 int orc_train_nparams(void* tp, int type) { return ((TrainSet*)tp)->m.nnh[type].nn.nparams(); }
@@ -1371,6 +1624,7 @@ void* orc_train_create(void* mp, int nstruc, const int* natoms, const double* ce
             neighbor_build(st.cell, st.n, st.pos.data(), st.type.data(), t->m.rc, nl);
             nnp_symm(t->m, st, nl);
             t->st.push_back(std::move(st));
+            t->nl.push_back(std::move(nl));
             t->eref.push_back(eref[s]);
             off += natoms[s];
         }
@@ -1403,7 +1657,6 @@ int orc_train_error(void* tp, const double* x, int nbatch, const int* batch, int
         for (int n = 0; n < T; ++n) t.m.nnh[n].nn.set_params(x + off[n]);
         if (grad)
             for (int q = 0; q < off[T]; ++q) grad[q] = 0.0;
-        const double deltai = 1.0 / delta, delta2 = delta * delta;
         err[0] = err[1] = 0.0;
         std::vector<std::vector<double>> a, d;
         for (int b = 0; b < nbatch; ++b) {
@@ -1412,22 +1665,7 @@ int orc_train_error(void* tp, const double* x, int nbatch, const int* batch, int
             for (int i = 0; i < s.n; ++i) energy += t.m.nnh[s.type[i]].nn.fpbp(s.symm[i].data(), a, d);
             const double norm = 1.0 / s.n;
             const double dE = (energy - t.eref[batch[b]]) * norm;
-            double dcdo = norm;
-            switch (loss) {
-                case 0: err[0] += 0.5 * dE * dE; dcdo *= dE; break;                           // MSE
-                case 1: err[0] += std::fabs(dE); dcdo *= (double)((0.0 < dE) - (dE < 0.0)); break;  // MAE
-                case 2: {                                                                     // HUBER
-                    const double arg = dE * deltai, sq = std::sqrt(1.0 + arg * arg);
-                    err[0] += delta2 * (sq - 1.0);
-                    dcdo *= dE / sq;
-                } break;
-                case 3: {                                                                     // ASINH
-                    const double arg = dE * deltai, sq = std::sqrt(1.0 + arg * arg), lg = std::log(arg + sq);
-                    err[0] += delta2 * (1.0 - sq + arg * lg);
-                    dcdo *= lg * delta;
-                } break;
-                default: throw std::invalid_argument("orc_train_error: unknown loss");
-            }
+            const double dcdo = norm * energy_loss(loss, dE, delta, err[0]);
             err[1] += dE * dE;
             if (grad)
                 for (int i = 0; i < s.n; ++i) {
@@ -1439,6 +1677,82 @@ int orc_train_error(void* tp, const double* x, int nbatch, const int* batch, int
         return 0;
     } catch (std::exception& e) {
         g_err = e.what();
+        return -1;
+    }
+}
+
+// Reference forces of every training atom (3 per atom, structures in creation order).
+int orc_train_set_forces(void* tp, const double* f) {
+    TrainSet& t = *(TrainSet*)tp;
+    t.fref.assign(t.st.size(), {});
+    size_t off = 0;
+    for (size_t b = 0; b < t.st.size(); ++b) {
+        t.fref[b].resize(t.st[b].n);
+        for (int i = 0; i < t.st[b].n; ++i, ++off) t.fref[b][i] = v3(f[3 * off], f[3 * off + 1], f[3 * off + 2]);
+    }
+    return 0;
+}
+
+// Energy + force training loss and its gradient (the training part of NNPTEF::error,
+// nnptef.cpp:1040-1293): per structure the energy loss of dE = (E - E_ref) / N (as
+// orc_train_error) plus beta/2 |fdiff_i|^2 per atom with fdiff = (F_i - F_ref,i) / N
+// (nnptef.cpp:1163-1166), the forces being NNP::compute's (reactions on minimum images only,
+// or on every image with all_images).  err = {loss, sum dE^2, sum |fdiff|^2}.  The gradient is
+// the exact derivative of that loss: the reference's own (nnptef.cpp:1138-1293, which does not
+// compile) keeps only the centre atom's share of each force.
+int orc_train_error_force(void* tp, const double* x, int nbatch, const int* batch, int loss, double delta, double beta,
+                          int all_images, double* err, double* grad) {
+    try {
+        TrainSet& t = *(TrainSet*)tp;
+        if (t.fref.size() != t.st.size()) throw std::invalid_argument("orc_train_error_force: no reference forces");
+        const int T = t.m.ntypes;
+        std::vector<int> off(T + 1, 0);
+        for (int n = 0; n < T; ++n) off[n + 1] = off[n] + t.m.nnh[n].nn.nparams();
+        for (int n = 0; n < T; ++n) t.m.nnh[n].nn.set_params(x + off[n]);
+        if (grad)
+            for (int q = 0; q < off[T]; ++q) grad[q] = 0.0;
+        err[0] = err[1] = err[2] = 0.0;
+        std::vector<std::vector<double>> a, d, e, v;
+        std::vector<V3> gF;
+        for (int b = 0; b < nbatch; ++b) {
+            const int sb = batch[b];
+            Structure s = t.st.at(sb);
+            const NeighborList& nl = t.nl[sb];
+            nnp_energy_force(t.m, s, nl, F_FORCE | (all_images ? F_ALL_IMAGES : 0));
+            const double norm = 1.0 / s.n;
+            const double dE = (s.energy - t.eref[sb]) * norm;
+            const double dcdo = norm * energy_loss(loss, dE, delta, err[0]);
+            err[1] += dE * dE;
+            gF.assign(s.n, v3(0, 0, 0));
+            for (int i = 0; i < s.n; ++i) {
+                const V3 fd = scl(norm, sub(s.force[i], t.fref[sb][i]));
+                err[0] += 0.5 * beta * sqn(fd);
+                err[2] += sqn(fd);
+                gF[i] = scl(beta * norm, fd);
+            }
+            if (!grad) continue;
+            force_adjoint(t.m, s, nl, all_images != 0, gF, v);
+            for (int i = 0; i < s.n; ++i) {
+                const ANN& nn = t.m.nnh[s.type[i]].nn;
+                nn.fpbp2(s.symm[i].data(), a, d, e);
+                nn.force_grad(s.symm[i].data(), v[i].data(), a, d, e, dcdo, grad + off[s.type[i]]);
+            }
+        }
+        return 0;
+    } catch (std::exception& ex) {
+        g_err = ex.what();
+        return -1;
+    }
+}
+
+// Second derivative of an activation (tests).
+double orc_activation_d2(int neuron, double z) { return activate_d2((Neuron)neuron, z); }
+int orc_activation(int neuron, double z, double* ad) {
+    try {
+        activate((Neuron)neuron, z, ad[0], ad[1]);
+        return 0;
+    } catch (std::exception& ex) {
+        g_err = ex.what();
         return -1;
     }
 }

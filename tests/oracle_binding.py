@@ -53,6 +53,11 @@ def lib():
         L.orc_train_nparams.argtypes = [vp, ip]
         L.orc_train_get_params.argtypes = [vp, vp]
         L.orc_train_error.argtypes = [vp, vp, ip, vp, ip, dp, vp, vp]
+        L.orc_train_set_forces.argtypes = [vp, vp]
+        L.orc_train_error_force.argtypes = [vp, vp, ip, vp, ip, dp, dp, ip, vp, vp]
+        L.orc_activation_d2.restype = dp
+        L.orc_activation_d2.argtypes = [ip, dp]
+        L.orc_activation.argtypes = [ip, dp, vp]
         _lib = L
     return _lib
 
@@ -185,6 +190,36 @@ class OracleTrain:
                                  g.ctypes.data if grad else None) != 0:
             raise ValueError(_err())
         return err, g
+
+    def set_forces(self, forces):
+        """Reference forces of every atom (structures in creation order, natoms x 3 each)."""
+        f = _arr(np.concatenate([np.asarray(x, dtype=np.float64).reshape(-1, 3) for x in forces]), np.float64)
+        lib().orc_train_set_forces(self.h, f.ctypes.data)
+
+    def error_force(self, x, batch, loss: int = LOSS_MSE, delta: float = 1.0, beta: float = 1.0,
+                    all_images: bool = False, grad: bool = True):
+        """Energy + force loss ({loss, sum dE^2, sum |fdiff|^2}) and its gradient."""
+        xx = _arr(x, np.float64)
+        b = _arr(batch, np.int32)
+        err = np.zeros(3)
+        g = np.zeros(xx.shape[0]) if grad else None
+        if lib().orc_train_error_force(self.h, xx.ctypes.data, b.shape[0], b.ctypes.data, loss, delta, beta,
+                                       1 if all_images else 0, err.ctypes.data, g.ctypes.data if grad else None) != 0:
+            raise ValueError(_err())
+        return err, g
+
+
+NEURONS = ["LINEAR", "SIGMOID", "TANH", "ISRU", "ARCTAN", "SOFTSIGN", "RELU", "SOFTPLUS", "ELU", "GELU", "SWISH",
+           "MISH", "TANHRE", "LOGCOSH", "IQUAD", "IFABS", "SINSIG"]
+
+
+def activation(name: str, z: float):
+    """(a, a', a'') of a hidden-layer activation."""
+    n = NEURONS.index(name) + 1
+    ad = np.zeros(2)
+    if lib().orc_activation(n, float(z), ad.ctypes.data) != 0:
+        raise ValueError(_err())
+    return ad[0], ad[1], lib().orc_activation_d2(n, float(z))
 
 
 def neighbors(types, pos, cell, rc: float) -> dict:

@@ -119,3 +119,122 @@ def test_bad_arguments(pair):
         t.error(x, [0], LOSS_HUBER, 0.0)
     with pytest.raises(ValueError):
         t.error(x[:-1], [0])
+
+
+# ---------------------------------------------------------------- force training
+def reference_forces(om, strucs, seed):
+    """Oracle forces at the model's parameters, perturbed (so the force loss is not zero)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for s in strucs:
+        _, _, f = om.compute(s.types, s.pos, s.cell, ob.F_FORCE)
+        out.append(f + rng.normal(0, 0.05, f.shape))
+    return out
+
+
+@pytest.fixture(scope="module")
+def fpair(potentials):
+    strucs, eref = training_set(5)
+    m = NNP.read(potentials["plain4096"])
+    om = ob.OracleModel(potentials["plain4096"])
+    fref = reference_forces(om, strucs, 8)
+    t = Train(m, strucs, eref)
+    t.set_forces(fref)
+    o = ob.OracleTrain(om, strucs, eref)
+    o.set_forces(fref)
+    yield m, t, o, strucs, eref
+    t.close()
+    m.close()
+
+
+def check_force_parity(t, o, x, batch, loss, delta, beta, ai):
+    e, g = t.error_force(x, batch, loss, delta, beta, all_images=ai)
+    eo, go = o.error_force(x, batch, loss, delta, beta, all_images=ai)
+    for k in range(3):
+        assert abs(e[k] - eo[k]) <= TOL * abs(eo[k]), (k, e, eo)
+    assert np.abs(g - go).max() <= TOL * np.abs(go).max()
+    e2, _ = t.error_force(x, batch, loss, delta, beta, all_images=ai, grad=False)
+    assert np.array_equal(e2, e)
+
+
+@pytest.mark.parametrize("loss,ai", [(LOSS_MSE, False), (LOSS_HUBER, False), (LOSS_MSE, True)])
+def test_force_training_matches_the_oracle(fpair, loss, ai):
+    m, t, o, strucs, eref = fpair
+    x = t.params() + np.random.default_rng(21).normal(0, 0.02, t.nparams())
+    for batch in ([0, 1, 2, 3, 4, 5, 6, 7], [6, 3]):
+        check_force_parity(t, o, x, batch, loss, 0.05, 2.5, ai)
+
+
+def test_force_training_without_force_weight_is_energy_training(fpair):
+    m, t, o, strucs, eref = fpair
+    x = t.params()
+    e, g = t.error_force(x, list(range(8)), LOSS_ASINH, 0.05, 0.0)
+    ee, ge = t.error(x, list(range(8)), LOSS_ASINH, 0.05)
+    assert abs(e[0] - ee[0]) <= 1e-12 * abs(ee[0]) and abs(e[1] - ee[1]) <= 1e-12 * abs(ee[1])
+    assert np.abs(g - ge).max() <= 1e-12 * np.abs(ge).max()
+
+
+def test_force_training_variants(tmp_path):
+    """Generic kernels: another activation (its second derivative), radial and non-separable
+    angular basis functions."""
+    pot = synth.make_potential(["A", "B"], 5.0, 4, [0.02, 0.2], [1.0, 2.0], [1, -1], [12, 9],
+                               np.random.default_rng(13))
+    pot.neuron, pot.phi_r, pot.phi_a = "SOFTPLUS", "TANH", "IPOWS"
+    path = str(tmp_path / "gen.ann")
+    pot.write(path)
+    strucs, eref = training_set(9)
+    om = ob.OracleModel(path)
+    fref = reference_forces(om, strucs, 10)
+    m = NNP.read(path)
+    try:
+        t = Train(m, strucs, eref)
+        t.set_forces(fref)
+        o = ob.OracleTrain(om, strucs, eref)
+        o.set_forces(fref)
+        x = t.params() + np.random.default_rng(2).normal(0, 0.02, t.nparams())
+        check_force_parity(t, o, x, [0, 2, 4, 6, 7], LOSS_MSE, 1.0, 1.5, False)
+        t.close()
+    finally:
+        m.close()
+
+
+def test_force_training_wide_networks(potentials):
+    """84-64-64-1 networks of three species (the tile kernel's largest LDS): a 64-atom cell
+    against the oracle, and 2 x 4096 atoms equal to the sum of the per-structure calls."""
+    rng = np.random.default_rng(23)
+    big = synth.mixed64x4096(2)
+    cell = synth.cubic_cell(64, 0.06)
+    small = synth.Structure(synth.species_exact(64, [0.4, 0.3, 0.3], rng), synth.random_positions(64, cell, 1.8, 5),
+                            cell)
+    strucs = [small] + big
+    eref = np.array([rng.normal(-3.0, 0.2) * s.natoms for s in strucs])
+    om = ob.OracleModel(potentials["mixed64x4096"])
+    fref = reference_forces(om, [small], 3) + [rng.normal(0, 0.3, (s.natoms, 3)) for s in big]
+    m = NNP.read(potentials["mixed64x4096"])
+    try:
+        t = Train(m, strucs, eref)
+        t.set_forces(fref)
+        x = t.params()
+        o = ob.OracleTrain(om, [small], eref[:1])
+        o.set_forces(fref[:1])
+        check_force_parity(t, o, x, [0], LOSS_MSE, 1.0, 2.0, False)
+        err, g = t.error_force(x, [1, 2], LOSS_HUBER, 0.05, 2.0)
+        e1, g1 = t.error_force(x, [1], LOSS_HUBER, 0.05, 2.0)
+        e2, g2 = t.error_force(x, [2], LOSS_HUBER, 0.05, 2.0)
+        assert np.allclose(err, e1 + e2, rtol=1e-12)
+        assert np.abs(g - (g1 + g2)).max() <= 1e-11 * np.abs(g).max()
+        t.close()
+    finally:
+        m.close()
+
+
+def test_force_training_bad_arguments(pair):
+    m, t, o, strucs, eref = pair
+    x = t.params()
+    with pytest.raises(NNPError, match="reference forces"):
+        t.error_force(x, [0])
+    t.set_forces([np.zeros((s.natoms, 3)) for s in strucs])
+    with pytest.raises(NNPError, match="beta"):
+        t.error_force(x, [0], beta=-1.0)
+    with pytest.raises(NNPError, match="out of range"):
+        t.error_force(x, [9])
