@@ -72,6 +72,26 @@ __device__ __forceinline__ int train_species(const TrainTiles& tt, int ntypes, i
     return s;
 }
 
+// Hidden-layer activation and its derivatives.  GEN = false: tanh hidden layers only (the common
+// case; the general switch over every activation costs the kernels about 50 VGPRs).
+template <bool GEN>
+__device__ __forceinline__ void train_act(int act, double z, double& a, double& d) {
+    if constexpr (GEN) {
+        act_eval<double>(act, z, a, d);
+    } else if (act == NEU_LINEAR) {
+        a = z;
+        d = 1.0;
+    } else {
+        a = tanh(z);
+        d = 1.0 - a * a;
+    }
+}
+template <bool GEN>
+__device__ __forceinline__ double train_act_d2(int act, double z, double a) {
+    if constexpr (GEN) return act_d2(act, z);
+    return act == NEU_LINEAR ? 0.0 : -2.0 * a * (1.0 - a * a);
+}
+
 // LDS: ins [16][nin_pad], then per layer A_l [16][wpad_l] and D_l [16][wpad_l] (dadz; dcdz in the
 // backward pass, in place).
 static size_t train_lds_bytes(const TrainNet& n) {
@@ -83,7 +103,7 @@ static size_t train_lds_bytes(const TrainNet& n) {
 // MODE 0: forward, out[slot] = outb + outw a_L.  MODE 1: forward and backward, the tile's
 // gradient row into `partial`.  MODE 2: forward and the input gradient (fpbp + DODZ::grad,
 // nn.cpp:1191-1233, 1620-1628): out[slot] and Y[slot] = dout/dG = inpw .* (w_0^T dodz_0).
-template <int MODE>
+template <int MODE, bool GEN>
 __global__ __launch_bounds__(kTrainThreads) void k_train_tile(
     const TrainPlan* __restrict__ tp, const double* __restrict__ Wm, const double* __restrict__ par, TrainTiles tt,
     int tile0, const int* __restrict__ batom, const int* __restrict__ bslot, const int* __restrict__ bstruct,
@@ -125,7 +145,7 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_tile(
             for (int v = 0; v < 4; ++v) {
                 const int row = 4 * v + lk;
                 double av, dv;
-                act_eval<double>(act, acc[v] + par[N.boff[l] + col], av, dv);
+                train_act<GEN>(act, acc[v] + par[N.boff[l] + col], av, dv);
                 A[l][row * wp + col] = av;
                 D[l][row * wp + col] = dv;
             }
@@ -240,6 +260,7 @@ static size_t train_force_lds_bytes(const TrainNet& n) {
     return 8 * e;
 }
 
+template <bool GEN>
 __global__ __launch_bounds__(kTrainThreads) void k_train_force_tile(
     const TrainPlan* __restrict__ tp, const double* __restrict__ Wm, const double* __restrict__ par, TrainTiles tt,
     int tile0, const int* __restrict__ batom, const int* __restrict__ bstruct, const double* __restrict__ G,
@@ -294,8 +315,8 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_force_tile(
                 const int idx = (4 * v + lk) * wp + col;
                 const double z = az[v] + par[N.boff[l] + col];
                 double av, dv;
-                act_eval<double>(act, z, av, dv);
-                const double d2 = last ? 0.0 : act_d2(act, z);
+                train_act<GEN>(act, z, av, dv);
+                const double d2 = last ? 0.0 : train_act_d2<GEN>(act, z, av);
                 A[l][idx] = av;
                 D[l][idx] = dv;
                 T[l][idx] = dv * at[v];
@@ -473,6 +494,19 @@ __global__ __launch_bounds__(256) void k_train_reduce2(const TrainPlan* __restri
 }
 
 // ------------------------------------------------------------------ host side
+template <int MODE>
+static void launch_tile(bool gen, dim3 grid, size_t lds, hipStream_t st, const TrainPlan* tp, const double* Wm,
+                        const double* par, const TrainTiles& tt, int tile0, const int* batom, const int* bslot,
+                        const int* bstruct, const double* G, int ldg, double* out, const double* dcdo, double* partial,
+                        double* Y) {
+    if (gen)
+        hipLaunchKernelGGL((k_train_tile<MODE, true>), grid, dim3(kTrainThreads), lds, st, tp, Wm, par, tt, tile0, batom,
+                           bslot, bstruct, G, ldg, out, dcdo, partial, Y);
+    else
+        hipLaunchKernelGGL((k_train_tile<MODE, false>), grid, dim3(kTrainThreads), lds, st, tp, Wm, par, tt, tile0, batom,
+                           bslot, bstruct, G, ldg, out, dcdo, partial, Y);
+}
+
 class TrainSet {
   public:
     TrainSet(Engine& owner, int n, const nnp_structure* ss, const double* energy) : owner_(owner) {
@@ -498,6 +532,7 @@ class TrainSet {
             N.inpb_off = dn.inpb_off;
             N.outw = dn.outw;
             N.outb = dn.outb;
+            gen_ = gen_ || (N.L > 1 && N.act != NEU_TANH && N.act != NEU_LINEAR);
             if (ns.nodes.back() != 1) throw unsupported_error("nnp_train: networks with more than one output");
             int rb = 0, nb = 0;
             for (int l = 0; l < N.L; ++l) nb += ns.nodes[l];
@@ -740,11 +775,11 @@ class TrainSet {
         HIPCHK(hipMemcpyAsync(d_eref_.p, eb.data(), 8 * (size_t)nbatch, hipMemcpyHostToDevice, stream_));
         size_t lds = 0;
         for (int t = 0; t < T_; ++t) lds = std::max(lds, train_lds_bytes(plan_.net[t]));
-        check_lds(lds, (const void*)k_train_tile<0>);
-        check_lds(lds, (const void*)k_train_tile<1>);
+        check_lds(lds, gen_ ? (const void*)k_train_tile<0, true> : (const void*)k_train_tile<0, false>);
+        check_lds(lds, gen_ ? (const void*)k_train_tile<1, true> : (const void*)k_train_tile<1, false>);
         const double* Wm = (const double*)owner_.d_w64.p;
         if (tl.tiles > 0)
-            hipLaunchKernelGGL(k_train_tile<0>, dim3(tl.tiles), dim3(kTrainThreads), lds, stream_, (const TrainPlan*)d_plan_.p,
+            launch_tile<0>(gen_, dim3(tl.tiles), lds, stream_, (const TrainPlan*)d_plan_.p,
                                Wm, (const double*)d_par_.p, tl.tt, 0, (const int*)d_batom_.p, (const int*)d_bslot_.p,
                                (const int*)d_bstruct_.p, (const double*)d_G_.p, ldg_, (double*)d_out_.p, nullptr,
                                nullptr, nullptr);
@@ -755,7 +790,7 @@ class TrainSet {
         if (grad) {
             HIPCHK(hipMemsetAsync(d_grad_.p, 0, 8 * (size_t)total(), stream_));
             sum_tiles(tl, stream_, [&](const TrainTiles& tc, int c0, int c1) {
-                hipLaunchKernelGGL(k_train_tile<1>, dim3(c1 - c0), dim3(kTrainThreads), lds, stream_,
+                launch_tile<1>(gen_, dim3(c1 - c0), lds, stream_,
                                    (const TrainPlan*)d_plan_.p, Wm, (const double*)d_par_.p, tc, c0,
                                    (const int*)d_batom_.p, (const int*)d_bslot_.p, (const int*)d_bstruct_.p,
                                    (const double*)d_G_.p, ldg_, nullptr, (const double*)d_dcdo_.p,
@@ -794,8 +829,8 @@ class TrainSet {
             lds = std::max(lds, train_lds_bytes(plan_.net[t]));
             flds = std::max(flds, train_force_lds_bytes(plan_.net[t]));
         }
-        check_lds(lds, (const void*)k_train_tile<2>);
-        if (grad) check_lds(flds, (const void*)k_train_force_tile);
+        check_lds(lds, gen_ ? (const void*)k_train_tile<2, true> : (const void*)k_train_tile<2, false>);
+        if (grad) check_lds(flds, gen_ ? (const void*)k_train_force_tile<true> : (const void*)k_train_force_tile<false>);
         const double* Wm = (const double*)owner_.d_w64.p;
         if (grad) {
             d_grad_.reserve(8 * (size_t)total());
@@ -853,7 +888,7 @@ class TrainSet {
             HIPCHK(hipMemcpyAsync(d_eref_.p, eb.data(), 8 * (size_t)nb, hipMemcpyHostToDevice, st));
             // outputs and dout/dG at x, then the forces by the model's force kernels
             if (tl.tiles > 0)
-                hipLaunchKernelGGL(k_train_tile<2>, dim3(tl.tiles), dim3(kTrainThreads), lds, st,
+                launch_tile<2>(gen_, dim3(tl.tiles), lds, st,
                                    (const TrainPlan*)d_plan_.p, Wm, (const double*)d_par_.p, tl.tt, 0,
                                    (const int*)d_batom_.p, (const int*)d_bslot_.p, (const int*)d_bstruct_.p,
                                    (const double*)E.d_G.p, ldg_, (double*)d_out_.p, nullptr, nullptr,
@@ -876,10 +911,17 @@ class TrainSet {
                                        (const double4*)E.d_nbrr.p, E.maxp, E.lmaxp, (const double*)d_gF_.p,
                                        all_images ? 1 : 0, (double*)d_V_.p, ldg_, st));
                 sum_tiles(tl, st, [&](const TrainTiles& tc, int c0, int c1) {
-                    hipLaunchKernelGGL(k_train_force_tile, dim3(c1 - c0), dim3(kTrainThreads), flds, st,
-                                       (const TrainPlan*)d_plan_.p, Wm, (const double*)d_par_.p, tc, c0,
-                                       (const int*)d_batom_.p, (const int*)d_bstruct_.p, (const double*)E.d_G.p,
-                                       (const double*)d_V_.p, ldg_, (const double*)d_dcdo_.p, (double*)d_partial_.p);
+                    const TrainPlan* tpp = (const TrainPlan*)d_plan_.p;
+                    const double* par = (const double*)d_par_.p;
+                    const int *ba = (const int*)d_batom_.p, *bs = (const int*)d_bstruct_.p;
+                    const double *Gp = (const double*)E.d_G.p, *Vp = (const double*)d_V_.p, *dc = (const double*)d_dcdo_.p;
+                    double* pp = (double*)d_partial_.p;
+                    if (gen_)
+                        hipLaunchKernelGGL(k_train_force_tile<true>, dim3(c1 - c0), dim3(kTrainThreads), flds, st, tpp, Wm,
+                                           par, tc, c0, ba, bs, Gp, Vp, ldg_, dc, pp);
+                    else
+                        hipLaunchKernelGGL(k_train_force_tile<false>, dim3(c1 - c0), dim3(kTrainThreads), flds, st, tpp, Wm,
+                                           par, tc, c0, ba, bs, Gp, Vp, ldg_, dc, pp);
                 });
             }
             std::vector<double> lv(nb), e2(nb), f2(nb);
@@ -917,6 +959,7 @@ class TrainSet {
     std::vector<int> hbatom_, hbslot_, hbstruct_;
     bool has_forces_ = false;
     size_t lds_limit_ = 0;
+    bool gen_ = false;  // some hidden layer is not tanh
     std::unique_ptr<Engine> ws_;  // workspace of the force passes
 };
 

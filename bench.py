@@ -142,6 +142,23 @@ def train_line(tmp, args):
         "nparams": tr.nparams(),
         "network_tflops": flops / sec / 1e12,
     }
+    # force training on a quarter of the set: each call rebuilds neighbour lists and descriptors,
+    # evaluates the forces at x, and runs the force pass backwards for the gradient
+    fb = list(range(max(1, len(ts) // 4)))
+    fat = sum(ts[b].natoms for b in fb)
+    frng = np.random.default_rng(8)
+    tr.set_forces([frng.normal(0, 0.5, (s.natoms, 3)) for s in ts])
+    tr.error_force(x, fb, LOSS_MSE, 1.0, 1.0)
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        tr.error_force(x, fb, LOSS_MSE, 1.0, 1.0)
+    fsec = (time.perf_counter() - t0) / calls
+    line["force_training"] = {
+        "workload": f"{len(fb)} mixed64x4096 structures ({fat} atoms), MSE energy + force loss (beta 1), "
+                    "error + exact gradient per call, FP64",
+        "atoms_per_s": fat / fsec,
+        "ms_per_call": fsec * 1e3,
+    }
     tr.close()
     model_t.close()
     if not args.no_cpu_baseline:
