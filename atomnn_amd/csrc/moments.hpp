@@ -220,6 +220,124 @@ __device__ __forceinline__ void symm_angular_moments(const DevPlan* __restrict__
     }
 }
 
+// Tangents of the moments along entry displacements (force training).  tg[4p..4p+3] holds
+// (u_x', u_y', u_z', d') of entry p, Rf / Rd hold R_e(p) and R_e'(p).  Per species and exponent:
+//   M[s][e][q]  = sum_p R_e u^q,
+//   dM[s][e][q] = sum_p (R_e' d' u^q + R_e grad(u^q) . u'),  grad(u^q) . u' = sum_c q_c u^(q - e_c) u_c',
+//   dD[s][e]    = sum_p 2 R_e R_e' d'   (the tangent of the diagonal D = sum_p R_e^2).
+// Lane q < 35 owns monomial q; entries in index order; the power table goes through pw in chunks.
+template <int NE>
+__device__ __forceinline__ void build_moments_jvp(const double* su, const double* tg, const double* Rf,
+                                                  const double* Rd, int P, int ch, int ntypes, const int* seg,
+                                                  double* pw, double* Mom, double* dMom, double* D, double* dD,
+                                                  int lane) {
+    ch = max(ch, 1);
+    const int q = lane;
+    int ax = 0, ay = 0, az = 0;
+    if (q < kMomK) {
+        ax = kMomTab.a[q][0];
+        ay = kMomTab.a[q][1];
+        az = kMomTab.a[q][2];
+    }
+    // power-table offsets of u_c^(a_c) and u_c^(a_c - 1) (the latter only read when a_c > 0)
+    const int ox = ax, oy = 5 + ay, oz = 10 + az;
+    const int lx = ax > 0 ? ox - 1 : ox, ly = ay > 0 ? oy - 1 : oy, lz = az > 0 ? oz - 1 : oz;
+    const double fx = ax, fy = ay, fz = az;
+    for (int c0 = 0; c0 < P || c0 == 0; c0 += ch) {
+        const int c1 = min(P, c0 + ch);
+        __syncthreads();
+        power_rows<double>(su, c0, c1, pw, lane);
+        __syncthreads();
+        if (q < kMomK) {
+            for (int s = 0; s < ntypes; ++s) {
+                const int lo = max(seg[s], c0), hi = min(seg[s + 1], c1);
+                double a[NE], da[NE];
+#pragma unroll
+                for (int e = 0; e < NE; ++e) {
+                    a[e] = c0 == 0 ? 0.0 : Mom[(s * NE + e) * kMomK + q];
+                    da[e] = c0 == 0 ? 0.0 : dMom[(s * NE + e) * kMomK + q];
+                }
+                for (int p = lo; p < hi; ++p) {
+                    const double* row = pw + (p - c0) * kMomPw;
+                    const double X = row[ox], Y = row[oy], Z = row[oz];
+                    const double m = X * Y * Z;
+                    const double dm = fx * row[lx] * Y * Z * tg[4 * p] + fy * X * row[ly] * Z * tg[4 * p + 1] +
+                                      fz * X * Y * row[lz] * tg[4 * p + 2];
+                    const double dd = tg[4 * p + 3];
+#pragma unroll
+                    for (int e = 0; e < NE; ++e) {
+                        const double r = Rf[p * NE + e];
+                        a[e] += r * m;
+                        da[e] += Rd[p * NE + e] * dd * m + r * dm;
+                    }
+                }
+#pragma unroll
+                for (int e = 0; e < NE; ++e) {
+                    Mom[(s * NE + e) * kMomK + q] = a[e];
+                    dMom[(s * NE + e) * kMomK + q] = da[e];
+                }
+            }
+        }
+        if (c1 >= P) break;
+    }
+    for (int it = lane; it < ntypes * NE; it += kWave) {
+        const int e = it % NE, s = it / NE;
+        double a = 0.0, da = 0.0;
+        for (int p = seg[s]; p < seg[s + 1]; ++p) {
+            const double r = Rf[p * NE + e];
+            a += r * r;
+            da += 2.0 * r * Rd[p * NE + e] * tg[4 * p + 3];
+        }
+        if (D) D[it] = a;
+        dD[it] = da;
+    }
+    __syncthreads();
+}
+
+// Tangents of the angular descriptors from the moments and their tangents: per block (a, b),
+// exponent e and power p, dS = sum_{|q| = p} mult(q) (dM_a M_b + M_a dM_b), dX_p = dS for a != b
+// and (dS - dD_a) / 2 for a == b; G_f' = norm 2^-zeta sum_p C(zeta, p) lambda^p dX_p.  Writes
+// -G_f' into Vrow (the sign of the force-training adjoint).  S: nblk NE 5 doubles of scratch.
+template <int NE>
+__device__ __forceinline__ void symm_angular_moments_jvp(const DevPlan* __restrict__ plan, int ti, int ntypes,
+                                                         const double* Mom, const double* dMom, const double* dD,
+                                                         double* S, double* Vrow, int lane) {
+    const int nblk = ntypes * (ntypes + 1) / 2;
+    for (int it = lane; it < nblk * NE * (kMomZ + 1); it += kWave) {
+        const int pd = it % (kMomZ + 1), e = (it / (kMomZ + 1)) % NE, blk = it / ((kMomZ + 1) * NE);
+        int a, bb;
+        blk_species(blk, a, bb);
+        const double* Ma = Mom + (a * NE + e) * kMomK;
+        const double* Mb = Mom + (bb * NE + e) * kMomK;
+        const double* dMa = dMom + (a * NE + e) * kMomK;
+        const double* dMb = dMom + (bb * NE + e) * kMomK;
+        double acc = 0.0;
+        const int q0 = pd * (pd + 1) * (pd + 2) / 6, q1 = (pd + 1) * (pd + 2) * (pd + 3) / 6;
+        for (int q = q0; q < q1; ++q) acc += double(kMomTab.mult[q]) * (dMa[q] * Mb[q] + Ma[q] * dMb[q]);
+        S[it] = acc;
+    }
+    __syncthreads();
+    for (int it = lane; it < nblk * NE * kMaxC; it += kWave) {
+        const int c = it % kMaxC, e = (it / kMaxC) % NE, blk = it / (kMaxC * NE);
+        const DevAngular& A = plan->ang[ti][blk];
+        if (e >= A.ne || c >= A.nc) continue;
+        const int f = A.fmap[e * kMaxC + c];
+        if (f < 0) continue;
+        int a, bb;
+        blk_species(blk, a, bb);
+        const int z = A.zint[c];
+        const double lam = A.lambda[c];
+        double acc = 0.0, lp = 1.0;
+        for (int p = 0; p <= z; ++p) {
+            double x = S[(blk * NE + e) * (kMomZ + 1) + p];
+            if (a == bb) x = 0.5 * (x - dD[a * NE + e]);
+            acc += double(binom4(z, p)) * lp * x;
+            lp *= lam;
+        }
+        Vrow[A.goff + f] = -acc * A.norm * (1.0 / (double)(1 << z));
+    }
+}
+
 // Adjoint tables from dE/dG (row W): w table, Delta[s][e] (into Delta) and L[s][q][e] (into L,
 // which held the power table: the caller's barrier after the moments separates the two uses).
 // The moments are the sum of `nparts` partial tables `pstride` elements apart, added in order.

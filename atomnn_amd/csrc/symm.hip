@@ -837,9 +837,16 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
 // Radial: G_f' = norm sum_j d_j' (phi_f fc)'(d_j).  Angular, per pair and (exponent e, term c):
 // (A_c R_e(j) R_e(k))' = A_c' cos' R_e(j) R_e(k) + A_c (R_e'(j) d_j' R_e(k) + R_e(j) R_e'(k) d_k').
 // One wavefront per centre, the same staging, tables and fixed-order lane reductions as k_symm.
+// Moment form (centre species with mom_ok): the tangents of the moments instead of the pair
+// loops, O(35 NE) per entry; LDS adds Mom, dMom [S][NE][35], dD [S][NE] and S [blocks][NE][5].
+template <int NE>
+__host__ __device__ constexpr int jvp_mom_elems(int ntypes) {
+    return 2 * ntypes * NE * kMomK + ntypes * NE + (ntypes * (ntypes + 1) / 2) * NE * (kMomZ + 1);
+}
 template <int NE, int NC, bool GEN>
-__host__ __device__ constexpr size_t jvp_lds_bytes(int maxp, int ldg) {
-    return 8 * (size_t)(8 * maxp + 2 * NE * maxp + kWave * NE * NC + ((ldg + 1) & ~1) + (GEN ? 2 * maxp : 0)) +
+__host__ __device__ constexpr size_t jvp_lds_bytes(int maxp, int ldg, int mom_elems = 0) {
+    return 8 * (size_t)(8 * maxp + 2 * NE * maxp + kWave * NE * NC + ((ldg + 1) & ~1) + (GEN ? 2 * maxp : 0) +
+                        mom_elems) +
            4 * (size_t)((maxp + 3) & ~3);
 }
 
@@ -849,7 +856,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 8))) v
     const int* __restrict__ atom_type, const DevStruc* __restrict__ S, const int* __restrict__ atom_struc,
     const int* __restrict__ cnt, const int* __restrict__ nbr_j, const int* __restrict__ nbr_meta,
     const double4* __restrict__ nbr_r, int maxp, int lmaxp, const double* __restrict__ gF, int all_images,
-    double* __restrict__ V, int ldg) {
+    double* __restrict__ V, int ldg, int use_mom) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     double* su = (double*)lds_raw;
     double* tg = su + 4 * lmaxp;  // (w_x, w_y, w_z, d')
@@ -858,7 +865,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 8))) v
     double* red = Rd + NE * lmaxp;
     double* Vrow = red + kWave * NE * NC;
     double* nsx = Vrow + ((ldg + 1) & ~1);
-    int* orig = (int*)(nsx + (GEN ? 2 * lmaxp : 0));
+    double* momr = nsx + (GEN ? 2 * lmaxp : 0);
+    int* orig = (int*)(momr + (use_mom ? jvp_mom_elems<NE>(plan->T) : 0));
     const int i = blockIdx.x;
     const int lane = threadIdx.x;
     const int ti = atom_type[i];
@@ -942,6 +950,26 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(1, 8))) v
             }
             __syncthreads();
         }
+    }
+    // angular blocks: tangents of the moments when the centre species takes them
+    if (use_mom && plan->mom_ok[ti]) {
+        // tg: (w, d') -> (u' = w - (d'/d) u, d')
+        for (int p = lane; p < P; p += kWave) {
+            const double sp = tg[4 * p + 3] / su[4 * p + 3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) tg[4 * p + c] -= sp * su[4 * p + c];
+        }
+        radial_tables<double, NE, GEN>(plan->ang[ti][0], su, 0, P, Rf, Rd, lane);
+        double* Mom = momr;
+        double* dMom = Mom + ntypes * NE * kMomK;
+        double* dD = dMom + ntypes * NE * kMomK;
+        double* S = dD + ntypes * NE;
+        build_moments_jvp<NE>(su, tg, Rf, Rd, P, (kWave * NE * NC) / kMomPw, ntypes, sg.start, red, Mom, dMom, nullptr,
+                              dD, lane);
+        symm_angular_moments_jvp<NE>(plan, ti, ntypes, Mom, dMom, dD, S, Vrow, lane);
+        __syncthreads();
+        for (int k = lane; k < ldg; k += kWave) V[(int64_t)i * ldg + k] = Vrow[k];
+        return;
     }
     // angular blocks in LMat order
     const bool shared = plan->ang_shared[ti];
@@ -1066,11 +1094,16 @@ template <int NE, int NC, bool GEN>
 static hipError_t launch_jvp_t(const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
                                const DevStruc* S, const int* atom_struc, int ntot, const int* cnt, const int* nbr_j,
                                const int* meta, const double4* r, int maxp, int lmaxp, const double* gF, int all_images,
-                               double* V, int ldg, hipStream_t st) {
-    const size_t lds = jvp_lds_bytes<NE, NC, GEN>(lmaxp, ldg);
+                               double* V, int ldg, int use_mom, int ntypes, hipStream_t st) {
+    size_t lds = jvp_lds_bytes<NE, NC, GEN>(lmaxp, ldg);
     if (lds > 65536) return hipErrorInvalidConfiguration;
+    if (use_mom) {
+        const size_t ml = jvp_lds_bytes<NE, NC, GEN>(lmaxp, ldg, jvp_mom_elems<NE>(ntypes));
+        if (ml <= 65536) lds = ml;
+        else use_mom = 0;
+    }
     hipLaunchKernelGGL((k_symm_jvp<NE, NC, GEN>), dim3(ntot), dim3(kWave), lds, st, plan, rfe, rfr, atom_type, S,
-                       atom_struc, cnt, nbr_j, meta, r, maxp, lmaxp, gF, all_images, V, ldg);
+                       atom_struc, cnt, nbr_j, meta, r, maxp, lmaxp, gF, all_images, V, ldg, use_mom);
     return hipGetLastError();
 }
 
@@ -1083,8 +1116,9 @@ size_t symm_jvp_lds(int shape, bool ns, int lmaxp, int ldg) {
 hipError_t launch_symm_jvp(int shape, int ns, const DevPlan* plan, const double* rfe, const double* rfr,
                            const int* atom_type, const DevStruc* S, const int* atom_struc, int ntot, const int* cnt,
                            const int* nbr_j, const int* meta, const double4* r, int maxp, int lmaxp, const double* gF,
-                           int all_images, double* V, int ldg, hipStream_t st) {
-#define JVP_ARGS plan, rfe, rfr, atom_type, S, atom_struc, ntot, cnt, nbr_j, meta, r, maxp, lmaxp, gF, all_images, V, ldg, st
+                           int all_images, double* V, int ldg, int use_mom, int ntypes, hipStream_t st) {
+#define JVP_ARGS \
+    plan, rfe, rfr, atom_type, S, atom_struc, ntot, cnt, nbr_j, meta, r, maxp, lmaxp, gF, all_images, V, ldg, use_mom, ntypes, st
     if (shape == 0) return ns ? launch_jvp_t<2, 4, true>(JVP_ARGS) : launch_jvp_t<2, 4, false>(JVP_ARGS);
     if (shape == 1) return ns ? launch_jvp_t<4, 4, true>(JVP_ARGS) : launch_jvp_t<4, 4, false>(JVP_ARGS);
     return ns ? launch_jvp_t<8, 8, true>(JVP_ARGS) : launch_jvp_t<8, 8, false>(JVP_ARGS);

@@ -39,7 +39,7 @@ namespace atomnn {
 hipError_t launch_symm_jvp(int shape, int ns, const DevPlan* plan, const double* rfe, const double* rfr,
                            const int* atom_type, const DevStruc* S, const int* atom_struc, int ntot, const int* cnt,
                            const int* nbr_j, const int* meta, const double4* r, int maxp, int lmaxp, const double* gF,
-                           int all_images, double* V, int ldg, hipStream_t st);
+                           int all_images, double* V, int ldg, int use_mom, int ntypes, hipStream_t st);
 
 // Per-species network for training.  W_l (padded, row-major wpad_l x in_pad_l) and b_l (padded)
 // live in the training parameter buffer; inpw / inpb stay in the model's weight buffer.
@@ -909,7 +909,7 @@ class TrainSet {
                                        (const DevStruc*)E.d_strucs.p, (const int*)E.d_astruc.p, n,
                                        (const int*)E.d_cnt.p, (const int*)E.d_nbrj.p, (const int*)E.d_meta.p,
                                        (const double4*)E.d_nbrr.p, E.maxp, E.lmaxp, (const double*)d_gF_.p,
-                                       all_images ? 1 : 0, (double*)d_V_.p, ldg_, st));
+                                       all_images ? 1 : 0, (double*)d_V_.p, ldg_, (E.moments & 1) ? 1 : 0, T_, st));
                 sum_tiles(tl, st, [&](const TrainTiles& tc, int c0, int c1) {
                     const TrainPlan* tpp = (const TrainPlan*)d_plan_.p;
                     const double* par = (const double*)d_par_.p;

@@ -165,6 +165,22 @@ def test_force_training_matches_the_oracle(fpair, loss, ai):
         check_force_parity(t, o, x, batch, loss, 0.05, 2.5, ai)
 
 
+def test_force_training_triplet_path(potentials, fpair):
+    """With the moment expansion switched off the descriptor tangents take the pair loops; the
+    same parity bar."""
+    _, _, o, strucs, eref = fpair
+    m2 = NNP.read(potentials["plain4096"])
+    try:
+        m2.set_option("moments", 0)
+        t2 = Train(m2, strucs, eref)
+        t2.set_forces(reference_forces(ob.OracleModel(potentials["plain4096"]), strucs, 8))
+        x = t2.params() + np.random.default_rng(22).normal(0, 0.02, t2.nparams())
+        check_force_parity(t2, o, x, [0, 3, 5, 7], LOSS_MSE, 1.0, 2.5, False)
+        t2.close()
+    finally:
+        m2.close()
+
+
 def test_force_training_without_force_weight_is_energy_training(fpair):
     m, t, o, strucs, eref = fpair
     x = t.params()
