@@ -1165,6 +1165,9 @@ int nnp_set_option(nnp_model* m, const char* key, int64_t value) {
             E.force_waves = (int)value;
             E.drop_graph();
             E.eager_runs = 0;
+        } else if (k == "train_one_pass") {
+            if (value != 0 && value != 1) throw std::invalid_argument("nnp_set_option: train_one_pass is 0 or 1");
+            E.train_one_pass = (int)value;
         } else if (k == "fused_blocks_per_cu") {
             if (value < 0 || value > 64) throw std::invalid_argument("nnp_set_option: fused_blocks_per_cu in 0..64");
             E.fused_blocks_per_cu = (int)value;

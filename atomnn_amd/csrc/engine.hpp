@@ -136,6 +136,9 @@ class Engine {
     // mixed precision networks: BF16 MFMA products with operands split into hi + lo BF16 parts
     // (3 products, about 16 mantissa bits per operand) or single BF16 products (1)
     int net_bf16_terms = 3;
+    // energy training: one forward + backward pass over single-structure tiles when it fits (1),
+    // or always a forward pass and a second pass with the loss seeds (0)
+    int train_one_pass = 1;
     int last_mom = 0;  // the moment bits the last pass used
     int mom_flag(size_t base_lds, bool fp64, int bits);
     int num_cus = 0;

@@ -103,6 +103,9 @@ static size_t train_lds_bytes(const TrainNet& n) {
 // MODE 0: forward, out[slot] = outb + outw a_L.  MODE 1: forward and backward, the tile's
 // gradient row into `partial`.  MODE 2: forward and the input gradient (fpbp + DODZ::grad,
 // nn.cpp:1191-1233, 1620-1628): out[slot] and Y[slot] = dout/dG = inpw .* (w_0^T dodz_0).
+// MODE 3: out[slot] and the gradient row of the outputs (seed outw instead of dcdo outw), for
+// tiles whose atoms belong to one structure: the reduction then weights the row by dcdo.
+// Rows beyond the species' atoms or with batom < 0 (padding) carry nothing.
 template <int MODE, bool GEN>
 __global__ __launch_bounds__(kTrainThreads) void k_train_tile(
     const TrainPlan* __restrict__ tp, const double* __restrict__ Wm, const double* __restrict__ par, TrainTiles tt,
@@ -130,7 +133,8 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_tile(
     for (int idx = tid; idx < 16 * N.nin_pad; idx += kTrainThreads) {
         const int r = idx / N.nin_pad, k = idx % N.nin_pad;
         double v = 0.0;
-        if (r0 + r < M && k < N.nin) v = Wm[N.inpw_off + k] * (G[(int64_t)batom[a0 + r0 + r] * ldg + k] + Wm[N.inpb_off + k]);
+        const int src = r0 + r < M ? batom[a0 + r0 + r] : -1;  // -1: padding row
+        if (src >= 0 && k < N.nin) v = Wm[N.inpw_off + k] * (G[(int64_t)src * ldg + k] + Wm[N.inpb_off + k]);
         ins[r * N.nin_pad + k] = v;
     }
     __syncthreads();
@@ -153,8 +157,9 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_tile(
         __syncthreads();
     }
     const int L = N.L;
-    if constexpr (MODE == 0 || MODE == 2) {
-        if (tid < 16 && r0 + tid < M) out[bslot[a0 + r0 + tid]] = N.outb + A[L - 1][tid * N.wpad[L - 1]] * N.outw;
+    if constexpr (MODE != 1) {
+        if (tid < 16 && r0 + tid < M && batom[a0 + r0 + tid] >= 0)
+            out[bslot[a0 + r0 + tid]] = N.outb + A[L - 1][tid * N.wpad[L - 1]] * N.outw;
     }
     if constexpr (MODE == 0) {
         return;
@@ -181,7 +186,8 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_tile(
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int row = 4 * v + lk;
-                if (r0 + row < M && col < N.nin) Y[(int64_t)bslot[a0 + r0 + row] * ldg + col] = acc[v] * Wm[N.inpw_off + col];
+                if (r0 + row < M && col < N.nin && batom[a0 + r0 + row] >= 0)
+                    Y[(int64_t)bslot[a0 + r0 + row] * ldg + col] = acc[v] * Wm[N.inpw_off + col];
             }
         }
     } else {
@@ -190,7 +196,8 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_tile(
             const int wp = N.wpad[L - 1];
             for (int idx = tid; idx < 16 * wp; idx += kTrainThreads) {
                 const int r = idx / wp, n = idx % wp;
-                const double c = (r0 + r < M && n < N.nodes[L - 1]) ? dcdo[bstruct[a0 + r0 + r]] * N.outw : 0.0;
+                const bool real = r0 + r < M && n < N.nodes[L - 1] && batom[a0 + r0 + r] >= 0;
+                const double c = real ? (MODE == 3 ? N.outw : dcdo[bstruct[a0 + r0 + r]] * N.outw) : 0.0;
                 D[L - 1][idx] *= c;
             }
         }
@@ -467,7 +474,8 @@ __global__ __launch_bounds__(256) void k_train_struct(const double* __restrict__
 constexpr int kReduceGroup = 32;
 __global__ __launch_bounds__(256) void k_train_reduce1(const TrainPlan* __restrict__ tp, TrainTiles tt, int tile0,
                                                        int tile1, const double* __restrict__ partial, int gmax,
-                                                       int64_t gstride, double* __restrict__ grp) {
+                                                       int64_t gstride, double* __restrict__ grp,
+                                                       const int* __restrict__ tstruct, const double* __restrict__ dcdo) {
     const int s = blockIdx.z, g = blockIdx.y;
     const TrainNet& N = tp->net[s];
     const int p = blockIdx.x * 256 + threadIdx.x;
@@ -475,7 +483,12 @@ __global__ __launch_bounds__(256) void k_train_reduce1(const TrainPlan* __restri
     const int t0 = max(tile0, tt.tile_start[s]), t1 = min(tile1, tt.tile_start[s + 1]);
     const int a = t0 + g * kReduceGroup, b = min(t1, a + kReduceGroup);
     double acc = 0.0;
-    for (int t = a; t < b; ++t) acc += partial[tt.pbase[s] + (int64_t)(t - tt.tile_start[s]) * N.nparams + p];
+    if (tstruct) {  // rows of single-structure tiles, weighted by their structure's dcdo
+        for (int t = a; t < b; ++t)
+            acc += dcdo[tstruct[t]] * partial[tt.pbase[s] + (int64_t)(t - tt.tile_start[s]) * N.nparams + p];
+    } else {
+        for (int t = a; t < b; ++t) acc += partial[tt.pbase[s] + (int64_t)(t - tt.tile_start[s]) * N.nparams + p];
+    }
     grp[(int64_t)s * gstride + (int64_t)g * N.nparams + p] = acc;
 }
 __global__ __launch_bounds__(256) void k_train_reduce2(const TrainPlan* __restrict__ tp, TrainTiles tt, int tile0,
@@ -711,16 +724,27 @@ class TrainSet {
             }
             launch(tc, c0, c1);
             HIPCHK(hipGetLastError());
-            const int gmax = (c1 - c0 + kReduceGroup - 1) / kReduceGroup;
-            const int64_t gstride = (int64_t)gmax * maxnp;
-            d_grp_.reserve(8 * (size_t)(T_ * gstride));
-            const unsigned pblocks = (unsigned)((maxnp + 255) / 256);
-            hipLaunchKernelGGL(k_train_reduce1, dim3(pblocks, gmax, T_), dim3(256), 0, st, (const TrainPlan*)d_plan_.p, tc,
-                               c0, c1, (const double*)d_partial_.p, gmax, gstride, (double*)d_grp_.p);
-            hipLaunchKernelGGL(k_train_reduce2, dim3(pblocks, T_), dim3(256), 0, st, (const TrainPlan*)d_plan_.p, tc, c0,
-                               c1, (const double*)d_grp_.p, gstride, (const int64_t*)d_goff_.p, (double*)d_grad_.p, 1);
-            HIPCHK(hipGetLastError());
+            reduce_rows(tl, tc, c0, c1, st, nullptr, nullptr);
         }
+    }
+
+    // d_grad_ += the rows of tiles [c0, c1) (bases tc), optionally weighted by dcdo[tstruct[tile]].
+    void reduce_rows(const Tiles& tl, const TrainTiles& tc, int c0, int c1, hipStream_t st, const int* tstruct,
+                     const double* dcdo) {
+        (void)tl;
+        if (c1 <= c0) return;
+        int64_t maxnp = 1;
+        for (int t = 0; t < T_; ++t) maxnp = std::max<int64_t>(maxnp, plan_.net[t].nparams);
+        const int gmax = (c1 - c0 + kReduceGroup - 1) / kReduceGroup;
+        if (gmax > 65535) throw std::invalid_argument("nnp_train: too many tiles in one reduction");
+        const int64_t gstride = (int64_t)gmax * maxnp;
+        d_grp_.reserve(8 * (size_t)(T_ * gstride));
+        const unsigned pblocks = (unsigned)((maxnp + 255) / 256);
+        hipLaunchKernelGGL(k_train_reduce1, dim3(pblocks, gmax, T_), dim3(256), 0, st, (const TrainPlan*)d_plan_.p, tc, c0,
+                           c1, (const double*)d_partial_.p, gmax, gstride, (double*)d_grp_.p, tstruct, dcdo);
+        hipLaunchKernelGGL(k_train_reduce2, dim3(pblocks, T_), dim3(256), 0, st, (const TrainPlan*)d_plan_.p, tc, c0, c1,
+                           (const double*)d_grp_.p, gstride, (const int64_t*)d_goff_.p, (double*)d_grad_.p, 1);
+        HIPCHK(hipGetLastError());
     }
 
     // A tile kernel's dynamic LDS within the device's per-workgroup limit (above 64 KiB the kernel
@@ -747,55 +771,128 @@ class TrainSet {
     void error(const double* x, int nbatch, const int* batch, int loss, double delta, double* err, double* grad) {
         check_args("nnp_train_error", x, nbatch, batch, loss, delta, err);
         pack_params(x, stream_);
-        // batch atoms: structure order (slots) and species order (tiles)
-        std::vector<int> bstart(nbatch + 1, 0);
-        for (int b = 0; b < nbatch; ++b) bstart[b + 1] = bstart[b] + natoms_[batch[b]];
-        const int nba = bstart[nbatch];
-        std::vector<std::vector<int>> atom_s(T_), slot_s(T_), struct_s(T_);
-        for (int b = 0; b < nbatch; ++b) {
-            const int st = batch[b];
-            for (int i = 0; i < natoms_[st]; ++i) {
-                const int g = (int)aoff_[st] + i, t = types_[g];
-                atom_s[t].push_back(g);
-                slot_s[t].push_back(bstart[b] + i);
-                struct_s[t].push_back(b);
+        int64_t maxnp = 1;
+        for (int t = 0; t < T_; ++t) maxnp = std::max<int64_t>(maxnp, plan_.net[t].nparams);
+        // Tiles: with the gradient, single-structure tiles (each structure's atoms of a species
+        // padded to whole tiles) when that adds few tiles and every row fits in one buffer, so one
+        // forward + backward pass per tile gives the outputs and the gradient rows of the outputs,
+        // which the reduction weights by dcdo; otherwise species tiles across structures, a forward
+        // pass for the energies and a second pass with the loss seeds.  The tiles of the previous
+        // call are reused when the batch and mode repeat.
+        const int mode = grad ? (owner_.train_one_pass ? 2 : 1) : 0;
+        std::vector<int> key(batch, batch + nbatch);
+        key.push_back(mode);
+        if (key != tiles_key_) {
+            tiles_key_.clear();
+            std::vector<int> bstart(nbatch + 1, 0);
+            for (int b = 0; b < nbatch; ++b) bstart[b + 1] = bstart[b] + natoms_[batch[b]];
+            std::vector<std::vector<int>> atom_s(T_), slot_s(T_), struct_s(T_);
+            for (int b = 0; b < nbatch; ++b) {
+                const int st = batch[b];
+                for (int i = 0; i < natoms_[st]; ++i) {
+                    const int g = (int)aoff_[st] + i, t = types_[g];
+                    atom_s[t].push_back(g);
+                    slot_s[t].push_back(bstart[b] + i);
+                    struct_s[t].push_back(b);
+                }
             }
+            one_pass_ = false;
+            if (grad) {
+                int64_t tm = 0, th = 0;
+                for (int t = 0; t < T_; ++t) {
+                    tm += ((int64_t)atom_s[t].size() + 15) / 16;
+                    for (size_t k = 0; k < atom_s[t].size();) {  // runs of one structure
+                        size_t e = k;
+                        while (e < atom_s[t].size() && struct_s[t][e] == struct_s[t][k]) ++e;
+                        th += ((int64_t)(e - k) + 15) / 16;
+                        k = e;
+                    }
+                }
+                one_pass_ = mode == 2 && th <= tm + tm / 8 + 8 && th * maxnp * 8 <= kOnePassBytes;
+            }
+            if (one_pass_) {
+                // pad each (structure, species) run to whole tiles with rows marked -1
+                for (int t = 0; t < T_; ++t) {
+                    std::vector<int> pa, ps, pst;
+                    for (size_t k = 0; k < atom_s[t].size();) {
+                        size_t e = k;
+                        while (e < atom_s[t].size() && struct_s[t][e] == struct_s[t][k]) ++e;
+                        for (size_t q = k; q < e; ++q) {
+                            pa.push_back(atom_s[t][q]);
+                            ps.push_back(slot_s[t][q]);
+                            pst.push_back(struct_s[t][q]);
+                        }
+                        while (pa.size() % 16) {
+                            pa.push_back(-1);
+                            ps.push_back(0);
+                            pst.push_back(struct_s[t][k]);
+                        }
+                        k = e;
+                    }
+                    atom_s[t].swap(pa);
+                    slot_s[t].swap(ps);
+                    struct_s[t].swap(pst);
+                }
+            }
+            tiles_ = make_tiles(atom_s, slot_s, struct_s, stream_);
+            if (one_pass_) {
+                std::vector<int> ts;
+                for (int t = 0; t < T_; ++t)
+                    for (size_t k = 0; k < struct_s[t].size(); k += 16) ts.push_back(struct_s[t][k]);
+                d_tstruct_.reserve(4 * std::max<size_t>(ts.size(), 1));
+                HIPCHK(hipMemcpyAsync(d_tstruct_.p, ts.data(), 4 * ts.size(), hipMemcpyHostToDevice, stream_));
+            }
+            d_bstart_.reserve(4 * (size_t)(nbatch + 1));
+            d_out_.reserve(8 * (size_t)std::max(bstart[nbatch], 1));
+            HIPCHK(hipMemcpyAsync(d_bstart_.p, bstart.data(), 4 * (size_t)(nbatch + 1), hipMemcpyHostToDevice, stream_));
+            HIPCHK(hipStreamSynchronize(stream_));  // the host vectors above go out of scope
+            tiles_key_ = key;
         }
-        const Tiles tl = make_tiles(atom_s, slot_s, struct_s, stream_);
+        const Tiles& tl = tiles_;
         std::vector<double> eb(nbatch);
         for (int b = 0; b < nbatch; ++b) eb[b] = eref_[batch[b]];
-        d_bstart_.reserve(4 * (size_t)(nbatch + 1));
         d_eref_.reserve(8 * (size_t)nbatch);
-        d_out_.reserve(8 * (size_t)std::max(nba, 1));
         d_dcdo_.reserve(8 * (size_t)nbatch);
         d_loss_.reserve(8 * (size_t)nbatch);
         d_dE2_.reserve(8 * (size_t)nbatch);
         d_grad_.reserve(8 * (size_t)total());
-        HIPCHK(hipMemcpyAsync(d_bstart_.p, bstart.data(), 4 * (size_t)(nbatch + 1), hipMemcpyHostToDevice, stream_));
         HIPCHK(hipMemcpyAsync(d_eref_.p, eb.data(), 8 * (size_t)nbatch, hipMemcpyHostToDevice, stream_));
         size_t lds = 0;
         for (int t = 0; t < T_; ++t) lds = std::max(lds, train_lds_bytes(plan_.net[t]));
-        check_lds(lds, gen_ ? (const void*)k_train_tile<0, true> : (const void*)k_train_tile<0, false>);
-        check_lds(lds, gen_ ? (const void*)k_train_tile<1, true> : (const void*)k_train_tile<1, false>);
+        for (int md = 0; md < 4; md += (md == 1 ? 2 : 1)) {  // modes 0, 1, 3
+            const void* fn = md == 0 ? (gen_ ? (const void*)k_train_tile<0, true> : (const void*)k_train_tile<0, false>)
+                             : md == 1 ? (gen_ ? (const void*)k_train_tile<1, true> : (const void*)k_train_tile<1, false>)
+                                       : (gen_ ? (const void*)k_train_tile<3, true> : (const void*)k_train_tile<3, false>);
+            check_lds(lds, fn);
+        }
         const double* Wm = (const double*)owner_.d_w64.p;
-        if (tl.tiles > 0)
-            launch_tile<0>(gen_, dim3(tl.tiles), lds, stream_, (const TrainPlan*)d_plan_.p,
-                               Wm, (const double*)d_par_.p, tl.tt, 0, (const int*)d_batom_.p, (const int*)d_bslot_.p,
-                               (const int*)d_bstruct_.p, (const double*)d_G_.p, ldg_, (double*)d_out_.p, nullptr,
-                               nullptr, nullptr);
+        const TrainPlan* tpp = (const TrainPlan*)d_plan_.p;
+        const double* par = (const double*)d_par_.p;
+        const int *ba = (const int*)d_batom_.p, *bsl = (const int*)d_bslot_.p, *bst = (const int*)d_bstruct_.p;
+        const double* Gp = (const double*)d_G_.p;
+        if (one_pass_) {
+            d_partial_.reserve(8 * (size_t)std::max<int64_t>(tl.pb, 1));
+            if (tl.tiles > 0)
+                launch_tile<3>(gen_, dim3(tl.tiles), lds, stream_, tpp, Wm, par, tl.tt, 0, ba, bsl, bst, Gp, ldg_,
+                               (double*)d_out_.p, nullptr, (double*)d_partial_.p, nullptr);
+        } else if (tl.tiles > 0) {
+            launch_tile<0>(gen_, dim3(tl.tiles), lds, stream_, tpp, Wm, par, tl.tt, 0, ba, bsl, bst, Gp, ldg_,
+                           (double*)d_out_.p, nullptr, nullptr, nullptr);
+        }
         hipLaunchKernelGGL(k_train_struct, dim3(nbatch), dim3(256), 0, stream_, (const double*)d_out_.p,
                            (const int*)d_bstart_.p, (const double*)d_eref_.p, loss, delta, (double*)d_dcdo_.p,
                            (double*)d_loss_.p, (double*)d_dE2_.p);
         HIPCHK(hipGetLastError());
         if (grad) {
             HIPCHK(hipMemsetAsync(d_grad_.p, 0, 8 * (size_t)total(), stream_));
-            sum_tiles(tl, stream_, [&](const TrainTiles& tc, int c0, int c1) {
-                launch_tile<1>(gen_, dim3(c1 - c0), lds, stream_,
-                                   (const TrainPlan*)d_plan_.p, Wm, (const double*)d_par_.p, tc, c0,
-                                   (const int*)d_batom_.p, (const int*)d_bslot_.p, (const int*)d_bstruct_.p,
-                                   (const double*)d_G_.p, ldg_, nullptr, (const double*)d_dcdo_.p,
-                                   (double*)d_partial_.p, nullptr);
-            });
+            if (one_pass_) {
+                reduce_rows(tl, tl.tt, 0, tl.tiles, stream_, (const int*)d_tstruct_.p, (const double*)d_dcdo_.p);
+            } else {
+                sum_tiles(tl, stream_, [&](const TrainTiles& tc, int c0, int c1) {
+                    launch_tile<1>(gen_, dim3(c1 - c0), lds, stream_, tpp, Wm, par, tc, c0, ba, bsl, bst, Gp, ldg_,
+                                   nullptr, (const double*)d_dcdo_.p, (double*)d_partial_.p, nullptr);
+                });
+            }
         }
         std::vector<double> lv(nbatch), e2(nbatch);
         HIPCHK(hipMemcpyAsync(lv.data(), d_loss_.p, 8 * (size_t)nbatch, hipMemcpyDeviceToHost, stream_));
@@ -871,6 +968,7 @@ class TrainSet {
                     slot_s[t].push_back(i);
                     struct_s[t].push_back(b);
                 }
+            tiles_key_.clear();  // the energy path's cached tiles are overwritten
             const Tiles tl = make_tiles(atom_s, slot_s, struct_s, st);
             d_bstart_.reserve(4 * (size_t)(nb + 1));
             d_fstart_.reserve(8 * (size_t)nb);
@@ -960,6 +1058,12 @@ class TrainSet {
     bool has_forces_ = false;
     size_t lds_limit_ = 0;
     bool gen_ = false;  // some hidden layer is not tanh
+    // tiles of the last energy-training call (batch + mode) and whether they are single-structure
+    std::vector<int> tiles_key_;
+    Tiles tiles_;
+    bool one_pass_ = false;
+    DevMem d_tstruct_;
+    static constexpr int64_t kOnePassBytes = (int64_t)2 << 30;
     std::unique_ptr<Engine> ws_;  // workspace of the force passes
 };
 

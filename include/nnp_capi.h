@@ -167,7 +167,9 @@ int nnp_profile_read(nnp_model* m, double* ms, int64_t* calls, int nstage);
  *               wavefronts split a centre's exponents, radial functions and moment entries
  *               (moment path on fast-path models only)
  *   "net_bf16_terms"  mixed-precision networks: 3 = BF16 MFMA products of split operands
- *               (hi + lo, about 16 mantissa bits; default), 1 = single BF16 products */
+ *               (hi + lo, about 16 mantissa bits; default), 1 = single BF16 products
+ *   "train_one_pass"  energy training: 1 = one forward + backward pass over single-structure
+ *               tiles when it fits (default), 0 = a forward pass, then a pass with the loss seeds */
 int nnp_set_option(nnp_model* m, const char* key, int64_t value);
 /* Engine state: maxp, max entries, atoms, structures, kernel shape, NE, NC, descriptor width,
  * pipeline, fused blocks per CU (tuned), compute units, moment bits of the last pass. */

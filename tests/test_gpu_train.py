@@ -84,17 +84,22 @@ def test_training_leaves_the_model_unchanged_and_descends(pair):
     assert E1 == E0 and np.array_equal(Ea0, Ea1)
 
 
-def test_large_batch_gradient_in_chunks(potentials):
-    """32768 atoms of a three-species potential (84-64-64-1 networks): the gradient is summed in
-    chunks of tiles; it equals the sum of the per-structure gradients."""
+@pytest.mark.parametrize("one_pass", [1, 0])
+def test_large_batch_gradient_in_chunks(potentials, one_pass):
+    """32768 atoms of a three-species potential (84-64-64-1 networks): one pass over
+    single-structure tiles, or two passes with the rows summed in chunks of tiles; either equals
+    the sum of the per-structure gradients, and the two agree."""
     rng = np.random.default_rng(19)
     strucs = [s for s in synth.mixed64x4096(8)]
     eref = np.array([rng.normal(-3.0, 0.2) * s.natoms for s in strucs])
     m = NNP.read(potentials["mixed64x4096"])
     try:
+        m.set_option("train_one_pass", one_pass)
         t = Train(m, strucs, eref)
         x = t.params()
         err, g = t.error(x, list(range(8)), LOSS_HUBER, 0.05)
+        err2, g2 = t.error(x, list(range(8)), LOSS_HUBER, 0.05)  # tiles reused
+        assert np.array_equal(err, err2) and np.array_equal(g, g2)
         parts_e = np.zeros(2)
         parts_g = np.zeros_like(g)
         for b in range(8):
@@ -103,6 +108,35 @@ def test_large_batch_gradient_in_chunks(potentials):
             parts_g += gb
         assert np.allclose(err, parts_e, rtol=1e-12)
         assert np.abs(g - parts_g).max() <= 1e-11 * np.abs(g).max()
+        m.set_option("train_one_pass", 1 - one_pass)
+        e3, g3 = t.error(x, list(range(8)), LOSS_HUBER, 0.05)
+        assert np.allclose(err, e3, rtol=1e-12)
+        assert np.abs(g - g3).max() <= 1e-11 * np.abs(g).max()
+        t.close()
+    finally:
+        m.close()
+
+
+def test_many_small_structures_take_two_passes(potentials):
+    """40 structures of 3-6 atoms: single-structure tiles would mostly be padding, so the
+    species tiles across structures are used; the oracle bar holds."""
+    rng = np.random.default_rng(29)
+    strucs = []
+    for k in range(40):
+        n = int(rng.integers(3, 7))
+        box = synth.cubic_cell(n, 0.05)
+        strucs.append(synth.Structure(rng.integers(0, 2, n).astype(np.int32), synth.random_positions(n, box, 1.8, 300 + k),
+                                      box if k % 3 else None))
+    eref = np.array([rng.normal(-4.0, 0.5) * s.natoms for s in strucs])
+    m = NNP.read(potentials["plain4096"])
+    try:
+        t = Train(m, strucs, eref)
+        o = ob.OracleTrain(ob.OracleModel(potentials["plain4096"]), strucs, eref)
+        x = t.params() + rng.normal(0, 0.02, t.nparams())
+        e, g = t.error(x, list(range(40)), LOSS_MSE)
+        eo, go = o.error(x, list(range(40)), LOSS_MSE)
+        assert abs(e[0] - eo[0]) <= TOL * abs(eo[0])
+        assert np.abs(g - go).max() <= TOL * np.abs(go).max()
         t.close()
     finally:
         m.close()
