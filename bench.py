@@ -266,6 +266,18 @@ def main():
             "ops_per_launch": ops,
         },
     }
+    # the synchronous C ABI call with host buffers (upload of positions, the pass, download of
+    # energies and forces): the PCIe-inclusive rate, reported beside `value`
+    model.compute(strucs[0], flags)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        model.compute(strucs[0], flags)
+    hsec = (time.perf_counter() - t0) / args.steps
+    out["host_buffers"] = {
+        "call": "nnp_compute (host positions in; host energy, per-atom energies and forces out)",
+        "atoms_per_s": natoms / hsec,
+        "ms_per_call": hsec * 1e3,
+    }
     model_mixed = None
     if not args.no_mixed:
         mpath = os.path.join(tmp, "mixed64x4096.ann")
