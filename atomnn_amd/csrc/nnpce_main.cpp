@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "../../include/nnp_capi.h"
+#include "../../include/nnp_pot.hpp"
 
 namespace {
 
@@ -237,33 +238,32 @@ int main(int argc, char** argv) {
         for (const auto& fn : files)
             strucs.push_back(format == "XYZ" ? read_xyz(fn, s_len, s_energy) : read_poscar(fn, s_len));
 
-        nnp_model* m = nullptr;
-        if (nnp_load(pot.c_str(), &m) != NNP_OK) throw std::runtime_error(nnp_last_error());
+        atomnn::GpuPot gpu(pot);
         const size_t ns = strucs.size();
         std::vector<std::vector<int32_t>> types(ns);
         std::vector<std::vector<double>> eatom(ns), forces(ns);
         std::vector<nnp_structure> in(ns);
-        std::vector<nnp_result> out(ns);
+        std::vector<double*> ea_ptr(ns), f_ptr;
         for (size_t k = 0; k < ns; ++k) {
             const Struc& s = strucs[k];
             const int n = (int)s.names.size();
             for (const auto& nm : s.names) {
-                const int t = nnp_type_index(m, nm.c_str());
+                const int t = gpu.index(nm);
                 if (t < 0) throw std::runtime_error("No match for key in map");
                 types[k].push_back(t);
             }
             eatom[k].resize(n);
             forces[k].resize(3 * (size_t)n);
             in[k] = nnp_structure{n, types[k].data(), s.pos.data(), s.periodic ? s.cell : nullptr};
-            out[k] = nnp_result{0.0, eatom[k].data(), forces[k].data()};
+            ea_ptr[k] = eatom[k].data();
+            if (force) f_ptr.push_back(forces[k].data());
         }
-        const uint32_t flags = NNP_VACUUM_ENERGY | NNP_ATOM_ENERGY | (force ? NNP_FORCE : 0u);
-        if (ns > 0 && nnp_compute_batch(m, (int)ns, in.data(), flags, out.data()) != NNP_OK)
-            throw std::runtime_error(nnp_last_error());
+        std::vector<double> energy;
+        gpu.compute_batch(in, energy, ea_ptr, f_ptr);
         for (size_t k = 0; k < ns; ++k) {
             const double n = (double)strucs[k].names.size();
-            if (!norm) std::printf("energy %zu %.6f %.6f\n", k, strucs[k].energy, out[k].energy);
-            else std::printf("energy/atom %zu %.6f %.6f\n", k, strucs[k].energy / n, out[k].energy / n);
+            if (!norm) std::printf("energy %zu %.6f %.6f\n", k, strucs[k].energy, energy[k]);
+            else std::printf("energy/atom %zu %.6f %.6f\n", k, strucs[k].energy / n, energy[k] / n);
         }
         std::fflush(stdout);
         if (force) {
@@ -273,7 +273,6 @@ int main(int argc, char** argv) {
                     std::cout << "force[" << k << "] = " << eigen_row(&forces[k][3 * j]) << "\n";
             }
         }
-        nnp_free(m);
         const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         std::cout << "time = " << sec << "\n";
     } catch (std::exception& e) {
