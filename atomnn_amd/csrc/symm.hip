@@ -1588,9 +1588,34 @@ __global__ __launch_bounds__(kWave) void k_assemble(const DevStruc* __restrict__
                 const int64_t kj = nbr_j[rowj + idx];
                 return multi ? kj * st.nimg + (nbr_meta[rowj + idx] >> 8) : kj;
             };
+            // One image per neighbour (the key is the neighbour index): rows are sorted by key, so
+            // the mirror sits near want / natoms of the way along row j.  The 16 keys around that
+            // guess come in four aligned 16-byte loads; when they bracket want their count places
+            // it, otherwise the search below takes the side that holds it.
+            int lo = 0, hi = Pj;
+            if (!multi && Pj >= 16) {
+                const int g = (int)((want * Pj) / st.natoms);
+                const int w0 = min(max(0, g - 8), Pj - 16) & ~3;  // 4-aligned; w0 + 16 <= Pj + 3 <= maxp
+                const int4* kv = reinterpret_cast<const int4*>(nbr_j + rowj + w0);
+                int c = 0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int4 k4 = kv[q];
+                    const int b = w0 + 4 * q;
+                    c += (b + 0 < Pj && k4.x < want) + (b + 1 < Pj && k4.y < want) + (b + 2 < Pj && k4.z < want) +
+                         (b + 3 < Pj && k4.w < want);
+                }
+                const int w1 = min(Pj, w0 + 16);
+                if (c > 0 && (w0 + c < w1 || w1 == Pj)) {
+                    lo = hi = w0 + c;
+                } else if (c == 0) {
+                    hi = w0;
+                } else {
+                    lo = w1;
+                }
+            }
             // 8-ary search: 7 independent probes per level narrow [lo, hi) to one of 8 parts,
             // then the last <= 16 keys are counted; the first index with key >= want is lo
-            int lo = 0, hi = Pj;
             while (hi - lo > 16) {
                 const int len = hi - lo;
                 int c = 0;
