@@ -1588,22 +1588,31 @@ __global__ __launch_bounds__(kWave) void k_assemble(const DevStruc* __restrict__
                 const int64_t kj = nbr_j[rowj + idx];
                 return multi ? kj * st.nimg + (nbr_meta[rowj + idx] >> 8) : kj;
             };
-            // One image per neighbour (the key is the neighbour index): rows are sorted by key, so
-            // the mirror sits near want / natoms of the way along row j.  The 16 keys around that
-            // guess come in four aligned 16-byte loads; when they bracket want their count places
-            // it, otherwise the search below takes the side that holds it.
+            // Rows are sorted by key (neighbour index, then image), so the mirror sits near
+            // want / (natoms nimg) of the way along row j.  The 16 keys around that guess come in
+            // aligned 16-byte loads (indices, and image fields when there are several images);
+            // when they bracket want their count places it, otherwise the search below takes the
+            // side that holds it.
             int lo = 0, hi = Pj;
-            if (!multi && Pj >= 16) {
-                const int g = (int)((want * Pj) / st.natoms);
-                const int w0 = min(max(0, g - 8), Pj - 16) & ~3;  // 4-aligned; w0 + 16 <= Pj + 3 <= maxp
+            if (Pj >= 16) {
+                const int64_t kend = multi ? (int64_t)st.natoms * st.nimg : (int64_t)st.natoms;
+                const int g = (int)((want * Pj) / kend);
+                const int w0 = min(max(0, g - 8), Pj - 16) & ~3;  // 4-aligned, and w0 + 16 <= Pj
                 const int4* kv = reinterpret_cast<const int4*>(nbr_j + rowj + w0);
+                const int4* mv = reinterpret_cast<const int4*>(nbr_meta + rowj + w0);
                 int c = 0;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const int4 k4 = kv[q];
-                    const int b = w0 + 4 * q;
-                    c += (b + 0 < Pj && k4.x < want) + (b + 1 < Pj && k4.y < want) + (b + 2 < Pj && k4.z < want) +
-                         (b + 3 < Pj && k4.w < want);
+                    int64_t kx = k4.x, ky = k4.y, kz = k4.z, kw = k4.w;
+                    if (multi) {
+                        const int4 m4 = mv[q];
+                        kx = kx * st.nimg + (m4.x >> 8);
+                        ky = ky * st.nimg + (m4.y >> 8);
+                        kz = kz * st.nimg + (m4.z >> 8);
+                        kw = kw * st.nimg + (m4.w >> 8);
+                    }
+                    c += (kx < want) + (ky < want) + (kz < want) + (kw < want);
                 }
                 const int w1 = min(Pj, w0 + 16);
                 if (c > 0 && (w0 + c < w1 || w1 == Pj)) {
