@@ -244,7 +244,7 @@ __global__ __launch_bounds__(1024) void k_bins_small(const DevStruc* __restrict_
                                                      int* __restrict__ bin_start, int* __restrict__ bin_atoms) {
     __shared__ int cnt_l[kBinsSmallBins];
     __shared__ int start_l[kBinsSmallBins];
-    __shared__ int part[1024];
+    __shared__ int part[1024 / kWave];
     const int t = threadIdx.x;
     if (t == 0) {
         *maxcnt = 0;
@@ -267,15 +267,18 @@ __global__ __launch_bounds__(1024) void k_bins_small(const DevStruc* __restrict_
         loc[k] = sum;
         sum += b < nbins ? cnt_l[b] : 0;
     }
-    part[t] = sum;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-        const int y = t >= o ? part[t - o] : 0;
-        __syncthreads();
-        part[t] += y;
-        __syncthreads();
+    // inclusive scan inside each wavefront (shuffles), then the 16 wavefront totals
+    const int lane = t & (kWave - 1), wv = t / kWave;
+    int inc = sum;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int y = __shfl_up(inc, o, kWave);
+        if (lane >= o) inc += y;
     }
-    const int excl = part[t] - sum;
+    if (lane == kWave - 1) part[wv] = inc;
+    __syncthreads();
+    int excl = inc - sum;
+    for (int k = 0; k < wv; ++k) excl += part[k];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const int b = 4 * t + k;
@@ -530,25 +533,26 @@ __global__ void k_zero(int* __restrict__ maxcnt, int* __restrict__ work, int* __
 
 // Exclusive scan of n ints by one block (bin starts of the cell list).
 __global__ __launch_bounds__(1024) void k_scan_excl(const int* __restrict__ in, int* __restrict__ out, int n) {
-    __shared__ int part[1024];
-    __shared__ int carry;
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
+    // blocks of 1024: inclusive scans inside each wavefront (shuffles), then the 16 wavefront
+    // totals, then the running carry of the blocks before
+    __shared__ int part[1024 / kWave];
+    const int t = threadIdx.x, lane = t & (kWave - 1), wv = t / kWave;
+    int carry = 0;
     for (int base = 0; base < n; base += 1024) {
-        const int t = threadIdx.x;
         const int v = base + t < n ? in[base + t] : 0;
-        part[t] = v;
-        __syncthreads();
-        for (int o = 1; o < 1024; o <<= 1) {
-            const int y = t >= o ? part[t - o] : 0;
-            __syncthreads();
-            part[t] += y;
-            __syncthreads();
+        int inc = v;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const int y = __shfl_up(inc, o, kWave);
+            if (lane >= o) inc += y;
         }
-        if (base + t < n) out[base + t] = carry + part[t] - v;
+        if (lane == kWave - 1) part[wv] = inc;
         __syncthreads();
-        if (t == 0) carry += part[1023];
-        __syncthreads();
+        int off = carry;
+        for (int k = 0; k < wv; ++k) off += part[k];
+        if (base + t < n) out[base + t] = off + inc - v;
+        for (int k = 0; k < 1024 / kWave; ++k) carry += part[k];
+        __syncthreads();  // part is rewritten by the next block
     }
 }
 
