@@ -34,7 +34,7 @@ sys.path.insert(0, ROOT)
 
 from atomnn_amd import serve, synth  # noqa: E402
 from atomnn_amd import nnp as nnpmod  # noqa: E402
-from atomnn_amd.nnp import LOSS_MSE, NNP, NNP_FORCE, NNP_PREC_MIXED, NNP_VACUUM_ENERGY, Resident, Train  # noqa: E402
+from atomnn_amd.nnp import LOSS_MSE, MD, NNP, NNP_FORCE, NNP_PREC_MIXED, NNP_VACUUM_ENERGY, Resident, Train  # noqa: E402
 
 METRIC = "E+F atoms/s, mixed vs plain FP64 at ≤1 meV/atom; p50/p99 request latency"
 FP64_PEAK_TFLOPS = 78.6
@@ -180,6 +180,36 @@ def train_line(tmp, args):
     return line
 
 
+def md_line(model, s, args):
+    """Velocity-Verlet MD on the resident 4096-atom cell (half kick, drift and wrap, the full
+    pass, half kick, kinetic energy; one captured graph per step): steps/s from a 300 K
+    Maxwell-Boltzmann start, dt 0.5 fs.  (The synthetic cell is far from equilibrium, so it heats
+    quickly; energy conservation is a test, not a benchmark figure.)"""
+    rng = np.random.default_rng(11)
+    masses = [30.0, 60.0]
+    kT = 8.617333262e-5 * 300.0
+    mv2e = 103.642696  # eV per amu A^2 / fs^2
+    m_at = np.array(masses)[s.types]
+    vel = rng.normal(0.0, 1.0, (s.natoms, 3)) * np.sqrt(kT / (m_at * mv2e))[:, None]
+    vel -= vel.mean(axis=0)
+    md = MD(model, s, masses, vel, dt=0.5)
+    md.step(5)
+    e0 = md.state()
+    nsteps = max(50, args.steps * 5)
+    t0 = time.perf_counter()
+    md.step(nsteps)
+    st = md.state()  # waits for the steps
+    sec = time.perf_counter() - t0
+    md.close()
+    assert st["steps"] == e0["steps"] + nsteps
+    return {
+        "workload": f"plain4096 cell, velocity Verlet, dt 0.5 fs, {nsteps} steps after 5 warm-up steps, FP64",
+        "steps_per_s": nsteps / sec,
+        "atom_steps_per_s": nsteps * s.natoms / sec,
+        "us_per_step": sec / nsteps * 1e6,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -191,6 +221,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-train", action="store_true")
+    ap.add_argument("--no-md", action="store_true")
     ap.add_argument("--train-structures", type=int, default=64)
     ap.add_argument("--serve-requests", type=int, default=20000)
     ap.add_argument("--serve-pool", type=int, default=1024)
@@ -278,6 +309,8 @@ def main():
         "atoms_per_s": natoms / hsec,
         "ms_per_call": hsec * 1e3,
     }
+    if not args.no_md:
+        out["md"] = md_line(model, strucs[0], args)
     model_mixed = None
     if not args.no_mixed:
         mpath = os.path.join(tmp, "mixed64x4096.ann")
