@@ -30,26 +30,23 @@ namespace atomnn {
 constexpr double kAmuA2PerFs2 = 103.642696;  // eV per amu A^2 / fs^2
 constexpr double kBoltzmann = 8.617333262e-5;  // eV / K
 
-// v += c_i F, c_i = dt / (2 m_i) in A/fs per eV/A
-__global__ __launch_bounds__(256) void k_half_kick(double* __restrict__ v, const double* __restrict__ F,
-                                                  const double* __restrict__ c, int n) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const double ci = c[i];
-    v[3 * i] += ci * F[3 * i];
-    v[3 * i + 1] += ci * F[3 * i + 1];
-    v[3 * i + 2] += ci * F[3 * i + 2];
-}
-
 // math/special.hpp:46-47, mod(x, 0, 1)
 __device__ __forceinline__ double mod01(double x) { return fmod(fmod(x, 1.0) + 1.0, 1.0); }
 
-// x += v dt, then back into the cell (Cell::returnToCell, struc/cell.cpp:84-94)
-__global__ __launch_bounds__(256) void k_drift(double* __restrict__ x, const double* __restrict__ v, double dt,
-                                              int n, const DevStruc* __restrict__ S) {
+// The first half kick and the drift of a step in one launch: v += c_i F with c_i = dt / (2 m_i)
+// (A/fs per eV/A), then x += v dt and back into the cell (Cell::returnToCell,
+// struc/cell.cpp:84-94).
+__global__ __launch_bounds__(256) void k_kick_drift(double* __restrict__ x, double* __restrict__ v,
+                                                   const double* __restrict__ F, const double* __restrict__ c, double dt,
+                                                   int n, const DevStruc* __restrict__ S) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    double px = x[3 * i] + v[3 * i] * dt, py = x[3 * i + 1] + v[3 * i + 1] * dt, pz = x[3 * i + 2] + v[3 * i + 2] * dt;
+    const double ci = c[i];
+    const double vx = v[3 * i] + ci * F[3 * i], vy = v[3 * i + 1] + ci * F[3 * i + 1], vz = v[3 * i + 2] + ci * F[3 * i + 2];
+    v[3 * i] = vx;
+    v[3 * i + 1] = vy;
+    v[3 * i + 2] = vz;
+    double px = x[3 * i] + vx * dt, py = x[3 * i + 1] + vy * dt, pz = x[3 * i + 2] + vz * dt;
     const DevStruc& s = S[0];
     if (s.periodic) {
         const double f0 = mod01(s.RInv[0] * px + s.RInv[1] * py + s.RInv[2] * pz);
@@ -62,6 +59,57 @@ __global__ __launch_bounds__(256) void k_drift(double* __restrict__ x, const dou
     x[3 * i] = px;
     x[3 * i + 1] = py;
     x[3 * i + 2] = pz;
+}
+
+// The second half kick and the kinetic energy in one launch: each block kicks its 256 atoms and
+// sums m v^2 over them (fixed tree), the last block to finish adds the block sums in block order
+// (ke = 0.5 of that), folds the pass's neighbour maximum into the sticky maximum and resets the
+// block counter for the next step (no memset node in the captured step).
+__global__ __launch_bounds__(256) void k_kick_kinetic(double* __restrict__ v, const double* __restrict__ F,
+                                                     const double* __restrict__ c, const double* __restrict__ m, int n,
+                                                     double* __restrict__ bsum, unsigned* __restrict__ done,
+                                                     double* __restrict__ ke, const int* __restrict__ maxcnt,
+                                                     int* __restrict__ maxseen) {
+    __shared__ double part[256];
+    __shared__ bool last;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double s = 0.0;
+    if (i < n) {
+        const double ci = c[i];
+        const double vx = v[3 * i] + ci * F[3 * i], vy = v[3 * i + 1] + ci * F[3 * i + 1],
+                     vz = v[3 * i + 2] + ci * F[3 * i + 2];
+        v[3 * i] = vx;
+        v[3 * i + 1] = vy;
+        v[3 * i + 2] = vz;
+        s = m[i] * (vx * vx + vy * vy + vz * vz);
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        bsum[blockIdx.x] = part[0];
+        __threadfence();
+        last = atomicAdd(done, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double t = 0.0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += 256) t += __hip_atomic_load(&bsum[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    part[threadIdx.x] = t;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        *ke = 0.5 * part[0];
+        if (*maxcnt > *maxseen) *maxseen = *maxcnt;
+        *done = 0u;
+    }
 }
 
 // ke = 0.5 sum_i m_i v_i^2 (amu A^2/fs^2), one block, fixed-order tree sum; also tracks the
@@ -91,9 +139,13 @@ struct Md {
     double dt = 1.0;
     int n = 0;
     std::vector<double> mass;  // per atom, amu
-    DevMem d_vel, d_half, d_mass, d_ke, d_maxseen, d_save;
+    DevMem d_vel, d_half, d_mass, d_ke, d_maxseen, d_save, d_bsum, d_done;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // kStepsPerGraph consecutive steps in one graph: one launch instead of eight
+    static constexpr int kStepsPerGraph = 8;
+    hipGraph_t graph8 = nullptr;
+    hipGraphExec_t exec8 = nullptr;
     int graph_lmaxp = 0, graph_maxp = 0;
     bool eager_done = false;
     int64_t steps = 0;
@@ -102,8 +154,12 @@ struct Md {
     void drop() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
+        if (exec8) (void)hipGraphExecDestroy(exec8);
+        if (graph8) (void)hipGraphDestroy(graph8);
         exec = nullptr;
         graph = nullptr;
+        exec8 = nullptr;
+        graph8 = nullptr;
     }
     double* pos() { return (double*)eng->d_pos.p; }
     double* force() { return (double*)eng->d_F.p; }
@@ -111,17 +167,13 @@ struct Md {
     void enqueue_step() {
         Engine& E = *eng;
         const int nb = (n + 255) / 256;
-        hipLaunchKernelGGL(k_half_kick, dim3(nb), dim3(256), 0, E.stream, (double*)d_vel.p, force(),
-                           (const double*)d_half.p, n);
-        hipLaunchKernelGGL(k_drift, dim3(nb), dim3(256), 0, E.stream, pos(), (const double*)d_vel.p, dt, n,
-                           (const DevStruc*)E.d_strucs.p);
+        hipLaunchKernelGGL(k_kick_drift, dim3(nb), dim3(256), 0, E.stream, pos(), (double*)d_vel.p, force(),
+                           (const double*)d_half.p, dt, n, (const DevStruc*)E.d_strucs.p);
         HIPCHK(hipGetLastError());
         E.enqueue_pass(flags);
-        hipLaunchKernelGGL(k_half_kick, dim3(nb), dim3(256), 0, E.stream, (double*)d_vel.p, force(),
-                           (const double*)d_half.p, n);
-        hipLaunchKernelGGL(k_kinetic, dim3(1), dim3(256), 0, E.stream, (const double*)d_vel.p,
-                           (const double*)d_mass.p, n, (double*)d_ke.p, (const int*)E.d_maxcnt.p,
-                           (int*)d_maxseen.p);
+        hipLaunchKernelGGL(k_kick_kinetic, dim3(nb), dim3(256), 0, E.stream, (double*)d_vel.p, force(),
+                           (const double*)d_half.p, (const double*)d_mass.p, n, (double*)d_bsum.p,
+                           (unsigned*)d_done.p, (double*)d_ke.p, (const int*)E.d_maxcnt.p, (int*)d_maxseen.p);
         HIPCHK(hipGetLastError());
     }
 
@@ -155,6 +207,25 @@ struct Md {
         HIPCHK(hipGraphLaunch(exec, E.stream));
     }
 
+    // kStepsPerGraph steps (after the eager step has sized the buffers for these rows)
+    void step_block() {
+        Engine& E = *eng;
+        if (!exec8) {
+            HIPCHK(hipStreamBeginCapture(E.stream, hipStreamCaptureModeThreadLocal));
+            try {
+                for (int q = 0; q < kStepsPerGraph; ++q) enqueue_step();
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(E.stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            HIPCHK(hipStreamEndCapture(E.stream, &graph8));
+            HIPCHK(hipGraphInstantiate(&exec8, graph8, nullptr, nullptr, 0));
+        }
+        HIPCHK(hipGraphLaunch(exec8, E.stream));
+    }
+
     void run(int nsteps) {
         Engine& E = *eng;
         const size_t vb = (size_t)n * 24;
@@ -165,9 +236,17 @@ struct Md {
         HIPCHK(hipMemcpyAsync((char*)d_save.p + 2 * vb, force(), vb, hipMemcpyDeviceToDevice, E.stream));
         HIPCHK(hipMemsetAsync(d_maxseen.p, 0, 4, E.stream));
         for (int attempt = 0; attempt < 3; ++attempt) {
-            for (int k = 0; k < nsteps; ++k) {
-                step_once();
-                ++steps;
+            for (int k = 0; k < nsteps;) {
+                const bool ready = eager_done && graph_lmaxp == E.lmaxp && graph_maxp == E.maxp;
+                if (ready && nsteps - k >= kStepsPerGraph) {
+                    step_block();
+                    k += kStepsPerGraph;
+                    steps += kStepsPerGraph;
+                } else {
+                    step_once();
+                    ++k;
+                    ++steps;
+                }
             }
             int seen = 0;
             HIPCHK(hipMemcpyAsync(&seen, d_maxseen.p, 4, hipMemcpyDeviceToHost, E.stream));
@@ -258,6 +337,9 @@ int nnp_md_create(nnp_model* m, const nnp_structure* s, const double* masses, co
         md->d_mass.reserve((size_t)n * 8);
         md->d_ke.reserve(8);
         md->d_maxseen.reserve(4);
+        md->d_bsum.reserve((size_t)((n + 255) / 256) * 8);
+        md->d_done.reserve(4);
+        HIPCHK(hipMemsetAsync(md->d_done.p, 0, 4, E.stream));  // block counter; each step leaves it at 0
         HIPCHK(hipMemcpyAsync(md->d_half.p, half.data(), (size_t)n * 8, hipMemcpyHostToDevice, E.stream));
         HIPCHK(hipMemcpyAsync(md->d_mass.p, md->mass.data(), (size_t)n * 8, hipMemcpyHostToDevice, E.stream));
         if (vel) HIPCHK(hipMemcpyAsync(md->d_vel.p, vel, (size_t)n * 24, hipMemcpyHostToDevice, E.stream));

@@ -124,3 +124,26 @@ def test_md_rejects_bad_arguments(potentials):
             MD(m, s, MASSES, None, -1.0)
     finally:
         m.close()
+
+
+def test_md_steps_in_one_call_equal_single_steps(potentials):
+    """20 steps in one call (graphs of several steps) leave exactly the state of 20 one-step
+    calls (one step per graph)."""
+    m = NNP.read(potentials["plain4096"])
+    try:
+        s, rng = cell_300()
+        v0 = velocities(s, rng, 300.0)
+        a = MD(m, s, MASSES, v0, 0.5, NNP_FORCE)
+        b = MD(m, s, MASSES, v0, 0.5, NNP_FORCE)
+        a.step(20)
+        for _ in range(20):
+            b.step(1)
+        sa, sb = a.state(), b.state()
+        assert sa["steps"] == sb["steps"] == 20
+        for k in ("pos", "vel", "force"):
+            assert np.array_equal(sa[k], sb[k]), k
+        assert sa["epot"] == sb["epot"] and sa["ekin"] == sb["ekin"]
+        a.close()
+        b.close()
+    finally:
+        m.close()
