@@ -189,6 +189,8 @@ int nnp_info(nnp_model* m, int64_t* out, int nout);
 #define NNP_LOSS_HUBER 2 /* delta^2 (sqrt(1 + (dE/delta)^2) - 1) */
 #define NNP_LOSS_ASINH 3 /* delta^2 (1 - sqrt(1 + a^2) + a asinh(a)), a = dE/delta */
 typedef struct nnp_train nnp_train;
+/* The structures are copied; the model m must outlive t (t uses its input scalings, basis
+ * tables and force kernels).  One host thread per training set at a time. */
 int nnp_train_create(nnp_model* m, int nstruc, const nnp_structure* s, const double* energy, nnp_train** out);
 /* parameters of one species' network, or of all species (species < 0); negative on error */
 int64_t nnp_train_nparams(nnp_train* t, int species);
@@ -225,7 +227,7 @@ void nnp_train_destroy(nnp_train* t);
  * state has its own workspace.  nnp_md_step advances nsteps steps on the device (each step
  * one replayed graph; the host waits once per call).  nnp_md_state copies positions,
  * velocities and forces (any may be NULL) and energies[4] = {potential, kinetic (eV),
- * temperature (K), steps taken}. */
+ * temperature (K), steps taken}.  The model m must outlive the MD state. */
 typedef struct nnp_md nnp_md;
 int nnp_md_create(nnp_model* m, const nnp_structure* s, const double* masses, const double* vel, double dt,
                   uint32_t flags, nnp_md** out);
