@@ -1165,6 +1165,9 @@ int nnp_set_option(nnp_model* m, const char* key, int64_t value) {
             E.force_waves = (int)value;
             E.drop_graph();
             E.eager_runs = 0;
+        } else if (k == "train_cache_atoms") {
+            if (value < 0) throw std::invalid_argument("nnp_set_option: train_cache_atoms must be >= 0");
+            E.train_cache_atoms = value;
         } else if (k == "train_one_pass") {
             if (value != 0 && value != 1) throw std::invalid_argument("nnp_set_option: train_one_pass is 0 or 1");
             E.train_one_pass = (int)value;

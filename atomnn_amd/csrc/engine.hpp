@@ -139,6 +139,8 @@ class Engine {
     // energy training: one forward + backward pass over single-structure tiles when it fits (1),
     // or always a forward pass and a second pass with the loss seeds (0)
     int train_one_pass = 1;
+    // force training: atoms whose neighbour rows and descriptors are kept between calls
+    int64_t train_cache_atoms = (int64_t)1 << 20;
     int last_mom = 0;  // the moment bits the last pass used
     int mom_flag(size_t base_lds, bool fp64, int bits);
     int num_cus = 0;

@@ -917,8 +917,7 @@ class TrainSet {
         if (!has_forces_) throw std::invalid_argument("nnp_train_error_force: reference forces not set");
         if (!(beta >= 0)) throw std::invalid_argument("nnp_train_error_force: beta must be non-negative");
         if (!ws_) ws_ = std::make_unique<Engine>(owner_, 1001);
-        Engine& E = *ws_;
-        const hipStream_t st = E.stream;
+        const hipStream_t st = ws_->stream;  // every chunk's work is ordered on this stream
         const bool all_images = flags & NNP_ALL_IMAGES;
         pack_params(x, st);
         size_t lds = 0, flds = 0;
@@ -955,9 +954,38 @@ class TrainSet {
                 fstart[b] = aoff_[sb];
                 eb[b] = eref_[sb];
             }
-            E.upload(cs);
-            E.build_neighbors(all_images);
-            E.descriptors<double>();
+            // The structures do not move between calls: a chunk seen before keeps its neighbour
+            // rows and descriptors in a workspace of its own (up to the model's train_cache_atoms
+            // option in all, 2^20 by default);
+            // otherwise the scratch workspace builds them.
+            std::vector<int> key(batch + k0, batch + k1);
+            key.push_back(all_images ? 1 : 0);
+            Engine* Ep = nullptr;
+            for (auto& c : fcache_)
+                if (c.first == key) Ep = c.second.get();
+            if (!Ep) {
+                const bool keep = fcache_atoms_ + atoms <= owner_.train_cache_atoms;
+                if (keep) {
+                    fcache_.emplace_back(key, std::make_unique<Engine>(owner_, 1002));
+                    fcache_atoms_ += atoms;
+                    Ep = fcache_.back().second.get();
+                } else {
+                    Ep = ws_.get();
+                }
+                Engine& B = *Ep;
+                B.upload(cs);
+                B.build_neighbors(all_images);
+                B.descriptors<double>();
+                if (keep) HIPCHK(hipStreamSynchronize(B.stream));
+            }
+            Engine& E = *Ep;
+            // the chunk's kernels, the workspace's force kernels included, run in order on st
+            struct OnStream {
+                Engine& e;
+                hipStream_t own;
+                OnStream(Engine& w, hipStream_t s) : e(w), own(w.stream) { e.stream = s; }
+                ~OnStream() { e.stream = own; }
+            } on_stream(E, st);
             const int n = bstart[nb];
             // species tiles over the workspace atoms (source row = slot = workspace index)
             std::vector<std::vector<int>> atom_s(T_), slot_s(T_), struct_s(T_);
@@ -1065,6 +1093,9 @@ class TrainSet {
     DevMem d_tstruct_;
     static constexpr int64_t kOnePassBytes = (int64_t)2 << 30;
     std::unique_ptr<Engine> ws_;  // workspace of the force passes
+    // workspaces that keep the neighbour rows and descriptors of force-training chunks
+    std::vector<std::pair<std::vector<int>, std::unique_ptr<Engine>>> fcache_;
+    int64_t fcache_atoms_ = 0;
 };
 
 Engine& engine_of(nnp_model* m);

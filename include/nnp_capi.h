@@ -169,7 +169,10 @@ int nnp_profile_read(nnp_model* m, double* ms, int64_t* calls, int nstage);
  *   "net_bf16_terms"  mixed-precision networks: 3 = BF16 MFMA products of split operands
  *               (hi + lo, about 16 mantissa bits; default), 1 = single BF16 products
  *   "train_one_pass"  energy training: 1 = one forward + backward pass over single-structure
- *               tiles when it fits (default), 0 = a forward pass, then a pass with the loss seeds */
+ *               tiles when it fits (default), 0 = a forward pass, then a pass with the loss seeds
+ *   "train_cache_atoms"  force training keeps the neighbour rows and descriptors of the
+ *               structure chunks it has seen, up to this many atoms (default 2^20; 0 = rebuild
+ *               them every call) */
 int nnp_set_option(nnp_model* m, const char* key, int64_t value);
 /* Engine state: maxp, max entries, atoms, structures, kernel shape, NE, NC, descriptor width,
  * pipeline, fused blocks per CU (tuned), compute units, moment bits of the last pass. */

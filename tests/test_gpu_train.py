@@ -215,6 +215,27 @@ def test_force_training_triplet_path(potentials, fpair):
         m2.close()
 
 
+def test_force_training_cache_gives_the_same_results(potentials, fpair):
+    """Chunks whose neighbour rows and descriptors are kept between calls, and chunks rebuilt
+    every call (cache size 0), give identical losses and gradients."""
+    _, t, _, strucs, eref = fpair
+    x = t.params() + np.random.default_rng(31).normal(0, 0.02, t.nparams())
+    e1, g1 = t.error_force(x, [1, 4, 6], LOSS_MSE, 1.0, 2.0)
+    e1b, g1b = t.error_force(x, [1, 4, 6], LOSS_MSE, 1.0, 2.0)  # the cached chunk
+    m2 = NNP.read(potentials["plain4096"])
+    try:
+        m2.set_option("train_cache_atoms", 0)
+        t2 = Train(m2, strucs, eref)
+        t2.set_forces(reference_forces(ob.OracleModel(potentials["plain4096"]), strucs, 8))
+        e2, g2 = t2.error_force(x, [1, 4, 6], LOSS_MSE, 1.0, 2.0)
+        e2b, g2b = t2.error_force(x, [1, 4, 6], LOSS_MSE, 1.0, 2.0)
+        for a, b in ((e1, e1b), (e1, e2), (e1, e2b), (g1, g1b), (g1, g2), (g1, g2b)):
+            assert np.array_equal(a, b)
+        t2.close()
+    finally:
+        m2.close()
+
+
 def test_force_training_without_force_weight_is_energy_training(fpair):
     m, t, o, strucs, eref = fpair
     x = t.params()
