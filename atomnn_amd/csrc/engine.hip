@@ -1165,6 +1165,9 @@ int nnp_set_option(nnp_model* m, const char* key, int64_t value) {
             E.force_waves = (int)value;
             E.drop_graph();
             E.eager_runs = 0;
+        } else if (k == "train_chunk_elems") {
+            if (value < (1 << 16)) throw std::invalid_argument("nnp_set_option: train_chunk_elems must be >= 65536");
+            E.train_chunk_elems = value;
         } else if (k == "train_cache_atoms") {
             if (value < 0) throw std::invalid_argument("nnp_set_option: train_cache_atoms must be >= 0");
             E.train_cache_atoms = value;

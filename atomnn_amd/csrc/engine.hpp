@@ -141,6 +141,8 @@ class Engine {
     int train_one_pass = 1;
     // force training: atoms whose neighbour rows and descriptors are kept between calls
     int64_t train_cache_atoms = (int64_t)1 << 20;
+    // training: gradient-row elements (doubles) held at a time
+    int64_t train_chunk_elems = (int64_t)1 << 28;
     int last_mom = 0;  // the moment bits the last pass used
     int mom_flag(size_t base_lds, bool fp64, int bits);
     int num_cus = 0;

@@ -701,7 +701,8 @@ class TrainSet {
         return r;
     }
 
-    // Gradient rows of every tile, in chunks of tiles (at most ~128 MB of partial sums at a time):
+    // Gradient rows of every tile, in chunks of tiles (at most the model's train_chunk_elems row
+    // elements at a time, 2^28 = 2 GB by default):
     // launch(tc, c0, c1) writes the rows of tiles [c0, c1) at the shifted bases of tc; the rows are
     // then added into d_grad_ in a fixed order (groups of consecutive tiles, then the groups).
     template <typename F>
@@ -710,7 +711,7 @@ class TrainSet {
         int64_t maxnp = 1;
         for (int t = 0; t < T_; ++t) maxnp = std::max<int64_t>(maxnp, plan_.net[t].nparams);
         const int chunk = (int)std::max<int64_t>(
-            1, std::min<int64_t>({(int64_t)tl.tiles, ((int64_t)16 << 20) / maxnp, (int64_t)kReduceGroup * 65535}));
+            1, std::min<int64_t>({(int64_t)tl.tiles, owner_.train_chunk_elems / maxnp, (int64_t)kReduceGroup * 65535}));
         d_partial_.reserve(8 * (size_t)std::min<int64_t>(tl.pb, (int64_t)chunk * maxnp + maxnp));
         for (int c0 = 0; c0 < tl.tiles; c0 += chunk) {
             const int c1 = std::min(tl.tiles, c0 + chunk);
@@ -808,7 +809,7 @@ class TrainSet {
                         k = e;
                     }
                 }
-                one_pass_ = mode == 2 && th <= tm + tm / 8 + 8 && th * maxnp * 8 <= kOnePassBytes;
+                one_pass_ = mode == 2 && th <= tm + tm / 8 + 8 && th * maxnp <= owner_.train_chunk_elems;
             }
             if (one_pass_) {
                 // pad each (structure, species) run to whole tiles with rows marked -1
@@ -1091,7 +1092,6 @@ class TrainSet {
     Tiles tiles_;
     bool one_pass_ = false;
     DevMem d_tstruct_;
-    static constexpr int64_t kOnePassBytes = (int64_t)2 << 30;
     std::unique_ptr<Engine> ws_;  // workspace of the force passes
     // workspaces that keep the neighbour rows and descriptors of force-training chunks
     std::vector<std::pair<std::vector<int>, std::unique_ptr<Engine>>> fcache_;

@@ -172,7 +172,9 @@ int nnp_profile_read(nnp_model* m, double* ms, int64_t* calls, int nstage);
  *               tiles when it fits (default), 0 = a forward pass, then a pass with the loss seeds
  *   "train_cache_atoms"  force training keeps the neighbour rows and descriptors of the
  *               structure chunks it has seen, up to this many atoms (default 2^20; 0 = rebuild
- *               them every call) */
+ *               them every call)
+ *   "train_chunk_elems"  training: gradient-row elements (doubles) held at a time (default
+ *               2^28, 2 GB; batches with more rows are summed in chunks; at least 65536) */
 int nnp_set_option(nnp_model* m, const char* key, int64_t value);
 /* Engine state: maxp, max entries, atoms, structures, kernel shape, NE, NC, descriptor width,
  * pipeline, fused blocks per CU (tuned), compute units, moment bits of the last pass. */

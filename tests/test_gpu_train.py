@@ -95,6 +95,8 @@ def test_large_batch_gradient_in_chunks(potentials, one_pass):
     m = NNP.read(potentials["mixed64x4096"])
     try:
         m.set_option("train_one_pass", one_pass)
+        if not one_pass:
+            m.set_option("train_chunk_elems", 1 << 22)  # the rows of 2048 tiles in five chunks
         t = Train(m, strucs, eref)
         x = t.params()
         err, g = t.error(x, list(range(8)), LOSS_HUBER, 0.05)
