@@ -252,11 +252,18 @@ __global__ __launch_bounds__(1024) void k_bins_small(const DevStruc* __restrict_
     }
     for (int b = t; b < nbins; b += 1024) cnt_l[b] = 0;
     __syncthreads();
-#pragma unroll 4
-    for (int i = t; i < ntot; i += 1024) {
-        const int bin = bin_of(S, atom_struc, pos, i);
-        atom_bin[i] = bin;
-        if (bin >= 0) atomicAdd(&cnt_l[bin], 1);
+    // each thread's atoms i = t + 1024 k keep their bins in registers for the fill below
+    int mybin[kBinsSmallAtoms / 1024];
+#pragma unroll
+    for (int k = 0; k < kBinsSmallAtoms / 1024; ++k) {
+        const int i = t + 1024 * k;
+        mybin[k] = -1;
+        if (i < ntot) {
+            const int bin = bin_of(S, atom_struc, pos, i);
+            atom_bin[i] = bin;
+            mybin[k] = bin;
+            if (bin >= 0) atomicAdd(&cnt_l[bin], 1);
+        }
     }
     __syncthreads();
     // exclusive scan: four consecutive bins per thread, then a scan of the 1024 thread sums
@@ -292,12 +299,12 @@ __global__ __launch_bounds__(1024) void k_bins_small(const DevStruc* __restrict_
     __syncthreads();
     for (int b = t; b < nbins; b += 1024) cnt_l[b] = 0;  // now the fill counters
     __syncthreads();
-#pragma unroll 4
-    for (int i = t; i < ntot; i += 1024) {
-        const int b = atom_bin[i];  // written above by this thread
+#pragma unroll
+    for (int k = 0; k < kBinsSmallAtoms / 1024; ++k) {
+        const int b = mybin[k];
         if (b < 0) continue;
         const int slot = atomicAdd(&cnt_l[b], 1);
-        bin_atoms[start_l[b] + slot] = i;
+        bin_atoms[start_l[b] + slot] = t + 1024 * k;
     }
 }
 
