@@ -210,6 +210,20 @@ def md_line(model, s, args):
     }
 
 
+def valu_floor_from_profile(path, kernel):
+    """The kernel's VALU-issue floor (us per launch: its SQ_INSTS_VALU x 4 cycles over the 1024
+    SIMDs at 2.4 GHz) from the committed SQ-counter summary, or None."""
+    try:
+        with open(path) as f:
+            for line in f:
+                cells = [c.strip() for c in line.split("|")]
+                if len(cells) > 8 and cells[1] == kernel:
+                    return float(cells[7])
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -296,6 +310,12 @@ def main():
             "frac": achieved / FP64_PEAK_TFLOPS,
             "traffic": traffic,
             "ops_per_launch": ops,
+            # counter view: the VALU instructions the kernel issues need this long at one per
+            # 4 cycles on every SIMD; the launch time over it is the issue efficiency
+            "valu_issue_floor_us": valu_floor_from_profile(
+                os.path.join(ROOT, "profiles", "plain4096_fp64_sq_counters.md"),
+                "atomnn::k_force_mom<double, 4, 1>" if mom_forces else "atomnn::k_force<double, 4, 4, false>"),
+            "launch_us": t_force * 1e6,
         },
     }
     # the synchronous C ABI call with host buffers (upload of positions, the pass, download of
