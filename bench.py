@@ -11,9 +11,17 @@ stream synchronisations; a second, profiled pass of K steps brackets every stage
 events to obtain the force kernel's launch duration for the roofline line.
 
 Also reported:
-  * "mixed": the precision form (FP32 descriptors, BF16 MFMA networks, FP64 reductions) on
-    the 3-species 64 x 4096-atom batch beside the plain FP64 form on the same batch, with the
-    energy / force error of the mixed form against the FP64 result;
+  * "roofline": the force-term kernel's algorithmic FP64 throughput against the FP64 peak, its
+    HBM traffic and VALU-issue floor (from the committed rocprofv3 summaries in profiles/);
+  * "host_buffers": the synchronous nnp_compute call with host buffers (PCIe included);
+  * "md": velocity-Verlet steps per second on the resident cell;
+  * "mixed": the precision form (FP32 descriptors, split-BF16 MFMA networks, FP64 reductions)
+    on the 3-species 64 x 4096-atom batch beside the plain FP64 form on the same batch, with
+    the energy / force error of the mixed form against the FP64 result;
+  * "batching": 512 uneven triclinic cells in one pass (separate kernels, and the persistent
+    fused kernel);
+  * "training": error + gradient calls of the energy-training loop, and of force training;
+  * "serving": open-loop Poisson requests at fractions of the burst capacity, p50 / p99;
   * "cpu_baseline": the CPU oracle (a restatement of the reference evaluation loop, the
     reference itself cannot be built) timed on this host's cores on one 4096-atom structure.
 
