@@ -30,14 +30,15 @@ namespace atomnn {
 typedef float f4_t __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
 
-// LDS: two activation buffers [16][maxw] and the derivatives of every layer [16][wpad[l]]
+// LDS: two activation buffers [16][maxw], the derivatives of every layer [16][wpad[l]], then
+// (FP64 form) the input scale inpw [nin_pad] and the tile's 16 atom indices (16 slots)
 size_t net_lds_bytes(const DevNet& n, bool fp64) {
     int maxw = n.nin_pad, dsum = 0;
     for (int l = 0; l < n.L; ++l) {
         if (n.wpad[l] > maxw) maxw = n.wpad[l];
         dsum += n.wpad[l];
     }
-    return (size_t)(fp64 ? 8 : 4) * (size_t)(2 * 16 * maxw + 16 * dsum);
+    return (size_t)(fp64 ? 8 : 4) * (size_t)(2 * 16 * maxw + 16 * dsum + n.nin_pad + 16);
 }
 
 // Elements of the LDS copy of the weights-in-LDS form: every layer's matrix and bias.
@@ -47,6 +48,29 @@ static __device__ __forceinline__ int net_maxw(const DevNet& n) {
     int maxw = n.nin_pad;
     for (int l = 0; l < n.L; ++l) maxw = n.wpad[l] > maxw ? n.wpad[l] : maxw;
     return maxw;
+}
+
+// Activation of the 4 result values of a lane.  GEN = false: tanh hidden layers (and the linear
+// output layer) only, straight-line code whose four dependent FP64 / FP32 chains interleave;
+// GEN = true: any activation kind, one switch per value.
+template <bool GEN, typename T>
+__device__ __forceinline__ void net_act4(int act, const T (&z)[4], T (&a)[4], T (&d)[4]) {
+    if constexpr (GEN) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) act_eval<T>(act, z[v], a[v], d[v]);
+    } else if (act == NEU_LINEAR) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            a[v] = z[v];
+            d[v] = T(1);
+        }
+    } else {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            a[v] = tanh(z[v]);
+            d[v] = T(1) - a[v] * a[v];
+        }
+    }
 }
 
 // ------------------------------------------------------------------ FP64 (tile_mm_f64: netmm.hpp)
@@ -69,7 +93,7 @@ __device__ __forceinline__ int tile_species(const NetTiles& nt, int ntypes, int 
 // WL: the species' weight matrices are copied into LDS once per tile (all loads in flight
 // together), so the layers read weights from LDS instead of waiting on global loads layer by
 // layer (small batches, where the network kernel is a latency chain of one tile per CU).
-template <bool WL>
+template <bool WL, bool GEN>
 __global__ __launch_bounds__(kNetWaves * kWave) void k_net_f64(const DevPlan* __restrict__ plan, const double* __restrict__ Wb,
                                                    NetTiles nt, const int* __restrict__ spatoms,
                                                    const double* __restrict__ G, int ldg, int vac,
@@ -86,11 +110,18 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_f64(const DevPlan* __
     double* nxt = shd + 16 * maxw;
     double* der = shd + 32 * maxw;
     const int li = lane & 15, lk = lane >> 4;
+    int dsum = 0;
+    for (int l = 0; l < N.L; ++l) dsum += N.wpad[l];
+    double* sinw = der + 16 * dsum;                          // inpw, for dE/dG
+    int* sat = reinterpret_cast<int*>(sinw + N.nin_pad);     // atom index of each tile row, -1 past M
+    // input staging: thread (gr, gk) fills row gr, columns gk + 16 j; its atom index is loaded
+    // first, so that load is in flight together with the weight copy
+    static_assert(kNetThreads == 16 * 16, "input staging maps 16 threads to each of the 16 rows");
+    const int gr = tid >> 4, gk = tid & 15;
+    const int gat = r0 + gr < M ? atoms[r0 + gr] : -1;
     double* wl = nullptr;
     if constexpr (WL) {
-        int dsum = 0;
-        for (int l = 0; l < N.L; ++l) dsum += N.wpad[l];
-        wl = der + 16 * dsum;
+        wl = sinw + N.nin_pad + 16;
         // one flat 16-byte copy of [W_0 | W_1 | ... | b_0 | ...]: all loads in flight together
         const double2* src = reinterpret_cast<const double2*>(Wb + N.pack_off);
         double2* dst = reinterpret_cast<double2*>(wl);
@@ -108,16 +139,16 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_f64(const DevPlan* __
         else return Wb + N.boff[l];
     };
     // scaled input
-    for (int idx = tid; idx < 16 * N.nin_pad; idx += kNetThreads) {
-        const int r = idx / N.nin_pad, k = idx % N.nin_pad;
-        const int row = r0 + r;
-        double v = 0.0;
-        if (row < M && k < N.nin) {
-            const int at = atoms[row];
-            v = Wb[N.inpw_off + k] * (G[(int64_t)at * ldg + k] + Wb[N.inpb_off + k]);
+    for (int k = gk; k < N.nin_pad; k += 16) {
+        double v = 0.0, w = 0.0;
+        if (k < N.nin) {
+            w = Wb[N.inpw_off + k];
+            if (gat >= 0) v = w * (G[(int64_t)gat * ldg + k] + Wb[N.inpb_off + k]);
         }
-        cur[r * maxw + k] = v;
+        cur[gr * maxw + k] = v;
+        if (gr == 0) sinw[k] = w;
     }
+    if (gk == 0) sat[gr] = gat;
     __syncthreads();
     int in_pad = N.nin_pad;
     int doff = 0;
@@ -129,14 +160,16 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_f64(const DevPlan* __
         for (int jt = wv; jt < out_pad / 16; jt += kNetWaves) {
             const d4_t acc = tile_mm_f64<true>(cur, maxw, W, in_pad, in_pad, jt, li, lk);
             const int col = jt * 16 + li;
+            const double bc = b[col];
+            double z[4], av[4], dv[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) z[v] = acc[v] + bc;
+            net_act4<GEN>(act, z, av, dv);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int row = 4 * v + lk;
-                const double z = acc[v] + b[col];
-                double av, dv;
-                act_eval<double>(act, z, av, dv);
-                nxt[row * maxw + col] = av;
-                der[doff + row * out_pad + col] = dv;
+                nxt[row * maxw + col] = av[v];
+                der[doff + row * out_pad + col] = dv[v];
             }
         }
         __syncthreads();
@@ -147,9 +180,9 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_f64(const DevPlan* __
         doff += 16 * out_pad;
     }
     // energies
-    if (tid < 16 && r0 + tid < M) {
-        const int at = atoms[r0 + tid];
-        eatom[at] = N.outb + cur[tid * maxw] * N.outw + (vac ? N.evac : 0.0);
+    if (tid < 16) {
+        const int at = sat[tid];
+        if (at >= 0) eatom[at] = N.outb + cur[tid * maxw] * N.outw + (vac ? N.evac : 0.0);
     }
     // backward: delta_L = outw .* d_L (single output)
     const int last = N.wpad[N.L - 1];
@@ -191,8 +224,8 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_f64(const DevPlan* __
             const int col = jt * 16 + li;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int row = r0 + 4 * v + lk;
-                if (row < M && col < N.nin) dEdG[(int64_t)atoms[row] * ldg + col] = acc[v] * Wb[N.inpw_off + col];
+                const int at = sat[4 * v + lk];
+                if (at >= 0 && col < N.nin) dEdG[(int64_t)at * ldg + col] = acc[v] * sinw[col];
             }
         }
     }
@@ -228,7 +261,7 @@ __device__ __forceinline__ f4_t mfma_split(const float (&a)[8], const float (&b)
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc, 0, 0, 0);
 }
 
-template <int TERMS>
+template <int TERMS, bool GEN>
 __global__ __launch_bounds__(kNetWaves * kWave) void k_net_mixed(const DevPlan* __restrict__ plan, const float* __restrict__ Wf,
                                                      NetTiles nt, const int* __restrict__ spatoms,
                                                      const float* __restrict__ G, int ldg, int vac,
@@ -278,14 +311,16 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_mixed(const DevPlan* 
                 acc = mfma_split<TERMS>(a, bv, acc);
             }
             const int col = jt * 16 + li;
+            const float bc = b[col];
+            float z[4], av[4], dv[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) z[v] = acc[v] + bc;
+            net_act4<GEN>(act, z, av, dv);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int row = 4 * lk + v;
-                const float z = acc[v] + b[col];
-                float av, dv;
-                act_eval<float>(act, z, av, dv);
-                nxt[row * maxw + col] = av;
-                der[doff + row * out_pad + col] = dv;
+                nxt[row * maxw + col] = av[v];
+                der[doff + row * out_pad + col] = dv[v];
             }
         }
         __syncthreads();
@@ -378,19 +413,25 @@ hipError_t launch_net(bool fp64, const DevPlan* plan, const DevPlan& hplan, cons
         lds = std::max(lds, net_lds_bytes(hplan.net[s], fp64));
         ldsw = std::max(ldsw, net_lds_bytes(hplan.net[s], fp64) + 8 * (size_t)net_weight_elems(hplan.net[s]));
     }
+    // the tanh-only kernels when every species' hidden layers are tanh
+    bool gen = false;
+    for (int s = 0; s < hplan.T; ++s) gen = gen || hplan.net[s].act != NEU_TANH;
     // weights in LDS: a few tiles per CU at most, and the copy fits beside the activations
-    if (fp64 && ntiles <= wl_tiles_max && ldsw <= 65536)
-        hipLaunchKernelGGL(k_net_f64<true>, dim3(ntiles), dim3(kNetThreads), ldsw, st, plan, (const double*)W, nt,
-                           spatoms, (const double*)G, ldg, vac, eatom, (double*)dEdG);
-    else if (fp64)
-        hipLaunchKernelGGL(k_net_f64<false>, dim3(ntiles), dim3(kNetThreads), lds, st, plan, (const double*)W, nt,
-                           spatoms, (const double*)G, ldg, vac, eatom, (double*)dEdG);
+    const bool wl = fp64 && ntiles <= wl_tiles_max && ldsw <= 65536;
+    const size_t bytes = wl ? ldsw : lds;
+    auto kern = [&](auto k) { return (const void*)k; };
+    const void* f;
+    if (fp64)
+        f = wl ? (gen ? kern(k_net_f64<true, true>) : kern(k_net_f64<true, false>))
+               : (gen ? kern(k_net_f64<false, true>) : kern(k_net_f64<false, false>));
     else if (bf16_terms == 1)
-        hipLaunchKernelGGL(k_net_mixed<1>, dim3(ntiles), dim3(kNetThreads), lds, st, plan, (const float*)W, nt, spatoms,
-                           (const float*)G, ldg, vac, eatom, (float*)dEdG);
+        f = gen ? kern(k_net_mixed<1, true>) : kern(k_net_mixed<1, false>);
     else
-        hipLaunchKernelGGL(k_net_mixed<3>, dim3(ntiles), dim3(kNetThreads), lds, st, plan, (const float*)W, nt, spatoms,
-                           (const float*)G, ldg, vac, eatom, (float*)dEdG);
+        f = gen ? kern(k_net_mixed<3, true>) : kern(k_net_mixed<3, false>);
+    void* args[] = {(void*)&plan, (void*)&W, (void*)&nt, (void*)&spatoms, (void*)&G, (void*)&ldg, (void*)&vac,
+                    (void*)&eatom, (void*)&dEdG};
+    const hipError_t e = hipLaunchKernel(f, dim3(ntiles), dim3(kNetThreads), args, bytes, st);
+    if (e != hipSuccess) return e;
     return hipGetLastError();
 }
 
