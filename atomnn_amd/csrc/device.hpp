@@ -41,4 +41,41 @@ struct DevStruc {
 
 constexpr int kWave = 64;
 
+// End-of-pass reductions, run by single-wavefront blocks appended to the assembly launch (or on
+// their own when no forces are computed): the total energy of each structure, and the largest
+// neighbour count of the pass (the row-overflow check of resident passes).
+struct EnergyTail {
+    const double* eatom;  // E_i, batch order
+    const int* sstart;    // first atom of each structure, nstruc + 1 entries
+    double* E;            // total energy of each structure
+    int nstruc;
+    int ncount;           // count blocks after the energy blocks (0: no count check)
+    const int* cnt;
+    int ntot;
+    int* maxcnt;
+};
+
+// Block b of the tail (one wavefront).  Energy: lane t sums its contiguous chunk of the
+// structure's E_i in order, then a fixed shuffle tree; the sum depends only on that structure's
+// E_i, so batch and single passes agree bit for bit.  Count: strided slices, one atomicMax each.
+__device__ __forceinline__ void energy_tail_block(const EnergyTail& t, int b, int lane) {
+    if (b < t.nstruc) {
+        const int a0 = t.sstart[b], a1 = t.sstart[b + 1];
+        const int n = a1 - a0, chunk = (n + kWave - 1) / kWave;
+        const int i0 = a0 + lane * chunk, i1 = a0 + min(n, (lane + 1) * chunk);
+        double acc = 0.0;
+        for (int i = i0; i < i1; ++i) acc += t.eatom[i];
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) acc += __shfl_down(acc, o);
+        if (lane == 0) t.E[b] = acc;
+        return;
+    }
+    const int c = b - t.nstruc;
+    int m = 0;
+    for (int i = c * kWave + lane; i < t.ntot; i += t.ncount * kWave) m = max(m, t.cnt[i]);
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) m = max(m, __shfl_down(m, o));
+    if (lane == 0) atomicMax(t.maxcnt, m);
+}
+
 }  // namespace atomnn

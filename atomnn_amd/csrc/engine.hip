@@ -48,7 +48,7 @@ hipError_t launch_force_mom(int shape, int waves, const DevPlan* plan, const dou
                             hipStream_t st);
 template <typename T>
 hipError_t launch_assemble(const DevStruc* S, const int* atom_struc, const int* cnt, const int* nbr_j, const int* meta,
-                           const T* X, int maxp, int ntot, int all_images, double* F, hipStream_t st);
+                           const T* X, int maxp, int ntot, int all_images, double* F, const EnergyTail* tail, hipStream_t st);
 hipError_t launch_net(bool fp64, const DevPlan* plan, const DevPlan& hplan, const void* W, const int* sp_off,
                       const int* spatoms, const void* G, int ldg, int vac, double* eatom, void* dEdG, int bf16_terms,
                       int wl_tiles_max, hipStream_t st);
@@ -61,40 +61,8 @@ hipError_t launch_fused(int shape, int ns, int nblocks, const DevPlan* plan, con
                         const double4* r, int maxp, int lmaxp, int ldg, int maxw, int dersum, int vac, int do_force,
                         double* eatom, T* X, int use_mom, int ntypes, hipStream_t st);
 
-// Total energy per structure: one block per structure, fixed-order tree sum of E_i (FP64).
-// With cnt given, the blocks past the structures fold the pass's largest neighbour count into
-// *maxcnt (the row-overflow check of resident passes, which need it only after the pass).
-__global__ __launch_bounds__(256) void k_energy(const double* __restrict__ eatom, const int* __restrict__ sstart,
-                                                double* __restrict__ E, int nstruc, const int* __restrict__ cnt,
-                                                int ntot, int* __restrict__ maxcnt) {
-    __shared__ double part[256];
-    const int s = blockIdx.x, t = threadIdx.x;
-    if (cnt && s >= nstruc) {  // the count blocks: strided slices, one atomicMax each
-        __shared__ int pm[256];
-        const int nb = (int)gridDim.x - nstruc, b = s - nstruc;
-        int m = 0;
-        for (int i = b * 256 + t; i < ntot; i += nb * 256) m = max(m, cnt[i]);
-        pm[t] = m;
-        __syncthreads();
-        for (int o = 128; o > 0; o >>= 1) {
-            if (t < o) pm[t] = max(pm[t], pm[t + o]);
-            __syncthreads();
-        }
-        if (t == 0) atomicMax(maxcnt, pm[0]);
-        return;
-    }
-    const int a0 = sstart[s], a1 = sstart[s + 1];
-    const int n = a1 - a0, chunk = (n + 255) / 256;
-    double acc = 0.0;
-    for (int i = a0 + t * chunk; i < a0 + min(n, (t + 1) * chunk); ++i) acc += eatom[i];
-    part[t] = acc;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (t < o) part[t] += part[t + o];
-        __syncthreads();
-    }
-    if (t == 0) E[s] = part[0];
-}
+// The end-of-pass reductions on their own (passes without forces): one wavefront per block.
+__global__ __launch_bounds__(kWave) void k_energy(EnergyTail t) { energy_tail_block(t, blockIdx.x, threadIdx.x); }
 
 // ------------------------------------------------------------------ errors
 static thread_local std::string g_err;
@@ -530,11 +498,11 @@ void Engine::force_terms() {
 }
 
 template <typename T>
-void Engine::assemble() {
+void Engine::assemble(const EnergyTail* tail) {
     d_F.reserve((size_t)ntot * 3 * 8);
     HIPCHK(launch_assemble<T>((const DevStruc*)d_strucs.p, (const int*)d_astruc.p, (const int*)d_cnt.p,
                               (const int*)d_nbrj.p, (const int*)d_meta.p, (const T*)d_X.p, maxp, ntot,
-                              all_images_ ? 1 : 0, (double*)d_F.p, stream));
+                              all_images_ ? 1 : 0, (double*)d_F.p, tail, stream));
 }
 
 template <typename T>
@@ -555,11 +523,11 @@ void Engine::run(const std::vector<nnp_structure>& ss, uint32_t flags) {
             if (mixed)
                 HIPCHK(launch_assemble<float>((const DevStruc*)d_strucs.p, (const int*)d_astruc.p, (const int*)d_cnt.p,
                                               (const int*)d_nbrj.p, (const int*)d_meta.p, (const float*)d_X.p, maxp,
-                                              ntot, ai, (double*)d_F.p, stream));
+                                              ntot, ai, (double*)d_F.p, nullptr, stream));
             else
                 HIPCHK(launch_assemble<double>((const DevStruc*)d_strucs.p, (const int*)d_astruc.p,
                                                (const int*)d_cnt.p, (const int*)d_nbrj.p, (const int*)d_meta.p,
-                                               (const double*)d_X.p, maxp, ntot, ai, (double*)d_F.p, stream));
+                                               (const double*)d_X.p, maxp, ntot, ai, (double*)d_F.p, nullptr, stream));
         }
         return;
     }
@@ -692,12 +660,23 @@ void Engine::profile_collect() {
     }
 }
 
-void Engine::energies(bool count_max) {
+EnergyTail Engine::energy_tail(bool count_max) {
     d_E.reserve((size_t)nstruc * 8);
-    const int ncount = count_max ? std::min(256, (ntot + 4095) / 4096) : 0;
-    hipLaunchKernelGGL(k_energy, dim3(nstruc + ncount), dim3(256), 0, stream, (const double*)d_eatom.p,
-                       (const int*)d_sstart.p, (double*)d_E.p, nstruc, count_max ? (const int*)d_cnt.p : nullptr,
-                       ntot, (int*)d_maxcnt.p);
+    EnergyTail t;
+    t.eatom = (const double*)d_eatom.p;
+    t.sstart = (const int*)d_sstart.p;
+    t.E = (double*)d_E.p;
+    t.nstruc = nstruc;
+    t.ncount = count_max ? std::min(1024, (ntot + 1023) / 1024) : 0;
+    t.cnt = (const int*)d_cnt.p;
+    t.ntot = ntot;
+    t.maxcnt = (int*)d_maxcnt.p;
+    return t;
+}
+
+void Engine::energies(bool count_max) {
+    const EnergyTail t = energy_tail(count_max);
+    hipLaunchKernelGGL(k_energy, dim3(t.nstruc + t.ncount), dim3(kWave), 0, stream, t);
     HIPCHK(hipGetLastError());
 }
 
@@ -804,17 +783,19 @@ void Engine::enqueue_pass(uint32_t flags) {
         fused(mixed, flags, fused_nb);
         stage_end(ST_SYMM);
         if (flags & NNP_FORCE) {
-            stage_begin(ST_ASSEMBLE);
+            stage_begin(ST_ASSEMBLE);  // forces, energies and the count check in one launch
             const int ai = (flags & NNP_ALL_IMAGES) ? 1 : 0;
+            const EnergyTail tail = energy_tail(true);
             if (mixed)
                 HIPCHK(launch_assemble<float>((const DevStruc*)d_strucs.p, (const int*)d_astruc.p, (const int*)d_cnt.p,
                                               (const int*)d_nbrj.p, (const int*)d_meta.p, (const float*)d_X.p, maxp,
-                                              ntot, ai, (double*)d_F.p, stream));
+                                              ntot, ai, (double*)d_F.p, &tail, stream));
             else
                 HIPCHK(launch_assemble<double>((const DevStruc*)d_strucs.p, (const int*)d_astruc.p,
                                                (const int*)d_cnt.p, (const int*)d_nbrj.p, (const int*)d_meta.p,
-                                               (const double*)d_X.p, maxp, ntot, ai, (double*)d_F.p, stream));
+                                               (const double*)d_X.p, maxp, ntot, ai, (double*)d_F.p, &tail, stream));
             stage_end(ST_ASSEMBLE);
+            return;
         }
         stage_begin(ST_ENERGY);
         energies(true);
@@ -833,10 +814,12 @@ void Engine::enqueue_pass(uint32_t flags) {
         if (mixed) force_terms<float>();
         else force_terms<double>();
         stage_end(ST_FORCE);
-        stage_begin(ST_ASSEMBLE);
-        if (mixed) assemble<float>();
-        else assemble<double>();
+        stage_begin(ST_ASSEMBLE);  // forces, energies and the count check in one launch
+        const EnergyTail tail = energy_tail(true);
+        if (mixed) assemble<float>(&tail);
+        else assemble<double>(&tail);
         stage_end(ST_ASSEMBLE);
+        return;
     }
     stage_begin(ST_ENERGY);
     energies(true);

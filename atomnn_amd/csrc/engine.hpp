@@ -99,8 +99,10 @@ class Engine {
     void forces();
     template <typename T>
     void force_terms();
+    // forces of the pass from the per-entry terms; with tail, the end-of-pass reductions run in
+    // the same launch
     template <typename T>
-    void assemble();
+    void assemble(const EnergyTail* tail = nullptr);
     void run(const std::vector<nnp_structure>& ss, uint32_t flags);
     void fetch(const std::vector<nnp_structure>& ss, uint32_t flags, nnp_result* res);
 
@@ -108,6 +110,7 @@ class Engine {
     void resident_load(const std::vector<nnp_structure>& ss, uint32_t flags);
     void resident_run(uint32_t flags);
     void resident_fetch(uint32_t flags, nnp_result* res);
+    EnergyTail energy_tail(bool count_max);
     void energies(bool count_max = false);
 
     // stage timing with HIP events on the engine's stream

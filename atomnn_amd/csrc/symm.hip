@@ -1559,9 +1559,15 @@ template <typename T>
 __global__ __launch_bounds__(kWave) void k_assemble(const DevStruc* __restrict__ S, const int* __restrict__ atom_struc,
                                                     const int* __restrict__ cnt, const int* __restrict__ nbr_j,
                                                     const int* __restrict__ nbr_meta, const T* __restrict__ X, int maxp,
-                                                    int all_images, double* __restrict__ F) {
-    const int a = blockIdx.x;
+                                                    int all_images, double* __restrict__ F, EnergyTail tail) {
+    // the end-of-pass reduction blocks come first, so they run beside the atoms' blocks
+    const int ntail = tail.nstruc + tail.ncount;
     const int lane = threadIdx.x;
+    if ((int)blockIdx.x < ntail) {
+        energy_tail_block(tail, blockIdx.x, lane);
+        return;
+    }
+    const int a = blockIdx.x - ntail;
     const DevStruc& st = S[atom_struc[a]];
     const int la = a - st.atom0;
     const int P = min(cnt[a], maxp);
@@ -1818,9 +1824,12 @@ template hipError_t launch_force_mom<float>(int, int, const DevPlan*, const doub
 
 template <typename T>
 hipError_t launch_assemble(const DevStruc* S, const int* atom_struc, const int* cnt, const int* nbr_j, const int* meta,
-                           const T* X, int maxp, int ntot, int all_images, double* F, hipStream_t st) {
-    hipLaunchKernelGGL((k_assemble<T>), dim3(ntot), dim3(kWave), 0, st, S, atom_struc, cnt, nbr_j, meta, X, maxp,
-                       all_images, F);
+                           const T* X, int maxp, int ntot, int all_images, double* F, const EnergyTail* tail,
+                           hipStream_t st) {
+    EnergyTail t{};
+    if (tail) t = *tail;  // without a tail, nstruc = ncount = 0: no extra blocks
+    hipLaunchKernelGGL((k_assemble<T>), dim3(ntot + t.nstruc + t.ncount), dim3(kWave), 0, st, S, atom_struc, cnt, nbr_j,
+                       meta, X, maxp, all_images, F, t);
     return hipGetLastError();
 }
 
@@ -1837,8 +1846,8 @@ template hipError_t launch_force<float>(int, int, const DevPlan*, const double*,
                                         int, const int*, const int*, const double4*, int, int, const float*, int,
                                         float*, int, int, hipStream_t);
 template hipError_t launch_assemble<double>(const DevStruc*, const int*, const int*, const int*, const int*,
-                                            const double*, int, int, int, double*, hipStream_t);
+                                            const double*, int, int, int, double*, const EnergyTail*, hipStream_t);
 template hipError_t launch_assemble<float>(const DevStruc*, const int*, const int*, const int*, const int*, const float*,
-                                           int, int, int, double*, hipStream_t);
+                                           int, int, int, double*, const EnergyTail*, hipStream_t);
 
 }  // namespace atomnn

@@ -306,8 +306,9 @@ def main():
             "neighbor_entries": P,
             "triplets": T,
         },
+        # with forces, the per-structure energy sums run as extra blocks of the assembly launch
         "stage_ms": {k: round(float(v), 5) for k, v in
-                     zip(["neighbors", "descriptors", "networks", "force_terms", "force_assembly", "energy"], stage)},
+                     zip(["neighbors", "descriptors", "networks", "force_terms", "force_assembly_and_energy"], stage)},
         "roofline": {
             "kernel": ("k_force_mom<double,4,1> (per-entry force terms, moment expansion)" if mom_forces
                        else "k_force<double,4,4> (per-entry force terms, triplet loops)"),
