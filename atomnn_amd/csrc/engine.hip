@@ -392,6 +392,7 @@ void Engine::build_neighbors(bool all_images) {
         a.pos = (const double*)d_pos.p;
         a.type = (const int*)d_type.p;
         a.ntot = ntot;
+        a.nstruc = nstruc;
         a.rc2 = host.rc * host.rc;
         a.maxp = maxp;
         a.cnt = (int*)d_cnt.p;
@@ -754,6 +755,7 @@ void Engine::enqueue_pass(uint32_t flags) {
     a.pos = (const double*)d_pos.p;
     a.type = (const int*)d_type.p;
     a.ntot = ntot;
+    a.nstruc = nstruc;
     a.rc2 = host.rc * host.rc;
     a.maxp = maxp;
     a.cnt = (int*)d_cnt.p;

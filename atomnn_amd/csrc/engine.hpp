@@ -56,6 +56,7 @@ struct NeighborArgs {
     const double* pos;
     const int* type;
     int ntot;
+    int nstruc = 0;  // structures in the batch (1: every atom in S[0]; 0: not stated)
     double rc2;
     int maxp;
     int* cnt;

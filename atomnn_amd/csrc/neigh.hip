@@ -206,9 +206,7 @@ __global__ __launch_bounds__(kWave) void k_neigh_allpairs(const DevStruc* __rest
 
 // ------------------------------------------------------------------ cell list
 // Bin of atom i (fractional coordinates wrapped into [0,1)), or -1 outside the cell list.
-__device__ __forceinline__ int bin_of(const DevStruc* __restrict__ S, const int* __restrict__ atom_struc,
-                                      const double* __restrict__ pos, int i) {
-    const DevStruc& s = S[atom_struc[i]];
+__device__ __forceinline__ int bin_in(const DevStruc& s, const double* __restrict__ pos, int i) {
     if (s.mode != 1) return -1;
     const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
     int b[3];
@@ -221,6 +219,10 @@ __device__ __forceinline__ int bin_of(const DevStruc* __restrict__ S, const int*
         b[r] = q;
     }
     return s.bin0 + (b[0] * s.nbin[1] + b[1]) * s.nbin[2] + b[2];
+}
+__device__ __forceinline__ int bin_of(const DevStruc* __restrict__ S, const int* __restrict__ atom_struc,
+                                      const double* __restrict__ pos, int i) {
+    return bin_in(S[atom_struc[i]], pos, i);
 }
 
 __global__ void k_bin_atoms(const DevStruc* __restrict__ S, const int* __restrict__ atom_struc,
@@ -235,8 +237,10 @@ __global__ void k_bin_atoms(const DevStruc* __restrict__ S, const int* __restric
 
 // Small batches: the counter reset, binning, bin scan and scatter of the four kernels below in
 // one workgroup with the bin counters in LDS (one launch instead of four at a few thousand atoms).
+// ONE: a single structure, whose cell is read once per wavefront (scalar loads) instead of per atom.
 constexpr int kBinsSmallAtoms = 32768;
 constexpr int kBinsSmallBins = 4096;
+template <bool ONE>
 __global__ __launch_bounds__(1024) void k_bins_small(const DevStruc* __restrict__ S, const int* __restrict__ atom_struc,
                                                      const double* __restrict__ pos, int ntot, int nbins,
                                                      int* __restrict__ maxcnt, int* __restrict__ work,
@@ -259,7 +263,7 @@ __global__ __launch_bounds__(1024) void k_bins_small(const DevStruc* __restrict_
         const int i = t + 1024 * k;
         mybin[k] = -1;
         if (i < ntot) {
-            const int bin = bin_of(S, atom_struc, pos, i);
+            const int bin = ONE ? bin_in(S[0], pos, i) : bin_of(S, atom_struc, pos, i);
             atom_bin[i] = bin;
             mybin[k] = bin;
             if (bin >= 0) atomicAdd(&cnt_l[bin], 1);
@@ -573,8 +577,12 @@ hipError_t launch_neighbors(const NeighborArgs& a, hipStream_t st) {
     hipError_t e;
     const bool small = a.ncell > 0 && a.ntot <= kBinsSmallAtoms && a.nbins <= kBinsSmallBins;
     if (small) {
-        hipLaunchKernelGGL(k_bins_small, dim3(1), dim3(1024), 0, st, a.S, a.atom_struc, a.pos, a.ntot, a.nbins,
-                           a.maxcnt, a.work, a.atom_bin, a.bin_count, a.bin_start, a.bin_atoms);
+        if (a.nstruc == 1)
+            hipLaunchKernelGGL(k_bins_small<true>, dim3(1), dim3(1024), 0, st, a.S, a.atom_struc, a.pos, a.ntot,
+                               a.nbins, a.maxcnt, a.work, a.atom_bin, a.bin_count, a.bin_start, a.bin_atoms);
+        else
+            hipLaunchKernelGGL(k_bins_small<false>, dim3(1), dim3(1024), 0, st, a.S, a.atom_struc, a.pos, a.ntot,
+                               a.nbins, a.maxcnt, a.work, a.atom_bin, a.bin_count, a.bin_start, a.bin_atoms);
     } else {
         const int nz = a.ncell > 0 ? a.nbins : 0;
         hipLaunchKernelGGL(k_zero, dim3((nz + 255) / 256 + 1), dim3(256), 0, st, a.maxcnt, a.work, a.bin_count,
