@@ -133,6 +133,15 @@ __device__ __forceinline__ void ang_pair(int kind, T aj, T ak, T& F, T& t) {
     }
 }
 
+// tanh z = -m / (2 + m) with m = expm1(-2|z|), sign of z: within 3 ulp of the library tanh
+// (expm1 keeps the relative accuracy near 0) in about a third of its FP64 instructions.  The
+// tanh-only network, fused and training kernels use it; FP32 keeps the library tanhf.
+__device__ __forceinline__ double tanh_m1(double z) {
+    const double m = expm1(-2.0 * fabs(z));
+    return copysign(-m / (2.0 + m), z);
+}
+__device__ __forceinline__ float tanh_m1(float z) { return tanhf(z); }
+
 // Hidden-layer activation a(z) and derivative d(z) (AFFPBP, ml/nn.cpp:144-766).
 template <typename T>
 __device__ __forceinline__ void act_eval(int kind, T z, T& a, T& d) {

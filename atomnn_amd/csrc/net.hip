@@ -50,14 +50,6 @@ static __device__ __forceinline__ int net_maxw(const DevNet& n) {
     return maxw;
 }
 
-// tanh z = -m / (2 + m) with m = expm1(-2|z|), sign of z: a few ulp everywhere (expm1 keeps the
-// relative accuracy near 0) in about a third of the instructions of the library tanh.
-__device__ __forceinline__ double tanh_m1(double z) {
-    const double m = expm1(-2.0 * fabs(z));
-    return copysign(-m / (2.0 + m), z);
-}
-__device__ __forceinline__ float tanh_m1(float z) { return tanhf(z); }
-
 // Activation of the 4 result values of a lane.  GEN = false: tanh hidden layers (and the linear
 // output layer) only, straight-line code whose four dependent FP64 / FP32 chains interleave;
 // GEN = true: any activation kind, one switch per value.

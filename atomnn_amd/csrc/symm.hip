@@ -1154,7 +1154,7 @@ __device__ __forceinline__ double net_body(const DevNet& N, const T* __restrict_
                 a = z;
                 d = T(1);
             } else {
-                a = tanh(z);
+                a = tanh_m1(z);
                 d = T(1) - a * a;
             }
             nxt[o] = a;

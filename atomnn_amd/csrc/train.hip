@@ -82,7 +82,7 @@ __device__ __forceinline__ void train_act(int act, double z, double& a, double& 
         a = z;
         d = 1.0;
     } else {
-        a = tanh(z);
+        a = tanh_m1(z);
         d = 1.0 - a * a;
     }
 }
