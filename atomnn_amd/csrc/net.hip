@@ -120,14 +120,40 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_f64(const DevPlan* __
     const int gr = tid >> 4, gk = tid & 15;
     const int gat = r0 + gr < M ? atoms[r0 + gr] : -1;
     double* wl = nullptr;
+    // Weights and the first 8 column groups of the descriptor row are loaded into registers
+    // before any LDS store, so the gather's loads (which wait only for the atom index) overlap
+    // the weight copy's instead of following them.
+    constexpr int kWReg = 16;  // 16-byte weight loads per thread: n2 <= 16 x 256 under the 64 KB LDS bound
+    constexpr int kGReg = 8;   // column groups of 16 gathered through registers (nin_pad <= 128)
+    double2 wr[kWReg];
+    const double2* src = nullptr;
+    double2* dst = nullptr;
+    int n2 = 0;
     if constexpr (WL) {
         wl = sinw + N.nin_pad + 16;
         // one flat 16-byte copy of [W_0 | W_1 | ... | b_0 | ...]: all loads in flight together
-        const double2* src = reinterpret_cast<const double2*>(Wb + N.pack_off);
-        double2* dst = reinterpret_cast<double2*>(wl);
-        const int n2 = N.pack_len / 2;
-#pragma unroll 8
-        for (int idx = tid; idx < n2; idx += kNetThreads) dst[idx] = src[idx];
+        src = reinterpret_cast<const double2*>(Wb + N.pack_off);
+        dst = reinterpret_cast<double2*>(wl);
+        n2 = N.pack_len / 2;
+        // unconditional loads of clamped (in-bounds) indices: no branch, so no wait, between them
+#pragma unroll
+        for (int j = 0; j < kWReg; ++j) wr[j] = src[min(tid + j * kNetThreads, n2 - 1)];
+    }
+    double gq[kGReg], wq[kGReg], bq[kGReg];
+    const int64_t grow = (int64_t)(gat >= 0 ? gat : 0) * ldg;  // row past M: any valid row, unused
+#pragma unroll
+    for (int j = 0; j < kGReg; ++j) {
+        const int kc = min(gk + 16 * j, N.nin - 1);
+        wq[j] = Wb[N.inpw_off + kc];
+        bq[j] = Wb[N.inpb_off + kc];
+        gq[j] = G[grow + kc];
+    }
+    if constexpr (WL) {
+        // unconditional too (threads past n2 rewrite the last element with its own value), so no
+        // load is sunk into a branch of its own
+#pragma unroll
+        for (int j = 0; j < kWReg; ++j) dst[min(tid + j * kNetThreads, n2 - 1)] = wr[j];
+        for (int idx = tid + kWReg * kNetThreads; idx < n2; idx += kNetThreads) dst[idx] = src[idx];
     }
     // layer l's weights and biases: LDS copy or global
     auto wmat = [&](int l) -> const double* {
@@ -138,8 +164,17 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_f64(const DevPlan* __
         if constexpr (WL) return wl + N.pb_off[l];
         else return Wb + N.boff[l];
     };
-    // scaled input
-    for (int k = gk; k < N.nin_pad; k += 16) {
+    // scaled input ins = inpw .* (G + inpb), zero past nin and on rows past M
+#pragma unroll
+    for (int j = 0; j < kGReg; ++j) {
+        const int k = gk + 16 * j;
+        if (k < N.nin_pad) {
+            const bool in = k < N.nin;
+            cur[gr * maxw + k] = in && gat >= 0 ? wq[j] * (gq[j] + bq[j]) : 0.0;
+            if (gr == 0) sinw[k] = in ? wq[j] : 0.0;
+        }
+    }
+    for (int k = gk + 16 * kGReg; k < N.nin_pad; k += 16) {
         double v = 0.0, w = 0.0;
         if (k < N.nin) {
             w = Wb[N.inpw_off + k];
