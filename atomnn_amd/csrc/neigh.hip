@@ -206,9 +206,26 @@ __global__ __launch_bounds__(kWave) void k_neigh_allpairs(const DevStruc* __rest
 
 // ------------------------------------------------------------------ cell list
 // Bin of atom i (fractional coordinates wrapped into [0,1)), or -1 outside the cell list.
-__device__ __forceinline__ int bin_in(const DevStruc& s, const double* __restrict__ pos, int i) {
+// The fields of a structure the binning reads (held in registers by the one-structure binning).
+struct BinGeom {
+    double RInv[9];
+    int nbin[3];
+    int bin0;
+    int mode;
+};
+__device__ __forceinline__ BinGeom bin_geom(const DevStruc& s) {
+    BinGeom g;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) g.RInv[q] = s.RInv[q];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) g.nbin[q] = s.nbin[q];
+    g.bin0 = s.bin0;
+    g.mode = s.mode;
+    return g;
+}
+template <class SG>
+__device__ __forceinline__ int bin_xyz(const SG& s, double x, double y, double z) {
     if (s.mode != 1) return -1;
-    const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
     int b[3];
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
@@ -222,7 +239,7 @@ __device__ __forceinline__ int bin_in(const DevStruc& s, const double* __restric
 }
 __device__ __forceinline__ int bin_of(const DevStruc* __restrict__ S, const int* __restrict__ atom_struc,
                                       const double* __restrict__ pos, int i) {
-    return bin_in(S[atom_struc[i]], pos, i);
+    return bin_xyz(S[atom_struc[i]], pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
 }
 
 __global__ void k_bin_atoms(const DevStruc* __restrict__ S, const int* __restrict__ atom_struc,
@@ -259,14 +276,32 @@ __global__ __launch_bounds__(1024) void k_bins_small(const DevStruc* __restrict_
     // each thread's atoms i = t + 1024 k keep their bins in registers for the fill below
     int mybin[kBinsSmallAtoms / 1024];
 #pragma unroll
-    for (int k = 0; k < kBinsSmallAtoms / 1024; ++k) {
-        const int i = t + 1024 * k;
-        mybin[k] = -1;
-        if (i < ntot) {
-            const int bin = ONE ? bin_in(S[0], pos, i) : bin_of(S, atom_struc, pos, i);
-            atom_bin[i] = bin;
-            mybin[k] = bin;
-            if (bin >= 0) atomicAdd(&cnt_l[bin], 1);
+    for (int k = 0; k < kBinsSmallAtoms / 1024; ++k) mybin[k] = -1;
+    BinGeom g1{};
+    if constexpr (ONE) g1 = bin_geom(S[0]);
+    // groups of four atoms per thread; the positions of a group are loaded unconditionally at
+    // clamped (in-bounds) indices, so the four loads go out together (a guarded load becomes a
+    // branch with a full wait after it); the exit test is the same for the whole block
+#pragma unroll
+    for (int k0 = 0; k0 < kBinsSmallAtoms / 1024; k0 += 4) {
+        if (k0 * 1024 >= ntot) break;
+        double px[4], py[4], pz[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int ic = min(t + 1024 * (k0 + q), ntot - 1);
+            px[q] = pos[3 * ic];
+            py[q] = pos[3 * ic + 1];
+            pz[q] = pos[3 * ic + 2];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = t + 1024 * (k0 + q);
+            if (i < ntot) {
+                const int bin = ONE ? bin_xyz(g1, px[q], py[q], pz[q]) : bin_of(S, atom_struc, pos, i);
+                atom_bin[i] = bin;
+                mybin[k0 + q] = bin;
+                if (bin >= 0) atomicAdd(&cnt_l[bin], 1);
+            }
         }
     }
     __syncthreads();
