@@ -10,7 +10,6 @@
 //     nbr_j    int32 [Ntot*maxp]    neighbour atom (index inside its structure)
 //     nbr_meta int32 [Ntot*maxp]    image << 8 | species << 1 | min-image flag
 //     nbr_r    double4[Ntot*maxp]   (r_i - r_j - T, |r|): points from neighbour to centre
-//     partner  int32 [Ntot*maxp]    entry of the mirrored pair (j -> i), -1 if none applies
 //   G          T[Ntot][ldg]          descriptors, dEdG same shape
 //   X          TX4[Ntot*maxp]       per-entry force term  -dE_i/dr_ij
 #pragma once
@@ -40,6 +39,27 @@ struct DevStruc {
 };
 
 constexpr int kWave = 64;
+
+// Force assembly (k_assemble).  react points at two per-pass counters beside the neighbour
+// maximum: react[0] = reactions the gather from the neighbours' rows missed (mode 0; cleared by
+// the neighbour launch), react[1] = longest receiver row (mode 1; cleared by that mode's own
+// launches).
+struct AssembleArgs {
+    const DevStruc* S;
+    const int* atom_struc;
+    const int* cnt;
+    const int* nbr_j;
+    const int* meta;
+    int maxp;
+    int ntot;
+    int all_images;
+    double* F;
+    int* react;
+    int mode;      // 0: gather from the neighbours' rows, 1: receiver rows
+    int* rcount;   // mode 1: [ntot] entries in each receiver row
+    int* rlist;    // mode 1: [ntot][rcap] entry indices
+    int rcap;
+};
 
 // End-of-pass reductions, run by single-wavefront blocks appended to the assembly launch (or on
 // their own when no forces are computed): the total energy of each structure, and the largest

@@ -6,7 +6,7 @@
 //   2. descriptors         k_symm<T, NE, NC>                 -> G
 //   3. networks            k_net_f64 | k_net_mixed, all species' 16-atom tiles -> E_i, dE/dG
 //   4. force terms         k_force<T, NE, NC>                 -> X (per neighbour entry)
-//   5. force assembly      k_assemble (finds each entry's mirrored partner) -> F (FP64)
+//   5. force assembly      k_assemble (gathers each atom's reactions from its neighbours' rows) -> F (FP64)
 //   6. energy              per structure, sum of E_i in atom order (FP64), as
 //                          NNP::energy accumulates (nnp.cpp:394-400)
 #include <hip/hip_runtime.h>
@@ -47,8 +47,7 @@ hipError_t launch_force_mom(int shape, int waves, const DevPlan* plan, const dou
                             const double4* r, int maxp, int lmaxp, const T* dEdG, int ldg, T* X, int ntypes,
                             hipStream_t st);
 template <typename T>
-hipError_t launch_assemble(const DevStruc* S, const int* atom_struc, const int* cnt, const int* nbr_j, const int* meta,
-                           const T* X, int maxp, int ntot, int all_images, double* F, const EnergyTail* tail, hipStream_t st);
+hipError_t launch_assemble(const AssembleArgs& a, const T* X, const EnergyTail* tail, hipStream_t st);
 hipError_t launch_net(bool fp64, const DevPlan* plan, const DevPlan& hplan, const void* W, const int* sp_off,
                       const int* spatoms, const void* G, int ldg, int vac, double* eatom, void* dEdG, int bf16_terms,
                       int wl_tiles_max, hipStream_t st);
@@ -241,6 +240,8 @@ Engine::Engine(const Engine& o, int) : host(o.host), pb(o.pb), shape(o.shape) {
     moments = o.moments;
     force_waves = o.force_waves;
     net_bf16_terms = o.net_bf16_terms;
+    react_rows = o.react_rows;
+    assemble_mode = o.react_rows;
     num_cus = o.num_cus;
     tune_cache = o.tune_cache;
     d_plan = o.d_plan;
@@ -254,7 +255,8 @@ Engine::Engine(const Engine& o, int) : host(o.host), pb(o.pb), shape(o.shape) {
 size_t Engine::workspace_bytes() const {
     size_t b = 0;
     for (const DevMem* m : {&d_up, &d_cnt, &d_nbrj, &d_meta, &d_nbrr, &d_maxcnt, &d_abin, &d_bcount, &d_bstart,
-                            &d_bfill, &d_batoms, &d_G, &d_dEdG, &d_X, &d_eatom, &d_F, &d_E, &d_sstart})
+                            &d_bfill, &d_batoms, &d_G, &d_dEdG, &d_X, &d_eatom, &d_F, &d_E, &d_sstart, &d_rcount,
+                            &d_rlist})
         b += m->cap;
     return b;
 }
@@ -271,7 +273,7 @@ Engine::~Engine() {
     if (owns_model)
         for (DevMem* m : {&d_plan, &d_rfe, &d_rfr, &d_w64, &d_w32}) m->release();
     for (DevMem* m : {&d_up, &d_cnt, &d_nbrj, &d_meta, &d_nbrr, &d_maxcnt, &d_abin, &d_bcount, &d_bstart,
-                      &d_bfill, &d_batoms, &d_G, &d_dEdG, &d_X, &d_eatom, &d_F})
+                      &d_bfill, &d_batoms, &d_G, &d_dEdG, &d_X, &d_eatom, &d_F, &d_rcount, &d_rlist})
         m->release();
     h_up.release();
     h_down.release();
@@ -302,6 +304,8 @@ void Engine::upload(const std::vector<nnp_structure>& ss) {
     resident_valid = false;
     drop_graph();
     eager_runs = 0;
+    assemble_mode = react_rows;
+    react_missed = 0;
     nstruc = (int)ss.size();
     cell_natoms_max = 0;
     h_natoms.clear();
@@ -385,7 +389,10 @@ void Engine::build_neighbors(bool all_images) {
         d_nbrj.reserve(ent * 4);
         d_meta.reserve(ent * 4);
         d_nbrr.reserve(ent * sizeof(double4));
-        d_maxcnt.reserve(64);
+        if (!d_maxcnt.p) {  // counters the current mode does not write are read back as zero
+            d_maxcnt.reserve(64);
+            HIPCHK(hipMemsetAsync(d_maxcnt.p, 0, 64, stream));
+        }
         NeighborArgs a;
         a.S = (const DevStruc*)d_strucs.p;
         a.atom_struc = (const int*)d_astruc.p;
@@ -501,9 +508,49 @@ void Engine::force_terms() {
 template <typename T>
 void Engine::assemble(const EnergyTail* tail) {
     d_F.reserve((size_t)ntot * 3 * 8);
-    HIPCHK(launch_assemble<T>((const DevStruc*)d_strucs.p, (const int*)d_astruc.p, (const int*)d_cnt.p,
-                              (const int*)d_nbrj.p, (const int*)d_meta.p, (const T*)d_X.p, maxp, ntot,
-                              all_images_ ? 1 : 0, (double*)d_F.p, tail, stream));
+    AssembleArgs a;
+    a.S = (const DevStruc*)d_strucs.p;
+    a.atom_struc = (const int*)d_astruc.p;
+    a.cnt = (const int*)d_cnt.p;
+    a.nbr_j = (const int*)d_nbrj.p;
+    a.meta = (const int*)d_meta.p;
+    a.maxp = maxp;
+    a.ntot = ntot;
+    a.all_images = all_images_ ? 1 : 0;
+    a.F = (double*)d_F.p;
+    a.react = (int*)d_maxcnt.p + 1;
+    a.mode = assemble_mode;
+    a.rcount = nullptr;
+    a.rlist = nullptr;
+    a.rcap = 0;
+    if (assemble_mode == 1) {
+        // a receiver row holds about as many entries as a centre's row
+        rcap = std::max(rcap, std::min(maxp, 8192));
+        if ((size_t)rcap * 8 > 65536) throw unsupported_error("too many reactions per atom for the receiver rows");
+        d_rcount.reserve((size_t)ntot * 4);
+        d_rlist.reserve((size_t)ntot * rcap * 4);
+        a.rcount = (int*)d_rcount.p;
+        a.rlist = (int*)d_rlist.p;
+        a.rcap = rcap;
+    }
+    HIPCHK(launch_assemble<T>(a, (const T*)d_X.p, tail, stream));
+}
+
+void Engine::assemble_prec(bool mixed, const EnergyTail* tail) {
+    if (mixed) assemble<float>(tail);
+    else assemble<double>(tail);
+}
+
+bool Engine::reactions_redo(const int* c) {
+    if (assemble_mode == 0) {
+        react_missed = c[1];
+        if (c[1] == 0) return false;
+        assemble_mode = 1;
+        return true;
+    }
+    if (c[2] <= rcap) return false;
+    rcap = (c[2] + c[2] / 8 + 8 + 31) / 32 * 32;
+    return true;
 }
 
 template <typename T>
@@ -519,17 +566,7 @@ void Engine::run(const std::vector<nnp_structure>& ss, uint32_t flags) {
     build_neighbors(flags & NNP_ALL_IMAGES);
     if (pipeline == 1) {
         fused(mixed, flags, fused_blocks(mixed, flags));
-        if (flags & NNP_FORCE) {
-            const int ai = (flags & NNP_ALL_IMAGES) ? 1 : 0;
-            if (mixed)
-                HIPCHK(launch_assemble<float>((const DevStruc*)d_strucs.p, (const int*)d_astruc.p, (const int*)d_cnt.p,
-                                              (const int*)d_nbrj.p, (const int*)d_meta.p, (const float*)d_X.p, maxp,
-                                              ntot, ai, (double*)d_F.p, nullptr, stream));
-            else
-                HIPCHK(launch_assemble<double>((const DevStruc*)d_strucs.p, (const int*)d_astruc.p,
-                                               (const int*)d_cnt.p, (const int*)d_nbrj.p, (const int*)d_meta.p,
-                                               (const double*)d_X.p, maxp, ntot, ai, (double*)d_F.p, nullptr, stream));
-        }
+        if (flags & NNP_FORCE) assemble_prec(mixed);
         return;
     }
     if (mixed) descriptors<float>();
@@ -786,16 +823,8 @@ void Engine::enqueue_pass(uint32_t flags) {
         stage_end(ST_SYMM);
         if (flags & NNP_FORCE) {
             stage_begin(ST_ASSEMBLE);  // forces, energies and the count check in one launch
-            const int ai = (flags & NNP_ALL_IMAGES) ? 1 : 0;
             const EnergyTail tail = energy_tail(true);
-            if (mixed)
-                HIPCHK(launch_assemble<float>((const DevStruc*)d_strucs.p, (const int*)d_astruc.p, (const int*)d_cnt.p,
-                                              (const int*)d_nbrj.p, (const int*)d_meta.p, (const float*)d_X.p, maxp,
-                                              ntot, ai, (double*)d_F.p, &tail, stream));
-            else
-                HIPCHK(launch_assemble<double>((const DevStruc*)d_strucs.p, (const int*)d_astruc.p,
-                                               (const int*)d_cnt.p, (const int*)d_nbrj.p, (const int*)d_meta.p,
-                                               (const double*)d_X.p, maxp, ntot, ai, (double*)d_F.p, &tail, stream));
+            assemble_prec(mixed, &tail);
             stage_end(ST_ASSEMBLE);
             return;
         }
@@ -818,8 +847,7 @@ void Engine::enqueue_pass(uint32_t flags) {
         stage_end(ST_FORCE);
         stage_begin(ST_ASSEMBLE);  // forces, energies and the count check in one launch
         const EnergyTail tail = energy_tail(true);
-        if (mixed) assemble<float>(&tail);
-        else assemble<double>(&tail);
+        assemble_prec(mixed, &tail);
         stage_end(ST_ASSEMBLE);
         return;
     }
@@ -828,20 +856,20 @@ void Engine::enqueue_pass(uint32_t flags) {
     stage_end(ST_ENERGY);
 }
 
-// Staged download: [maxcnt | E per structure | E_i | F] into pinned memory, one sync.
+// Staged download: [counters | E per structure | E_i | F] into pinned memory, one sync.
 int Engine::download(bool energies, bool forces) {
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
     const size_t o_e = 256, o_ea = o_e + (energies ? al((size_t)nstruc * 8) : 0), o_f = o_ea + al((size_t)ntot * 8),
                  total = o_f + (forces ? al((size_t)ntot * 24) : 0);
     h_down.reserve(total);
     char* hd = (char*)h_down.p;
-    HIPCHK(hipMemcpyAsync(hd, d_maxcnt.p, 4, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(hd, d_maxcnt.p, 16, hipMemcpyDeviceToHost, stream));
     if (energies) HIPCHK(hipMemcpyAsync(hd + o_e, d_E.p, (size_t)nstruc * 8, hipMemcpyDeviceToHost, stream));
     HIPCHK(hipMemcpyAsync(hd + o_ea, d_eatom.p, (size_t)ntot * 8, hipMemcpyDeviceToHost, stream));
     if (forces) HIPCHK(hipMemcpyAsync(hd + o_f, d_F.p, (size_t)ntot * 24, hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
-    int mc;
-    std::memcpy(&mc, hd, 4);
+    std::memcpy(counters, hd, 16);
+    const int mc = counters[0];
     if (energies) {
         h_E.resize(nstruc);
         std::memcpy(h_E.data(), hd + o_e, (size_t)nstruc * 8);
@@ -856,10 +884,11 @@ int Engine::download(bool energies, bool forces) {
 }
 
 void Engine::resident_fetch(uint32_t flags, nnp_result* res) {
-    int mc = 0;
-    HIPCHK(hipMemcpyAsync(&mc, d_maxcnt.p, 4, hipMemcpyDeviceToHost, stream));
+    int c[4] = {0, 0, 0, 0};
+    HIPCHK(hipMemcpyAsync(c, d_maxcnt.p, 16, hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
     profile_collect();
+    const int mc = c[0];
     if (mc > lmaxp) {  // the rows (or their LDS staging) overflowed: enlarge and redo the pass
         if (mc > maxp) {
             maxp = (mc + mc / 8 + 8 + 31) / 32 * 32;
@@ -870,6 +899,19 @@ void Engine::resident_fetch(uint32_t flags, nnp_result* res) {
         enqueue_pass(flags);
         HIPCHK(hipStreamSynchronize(stream));
         profile_collect();
+        HIPCHK(hipMemcpyAsync(c, d_maxcnt.p, 16, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+    // reactions the gather missed (or receiver rows too short): the batch takes receiver rows
+    for (int attempt = 0; (flags & NNP_FORCE) && reactions_redo(c); ++attempt) {
+        if (attempt == 3) throw std::runtime_error("force assembly: receiver rows did not converge");
+        drop_graph();
+        eager_runs = 0;
+        enqueue_pass(flags);
+        HIPCHK(hipStreamSynchronize(stream));
+        profile_collect();
+        HIPCHK(hipMemcpyAsync(c, d_maxcnt.p, 16, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
     }
     (void)download(true, flags & NNP_FORCE);
     int a0 = 0;
@@ -894,6 +936,12 @@ void Engine::fetch(const std::vector<nnp_structure>& ss, uint32_t flags, nnp_res
         }
         defer_count = true;
         (void)download(false, flags & NNP_FORCE);
+    }
+    // reactions the gather missed (or receiver rows too short): assemble again from receiver rows
+    for (int attempt = 0; (flags & NNP_FORCE) && reactions_redo(counters); ++attempt) {
+        if (attempt == 3) throw std::runtime_error("force assembly: receiver rows did not converge");
+        assemble_prec(flags & NNP_PREC_MIXED);
+        (void)download(false, true);
     }
     int a0 = 0;
     for (size_t k = 0; k < ss.size(); ++k) {
@@ -1150,6 +1198,13 @@ int nnp_set_option(nnp_model* m, const char* key, int64_t value) {
             E.force_waves = (int)value;
             E.drop_graph();
             E.eager_runs = 0;
+        } else if (k == "reaction_rows") {
+            if (value != 0 && value != 1) throw std::invalid_argument("nnp_set_option: reaction_rows is 0 or 1");
+            HIPCHK(hipStreamSynchronize(E.stream));
+            E.react_rows = (int)value;
+            E.assemble_mode = (int)value;
+            E.drop_graph();
+            E.eager_runs = 0;
         } else if (k == "train_chunk_elems") {
             if (value < (1 << 16)) throw std::invalid_argument("nnp_set_option: train_chunk_elems must be >= 65536");
             E.train_chunk_elems = value;
@@ -1178,7 +1233,8 @@ int nnp_info(nnp_model* m, int64_t* out, int nout) {
         int64_t tuned = E.fused_blocks_per_cu;
         if (!E.tune_cache.empty()) tuned = E.tune_cache.back().second;
         const int64_t v[] = {E.maxp, E.max_entries, E.ntot, E.nstruc, E.shape, E.pb.plan.ne_max, E.pb.plan.nc_max,
-                             E.pb.plan.ldg, E.pipeline, tuned, E.num_cus, E.last_mom};
+                             E.pb.plan.ldg, E.pipeline, tuned, E.num_cus, E.last_mom, E.assemble_mode,
+                             E.react_missed};
         for (int k = 0; k < nout && k < (int)(sizeof(v) / sizeof(v[0])); ++k) out[k] = v[k];
         return (int)(sizeof(v) / sizeof(v[0]));
     });

@@ -104,6 +104,19 @@ class Engine {
     // the same launch
     template <typename T>
     void assemble(const EnergyTail* tail = nullptr);
+    void assemble_prec(bool mixed, const EnergyTail* tail = nullptr);
+    // Reactions of the force assembly: the gather from the neighbours' rows (mode 0) reports in
+    // maxcnt[1] how many reactions it missed; receiver rows (mode 1) report their longest row in
+    // maxcnt[2].  Given the counters read back after a pass with forces, switch to receiver rows
+    // or lengthen them; true when the pass's forces must be assembled again.
+    bool reactions_redo(const int* counters);
+    // option: 0 = gather, receiver rows once a miss is seen in the batch; 1 = receiver rows always
+    int react_rows = 0;
+    int assemble_mode = 0;  // the current batch's mode (reset to react_rows by upload)
+    int rcap = 0;           // receiver-row length (mode 1)
+    int64_t react_missed = 0;  // reactions the last mode-0 pass missed (0 in the usual case)
+    DevMem d_rcount, d_rlist;
+    int counters[4] = {0, 0, 0, 0};  // maxcnt[0..3] as of the last download
     void run(const std::vector<nnp_structure>& ss, uint32_t flags);
     void fetch(const std::vector<nnp_structure>& ss, uint32_t flags, nnp_result* res);
 

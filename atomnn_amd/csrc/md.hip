@@ -107,7 +107,9 @@ __global__ __launch_bounds__(256) void k_kick_kinetic(double* __restrict__ v, co
     }
     if (threadIdx.x == 0) {
         *ke = 0.5 * part[0];
-        if (*maxcnt > *maxseen) *maxseen = *maxcnt;
+        // neighbour maximum, missed reactions, longest receiver row: maxima since the last reset
+        for (int q = 0; q < 3; ++q)
+            if (maxcnt[q] > maxseen[q]) maxseen[q] = maxcnt[q];
         *done = 0u;
     }
 }
@@ -129,7 +131,8 @@ __global__ __launch_bounds__(256) void k_kinetic(const double* __restrict__ v, c
     }
     if (threadIdx.x == 0) {
         *ke = 0.5 * part[0];
-        if (*maxcnt > *maxseen) *maxseen = *maxcnt;
+        for (int q = 0; q < 3; ++q)
+            if (maxcnt[q] > maxseen[q]) maxseen[q] = maxcnt[q];
     }
 }
 
@@ -234,8 +237,8 @@ struct Md {
         HIPCHK(hipMemcpyAsync(d_save.p, pos(), vb, hipMemcpyDeviceToDevice, E.stream));
         HIPCHK(hipMemcpyAsync((char*)d_save.p + vb, d_vel.p, vb, hipMemcpyDeviceToDevice, E.stream));
         HIPCHK(hipMemcpyAsync((char*)d_save.p + 2 * vb, force(), vb, hipMemcpyDeviceToDevice, E.stream));
-        HIPCHK(hipMemsetAsync(d_maxseen.p, 0, 4, E.stream));
-        for (int attempt = 0; attempt < 3; ++attempt) {
+        HIPCHK(hipMemsetAsync(d_maxseen.p, 0, 16, E.stream));
+        for (int attempt = 0; attempt < 4; ++attempt) {
             for (int k = 0; k < nsteps;) {
                 const bool ready = eager_done && graph_lmaxp == E.lmaxp && graph_maxp == E.maxp;
                 if (ready && nsteps - k >= kStepsPerGraph) {
@@ -248,25 +251,32 @@ struct Md {
                     ++steps;
                 }
             }
-            int seen = 0;
-            HIPCHK(hipMemcpyAsync(&seen, d_maxseen.p, 4, hipMemcpyDeviceToHost, E.stream));
+            int c[4] = {0, 0, 0, 0};
+            HIPCHK(hipMemcpyAsync(c, d_maxseen.p, 16, hipMemcpyDeviceToHost, E.stream));
             HIPCHK(hipStreamSynchronize(E.stream));
-            if (seen <= E.lmaxp) return;
-            // rows outgrown: restore, enlarge, redo
+            const int seen = c[0];
+            const bool rows = seen > E.lmaxp;
+            // reactions the assembly's gather missed (or receiver rows too short): the steps
+            // are redone with receiver rows
+            const bool react = !rows && E.reactions_redo(c);
+            if (!rows && !react) return;
+            // restore and redo
             steps -= nsteps;
             HIPCHK(hipMemcpyAsync(pos(), d_save.p, vb, hipMemcpyDeviceToDevice, E.stream));
             HIPCHK(hipMemcpyAsync(d_vel.p, (char*)d_save.p + vb, vb, hipMemcpyDeviceToDevice, E.stream));
             HIPCHK(hipMemcpyAsync(force(), (char*)d_save.p + 2 * vb, vb, hipMemcpyDeviceToDevice, E.stream));
-            HIPCHK(hipMemsetAsync(d_maxseen.p, 0, 4, E.stream));
-            if (seen > E.maxp) {
-                E.maxp = (seen + seen / 4 + 16 + 31) / 32 * 32;
-                E.build_neighbors(flags & NNP_ALL_IMAGES);
+            HIPCHK(hipMemsetAsync(d_maxseen.p, 0, 16, E.stream));
+            if (rows) {
+                if (seen > E.maxp) {
+                    E.maxp = (seen + seen / 4 + 16 + 31) / 32 * 32;
+                    E.build_neighbors(flags & NNP_ALL_IMAGES);
+                }
+                E.lmaxp = std::min(E.maxp, (seen + seen / 4 + 8 + 7) / 8 * 8);
             }
-            E.lmaxp = std::min(E.maxp, (seen + seen / 4 + 8 + 7) / 8 * 8);
             drop();
             eager_done = false;
         }
-        throw std::runtime_error("nnp_md_step: neighbour rows did not converge");
+        throw std::runtime_error("nnp_md_step: neighbour rows or reaction rows did not converge");
     }
 };
 
@@ -336,7 +346,7 @@ int nnp_md_create(nnp_model* m, const nnp_structure* s, const double* masses, co
         md->d_half.reserve((size_t)n * 8);
         md->d_mass.reserve((size_t)n * 8);
         md->d_ke.reserve(8);
-        md->d_maxseen.reserve(4);
+        md->d_maxseen.reserve(16);
         md->d_bsum.reserve((size_t)((n + 255) / 256) * 8);
         md->d_done.reserve(4);
         HIPCHK(hipMemsetAsync(md->d_done.p, 0, 4, E.stream));  // block counter; each step leaves it at 0
@@ -344,7 +354,7 @@ int nnp_md_create(nnp_model* m, const nnp_structure* s, const double* masses, co
         HIPCHK(hipMemcpyAsync(md->d_mass.p, md->mass.data(), (size_t)n * 8, hipMemcpyHostToDevice, E.stream));
         if (vel) HIPCHK(hipMemcpyAsync(md->d_vel.p, vel, (size_t)n * 24, hipMemcpyHostToDevice, E.stream));
         else HIPCHK(hipMemsetAsync(md->d_vel.p, 0, (size_t)n * 24, E.stream));
-        HIPCHK(hipMemsetAsync(md->d_maxseen.p, 0, 4, E.stream));
+        HIPCHK(hipMemsetAsync(md->d_maxseen.p, 0, 16, E.stream));
         // initial forces and energies at the starting positions
         E.enqueue_pass(md->flags);
         hipLaunchKernelGGL(k_kinetic, dim3(1), dim3(256), 0, E.stream, (const double*)md->d_vel.p,

@@ -268,7 +268,8 @@ __global__ __launch_bounds__(1024) void k_bins_small(const DevStruc* __restrict_
     __shared__ int part[1024 / kWave];
     const int t = threadIdx.x;
     if (t == 0) {
-        *maxcnt = 0;
+        maxcnt[0] = 0;  // neighbour maximum
+        maxcnt[1] = 0;  // reactions missed by the force assembly's gather
         if (work) *work = 0;
     }
     for (int b = t; b < nbins; b += 1024) cnt_l[b] = 0;
@@ -563,12 +564,14 @@ __global__ __launch_bounds__(256) void k_count_max(const int* __restrict__ cnt, 
     }
 }
 
-// Clears the per-pass counters (max entry count and the cell-list bin counters) in one launch.
+// Clears the per-pass counters (max entry count, missed reactions and the cell-list bin counters)
+// in one launch.
 __global__ void k_zero(int* __restrict__ maxcnt, int* __restrict__ work, int* __restrict__ bin_count,
                        int* __restrict__ bin_fill, int nbins) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t == 0) {
-        *maxcnt = 0;
+        maxcnt[0] = 0;  // neighbour maximum
+        maxcnt[1] = 0;  // reactions missed by the force assembly's gather
         if (work) *work = 0;
     }
     if (t < nbins) {

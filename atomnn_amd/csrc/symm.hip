@@ -1543,23 +1543,93 @@ template hipError_t launch_fused<float>(int, int, int, const DevPlan*, const dou
                                         const int*, const int*, int, int*, const int*, const int*, const double4*, int,
                                         int, int, int, int, int, int, double*, float*, int, int, hipStream_t);
 
-// F_a = sum_s X[a,s] - sum_{entries (j -> a) that carry a reaction} X[j -> a], in FP64.
-// One wavefront per atom, lanes over its entries.  The mirrored entry (j -> a, image
-// nimg-1-n) of each entry (a -> j, image n) has the same distance, so it carries the min flag
-// exactly when (a -> j) does; it is found by an 8-ary search in j's row, which is sorted by
-// (neighbour, image): independent probes per level keep the dependent load chain short.  With all_images every entry carries a reaction.  The lane partials are
-// summed in a fixed tree order (deterministic, no atomics).
+// F_a = sum_s X[a,s] - sum_{entries (i -> a) that carry a reaction} X[i -> a], in FP64.
+//
+// NNP::force (nnp.cpp:441-446, 469-478) subtracts the term of entry (i -> a, image n) from F_a
+// when that entry, in centre i's own row, carries the min flag (every entry with all_images).
+// One wavefront per atom a, lanes over its own entries.  The reactions are gathered from the
+// rows of a's neighbours: for the first entry of each neighbour j in a's row (rows are sorted by
+// neighbour index, then image), the run of entries with index a in j's row is located (lower
+// bound of a) and every entry of that run that carries a reaction is subtracted.  No image
+// correspondence is assumed, so images that tie in distance (half-lattice offsets, where both
+// differences wrap to -1/2) take exactly the entries the reference takes.
+//
+// That gathers everything unless some centre i lists a while a's own row does not list i (a
+// distance within rounding of rc, or an image table that is not symmetric after the 6-shift
+// fallback of Cell::diff).  Each atom therefore also counts its own reaction-carrying entries and
+// the entries it consumed; summed over atoms the difference is exactly the number of reactions
+// missed.  Atoms whose two counts differ add the difference to react[0] (zero atomics in the
+// usual case); the host reads it with the pass's results and, when it is not zero, redoes the
+// assembly from receiver rows (MODE 1 below), which does not depend on that symmetry.
+//
+// MODE 1: the receiver rows built by k_react_fill (the entries carrying a reaction onto a, in
+// an order set by atomics) are ranked by entry index in LDS and subtracted in that order, so the
+// result does not depend on the order of the atomics.  A row longer than rcap is reported in
+// react[1] (largest length seen) and the host redoes the pass with longer rows.
+//
+// Lane partials are summed in a fixed tree order (deterministic, no atomics on F).
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
     return v;
 }
 
-template <typename T>
+__device__ __forceinline__ int wave_sum_i(int v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// First index in [0, P) of row `rowj` whose neighbour index is >= want (P if none).  The row's
+// indices ascend, so the entry sits near want / natoms of the way along it: the 16 indices around
+// that guess come in four aligned 16-byte loads and, when they bracket want, their count places
+// it; otherwise an 8-ary search (7 independent probes per level, a short dependent chain) takes
+// the side that holds it and the last <= 16 indices are counted.
+__device__ __forceinline__ int row_lower_bound(const int* __restrict__ nbr_j, int64_t rowj, int P, int want,
+                                               int natoms) {
+    int lo = 0, hi = P;
+    if (P >= 16) {
+        const int g = (int)(((int64_t)want * P) / max(natoms, 1));
+        const int w0 = min(max(0, g - 8), P - 16) & ~3;  // 4-aligned, and w0 + 16 <= P
+        const int4* kv = reinterpret_cast<const int4*>(nbr_j + rowj + w0);
+        int c = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int4 k4 = kv[q];
+            c += (k4.x < want) + (k4.y < want) + (k4.z < want) + (k4.w < want);
+        }
+        const int w1 = w0 + 16;
+        if (c > 0 && (c < 16 || w1 == P)) {
+            lo = hi = w0 + c;
+        } else if (c == 0) {
+            hi = w0;
+        } else {
+            lo = w1;
+        }
+    }
+    while (hi - lo > 16) {
+        const int len = hi - lo;
+        int c = 0;
+#pragma unroll
+        for (int m = 1; m < 8; ++m) c += nbr_j[rowj + lo + (len * m) / 8] < want ? 1 : 0;
+        const int nlo = c == 0 ? lo : lo + (len * c) / 8 + 1;
+        const int nhi = c == 7 ? hi : lo + (len * (c + 1)) / 8;
+        lo = nlo;
+        hi = nhi;
+    }
+    int c = 0;
+#pragma unroll
+    for (int m = 0; m < 16; ++m) c += (lo + m < hi && nbr_j[rowj + lo + m] < want) ? 1 : 0;
+    return lo + c;
+}
+
+template <typename T, int MODE>
 __global__ __launch_bounds__(kWave) void k_assemble(const DevStruc* __restrict__ S, const int* __restrict__ atom_struc,
                                                     const int* __restrict__ cnt, const int* __restrict__ nbr_j,
                                                     const int* __restrict__ nbr_meta, const T* __restrict__ X, int maxp,
-                                                    int all_images, double* __restrict__ F, EnergyTail tail) {
+                                                    int all_images, double* __restrict__ F, EnergyTail tail,
+                                                    int* __restrict__ react, const int* __restrict__ rcount,
+                                                    const int* __restrict__ rlist, int rcap) {
     // the end-of-pass reduction blocks come first, so they run beside the atoms' blocks
     const int ntail = tail.nstruc + tail.ncount;
     const int lane = threadIdx.x;
@@ -1568,91 +1638,72 @@ __global__ __launch_bounds__(kWave) void k_assemble(const DevStruc* __restrict__
         return;
     }
     const int a = blockIdx.x - ntail;
-    const DevStruc& st = S[atom_struc[a]];
-    const int la = a - st.atom0;
     const int P = min(cnt[a], maxp);
     const int64_t row = (int64_t)a * maxp;
     double fx = 0, fy = 0, fz = 0;
-    for (int s = lane; s < P; s += kWave) {
-        const int64_t e = row + s;
-        {
-            const double4 v = load_x4<T>(X + 4 * e);
+    if (MODE == 0) {
+        const DevStruc& st = S[atom_struc[a]];
+        const int la = a - st.atom0;
+        int own = 0, used = 0;
+        for (int s = lane; s < P; s += kWave) {
+            const int64_t e = row + s;
+            {
+                const double4 v = load_x4<T>(X + 4 * e);
+                fx += v.x;
+                fy += v.y;
+                fz += v.z;
+            }
+            const int j = nbr_j[e];
+            own += (all_images || (nbr_meta[e] & 1)) ? 1 : 0;
+            // one lane per neighbour: the first of its entries.  Self entries carry no min flag
+            // (neighbor.cpp:131), so under the min-image rule the atom's own row is skipped.
+            const bool first = s == 0 || nbr_j[e - 1] != j;
+            if (first && (all_images || j != la)) {
+                const int gj = st.atom0 + j;
+                const int64_t rowj = (int64_t)gj * maxp;
+                const int Pj = min(cnt[gj], maxp);
+                for (int q = row_lower_bound(nbr_j, rowj, Pj, la, st.natoms); q < Pj && nbr_j[rowj + q] == la; ++q) {
+                    if (all_images || (nbr_meta[rowj + q] & 1)) {
+                        const double4 v = load_x4<T>(X + 4 * (rowj + q));
+                        fx -= v.x;
+                        fy -= v.y;
+                        fz -= v.z;
+                        ++used;
+                    }
+                }
+            }
+        }
+        const int miss = wave_sum_i(own - used);
+        if (lane == 0 && miss != 0) atomicAdd(react, miss);
+    } else {
+        extern __shared__ int rl_lds[];
+        int* ids = rl_lds;           // [rcap] receiver row as filled
+        int* sorted = rl_lds + rcap;  // [rcap] ranked by entry index
+        for (int s = lane; s < P; s += kWave) {
+            const double4 v = load_x4<T>(X + 4 * (row + s));
             fx += v.x;
             fy += v.y;
             fz += v.z;
         }
-        const int meta = nbr_meta[e];
-        if ((meta & 1) || all_images) {
-            const int img = meta >> 8;
-            const int want_img = st.periodic ? st.nimg - 1 - img : 0;
-            const int gj = st.atom0 + nbr_j[e];
-            const int64_t rowj = (int64_t)gj * maxp;
-            const int Pj = min(cnt[gj], maxp);
-            const int64_t want = (int64_t)la * st.nimg + want_img;
-            const bool multi = st.nimg > 1;  // one image per j: the key is the neighbour index
-            auto key = [&](int idx) -> int64_t {
-                const int64_t kj = nbr_j[rowj + idx];
-                return multi ? kj * st.nimg + (nbr_meta[rowj + idx] >> 8) : kj;
-            };
-            // Rows are sorted by key (neighbour index, then image), so the mirror sits near
-            // want / (natoms nimg) of the way along row j.  The 16 keys around that guess come in
-            // aligned 16-byte loads (indices, and image fields when there are several images);
-            // when they bracket want their count places it, otherwise the search below takes the
-            // side that holds it.
-            int lo = 0, hi = Pj;
-            if (Pj >= 16) {
-                const int64_t kend = multi ? (int64_t)st.natoms * st.nimg : (int64_t)st.natoms;
-                const int g = (int)((want * Pj) / kend);
-                const int w0 = min(max(0, g - 8), Pj - 16) & ~3;  // 4-aligned, and w0 + 16 <= Pj
-                const int4* kv = reinterpret_cast<const int4*>(nbr_j + rowj + w0);
-                const int4* mv = reinterpret_cast<const int4*>(nbr_meta + rowj + w0);
-                int c = 0;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int4 k4 = kv[q];
-                    int64_t kx = k4.x, ky = k4.y, kz = k4.z, kw = k4.w;
-                    if (multi) {
-                        const int4 m4 = mv[q];
-                        kx = kx * st.nimg + (m4.x >> 8);
-                        ky = ky * st.nimg + (m4.y >> 8);
-                        kz = kz * st.nimg + (m4.z >> 8);
-                        kw = kw * st.nimg + (m4.w >> 8);
-                    }
-                    c += (kx < want) + (ky < want) + (kz < want) + (kw < want);
-                }
-                const int w1 = min(Pj, w0 + 16);
-                if (c > 0 && (w0 + c < w1 || w1 == Pj)) {
-                    lo = hi = w0 + c;
-                } else if (c == 0) {
-                    hi = w0;
-                } else {
-                    lo = w1;
-                }
-            }
-            // 8-ary search: 7 independent probes per level narrow [lo, hi) to one of 8 parts,
-            // then the last <= 16 keys are counted; the first index with key >= want is lo
-            while (hi - lo > 16) {
-                const int len = hi - lo;
-                int c = 0;
-#pragma unroll
-                for (int m = 1; m < 8; ++m) c += key(lo + (len * m) / 8) < want ? 1 : 0;
-                const int nlo = c == 0 ? lo : lo + (len * c) / 8 + 1;
-                const int nhi = c == 7 ? hi : lo + (len * (c + 1)) / 8;
-                lo = nlo;
-                hi = nhi;
-            }
-            {
-                int c = 0;
-#pragma unroll
-                for (int m = 0; m < 16; ++m) c += (lo + m < hi && key(lo + m) < want) ? 1 : 0;
-                lo += c;
-            }
-            if (lo < Pj && key(lo) == want) {
-                const double4 v = load_x4<T>(X + 4 * (rowj + lo));
-                fx -= v.x;
-                fy -= v.y;
-                fz -= v.z;
-            }
+        int n = rcount[a];
+        if (n > rcap) {
+            if (lane == 0) atomicMax(react + 1, n);
+            n = rcap;
+        }
+        for (int q = lane; q < n; q += kWave) ids[q] = rlist[(int64_t)a * rcap + q];
+        __syncthreads();
+        for (int q = lane; q < n; q += kWave) {
+            const int id = ids[q];
+            int r = 0;
+            for (int k = 0; k < n; ++k) r += ids[k] < id ? 1 : 0;  // entry indices are distinct
+            sorted[r] = id;
+        }
+        __syncthreads();
+        for (int q = lane; q < n; q += kWave) {
+            const double4 v = load_x4<T>(X + 4 * (int64_t)sorted[q]);
+            fx -= v.x;
+            fy -= v.y;
+            fz -= v.z;
         }
     }
     fx = wave_sum(fx);
@@ -1662,6 +1713,32 @@ __global__ __launch_bounds__(kWave) void k_assemble(const DevStruc* __restrict__
         F[3 * (int64_t)a] = fx;
         F[3 * (int64_t)a + 1] = fy;
         F[3 * (int64_t)a + 2] = fz;
+    }
+}
+
+// Receiver rows for MODE 1 of k_assemble: rcount cleared, react[1] (longest row seen) cleared.
+__global__ void k_react_zero(int* __restrict__ rcount, int n, int* __restrict__ react) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) rcount[t] = 0;
+    if (t == 0) react[1] = 0;
+}
+
+// One wavefront per centre i: every entry (i -> j) that carries a reaction is appended to the
+// receiver row of j (slot order set by the atomics; k_assemble ranks the row before summing).
+__global__ __launch_bounds__(kWave) void k_react_fill(const DevStruc* __restrict__ S, const int* __restrict__ atom_struc,
+                                                      const int* __restrict__ cnt, const int* __restrict__ nbr_j,
+                                                      const int* __restrict__ nbr_meta, int maxp, int all_images,
+                                                      int* __restrict__ rcount, int* __restrict__ rlist, int rcap) {
+    const int i = blockIdx.x;
+    const int atom0 = S[atom_struc[i]].atom0;
+    const int P = min(cnt[i], maxp);
+    const int64_t row = (int64_t)i * maxp;
+    for (int s = threadIdx.x; s < P; s += kWave) {
+        const int64_t e = row + s;
+        if (!(all_images || (nbr_meta[e] & 1))) continue;
+        const int recv = atom0 + nbr_j[e];
+        const int slot = atomicAdd(&rcount[recv], 1);
+        if (slot < rcap) rlist[(int64_t)recv * rcap + slot] = (int)e;
     }
 }
 
@@ -1823,13 +1900,22 @@ template hipError_t launch_force_mom<float>(int, int, const DevPlan*, const doub
                                             const float*, int, float*, int, hipStream_t);
 
 template <typename T>
-hipError_t launch_assemble(const DevStruc* S, const int* atom_struc, const int* cnt, const int* nbr_j, const int* meta,
-                           const T* X, int maxp, int ntot, int all_images, double* F, const EnergyTail* tail,
-                           hipStream_t st) {
+hipError_t launch_assemble(const AssembleArgs& a, const T* X, const EnergyTail* tail, hipStream_t st) {
     EnergyTail t{};
     if (tail) t = *tail;  // without a tail, nstruc = ncount = 0: no extra blocks
-    hipLaunchKernelGGL((k_assemble<T>), dim3(ntot + t.nstruc + t.ncount), dim3(kWave), 0, st, S, atom_struc, cnt, nbr_j,
-                       meta, X, maxp, all_images, F, t);
+    const dim3 grid(a.ntot + t.nstruc + t.ncount);
+    if (a.mode == 0) {
+        hipLaunchKernelGGL((k_assemble<T, 0>), grid, dim3(kWave), 0, st, a.S, a.atom_struc, a.cnt, a.nbr_j, a.meta, X,
+                           a.maxp, a.all_images, a.F, t, a.react, nullptr, nullptr, 0);
+        return hipGetLastError();
+    }
+    const size_t lds = (size_t)a.rcap * 8;
+    if (lds > 65536) return hipErrorInvalidConfiguration;
+    hipLaunchKernelGGL(k_react_zero, dim3((a.ntot + 255) / 256), dim3(256), 0, st, a.rcount, a.ntot, a.react);
+    hipLaunchKernelGGL(k_react_fill, dim3(a.ntot), dim3(kWave), 0, st, a.S, a.atom_struc, a.cnt, a.nbr_j, a.meta,
+                       a.maxp, a.all_images, a.rcount, a.rlist, a.rcap);
+    hipLaunchKernelGGL((k_assemble<T, 1>), grid, dim3(kWave), lds, st, a.S, a.atom_struc, a.cnt, a.nbr_j, a.meta, X,
+                       a.maxp, a.all_images, a.F, t, a.react, a.rcount, a.rlist, a.rcap);
     return hipGetLastError();
 }
 
@@ -1845,9 +1931,7 @@ template hipError_t launch_force<double>(int, int, const DevPlan*, const double*
 template hipError_t launch_force<float>(int, int, const DevPlan*, const double*, const double*, const int*, const int*,
                                         int, const int*, const int*, const double4*, int, int, const float*, int,
                                         float*, int, int, hipStream_t);
-template hipError_t launch_assemble<double>(const DevStruc*, const int*, const int*, const int*, const int*,
-                                            const double*, int, int, int, double*, const EnergyTail*, hipStream_t);
-template hipError_t launch_assemble<float>(const DevStruc*, const int*, const int*, const int*, const int*, const float*,
-                                           int, int, int, double*, const EnergyTail*, hipStream_t);
+template hipError_t launch_assemble<double>(const AssembleArgs&, const double*, const EnergyTail*, hipStream_t);
+template hipError_t launch_assemble<float>(const AssembleArgs&, const float*, const EnergyTail*, hipStream_t);
 
 }  // namespace atomnn

@@ -1022,7 +1022,18 @@ class TrainSet {
                                    (double*)E.d_dEdG.p);
             HIPCHK(hipGetLastError());
             E.force_terms<double>();
+            // reactions: the missed-reaction counter is cleared here (a cached chunk has no
+            // neighbour launch in this call); a miss switches the chunk's workspace to receiver rows
+            HIPCHK(hipMemsetAsync((int*)E.d_maxcnt.p + 1, 0, 4, st));
             E.assemble<double>();
+            for (int attempt = 0;; ++attempt) {
+                int c[4] = {0, 0, 0, 0};
+                HIPCHK(hipMemcpyAsync(c, E.d_maxcnt.p, 16, hipMemcpyDeviceToHost, st));
+                HIPCHK(hipStreamSynchronize(st));
+                if (!E.reactions_redo(c)) break;
+                if (attempt == 3) throw std::runtime_error("force assembly: receiver rows did not converge");
+                E.assemble<double>();
+            }
             hipLaunchKernelGGL(k_train_struct, dim3(nb), dim3(256), 0, st, (const double*)d_out_.p,
                                (const int*)d_bstart_.p, (const double*)d_eref_.p, loss, delta, (double*)d_dcdo_.p,
                                (double*)d_loss_.p, (double*)d_dE2_.p);

@@ -204,10 +204,10 @@ class NNP:
         _check(lib().nnp_set_option(self._h, key.encode(), int(value)))
 
     def info(self) -> dict:
-        out = np.zeros(12, dtype=np.int64)
-        _check(lib().nnp_info(self._h, out.ctypes.data, 12))
+        out = np.zeros(14, dtype=np.int64)
+        _check(lib().nnp_info(self._h, out.ctypes.data, 14))
         keys = ["maxp", "max_entries", "atoms", "structures", "shape", "ne", "nc", "ldg", "pipeline",
-                "fused_blocks_per_cu", "compute_units", "moments"]
+                "fused_blocks_per_cu", "compute_units", "moments", "assemble_mode", "reactions_missed"]
         return dict(zip(keys, (int(v) for v in out)))
 
     def resident(self, strucs: Sequence[Structure], flags: int) -> "Resident":

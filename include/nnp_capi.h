@@ -174,10 +174,18 @@ int nnp_profile_read(nnp_model* m, double* ms, int64_t* calls, int nstage);
  *               structure chunks it has seen, up to this many atoms (default 2^20; 0 = rebuild
  *               them every call)
  *   "train_chunk_elems"  training: gradient-row elements (doubles) held at a time (default
- *               2^28, 2 GB; batches with more rows are summed in chunks; at least 65536) */
+ *               2^28, 2 GB; batches with more rows are summed in chunks; at least 65536)
+ *   "reaction_rows"  force assembly: 0 = each atom gathers its reactions from its neighbours'
+ *               rows, and a batch in which that misses one (a centre lists the atom but the
+ *               atom does not list the centre: a distance within rounding of rc) is assembled
+ *               again from receiver rows (default); 1 = receiver rows always.  Both apply the
+ *               reaction of every entry whose own min flag is set (NNP::force, nnp.cpp:441-478)
+ *               and give the same forces up to summation order. */
 int nnp_set_option(nnp_model* m, const char* key, int64_t value);
 /* Engine state: maxp, max entries, atoms, structures, kernel shape, NE, NC, descriptor width,
- * pipeline, fused blocks per CU (tuned), compute units, moment bits of the last pass. */
+ * pipeline, fused blocks per CU (tuned), compute units, moment bits of the last pass, force
+ * assembly mode of the current batch (0 gather, 1 receiver rows), reactions the last gather
+ * missed (0 in the usual case; a batch with misses is assembled again from receiver rows). */
 int nnp_info(nnp_model* m, int64_t* out, int nout);
 
 /* ---- energy-training inner loop ----

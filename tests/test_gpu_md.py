@@ -147,3 +147,37 @@ def test_md_steps_in_one_call_equal_single_steps(potentials):
         b.close()
     finally:
         m.close()
+
+
+def test_md_with_receiver_rows_on_a_small_crystal(potentials):
+    """A rattled 128-atom BCC cell (L = 11.5 A < 2 rc, so the min-image reaction rule is active)
+    with the force assembly on receiver rows (option reaction_rows = 1): the trajectory matches
+    the CPU Verlet driven by the oracle's forces."""
+    from reaction_cases import BCC, crystal
+
+    rng = np.random.default_rng(4)
+    c = crystal(2.87, BCC, 4, [0, 1])
+    s = synth.Structure(c.types, c.pos + rng.normal(0, 0.05, c.pos.shape), c.cell)
+    v0 = velocities(s, rng, 300.0)
+    dt, nsteps = 1.0, 4
+    m = NNP.read(potentials["plain4096"])
+    m.set_option("reaction_rows", 1)
+    o = ob.OracleModel(potentials["plain4096"])
+    md = MD(m, s, MASSES, v0, dt, NNP_FORCE)
+    try:
+        md.step(nsteps)
+        st = md.state()
+    finally:
+        md.close()
+        m.close()
+    mass = np.array([MASSES[t] for t in s.types])[:, None]
+    x, v = s.pos.copy(), v0.copy()
+    E, _, F = o.compute(s.types, x, s.cell, ob.F_FORCE)
+    for _ in range(nsteps):
+        v += 0.5 * dt * F / (mass * CONV)
+        x = wrap(x + v * dt, s.cell)
+        E, _, F = o.compute(s.types, x, s.cell, ob.F_FORCE)
+        v += 0.5 * dt * F / (mass * CONV)
+    assert np.abs(st["pos"] - x).max() < 1e-9
+    assert np.abs(st["vel"] - v).max() < 1e-11
+    assert abs(st["epot"] - E) <= 1e-9 * max(abs(E), 1.0)

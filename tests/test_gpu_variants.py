@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from atomnn_amd import synth
-from atomnn_amd.nnp import NNP, NNP_FORCE, NNP_PREC_MIXED, NNP_VACUUM_ENERGY
+from atomnn_amd.nnp import NNP, NNP_ATOM_ENERGY, NNP_FORCE, NNP_PREC_MIXED, NNP_VACUUM_ENERGY
 import oracle_binding as ob
 
 pytestmark = pytest.mark.gpu
@@ -26,7 +26,8 @@ def rel_f(F, Fo):
     return np.linalg.norm(F - Fo, axis=1).max() / np.linalg.norm(Fo, axis=1).max()
 
 
-def variant_potential(tmp_path, phi_r="GAUSSIAN", phi_a="GAUSS", neuron="TANH", seed=0, zetas=(1.0, 4.0)):
+def variant_potential(tmp_path, phi_r="GAUSSIAN", phi_a="GAUSS", neuron="TANH", seed=0, zetas=(1.0, 4.0), cut="COS",
+                      norm="UNIT"):
     """Writes the potential (preconditioned on the oracle's descriptors of a calibration cell)
     and returns its path."""
     rng = np.random.default_rng(seed)
@@ -35,8 +36,9 @@ def variant_potential(tmp_path, phi_r="GAUSSIAN", phi_a="GAUSS", neuron="TANH", 
     for key in pot.radial:
         pot.radial[key] = [(rs, eta) for rs, eta in zip(np.linspace(0.5, 5.5, 6), np.linspace(0.3, 2.0, 6))]
     pot.phi_r, pot.phi_a, pot.neuron = phi_r, phi_a, neuron
+    pot.cut, pot.norm = cut, norm
     pot.outw[0], pot.outb[1] = 0.9, 0.05
-    path = str(tmp_path / f"v_{phi_r}_{phi_a}_{neuron}_{'_'.join(str(z) for z in zetas)}.ann")
+    path = str(tmp_path / f"v_{phi_r}_{phi_a}_{neuron}_{cut}_{norm}_{'_'.join(str(z) for z in zetas)}.ann")
     pot.write(path)
     cell = synth.cubic_cell(128, 0.05)
     s = synth.Structure(synth.species_exact(128, [0.5, 0.5], rng), synth.random_positions(128, cell, 2.0, seed + 1),
@@ -57,10 +59,15 @@ def structures():
     return synth.batch512(3) + [big]
 
 
+# cutoffs and normalisations of src/nnp/basis.cpp:80-120, 160-168 (the common fast kernels and
+# the moment expansion take them like COS / UNIT)
+CUTS = [dict(cut=c, norm=n) for c in ("STEP", "TANH") for n in ("UNIT", "VOL")] + [dict(norm="VOL")]
+
 CASES = ([dict(zetas=(2.0, 8.0)), dict(zetas=(3.0, 16.0)), dict(zetas=(0.0, 2.0))] +
          [dict(phi_r=p) for p in ("TANH", "SOFTPLUS", "LOGCOSH", "SWISH", "MISH")] +
          [dict(phi_a=p) for p in ("IPOWP", "SECHP", "IPOWS", "SECHS")] +
-         [dict(neuron=n) for n in NEURONS if n != "TANH"])
+         [dict(neuron=n) for n in NEURONS if n != "TANH"] + CUTS +
+         [dict(phi_a="IPOWS", cut="TANH", norm="VOL"), dict(phi_r="SWISH", cut="STEP", norm="VOL")])
 
 
 def case_id(c):
@@ -110,5 +117,44 @@ def test_variant_mixed_precision(tmp_path):
         Eo, Eao, Fo = o.compute(s.types, s.pos, s.cell)
         assert abs(E - Eo) / s.natoms <= 1e-3
         assert np.sqrt(np.mean(np.sum((F - Fo) ** 2, axis=1))) <= 20e-3
+    finally:
+        m.close()
+
+
+@pytest.mark.parametrize("case", CUTS, ids=[case_id(c) for c in CUTS])
+def test_cutoffs_and_norms_triplet_path_and_resident(tmp_path, case):
+    """The same cutoff / norm cases with the G4 sums and force terms by the triplet loops
+    (moments 0), and through the resident graph-replayed pass with the moment expansion."""
+    path = variant_potential(tmp_path, **case)
+    o = ob.OracleModel(path)
+    m = NNP.read(path)
+    try:
+        strucs = structures()
+        ref = [o.compute(s.types, s.pos, s.cell) for s in strucs]
+        m.set_option("moments", 0)
+        for s, (Eo, Eao, Fo) in zip(strucs, ref):
+            E, Ea, F = m.compute(s)
+            assert rel_e(E, Eo, Eao) <= TOL and rel_f(F, Fo) <= TOL, (case, s.natoms)
+        m.set_option("moments", 3)
+        r = m.resident(strucs, NNP_FORCE | NNP_ATOM_ENERGY | NNP_VACUUM_ENERGY)
+        for _ in range(3):
+            r.run()
+        for (E, Ea, F), (Eo, Eao, Fo) in zip(r.fetch(), ref):
+            assert rel_e(E, Eo, Eao) <= TOL and rel_f(F, Fo) <= TOL, case
+    finally:
+        m.close()
+
+
+@pytest.mark.parametrize("case", CUTS, ids=[case_id(c) for c in CUTS])
+def test_cutoffs_and_norms_mixed_precision(tmp_path, case):
+    path = variant_potential(tmp_path, **case)
+    o = ob.OracleModel(path)
+    m = NNP.read(path)
+    try:
+        for s in structures():
+            E, Ea, F = m.compute(s, NNP_FORCE | NNP_VACUUM_ENERGY | NNP_PREC_MIXED)
+            Eo, Eao, Fo = o.compute(s.types, s.pos, s.cell)
+            assert abs(E - Eo) / s.natoms <= 1e-3, case
+            assert np.sqrt(np.mean(np.sum((F - Fo) ** 2, axis=1))) <= 20e-3, case
     finally:
         m.close()

@@ -58,6 +58,23 @@ def fc_cos(r, rc):
     return 0.5 * (math.cos(math.pi * r / rc) + 1.0) if r <= rc else 0.0
 
 
+def fc_closed(cut, r, rc):
+    """The reference's cutoffs (src/nnp/basis.cpp:80-99): COS, STEP (1 inside, 0 outside),
+    TANH = tanh^3(1 - r/rc)."""
+    if r > rc:
+        return 0.0
+    if cut == "COS":
+        return fc_cos(r, rc)
+    if cut == "STEP":
+        return 1.0
+    return math.tanh(1.0 - r / rc) ** 3
+
+
+def norm_closed(norm, rc):
+    """UNIT = 1; VOL = 1 / (1/2 4/3 (pi^2 - 6)/pi rc^3) (src/nnp/basis.cpp:160-168)."""
+    return 1.0 if norm == "UNIT" else 1.0 / (0.5 * 4.0 / 3.0 * (math.pi ** 2 - 6.0) / math.pi * rc ** 3)
+
+
 # ------------------------------------------------------------------ cell / neighbours
 def test_cell_diff_cubic_closed_form():
     cell = np.eye(3) * 10.0
@@ -164,6 +181,55 @@ def test_g2_dimer_closed_form_and_cutoff_integral(tmp_path):
         fcs.append(G[0, 0])
     integral = np.trapezoid(fcs, x[::400])
     assert abs(integral - rc / 2) < 2e-3
+
+
+@pytest.mark.parametrize("norm", ["UNIT", "VOL"])
+@pytest.mark.parametrize("cut", ["STEP", "TANH", "COS"])
+def test_g2_g4_closed_forms_every_cutoff_and_norm(tmp_path, cut, norm):
+    """Dimer G2 = norm exp(-eta (r - rs)^2) fc(r) and symmetric-trimer G4 = norm |(1 + lambda
+    cos)/2|^zeta fc(r)^2 exp(-2 r^2 / eta^2), for each cutoff and normalisation, on both sides of
+    rc (a STEP cutoff that leaked past rc, a TANH cutoff missing its cube, or a dropped VOL factor
+    fails here)."""
+    rc = 6.0
+    rad = [(0.0, 0.05), (0.0, 0.7), (2.0, 0.4)]
+    ang = [(3.0, 1.0, 1), (2.5, 4.0, -1)]
+    pot = tiny_potential(["A"], rc=rc, radial=rad, angular=ang, norm=norm, cut=cut)
+    m = ob.OracleModel(write(pot, tmp_path))
+    nv = norm_closed(norm, rc)
+    # TANH near rc: 1 - r/rc cancels, so its relative rounding grows by the condition number
+    # r / (rc - r) (the reference forms it as 1 - r * (1/rc))
+    cond = lambda r: 1.0 if cut != "TANH" or r > rc else max(1.0, 3.0 * r / (rc - r))
+    for r in [0.7, 1.9, 3.3, 4.8, 5.9, 5.999, 6.001, 6.6]:
+        G = m.symm([0, 0], np.array([[0.0, 0, 0], [r, 0, 0]]), None)
+        want = [nv * math.exp(-eta * (r - rs) ** 2) * fc_closed(cut, r, rc) for rs, eta in rad]
+        np.testing.assert_allclose(G[0, :3], want, rtol=1e-13 * cond(r), atol=1e-300)
+    for r, theta in [(1.5, 1.0), (2.6, 2.2), (3.9, 0.5), (5.95, 1.7), (6.2, 1.0)]:
+        pos = np.array([[0.0, 0, 0], [r, 0, 0], [r * math.cos(theta), r * math.sin(theta), 0]])
+        G = m.symm([0, 0, 0], pos, None)
+        for f, (eta, zeta, lam) in enumerate(ang):
+            want = (nv * abs(0.5 * (1 + lam * math.cos(theta))) ** zeta * fc_closed(cut, r, rc) ** 2
+                    * math.exp(-2 * r * r / eta ** 2))
+            assert abs(G[0, 3 + f] - want) <= 1e-13 * cond(r) * max(abs(want), 1e-300), (cut, norm, r, theta, f)
+
+
+@pytest.mark.parametrize("cut", ["STEP", "TANH"])
+def test_forces_are_minus_gradient_step_and_tanh_cutoffs(tmp_path, cut):
+    """F = -dE/dr on an L > 2 rc cell with the STEP and TANH cutoffs (VOL norm), so cut_grad is
+    pinned: STEP has derivative 0 inside rc (no pair lies within the difference step of rc), TANH
+    -3 tanh^2 (1 - tanh^2) / rc."""
+    pot = tiny_potential(["A", "B"], radial=[(0.0, 0.3), (0.0, 1.1), (1.0, 0.5)],
+                         angular=[(2.5, 1.0, 1), (3.0, 4.0, -1)], hidden=(6,), seed=4, norm="VOL", cut=cut)
+    m = ob.OracleModel(write(pot, tmp_path))
+    n = 120
+    cell = synth.cubic_cell(n, 0.05) * (14.0 / synth.cubic_cell(n, 0.05)[0, 0])
+    rng = np.random.default_rng(12)
+    s = synth.Structure(synth.species_exact(n, [0.5, 0.5], rng), synth.random_positions(n, cell, 2.0, 13), cell)
+    nl = ob.neighbors(s.types, s.pos, s.cell, 6.0)
+    assert np.abs(nl["d"] - 6.0).min() > 1e-4  # no pair crosses rc within the difference step
+    E, Ea, F = m.compute(s.types, s.pos, s.cell, ob.F_FORCE)
+    atoms = [0, 41, 119]
+    fd = fd_forces(m, s, atoms, 0)
+    np.testing.assert_allclose(F[atoms], fd, rtol=1e-6, atol=1e-7 * np.abs(F).max())
 
 
 def test_g4_symmetric_trimer_closed_form(tmp_path):
