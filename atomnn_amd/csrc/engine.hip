@@ -291,7 +291,20 @@ static int estimate_maxp(const std::vector<nnp_structure>& ss, double rc) {
                                          a[2] * (a[3] * a[7] - a[4] * a[6]));
             rho = vol > 0 ? s.natoms / vol : 0;
         }
-        const double p = rho > 0 ? rho * 4.0 / 3.0 * M_PI * rc * rc * rc : (double)s.natoms;
+        if (!(rho > 0) && s.natoms > 0) {
+            // open structure: the density of its bounding box (each side padded by 2 A) bounds
+            // the mean count; atoms in a denser region are caught by the count check after the
+            // build
+            double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+            for (int i = 0; i < s.natoms; ++i)
+                for (int c = 0; c < 3; ++c) {
+                    lo[c] = std::min(lo[c], s.pos[3 * i + c]);
+                    hi[c] = std::max(hi[c], s.pos[3 * i + c]);
+                }
+            const double vol = (hi[0] - lo[0] + 2.0) * (hi[1] - lo[1] + 2.0) * (hi[2] - lo[2] + 2.0);
+            rho = s.natoms / vol;
+        }
+        const double p = std::min((double)s.natoms, rho * 4.0 / 3.0 * M_PI * rc * rc * rc);
         worst = std::max(worst, p);
     }
     int m = (int)std::ceil(worst * 1.5 + 16);
@@ -377,7 +390,21 @@ void Engine::upload(const std::vector<nnp_structure>& ss) {
     d_centres.p = ds + o_cen;
     d_spatoms.p = ds + o_spa;
     HIPCHK(hipMemcpyAsync(d_up.p, h_up.p, total, hipMemcpyHostToDevice, stream));
-    if (maxp == 0) maxp = estimate_maxp(ss, host.rc);
+    // serving lanes size each batch's rows afresh (a batch of large neighbour counts must not
+    // leave every later batch on wide rows); other workspaces keep the stride they reached
+    if (maxp == 0 || defer_count) maxp = estimate_maxp(ss, host.rc);
+}
+
+// Whether the per-centre kernels' LDS holds rows of lm staged entries (FP64 sizes, which bound
+// the FP32 ones).
+bool Engine::lds_fits(int lm) const {
+    const bool gen = pb.plan.generic;
+    if (symm_lds(shape, true, gen, lm, pb.plan.ldg) > 65536) return false;
+    bool all = true;
+    for (int t = 0; t < pb.plan.T; ++t) all = all && pb.plan.mom_ok[t];
+    if (all && !gen && (moments & 2) && force_mom_lds(shape, true, force_waves, lm, pb.plan.ldg, pb.plan.T) <= 65536)
+        return true;
+    return force_lds(shape, true, gen, lm, pb.plan.ldg) <= 65536;
 }
 
 void Engine::build_neighbors(bool all_images) {
@@ -429,7 +456,10 @@ void Engine::build_neighbors(bool all_images) {
             a.bin_atoms = (int*)d_batoms.p;
         }
         HIPCHK(launch_neighbors(a, stream));
-        if (defer_count) {  // checked at the download instead (serving lanes)
+        // serving lanes check the count at the download instead (one host sync per batch), staging
+        // full rows in LDS; when full rows do not fit, the count is read here and the rows staged
+        // at the realised maximum
+        if (defer_count && lds_fits(maxp)) {
             lmaxp = maxp;
             return;
         }

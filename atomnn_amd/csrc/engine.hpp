@@ -92,6 +92,7 @@ class Engine {
     Engine& operator=(const Engine&) = delete;
 
     void upload(const std::vector<nnp_structure>& ss);
+    bool lds_fits(int lm) const;
     void build_neighbors(bool all_images);
     template <typename T>
     void descriptors();

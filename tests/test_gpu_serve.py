@@ -52,3 +52,43 @@ def test_budget_rejection_keeps_serving(potentials):
         assert E == E0
     srv.close()
     m.close()
+
+
+def cluster(n, rho, seed):
+    """An open (non-periodic) cluster of n atoms, random in a ball at density rho, 3 species."""
+    rng = np.random.default_rng(seed)
+    R = (3 * n / (4 * np.pi * rho)) ** (1 / 3)
+    pts = []
+    while len(pts) < n:
+        p = rng.uniform(-R, R, 3)
+        if np.linalg.norm(p) <= R and all(np.linalg.norm(p - q) >= 1.8 for q in pts[-400:]):
+            pts.append(p)
+    return synth.Structure(rng.integers(0, 3, n).astype(np.int32), np.array(pts), None)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_open_cluster_then_periodic_requests(potentials, mixed):
+    """A lane that takes an open cluster of several hundred atoms (whose neighbour count cannot
+    be estimated from a cell) keeps serving: the cluster and the periodic requests after it on the
+    same lanes give the results of nnp_compute alone, and the cluster matches the oracle."""
+    import oracle_binding as ob
+
+    flags = NNP_FORCE | NNP_VACUUM_ENERGY | (NNP_PREC_MIXED if mixed else 0)
+    m = NNP.read(potentials["serve20k"])
+    c = cluster(650, 0.055, 5)
+    periodic = [s for s in synth.serve_pool(40) if s.natoms <= 2500][:6]
+    alone = [m.compute(s, flags) for s in [c] + periodic]
+    srv = serve.Server(m, flags, lanes=2, max_batch_atoms=4000)
+    for rep in range(2):
+        got = [srv.submit(s) for s in [c] + periodic]
+        for (E0, Ea0, F0), p in zip(alone, got):
+            E, Ea, F = p.wait()
+            assert E == E0
+            np.testing.assert_array_equal(F, F0)
+    srv.close()
+    if not mixed:
+        Eo, Eao, Fo = ob.OracleModel(potentials["serve20k"]).compute(c.types, c.pos, None, ob.F_FORCE | ob.F_VACUUM)
+        E, Ea, F = alone[0]
+        assert abs(E - Eo) <= 1e-10 * max(abs(Eo), np.abs(Eao).sum())
+        assert np.linalg.norm(F - Fo, axis=1).max() <= 1e-10 * np.linalg.norm(Fo, axis=1).max()
+    m.close()
