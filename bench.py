@@ -1,29 +1,35 @@
 """Benchmark: energy+force atoms/s of the MI355X engine on one GPU.
 
-Default workload (the plain FP64 GPU configuration): one periodic cubic cell of 4096 atoms,
-two species 50/50 at 0.05 /A^3 (L = 43.4 A), rc = 6.0, 32 G2 + 48 G4 descriptors, one
-80-32-32-1 tanh network per species, energy and forces in FP64.
-
-A step is one full pass of the hot path over the resident batch: neighbour list (cell
-list), descriptors, per-species networks (MFMA), per-entry force terms, force assembly and
-the per-structure energy, with the structure already in HBM.  K steps are timed between two
-stream synchronisations; a second, profiled pass of K steps brackets every stage with HIP
-events to obtain the force kernel's launch duration for the roofline line.
+`value` (the plain FP64 GPU configuration): one periodic cubic cell of 4096 atoms, two species
+50/50 at 0.05 /A^3 (L = 43.4 A), rc = 6.0, 32 G2 + 48 G4 descriptors, one 80-32-32-1 tanh
+network per species, energy and forces in FP64, with the structure resident in HBM.  A step is
+one full pass of the hot path: neighbour list (cell list), descriptors, per-species networks
+(MFMA), per-entry force terms, force assembly and the per-structure energy.  K steps are timed
+between two stream synchronisations; a second, profiled pass of K steps brackets every stage with
+HIP events on the engine's stream to obtain the force kernel's launch duration for the roofline.
 
 Also reported:
-  * "roofline": the force-term kernel's algorithmic FP64 throughput against the FP64 peak, its
+  * "roofline": the force-term kernel's throughput in algorithmic operations (the fixed per-unit
+    convention: per neighbour entry 30 + 13 per radial function, per triplet 75 + 28 per angular
+    function) against the FP64 peak, the operations it actually executes (moment expansion), its
     HBM traffic and VALU-issue floor (from the committed rocprofv3 summaries in profiles/);
-  * "host_buffers": the synchronous nnp_compute call with host buffers (PCIe included);
+  * "end_to_end": the synchronous nnp_compute call from host positions to energies and forces in
+    host memory (PCIe included), the metric as the reference's driver loop sees it;
   * "md": velocity-Verlet steps per second on the resident cell;
   * "mixed": the precision form (FP32 descriptors, split-BF16 MFMA networks, FP64 reductions)
-    on the 3-species 64 x 4096-atom batch beside the plain FP64 form on the same batch, with
-    the energy / force error of the mixed form against the FP64 result;
+    on the 3-species 64 x 4096-atom batch beside the plain FP64 form on the same batch, with stage
+    times, its force kernel's roofline, and errors against the GPU FP64 form;
   * "batching": 512 uneven triclinic cells in one pass (separate kernels, and the persistent
     fused kernel);
   * "training": error + gradient calls of the energy-training loop, and of force training;
   * "serving": open-loop Poisson requests at fractions of the burst capacity, p50 / p99;
-  * "cpu_baseline": the CPU oracle (a restatement of the reference evaluation loop, the
-    reference itself cannot be built) timed on this host's cores on one 4096-atom structure.
+  * "ratios": the mixed serving form over the plain FP64 form (end to end and resident);
+  * "cpu_baseline": the CPU oracle (a restatement of the reference evaluation loop; the reference
+    itself cannot be built) timed on this host's cores: `value` on the 4096-atom cell with one
+    thread, plus bounded subsets of every configuration with up to 16 threads over structures,
+    the CPU model and the -march of the build.  The oracle's energies and forces of the first two
+    cells of the precision batch and of eight serving requests are the FP64 reference of the
+    "oracle_errors" lists (per structure) in "mixed" and "serving".
 
 Prints one JSON line.
 """
@@ -46,7 +52,13 @@ from atomnn_amd.nnp import LOSS_MSE, MD, NNP, NNP_FORCE, NNP_PREC_MIXED, NNP_VAC
 
 METRIC = "E+F atoms/s, mixed vs plain FP64 at ≤1 meV/atom; p50/p99 request latency"
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    """Progress line on stderr (the JSON result goes to stdout)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
 def timed_steps(R: Resident, steps: int, warmup: int):
@@ -96,9 +108,9 @@ def force_ops_moments(P, N, S, nR, NE):
     return P * (30 + 13 * nR + 13 * NE + 35 * (2 + 2 * NE) + 64 + 216 * NE) + N * 35 * NE * S * (3 * S + 1)
 
 
-def traffic_from_profile(path):
-    """HBM bytes per launch from a committed rocprofv3 --pmc summary (FETCH_SIZE / WRITE_SIZE in KiB,
-    already divided by the copy-kernel calibration factor); None when absent."""
+def traffic_from_profile(path, key="traffic_bytes_per_launch"):
+    """A figure of a committed rocprofv3 --pmc summary (HBM bytes from FETCH_SIZE / WRITE_SIZE, scaled
+    by the copy-kernel calibration factors); None when absent."""
     if not os.path.exists(path):
         return None
     vals = {}
@@ -108,7 +120,7 @@ def traffic_from_profile(path):
                 k, v = line.split(":", 1)
                 vals[k.strip()] = v.strip()
     try:
-        return float(vals["traffic_bytes_per_launch"])
+        return float(vals[key])
     except (KeyError, ValueError):
         return None
 
@@ -232,6 +244,90 @@ def valu_floor_from_profile(path, kernel):
     return None
 
 
+def host_cpu():
+    """CPU model name and the CPUs this process may run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        cpus = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cpus = os.cpu_count() or 1
+    return model, cpus
+
+
+def oracle_errors(got, ref, strucs):
+    """Per-structure errors of GPU results against oracle results: |dE| / N (meV/atom) and the RMS
+    over atoms of |dF| (meV/A)."""
+    out = []
+    for (E, _, F), (Eo, _, Fo), s in zip(got, ref, strucs):
+        out.append({"atoms": s.natoms, "energy_meV_per_atom": abs(E - Eo) / s.natoms * 1e3,
+                    "force_rms_meV_per_A": float(np.sqrt(np.mean(np.sum((F - Fo) ** 2, axis=1)))) * 1e3})
+    return out
+
+
+def cpu_baseline_leg(pot_paths, subsets, gpu_rates, march):
+    """The CPU oracle (a restatement of the reference's per-structure loop, nnpce.cpp:275-279:
+    NeighborList::build + NNP::symm + NNP::energy + NNP::force with both network passes) on this
+    host's cores, on bounded subsets of each configuration, with up to 16
+    threads over structures as nnpce spreads structures over MPI ranks (nnpce.cpp:222).  Also
+    returns the oracle's energies and forces of the accuracy samples (the FP64 reference the
+    mixed and serving results are checked against)."""
+    import oracle_binding as ob
+
+    model_name, cpus = host_cpu()
+    threads = min(16, cpus)
+    flags = ob.F_FORCE | ob.F_VACUUM
+    oracles = {k: ob.OracleModel(v) for k, v in pot_paths.items()}
+
+    def rate(key, strucs, nthreads, reps=1):
+        sec = 0.0
+        for _ in range(reps):
+            sec += oracles[key].bench(strucs, flags, nthreads=nthreads)[0]
+        return sum(s.natoms for s in strucs) * reps / sec, sec
+
+    leg = {"cpu_model": model_name, "host_cpus": cpus, "threads_max": threads, "march": march,
+           "compiler": "g++ -O3 -std=gnu++17 (oracle/Makefile)", "configs": {}}
+    c1_rate, c1_sec = rate("plain4096", subsets["plain4096"], 1)
+    leg["value"] = c1_rate
+    leg["sample"] = (f"one plain4096 structure (4096 atoms): neighbour list + descriptors + both network passes + "
+                     f"forces, 1 thread, {c1_sec:.2f} s")
+    cfg = leg["configs"]
+    r0, _ = rate("plain64", subsets["plain64"], 1, reps=10)
+    cfg["plain64"] = {"sample": "the 64-atom cell, 10 repetitions", "threads_1": r0}
+    cfg["plain4096"] = {"sample": "the 4096-atom cell", "threads_1": c1_rate}
+    b = subsets["batch512"]
+    r1, _ = rate("plain4096", b[:8], 1)
+    rn, _ = rate("plain4096", b[:64], threads)
+    cfg["batch512"] = {"sample": f"first 8 cells (1 thread) and first 64 cells ({sum(s.natoms for s in b[:64])} atoms, "
+                                 f"{threads} threads)", "threads_1": r1, f"threads_{threads}": rn}
+    # precision batch: the first structures one at a time (their forces are the accuracy reference)
+    m3 = subsets["mixed64x4096"]
+    t0 = time.perf_counter()
+    ref3 = [oracles["mixed64x4096"].compute(s.types, s.pos, s.cell, flags) for s in m3]
+    sec3 = time.perf_counter() - t0
+    cfg["mixed64x4096"] = {"sample": f"first {len(m3)} of the 64 cells ({sum(s.natoms for s in m3)} atoms), 1 thread",
+                           "threads_1": sum(s.natoms for s in m3) / sec3}
+    sv = subsets["serve20k"]
+    rn4, sec4 = rate("serve20k", sv, threads)
+    cfg["serve20k"] = {"sample": f"first {len(sv)} requests of the stream ({sum(s.natoms for s in sv)} atoms), "
+                                 f"{threads} threads", f"threads_{threads}": rn4}
+    acc = subsets["serve_accuracy"]
+    ref4 = [oracles["serve20k"].compute(s.types, s.pos, s.cell, flags) for s in acc]
+    for key, g in gpu_rates.items():
+        if key in cfg and g:
+            best = max(v for k, v in cfg[key].items() if k.startswith("threads_"))
+            cfg[key]["gpu_atoms_per_s"] = g
+            cfg[key]["gpu_speedup_vs_best_cpu"] = g / best
+    return leg, ref3, ref4
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -268,13 +364,25 @@ def main():
               "across GPUs with no exchange, so the whole-job figure is N times the one-GPU figure.",
               file=sys.stderr)
         sys.exit(2)
+    march = None
+    if not args.no_cpu_baseline:
+        # the oracle build for this host's widest ISA (the reference builds with -march=native);
+        # chosen before any oracle call of the process
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_binding as ob
+
+        march = ob.use_host_isa_build()
     tmp = tempfile.mkdtemp()
-    pot_path = os.path.join(tmp, "plain4096.ann")
-    synth.workload_potential("plain4096").write(pot_path)
+    pot_paths = {}
+    for name in ("plain64", "plain4096", "mixed64x4096", "serve20k"):
+        pot_paths[name] = os.path.join(tmp, f"{name}.ann")
+        synth.workload_potential(name).write(pot_paths[name])
+    pot_path = pot_paths["plain4096"]
     strucs = synth.plain4096()
     natoms = sum(s.natoms for s in strucs)
     model = NNP.read(pot_path)
     flags = NNP_FORCE | NNP_VACUUM_ENERGY
+    log("plain4096 resident passes")
     R = Resident(model, strucs, flags)
     sec = timed_steps(R, args.steps, args.warmup)
     value = natoms * args.steps / sec
@@ -282,7 +390,12 @@ def main():
     P, T = work_counts(model, strucs)
     nR, nA, NE, S = 16, 16, 4, 2
     mom_forces = bool(model.info()["moments"] & 2)
-    ops = force_ops_moments(P, natoms, S, nR, NE) if mom_forces else force_ops_triplets(P, T, nR, nA)
+    # roofline: the force pass's algorithmic operations by the fixed per-unit convention (per
+    # entry 30 + 13 per radial function, per triplet 75 + 28 per angular function, from the
+    # reference expressions), over the kernel's launch time measured with HIP events on the
+    # engine's stream; the operations the moment kernel actually executes are reported beside it
+    ops = force_ops_triplets(P, T, nR, nA)
+    ops_exec = force_ops_moments(P, natoms, S, nR, NE) if mom_forces else ops
     t_force = stage[3] * 1e-3
     achieved = ops / t_force / 1e12
     traffic = traffic_from_profile(os.path.join(ROOT, "profiles", "k_force_plain4096_traffic.txt"))
@@ -301,7 +414,7 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "plain4096: 1 cubic cell, 4096 atoms, 2 species, 0.05/A^3, rc 6.0, 32 G2 + 48 G4, "
-                        "80-32-32-1 tanh, FP64 energy + forces",
+                        "80-32-32-1 tanh, FP64 energy + forces, structure resident in HBM",
             "atoms_per_step": natoms,
             "neighbor_entries": P,
             "triplets": T,
@@ -319,6 +432,10 @@ def main():
             "frac": achieved / FP64_PEAK_TFLOPS,
             "traffic": traffic,
             "ops_per_launch": ops,
+            "ops_convention": "P (30 + 13 nR) + T (75 + 28 nA): the reference's pair and triplet force expressions",
+            # what the moment kernel executes (35 monomials per entry instead of the triplet loops)
+            "executed_ops_per_launch": ops_exec,
+            "executed_frac": ops_exec / t_force / 1e12 / FP64_PEAK_TFLOPS,
             # counter view: the VALU instructions the kernel issues need this long at one per
             # 4 cycles on every SIMD; the launch time over it is the issue efficiency
             "valu_issue_floor_us": valu_floor_from_profile(
@@ -327,27 +444,31 @@ def main():
             "launch_us": t_force * 1e6,
         },
     }
-    # the synchronous C ABI call with host buffers (upload of positions, the pass, download of
-    # energies and forces): the PCIe-inclusive rate, reported beside `value`
+    # the metric as the reference's driver sees it: the synchronous C ABI call from host positions
+    # to energies and forces in host memory (upload, the pass, download; PCIe included)
     model.compute(strucs[0], flags)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         model.compute(strucs[0], flags)
     hsec = (time.perf_counter() - t0) / args.steps
-    out["host_buffers"] = {
-        "call": "nnp_compute (host positions in; host energy, per-atom energies and forces out)",
-        "atoms_per_s": natoms / hsec,
+    e2e = natoms / hsec
+    out["end_to_end"] = {
+        "call": "nnp_compute on the plain4096 cell (host positions in; host energy, per-atom energies and forces "
+                "out), as nnpce's loop calls it",
+        "atoms_per_s": e2e,
         "ms_per_call": hsec * 1e3,
     }
     if not args.no_md:
+        log("md")
         out["md"] = md_line(model, strucs[0], args)
     model_mixed = None
+    gpu_rates = {"plain4096": value}
+    mixed_results = None
+    ms = synth.mixed64x4096(args.mixed_structures)
     if not args.no_mixed:
-        mpath = os.path.join(tmp, "mixed64x4096.ann")
-        synth.workload_potential("mixed64x4096").write(mpath)
-        ms = synth.mixed64x4096(args.mixed_structures)
+        log("mixed64x4096")
         mat = sum(s.natoms for s in ms)
-        model_mixed = NNP.read(mpath)
+        model_mixed = NNP.read(pot_paths["mixed64x4096"])
         res = {}
         for label, fl in (("plain_fp64", NNP_FORCE | NNP_VACUUM_ENERGY),
                           ("mixed", NNP_FORCE | NNP_VACUUM_ENERGY | NNP_PREC_MIXED)):
@@ -355,23 +476,51 @@ def main():
             steps = max(3, args.steps // 4)
             secm = timed_steps(Rm, steps, 1)
             Rm.run()
-            res[label] = (mat * steps / secm, Rm.fetch())
+            got = Rm.fetch()
+            st = profiled_steps(Rm, 2)
+            res[label] = (mat * steps / secm, got, st, secm / steps)
             del Rm
         ref = res["plain_fp64"][1]
-        mix = res["mixed"][1]
-        e_err = max(abs(a[0] - b[0]) / s.natoms for a, b, s in zip(mix, ref, ms))
-        f_rms = math.sqrt(sum(float(np.sum((a[2] - b[2]) ** 2)) for a, b in zip(mix, ref)) / mat)
+        mixed_results = res["mixed"][1]
+        e_err = max(abs(a[0] - b[0]) / s.natoms for a, b, s in zip(mixed_results, ref, ms))
+        f_rms = math.sqrt(sum(float(np.sum((a[2] - b[2]) ** 2)) for a, b in zip(mixed_results, ref)) / mat)
+        Pm, Tm = work_counts(model_mixed, ms)
+        stm = res["mixed"][2]
+        ops_m = force_ops_triplets(Pm, Tm, 12, 8)
         out["mixed"] = {
             "workload": f"mixed64x4096: {len(ms)} cubic cells x 4096 atoms, 3 species 40/30/30, 0.06/A^3, rc 6.5, "
                         "36 G2 + 48 G4, 84-64-64-1; FP32 descriptors, BF16 MFMA networks, FP64 reductions",
             "atoms_per_s": res["mixed"][0],
+            "ms_per_pass": res["mixed"][3] * 1e3,
             "plain_fp64_atoms_per_s": res["plain_fp64"][0],
+            "plain_fp64_ms_per_pass": res["plain_fp64"][3] * 1e3,
             "speedup_vs_plain_fp64": res["mixed"][0] / res["plain_fp64"][0],
-            "max_energy_err_meV_per_atom": e_err * 1e3,
-            "force_rms_err_meV_per_A": f_rms * 1e3,
-            "error_reference": "plain FP64 GPU form (parity-tested against the FP64 CPU oracle at 1e-10)",
+            "neighbor_entries": Pm,
+            "triplets": Tm,
+            "stage_ms": {k: round(float(v), 4) for k, v in
+                         zip(["neighbors", "descriptors", "networks", "force_terms", "force_assembly_and_energy"], stm)},
+            "plain_fp64_stage_ms": {k: round(float(v), 4) for k, v in
+                                    zip(["neighbors", "descriptors", "networks", "force_terms",
+                                         "force_assembly_and_energy"], res["plain_fp64"][2])},
+            "roofline": {
+                "kernel": "k_force_mom<float,2,1> (per-entry force terms, FP32)",
+                "bound": "valu",
+                "achieved": ops_m / (stm[3] * 1e-3) / 1e12,
+                "peak": FP32_PEAK_TFLOPS,
+                "unit": "TFLOP/s",
+                "frac": ops_m / (stm[3] * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
+                "ops_per_launch": ops_m,
+                "launch_us": stm[3] * 1e3,
+                # per-atom HBM bytes of the kernel (profiled on 16 of the cells) times the batch's atoms
+                "traffic": (lambda b: b * mat if b else None)(traffic_from_profile(
+                    os.path.join(ROOT, "profiles", "k_force_mixed64x4096_traffic.txt"), "traffic_bytes_per_atom")),
+            },
+            "max_energy_err_meV_per_atom_vs_gpu_fp64": e_err * 1e3,
+            "force_rms_err_meV_per_A_vs_gpu_fp64": f_rms * 1e3,
         }
+        gpu_rates["mixed64x4096"] = res["mixed"][0]
     if not args.no_batch:
+        log("batch512")
         bstrucs = synth.batch512()
         bat = sum(s.natoms for s in bstrucs)
         model_b = NNP.read(pot_path)
@@ -402,15 +551,18 @@ def main():
                 "mixed_blocks_per_cu": res[(1, "mixed")][1],
             },
         }
+        gpu_rates["batch512"] = res[(0, "fp64")][0]
         model_b.close()
     if not args.no_train:
+        log("training")
         out["training"] = train_line(tmp, args)
+    pool = synth.serve_pool(args.serve_pool)
+    serve_acc = [p for p in pool if p.natoms <= 3000][:8]
+    serve_acc_results = None
     if not args.no_serve:
-        spath = os.path.join(tmp, "serve20k.ann")
-        synth.workload_potential("serve20k").write(spath)
-        pool = synth.serve_pool(args.serve_pool)
+        log("serving")
         mean_atoms = float(np.mean([p.natoms for p in pool]))
-        model_s = NNP.read(spath)
+        model_s = NNP.read(pot_paths["serve20k"])
         sflags = NNP_FORCE | NNP_VACUUM_ENERGY | NNP_PREC_MIXED
         srv = serve.Server(model_s, sflags, lanes=3)
         # capacity probe: everything at t = 0, after one untimed burst that sizes the lanes'
@@ -428,13 +580,13 @@ def main():
             sweep.append((load, r))
         rate = args.serve_load * burst.atoms_per_s / mean_atoms
         res = [r for load, r in sweep if load == args.serve_load][0]
-        # accuracy of the served (mixed) results against FP64 on a sample of the pool
-        sample = [p for p in pool if p.natoms <= 6000][:24]
-        got = serve.run_stream(srv, sample, np.zeros(len(sample)), keep_results=True).results
-        ref = model_s.compute_batch(sample, NNP_FORCE | NNP_VACUUM_ENERGY)
-        e_err = max(abs(a[0] - b[0]) / s.natoms for a, b, s in zip(got, ref, sample))
-        f_rms = math.sqrt(sum(float(np.sum((a[2] - b[2]) ** 2)) for a, b in zip(got, ref)) /
-                          sum(s.natoms for s in sample))
+        # served (mixed) results of an accuracy sample, against the GPU's FP64 form here and
+        # against the oracle in the CPU-baseline leg
+        serve_acc_results = serve.run_stream(srv, serve_acc, np.zeros(len(serve_acc)), keep_results=True).results
+        ref = model_s.compute_batch(serve_acc, NNP_FORCE | NNP_VACUUM_ENERGY)
+        e_err = max(abs(a[0] - b[0]) / s.natoms for a, b, s in zip(serve_acc_results, ref, serve_acc))
+        f_rms = math.sqrt(sum(float(np.sum((a[2] - b[2]) ** 2)) for a, b in zip(serve_acc_results, ref)) /
+                          sum(s.natoms for s in serve_acc))
         srv.close()
         out["serving"] = {
             "workload": f"serve20k: {args.serve_requests} open-loop Poisson requests at {args.serve_load:g} of the burst "
@@ -451,32 +603,48 @@ def main():
             "batches": res.stats["batches"],
             "max_concurrent_batches": res.stats["max_concurrent_batches"],
             "peak_workspace_bytes": res.stats["peak_bytes"],
-            "max_energy_err_meV_per_atom": e_err * 1e3,
-            "force_rms_err_meV_per_A": f_rms * 1e3,
-            "error_reference": "plain FP64 GPU form on 24 pool structures",
+            "max_energy_err_meV_per_atom_vs_gpu_fp64": e_err * 1e3,
+            "force_rms_err_meV_per_A_vs_gpu_fp64": f_rms * 1e3,
             "load": args.serve_load,
             "load_sweep": [{"load": load, "arrival_rate_per_s": load * burst.atoms_per_s / mean_atoms,
                             "atoms_per_s": r.atoms_per_s, "p50_ms": r.p50_ms, "p99_ms": r.p99_ms}
                            for load, r in sweep],
         }
-        out["ratio_mixed_serving_vs_plain_fp64"] = res.atoms_per_s / value
+        gpu_rates["serve20k"] = burst.atoms_per_s
+        # the target ratio, mixed batched serving over the plain FP64 GPU form, both ways the plain
+        # form can be timed
+        out["ratios"] = {
+            "serving_capacity_vs_plain_fp64_end_to_end": burst.atoms_per_s / e2e,
+            "serving_capacity_vs_plain_fp64_resident": burst.atoms_per_s / value,
+            "serving_at_load_vs_plain_fp64_end_to_end": res.atoms_per_s / e2e,
+        }
+        if "mixed" in out:
+            out["ratios"]["mixed_vs_plain_fp64_same_batch_resident"] = out["mixed"]["speedup_vs_plain_fp64"]
         model_s.close()
     if not args.no_cpu_baseline:
-        sys.path.insert(0, os.path.join(ROOT, "tests"))
-        import oracle_binding as ob
-
-        om = ob.OracleModel(pot_path)
-        sec_cpu, _ = om.bench(strucs, ob.F_FORCE | ob.F_VACUUM, nthreads=1)
-        cpu_rate = natoms / sec_cpu
+        log("cpu baseline")
+        subsets = {
+            "plain64": synth.plain64(),
+            "plain4096": strucs,
+            "batch512": synth.batch512(64),
+            "mixed64x4096": ms[:2],
+            "serve20k": pool[:24],
+            "serve_accuracy": serve_acc,
+        }
+        leg, ref3, ref4 = cpu_baseline_leg(pot_paths, subsets, gpu_rates, march)
         out["cpu_baseline"] = {
-            "value": cpu_rate,
+            "value": leg.pop("value"),
             "unit": "atoms/s",
             "cores": 1,
             "kind": "port",
-            "sample": "one plain4096 structure (4096 atoms): neighbour list + descriptors + both network passes + "
-                      f"forces, single thread, {sec_cpu:.2f} s",
+            "sample": leg.pop("sample"),
+            **leg,
         }
-        out["speedup_vs_cpu"] = value / cpu_rate
+        out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
+        if mixed_results is not None:
+            out["mixed"]["oracle_errors"] = oracle_errors(mixed_results[:len(ref3)], ref3, ms[:len(ref3)])
+        if serve_acc_results is not None:
+            out["serving"]["oracle_errors"] = oracle_errors(serve_acc_results, ref4, serve_acc)
     print(json.dumps(out))
     model.close()
     if model_mixed is not None:

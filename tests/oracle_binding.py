@@ -14,6 +14,33 @@ import numpy as np
 
 _ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(_ROOT, "oracle", "lib", "libnnp_oracle.so")
+LIB_PATH_V4 = os.path.join(_ROOT, "oracle", "lib", "libnnp_oracle_v4.so")
+_V4_FLAGS = {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"}
+
+
+def cpu_has_v4() -> bool:
+    """Whether this host's CPU runs x86-64-v4 code (AVX-512 F/BW/CD/DQ/VL)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    return _V4_FLAGS <= set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    return False
+
+
+def use_host_isa_build() -> str:
+    """Load the oracle built for the widest ISA this host supports (the CPU-baseline timing; the
+    reference builds with -march=native).  Must be called before the first oracle call of the
+    process; returns the -march the loaded library was built with."""
+    global LIB_PATH
+    if _lib is not None:
+        raise RuntimeError("the oracle library is already loaded")
+    if cpu_has_v4() and os.path.exists(LIB_PATH_V4):
+        LIB_PATH = LIB_PATH_V4
+        return "x86-64-v4"
+    return "x86-64-v3"
 
 F_FORCE = 1
 F_ATOM_ENERGY = 2
