@@ -59,6 +59,11 @@ struct AssembleArgs {
     int* rcount;   // mode 1: [ntot] entries in each receiver row
     int* rlist;    // mode 1: [ntot][rcap] entry indices
     int rcap;
+    // processing order of the atoms: the cell-list atoms in bin order (bin_atoms[0, ncell)), then
+    // the other atoms (pair_centres[0, ntot - ncell)); each XCD's blocks take a contiguous chunk of it
+    const int* bin_atoms;
+    const int* pair_centres;
+    int ncell;
 };
 
 // End-of-pass reductions, run by single-wavefront blocks appended to the assembly launch (or on

@@ -553,6 +553,10 @@ void Engine::assemble(const EnergyTail* tail) {
     a.rcount = nullptr;
     a.rlist = nullptr;
     a.rcap = 0;
+    // atoms in bin order (cell-list structures) then the rest, for the XCD-local block order
+    a.ncell = nbins > 0 ? (int)h_cell_centres.size() : 0;
+    a.bin_atoms = a.ncell > 0 ? (const int*)d_batoms.p : nullptr;
+    a.pair_centres = (const int*)d_centres.p + a.ncell;
     if (assemble_mode == 1) {
         // a receiver row holds about as many entries as a centre's row
         rcap = std::max(rcap, std::min(maxp, 8192));

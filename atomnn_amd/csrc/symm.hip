@@ -1568,6 +1568,12 @@ template hipError_t launch_fused<float>(int, int, int, const DevPlan*, const dou
 // react[1] (largest length seen) and the host redoes the pass with longer rows.
 //
 // Lane partials are summed in a fixed tree order (deterministic, no atomics on F).
+//
+// Block order: workgroups are dealt to the 8 XCDs round-robin, each XCD with its own L2.  Block b
+// (after the tail blocks, padded to a multiple of 8) takes position (b mod 8) * chunk + b / 8 of the
+// processing order (cell-list atoms by bin, then the rest), so the blocks an XCD runs at any time
+// are neighbours in space and the rows and terms they look up in each other's rows are shared in
+// that XCD's L2, instead of every XCD touching the rows of the whole batch.
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
@@ -1623,21 +1629,31 @@ __device__ __forceinline__ int row_lower_bound(const int* __restrict__ nbr_j, in
     return lo + c;
 }
 
+constexpr int kXcds = 8;
+
 template <typename T, int MODE>
 __global__ __launch_bounds__(kWave) void k_assemble(const DevStruc* __restrict__ S, const int* __restrict__ atom_struc,
                                                     const int* __restrict__ cnt, const int* __restrict__ nbr_j,
                                                     const int* __restrict__ nbr_meta, const T* __restrict__ X, int maxp,
                                                     int all_images, double* __restrict__ F, EnergyTail tail,
                                                     int* __restrict__ react, const int* __restrict__ rcount,
-                                                    const int* __restrict__ rlist, int rcap) {
+                                                    const int* __restrict__ rlist, int rcap,
+                                                    const int* __restrict__ bin_atoms,
+                                                    const int* __restrict__ pair_centres, int ncell, int ntot,
+                                                    int chunk) {
     // the end-of-pass reduction blocks come first, so they run beside the atoms' blocks
     const int ntail = tail.nstruc + tail.ncount;
+    const int ntail_pad = (ntail + kXcds - 1) / kXcds * kXcds;
     const int lane = threadIdx.x;
     if ((int)blockIdx.x < ntail) {
         energy_tail_block(tail, blockIdx.x, lane);
         return;
     }
-    const int a = blockIdx.x - ntail;
+    if ((int)blockIdx.x < ntail_pad) return;
+    const int b = blockIdx.x - ntail_pad;
+    const int pos = (b % kXcds) * chunk + b / kXcds;
+    if (pos >= ntot) return;
+    const int a = pos < ncell ? bin_atoms[pos] : pair_centres[pos - ncell];
     const int P = min(cnt[a], maxp);
     const int64_t row = (int64_t)a * maxp;
     double fx = 0, fy = 0, fz = 0;
@@ -1903,10 +1919,14 @@ template <typename T>
 hipError_t launch_assemble(const AssembleArgs& a, const T* X, const EnergyTail* tail, hipStream_t st) {
     EnergyTail t{};
     if (tail) t = *tail;  // without a tail, nstruc = ncount = 0: no extra blocks
-    const dim3 grid(a.ntot + t.nstruc + t.ncount);
+    const int ntail_pad = (t.nstruc + t.ncount + kXcds - 1) / kXcds * kXcds;
+    const int chunk = (a.ntot + kXcds - 1) / kXcds;
+    const dim3 grid(ntail_pad + kXcds * chunk);
+    const int ncell = a.bin_atoms ? a.ncell : 0;
     if (a.mode == 0) {
         hipLaunchKernelGGL((k_assemble<T, 0>), grid, dim3(kWave), 0, st, a.S, a.atom_struc, a.cnt, a.nbr_j, a.meta, X,
-                           a.maxp, a.all_images, a.F, t, a.react, nullptr, nullptr, 0);
+                           a.maxp, a.all_images, a.F, t, a.react, nullptr, nullptr, 0, a.bin_atoms, a.pair_centres,
+                           ncell, a.ntot, chunk);
         return hipGetLastError();
     }
     const size_t lds = (size_t)a.rcap * 8;
@@ -1915,7 +1935,8 @@ hipError_t launch_assemble(const AssembleArgs& a, const T* X, const EnergyTail* 
     hipLaunchKernelGGL(k_react_fill, dim3(a.ntot), dim3(kWave), 0, st, a.S, a.atom_struc, a.cnt, a.nbr_j, a.meta,
                        a.maxp, a.all_images, a.rcount, a.rlist, a.rcap);
     hipLaunchKernelGGL((k_assemble<T, 1>), grid, dim3(kWave), lds, st, a.S, a.atom_struc, a.cnt, a.nbr_j, a.meta, X,
-                       a.maxp, a.all_images, a.F, t, a.react, a.rcount, a.rlist, a.rcap);
+                       a.maxp, a.all_images, a.F, t, a.react, a.rcount, a.rlist, a.rcap, a.bin_atoms, a.pair_centres,
+                       ncell, a.ntot, chunk);
     return hipGetLastError();
 }
 
