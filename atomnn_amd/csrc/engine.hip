@@ -420,41 +420,15 @@ void Engine::build_neighbors(bool all_images) {
             d_maxcnt.reserve(64);
             HIPCHK(hipMemsetAsync(d_maxcnt.p, 0, 64, stream));
         }
-        NeighborArgs a;
-        a.S = (const DevStruc*)d_strucs.p;
-        a.atom_struc = (const int*)d_astruc.p;
-        a.pos = (const double*)d_pos.p;
-        a.type = (const int*)d_type.p;
-        a.ntot = ntot;
-        a.nstruc = nstruc;
-        a.rc2 = host.rc * host.rc;
-        a.maxp = maxp;
-        a.cnt = (int*)d_cnt.p;
-        a.nbr_j = (int*)d_nbrj.p;
-        a.nbr_meta = (int*)d_meta.p;
-        a.nbr_r = (double4*)d_nbrr.p;
-        a.maxcnt = (int*)d_maxcnt.p;
         d_work.reserve(64);
-        a.work = (int*)d_work.p;
-        a.cell_centres = (const int*)d_centres.p;
-        a.ncell = (int)h_cell_centres.size();
-        a.pair_centres = (const int*)d_centres.p + h_cell_centres.size();
-        a.npair = (int)h_pair_centres.size();
-        a.nbins = nbins;
-        a.cell_natoms_max = cell_natoms_max;
-        a.all_images = all_images ? 1 : 0;
         if (nbins > 0) {
             d_abin.reserve((size_t)ntot * 4);
             d_bcount.reserve((size_t)nbins * 4);
             d_bstart.reserve((size_t)nbins * 4);
             d_bfill.reserve((size_t)nbins * 4);
             d_batoms.reserve((size_t)ntot * 4);
-            a.atom_bin = (int*)d_abin.p;
-            a.bin_count = (int*)d_bcount.p;
-            a.bin_start = (int*)d_bstart.p;
-            a.bin_fill = (int*)d_bfill.p;
-            a.bin_atoms = (int*)d_batoms.p;
         }
+        const NeighborArgs a = neighbor_args(all_images);
         HIPCHK(launch_neighbors(a, stream));
         // serving lanes check the count at the download instead (one host sync per batch), staging
         // full rows in LDS; when full rows do not fit, the count is read here and the rows staged
@@ -815,11 +789,8 @@ void Engine::resident_run(uint32_t flags) {
     HIPCHK(hipGraphLaunch(graph_exec, stream));
 }
 
-// Enqueue one full pass on the engine's stream (no host sync).
-void Engine::enqueue_pass(uint32_t flags) {
-    last_mom = 0;
-    const int fused_nb = pipeline == 1 ? fused_blocks(flags & NNP_PREC_MIXED, flags) : 0;
-    const bool mixed = flags & NNP_PREC_MIXED;
+// The neighbour launch's arguments over the current batch and rows (buffers already sized).
+NeighborArgs Engine::neighbor_args(bool all_images) const {
     NeighborArgs a;
     a.S = (const DevStruc*)d_strucs.p;
     a.atom_struc = (const int*)d_astruc.p;
@@ -841,16 +812,29 @@ void Engine::enqueue_pass(uint32_t flags) {
     a.npair = (int)h_pair_centres.size();
     a.nbins = nbins;
     a.cell_natoms_max = cell_natoms_max;
-    a.all_images = (flags & NNP_ALL_IMAGES) ? 1 : 0;
+    a.all_images = all_images ? 1 : 0;
     a.atom_bin = (int*)d_abin.p;
     a.bin_count = (int*)d_bcount.p;
     a.bin_start = (int*)d_bstart.p;
     a.bin_fill = (int*)d_bfill.p;
     a.bin_atoms = (int*)d_batoms.p;
-    a.count_max = false;  // folded into the energy launch at the end of the pass
-    stage_begin(ST_NEIGH);
-    HIPCHK(launch_neighbors(a, stream));
-    stage_end(ST_NEIGH);
+    return a;
+}
+
+// Enqueue one full pass on the engine's stream (no host sync).  With rows_ready the neighbour
+// rows (and the cleared per-pass counters) were provided by the caller (the MD step's list
+// refresh), so the pass starts with the descriptors.
+void Engine::enqueue_pass(uint32_t flags, bool rows_ready) {
+    last_mom = 0;
+    const int fused_nb = pipeline == 1 ? fused_blocks(flags & NNP_PREC_MIXED, flags) : 0;
+    const bool mixed = flags & NNP_PREC_MIXED;
+    if (!rows_ready) {
+        NeighborArgs a = neighbor_args(flags & NNP_ALL_IMAGES);
+        a.count_max = false;  // folded into the energy launch at the end of the pass
+        stage_begin(ST_NEIGH);
+        HIPCHK(launch_neighbors(a, stream));
+        stage_end(ST_NEIGH);
+    }
     if (pipeline == 1) {
         stage_begin(ST_SYMM);  // descriptors + networks + force terms in one kernel
         fused(mixed, flags, fused_nb);

@@ -191,7 +191,8 @@ class Engine {
     int graph_lmaxp = 0;
     int eager_runs = 0;
     void drop_graph();
-    void enqueue_pass(uint32_t flags);
+    void enqueue_pass(uint32_t flags, bool rows_ready = false);
+    NeighborArgs neighbor_args(bool all_images) const;
     std::vector<int> h_natoms;  // atoms per structure of the current batch
     bool resident_valid = false;
     std::vector<double> h_E;

@@ -89,6 +89,7 @@ def lib():
         L.nnp_set_option.argtypes = [vp, ctypes.c_char_p, i64]
         L.nnp_md_create.argtypes = [vp, ctypes.POINTER(_Struc), vp, vp, dp, ctypes.c_uint32, ctypes.POINTER(vp)]
         L.nnp_md_step.argtypes = [vp, ctypes.c_int]
+        L.nnp_md_vlist.argtypes = [vp, ctypes.c_double, ctypes.c_int]
         L.nnp_md_state.argtypes = [vp, vp, vp, vp, vp]
         L.nnp_md_destroy.argtypes = [vp]
         L.nnp_train_create.argtypes = [vp, ctypes.c_int, ctypes.POINTER(_Struc), vp, ctypes.POINTER(vp)]
@@ -272,6 +273,10 @@ class MD:
         _check(lib().nnp_md_create(model._h, ctypes.byref(pin.c_struct), m.ctypes.data,
                                    None if v is None else v.ctypes.data, float(dt), flags, ctypes.byref(h)))
         self._h = h
+
+    def vlist(self, rc_list: float, stride: int) -> None:
+        """Verlet list of radius rc_list (>= rc) rebuilt every `stride` steps (1: every step)."""
+        _check(lib().nnp_md_vlist(self._h, float(rc_list), int(stride)))
 
     def step(self, nsteps: int = 1) -> None:
         _check(lib().nnp_md_step(self._h, int(nsteps)))

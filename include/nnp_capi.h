@@ -244,6 +244,16 @@ void nnp_train_destroy(nnp_train* t);
 typedef struct nnp_md nnp_md;
 int nnp_md_create(nnp_model* m, const nnp_structure* s, const double* masses, const double* vel, double dt,
                   uint32_t flags, nnp_md** out);
+/* Verlet list for the MD steps (the reference's `vlist rc stride` input, struc/verlet.cpp:35-43,
+ * rebuilt every `stride` steps by its MD driver, torch/torch.cpp:269): the list holds every
+ * neighbour within rc_list >= rc and is built at the first step of each nnp_md_step call and
+ * every `stride` steps after; the other steps recompute the displacements of the list's entries
+ * and evaluate the potential on those within rc, the list a fresh build would give
+ * (NeighborList::build semantics, min-image flags included), so forces equal those of a build
+ * every step up to rounding.  Positions are returned to the cell only at builds.  A call in
+ * which some atom moved more than (rc_list - rc) / 2 since a build is redone with a build every
+ * step.  stride 1 (the default) builds a fresh list every step. */
+int nnp_md_vlist(nnp_md* md, double rc_list, int stride);
 int nnp_md_step(nnp_md* md, int nsteps);
 int nnp_md_state(nnp_md* md, double* pos, double* vel, double* force, double* energies);
 void nnp_md_destroy(nnp_md* md);

@@ -181,3 +181,87 @@ def test_md_with_receiver_rows_on_a_small_crystal(potentials):
     assert np.abs(st["pos"] - x).max() < 1e-9
     assert np.abs(st["vel"] - v).max() < 1e-11
     assert abs(st["epot"] - E) <= 1e-9 * max(abs(E), 1.0)
+
+
+def run_md(path, s, v0, dt, calls, flags=NNP_FORCE, vlist=None, masses=MASSES):
+    m = NNP.read(path)
+    md = MD(m, s, masses, v0, dt, flags)
+    try:
+        if vlist:
+            md.vlist(*vlist)
+        for k in calls:
+            md.step(k)
+        st = md.state()
+    finally:
+        md.close()
+        m.close()
+    return st
+
+
+@pytest.mark.parametrize("case", ["cell300", "plain64"])
+def test_md_verlet_list_matches_builds_every_step(potentials, case):
+    """A Verlet list of radius rc + 1 A rebuilt every 5 steps (the reference's MD driver,
+    torch/torch.cpp:269) gives the trajectory of a fresh list every step: the same entries in the
+    same order with the min-image flags recomputed, displacements equal up to rounding.  The
+    64-atom cell (L = 10.9 A < 2 rc, 27 images per pair) exercises the min-image rule's flags."""
+    if case == "cell300":
+        s, rng = cell_300()
+        path = potentials["plain4096"]
+        v0, masses = velocities(s, rng, 600.0), MASSES
+    else:
+        s = synth.plain64()[0]
+        rng = np.random.default_rng(3)
+        path = potentials["plain64"]
+        masses = [28.0855]
+        v0 = velocities(s, rng, 600.0, masses)
+    dt = 1.0
+    ref = run_md(path, s, v0, dt, [12], masses=masses)
+    for calls in ([12], [4, 4, 4], [7, 5]):
+        st = run_md(path, s, v0, dt, calls, vlist=(7.0, 5), masses=masses)
+        assert st["steps"] == 12
+        np.testing.assert_allclose(wrap(st["pos"], s.cell), wrap(ref["pos"], s.cell), rtol=0, atol=1e-9)
+        np.testing.assert_allclose(st["vel"], ref["vel"], rtol=0, atol=1e-10)
+        np.testing.assert_allclose(st["force"], ref["force"], rtol=0, atol=1e-8 * np.abs(ref["force"]).max())
+        assert abs(st["epot"] - ref["epot"]) <= 1e-10 * abs(ref["epot"])
+
+
+def test_md_verlet_list_against_oracle_verlet(potentials):
+    s, rng = cell_300(12)
+    v0 = velocities(s, rng, 600.0)
+    dt, nsteps = 1.0, 6
+    st = run_md(potentials["plain4096"], s, v0, dt, [nsteps], vlist=(7.5, 3))
+    o = ob.OracleModel(potentials["plain4096"])
+    mass = np.array([MASSES[t] for t in s.types])[:, None]
+    x, v = s.pos.copy(), v0.copy()
+    E, _, F = o.compute(s.types, x, s.cell, ob.F_FORCE)
+    for _ in range(nsteps):
+        v += 0.5 * dt * F / (mass * CONV)
+        x = wrap(x + v * dt, s.cell)
+        E, _, F = o.compute(s.types, x, s.cell, ob.F_FORCE)
+        v += 0.5 * dt * F / (mass * CONV)
+    np.testing.assert_allclose(wrap(st["pos"], s.cell), x, rtol=0, atol=1e-9)
+    assert np.abs(st["vel"] - v).max() < 1e-10
+    assert abs(st["epot"] - E) <= 1e-9 * max(abs(E), 1.0)
+
+
+def test_md_verlet_list_thin_skin_falls_back(potentials):
+    """A skin of 1e-4 A is outrun at once: each call is redone with a fresh list every step, which
+    is the stride-1 trajectory bit for bit."""
+    s, rng = cell_300(5)
+    v0 = velocities(s, rng, 900.0)
+    ref = run_md(potentials["plain4096"], s, v0, 1.0, [6, 3])
+    st = run_md(potentials["plain4096"], s, v0, 1.0, [6, 3], vlist=(6.0001, 4))
+    for k in ("pos", "vel", "force"):
+        assert np.array_equal(st[k], ref[k]), k
+    assert st["epot"] == ref["epot"] and st["steps"] == ref["steps"]
+
+
+def test_md_verlet_list_all_images(potentials):
+    s = synth.plain64()[0]
+    rng = np.random.default_rng(9)
+    v0 = velocities(s, rng, 400.0, [28.0855])
+    ref = run_md(potentials["plain64"], s, v0, 0.5, [10], flags=NNP_FORCE | NNP_ALL_IMAGES, masses=[28.0855])
+    st = run_md(potentials["plain64"], s, v0, 0.5, [10], flags=NNP_FORCE | NNP_ALL_IMAGES, vlist=(7.2, 10),
+                masses=[28.0855])
+    np.testing.assert_allclose(wrap(st["pos"], s.cell), wrap(ref["pos"], s.cell), rtol=0, atol=1e-9)
+    np.testing.assert_allclose(st["vel"], ref["vel"], rtol=0, atol=1e-10)
