@@ -70,9 +70,36 @@ void set_error(const std::string& s) { g_err = s; }
 const char* last_error() { return g_err.c_str(); }
 
 
+// ------------------------------------------------------------------ scratch
+// The workspace of nnp_compute_noalloc: one fixed array per device, held until the process ends.
+// Host code never uses the array's name as an address (in host code it is a stand-in, not device
+// memory): its device address is looked up once per device, with that device current, and kept.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+// The scratch array's address on the current device (looked up on the first call per device).
+static void* scratch_on_current_device(int& dev) {
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device index out of range for the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) throw hip_failure("hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_ptr[dev] = ptr;
+    }
+    return g_scratch_ptr[dev];
+}
+
 // ------------------------------------------------------------------ device buffers
 void DevMem::reserve(size_t bytes) {
     if (bytes <= cap) return;
+    if (bound)
+        throw unsupported_error("nnp_compute_noalloc: the structure needs more workspace than the library's " +
+                                std::to_string(kScratchBytes >> 20) + " MiB scratch array holds");
     if (p) (void)hipFree(p);
     p = nullptr;
     cap = 0;
@@ -105,9 +132,10 @@ void HostPinned::release() {
 }
 
 void DevMem::release() {
-    if (p) (void)hipFree(p);
+    if (p && !bound) (void)hipFree(p);
     p = nullptr;
     cap = 0;
+    bound = false;
 }
 
 // ------------------------------------------------------------------ structures
@@ -281,7 +309,7 @@ Engine::~Engine() {
 }
 
 // Estimated entries per atom from the density, plus head room; refined after the first build.
-static int estimate_maxp(const std::vector<nnp_structure>& ss, double rc) {
+static int estimate_maxp(Strucs ss, double rc) {
     double worst = 0;
     for (const auto& s : ss) {
         double rho = 0;
@@ -312,7 +340,7 @@ static int estimate_maxp(const std::vector<nnp_structure>& ss, double rc) {
     return std::max(64, std::min(m, 4096));
 }
 
-void Engine::upload(const std::vector<nnp_structure>& ss) {
+void Engine::upload(Strucs ss) {
     HIPCHK(hipStreamSynchronize(stream));  // the staging buffer may still feed an earlier copy
     resident_valid = false;
     drop_graph();
@@ -361,11 +389,9 @@ void Engine::upload(const std::vector<nnp_structure>& ss) {
     for (int i = 0; i < ntot; ++i) sp_off[h_type[i] + 1]++;
     for (int t = 0; t < host.T; ++t) sp_off[t + 1] += sp_off[t];
     h_spatoms.resize(ntot);
-    {
-        std::vector<int> fill(sp_off.begin(), sp_off.end() - 1);
-        for (int i = 0; i < ntot; ++i) h_spatoms[fill[h_type[i]]++] = i;
-    }
-    h_centres = h_cell_centres;
+    sp_fill.assign(sp_off.begin(), sp_off.end() - 1);
+    for (int i = 0; i < ntot; ++i) h_spatoms[sp_fill[h_type[i]]++] = i;
+    h_centres.assign(h_cell_centres.begin(), h_cell_centres.end());
     h_centres.insert(h_centres.end(), h_pair_centres.begin(), h_pair_centres.end());
     // one pinned staging buffer, one copy: positions | species | structure of each atom |
     // structures | centres (cell-list first) | species-sorted atoms, each 256-byte aligned
@@ -373,15 +399,7 @@ void Engine::upload(const std::vector<nnp_structure>& ss) {
     const size_t o_pos = 0, o_type = o_pos + al(h_pos.size() * 8), o_ast = o_type + al((size_t)ntot * 4),
                  o_st = o_ast + al((size_t)ntot * 4), o_cen = o_st + al(h_strucs.size() * sizeof(DevStruc)),
                  o_spa = o_cen + al((size_t)ntot * 4), total = o_spa + al((size_t)ntot * 4);
-    h_up.reserve(total);
     d_up.reserve(total);
-    char* hs = (char*)h_up.p;
-    std::memcpy(hs + o_pos, h_pos.data(), h_pos.size() * 8);
-    std::memcpy(hs + o_type, h_type.data(), (size_t)ntot * 4);
-    std::memcpy(hs + o_ast, h_astruc.data(), (size_t)ntot * 4);
-    std::memcpy(hs + o_st, h_strucs.data(), h_strucs.size() * sizeof(DevStruc));
-    std::memcpy(hs + o_cen, h_centres.data(), (size_t)ntot * 4);
-    std::memcpy(hs + o_spa, h_spatoms.data(), (size_t)ntot * 4);
     char* ds = (char*)d_up.p;
     d_pos.p = ds + o_pos;
     d_type.p = ds + o_type;
@@ -389,6 +407,26 @@ void Engine::upload(const std::vector<nnp_structure>& ss) {
     d_strucs.p = ds + o_st;
     d_centres.p = ds + o_cen;
     d_spatoms.p = ds + o_spa;
+    if (bound) {
+        // no staging buffer: each piece goes straight from host memory into its slice
+        HIPCHK(hipMemcpyAsync(d_pos.p, h_pos.data(), h_pos.size() * 8, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(d_type.p, h_type.data(), (size_t)ntot * 4, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(d_astruc.p, h_astruc.data(), (size_t)ntot * 4, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(d_strucs.p, h_strucs.data(), h_strucs.size() * sizeof(DevStruc), hipMemcpyHostToDevice,
+                              stream));
+        HIPCHK(hipMemcpyAsync(d_centres.p, h_centres.data(), (size_t)ntot * 4, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(d_spatoms.p, h_spatoms.data(), (size_t)ntot * 4, hipMemcpyHostToDevice, stream));
+        maxp = std::min(estimate_maxp(ss, host.rc), maxp_cap);
+        return;
+    }
+    h_up.reserve(total);
+    char* hs = (char*)h_up.p;
+    std::memcpy(hs + o_pos, h_pos.data(), h_pos.size() * 8);
+    std::memcpy(hs + o_type, h_type.data(), (size_t)ntot * 4);
+    std::memcpy(hs + o_ast, h_astruc.data(), (size_t)ntot * 4);
+    std::memcpy(hs + o_st, h_strucs.data(), h_strucs.size() * sizeof(DevStruc));
+    std::memcpy(hs + o_cen, h_centres.data(), (size_t)ntot * 4);
+    std::memcpy(hs + o_spa, h_spatoms.data(), (size_t)ntot * 4);
     HIPCHK(hipMemcpyAsync(d_up.p, h_up.p, total, hipMemcpyHostToDevice, stream));
     // serving lanes size each batch's rows afresh (a batch of large neighbour counts must not
     // leave every later batch on wide rows); other workspaces keep the stride they reached
@@ -405,6 +443,75 @@ bool Engine::lds_fits(int lm) const {
     if (all && !gen && (moments & 2) && force_mom_lds(shape, true, force_waves, lm, pb.plan.ldg, pb.plan.T) <= 65536)
         return true;
     return force_lds(shape, true, gen, lm, pb.plan.ldg) <= 65536;
+}
+
+// A workspace for calls that allocate nothing: the host bookkeeping reserved for max_atoms
+// atoms up front (so the calls' resizes stay within capacity), every batch buffer bound to a slice
+// of the scratch array per call.
+void Engine::make_bound(int max_atoms) {
+    bound = true;
+    pipeline = 0;
+    const size_t n = (size_t)max_atoms;
+    h_pos.reserve(3 * n);
+    h_type.reserve(n);
+    h_astruc.reserve(n);
+    h_cell_centres.reserve(n);
+    h_pair_centres.reserve(n);
+    h_centres.reserve(n);
+    h_spatoms.reserve(n);
+    h_eatom.reserve(n);
+    h_F.reserve(3 * n);
+    h_E.reserve(1);
+    h_natoms.reserve(1);
+    h_strucs.reserve(1);
+    sp_off.reserve(kMaxSpecies + 1);
+    sp_fill.reserve(kMaxSpecies + 1);
+}
+
+// Slices of [base, base + bytes) for one structure: the per-atom buffers, the cell-list bins of
+// its cell, the descriptor rows (FP64 widths, which hold the FP32 forms too), and what remains for
+// the neighbour rows, force terms and receiver rows at the largest stride that fits (maxp_cap).
+void Engine::bind_scratch(void* base, size_t bytes, const nnp_structure& s) {
+    const size_t N = (size_t)s.natoms;
+    const DevStruc d = make_struc(s, host.rc, 0);
+    const size_t nb = d.mode == 1 ? (size_t)d.nbin[0] * d.nbin[1] * d.nbin[2] : 1;
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t up = al(N * 24) + 4 * al(N * 4) + al(sizeof(DevStruc));
+    const size_t ldg = (size_t)pb.plan.ldg;
+    size_t off = 0;
+    char* b = (char*)base;
+    auto take = [&](DevMem& m, size_t sz) {
+        m.bind(b + off, al(sz));
+        off += al(sz);
+    };
+    take(d_up, up);
+    take(d_maxcnt, 64);
+    take(d_work, 64);
+    take(d_E, 8);
+    take(d_sstart, 8);
+    for (DevMem* m : {&d_cnt, &d_abin, &d_batoms, &d_rcount}) take(*m, N * 4);
+    take(d_eatom, N * 8);
+    take(d_F, N * 24);
+    for (DevMem* m : {&d_bcount, &d_bstart, &d_bfill}) take(*m, nb * 4);
+    take(d_G, N * ldg * 8);
+    take(d_dEdG, N * ldg * 8);
+    // per row slot: neighbour index 4, metadata 4, displacement 32, FP64 force term 32, receiver
+    // row entry 4 bytes
+    const size_t slot = 76;
+    const size_t rest = bytes > off + 5 * 256 ? bytes - off - 5 * 256 : 0;
+    const int fit = (int)std::min<size_t>(rest / (N * slot) / 32 * 32, 8192);
+    if (fit < 32)
+        throw std::invalid_argument("nnp_compute_noalloc: " + std::to_string(N) + " atoms need more than the " +
+                                    std::to_string(kScratchBytes >> 20) + " MiB scratch array holds");
+    maxp_cap = fit;
+    take(d_nbrj, N * fit * 4);
+    take(d_meta, N * fit * 4);
+    take(d_nbrr, N * fit * 32);
+    take(d_X, N * fit * 32);
+    take(d_rlist, N * fit * 4);
+    HIPCHK(hipMemsetAsync(d_maxcnt.p, 0, 64, stream));  // counters the mode does not write read as zero
+    maxp = 0;
+    rcap = 0;
 }
 
 void Engine::build_neighbors(bool all_images) {
@@ -567,7 +674,7 @@ void Engine::forces() {
     assemble<T>();
 }
 
-void Engine::run(const std::vector<nnp_structure>& ss, uint32_t flags) {
+void Engine::run(Strucs ss, uint32_t flags) {
     const bool mixed = flags & NNP_PREC_MIXED;
     last_mom = 0;
     upload(ss);
@@ -728,7 +835,7 @@ void Engine::energies(bool count_max) {
 
 // Upload a batch and size the neighbour rows once (with head room), so later passes run
 // without a host round trip.
-void Engine::resident_load(const std::vector<nnp_structure>& ss, uint32_t flags) {
+void Engine::resident_load(Strucs ss, uint32_t flags) {
     upload(ss);
     build_neighbors(flags & NNP_ALL_IMAGES);
     const int want = (max_entries + max_entries / 8 + 8 + 31) / 32 * 32;
@@ -876,6 +983,17 @@ void Engine::enqueue_pass(uint32_t flags, bool rows_ready) {
 
 // Staged download: [counters | E per structure | E_i | F] into pinned memory, one sync.
 int Engine::download(bool energies, bool forces) {
+    if (bound) {  // straight into the pre-reserved host vectors, no staging buffer
+        h_eatom.resize(ntot);
+        if (energies) h_E.resize(nstruc);
+        if (forces) h_F.resize(3 * (size_t)ntot);
+        HIPCHK(hipMemcpyAsync(counters, d_maxcnt.p, 16, hipMemcpyDeviceToHost, stream));
+        if (energies) HIPCHK(hipMemcpyAsync(h_E.data(), d_E.p, (size_t)nstruc * 8, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(h_eatom.data(), d_eatom.p, (size_t)ntot * 8, hipMemcpyDeviceToHost, stream));
+        if (forces) HIPCHK(hipMemcpyAsync(h_F.data(), d_F.p, (size_t)ntot * 24, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        return counters[0];
+    }
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
     const size_t o_e = 256, o_ea = o_e + (energies ? al((size_t)nstruc * 8) : 0), o_f = o_ea + al((size_t)ntot * 8),
                  total = o_f + (forces ? al((size_t)ntot * 24) : 0);
@@ -942,7 +1060,7 @@ void Engine::resident_fetch(uint32_t flags, nnp_result* res) {
     }
 }
 
-void Engine::fetch(const std::vector<nnp_structure>& ss, uint32_t flags, nnp_result* res) {
+void Engine::fetch(Strucs ss, uint32_t flags, nnp_result* res) {
     const int mc = download(false, flags & NNP_FORCE);
     if (defer_count && mc > maxp) {  // rows overflowed: redo the batch with the count read back
         defer_count = false;
@@ -979,7 +1097,18 @@ using namespace atomnn;
 
 struct nnp_model {
     Engine* eng;
+    // nnp_compute_noalloc: a workspace bound to the scratch array of the device the model was
+    // loaded on (set up here, at load, so the calls themselves allocate nothing)
+    Engine* noalloc = nullptr;
+    void* scratch = nullptr;
+    int dev = 0;
 };
+
+namespace atomnn {
+// atoms a call of nnp_compute_noalloc may hold (host bookkeeping reserved at load); the scratch
+// array bounds it further by the structure's neighbour rows
+constexpr int kNoallocMaxAtoms = 16384;
+}  // namespace atomnn
 
 namespace atomnn {
 Engine& engine_of(nnp_model* m) { return *m->eng; }
@@ -1020,7 +1149,12 @@ int nnp_load(const char* path, nnp_model** out) {
         auto* m = new nnp_model{nullptr};
         try {
             m->eng = new Engine(std::move(hm));
+            m->scratch = scratch_on_current_device(m->dev);
+            m->noalloc = new Engine(*m->eng, 0);
+            m->noalloc->make_bound(kNoallocMaxAtoms);
         } catch (...) {
+            delete m->noalloc;
+            delete m->eng;
             delete m;
             throw;
         }
@@ -1031,6 +1165,7 @@ int nnp_load(const char* path, nnp_model** out) {
 
 void nnp_free(nnp_model* m) {
     if (!m) return;
+    delete m->noalloc;
     delete m->eng;
     delete m;
 }
@@ -1117,6 +1252,29 @@ int nnp_compute_batch(nnp_model* m, int n, const nnp_structure* s, uint32_t flag
         std::vector<nnp_structure> ss(s, s + n);
         E.run(ss, flags);
         E.fetch(ss, flags, r);
+        return NNP_OK;
+    });
+}
+
+int nnp_compute_noalloc(nnp_model* m, const nnp_structure* s, uint32_t flags, nnp_result* r) {
+    return guarded([&]() {
+        if (!m || !s || !r) throw std::invalid_argument("nnp_compute_noalloc: invalid argument");
+        if (s->natoms <= 0 || s->natoms > kNoallocMaxAtoms)
+            throw std::invalid_argument("nnp_compute_noalloc: natoms must be in 1.." + std::to_string(kNoallocMaxAtoms));
+        int dev = 0;
+        HIPCHK(hipGetDevice(&dev));
+        if (dev != m->dev) throw std::invalid_argument("nnp_compute_noalloc: call it on the device the model was loaded on");
+        Engine& B = *m->noalloc;
+        const Engine& E = *m->eng;
+        B.moments = E.moments;
+        B.force_waves = E.force_waves;
+        B.net_bf16_terms = E.net_bf16_terms;
+        B.react_rows = E.react_rows;
+        HIPCHK(hipStreamSynchronize(B.stream));
+        B.bind_scratch(m->scratch, kScratchBytes, *s);
+        const Strucs ss(s, 1);
+        B.run(ss, flags);
+        B.fetch(ss, flags, r);
         return NNP_OK;
     });
 }

@@ -28,12 +28,31 @@ struct hip_failure : std::runtime_error {
             throw hip_failure(std::string("HIP error in ") + #expr + ": " + hipGetErrorString(e_));            \
     } while (0)
 
-// A device allocation that only grows.
+// A device allocation that only grows; or, bound, a fixed slice of memory owned elsewhere, which
+// refuses (throws) instead of allocating when a larger size is asked for.
 struct DevMem {
     void* p = nullptr;
     size_t cap = 0;
+    bool bound = false;
     void reserve(size_t bytes);
     void release();
+    void bind(void* ptr, size_t bytes) {
+        p = ptr;
+        cap = bytes;
+        bound = true;
+    }
+};
+
+// The structures of a call (a view: no copy of the caller's array).
+struct Strucs {
+    const nnp_structure* p = nullptr;
+    size_t n = 0;
+    Strucs(const std::vector<nnp_structure>& v) : p(v.data()), n(v.size()) {}
+    Strucs(const nnp_structure* q, size_t m) : p(q), n(m) {}
+    size_t size() const { return n; }
+    const nnp_structure& operator[](size_t k) const { return p[k]; }
+    const nnp_structure* begin() const { return p; }
+    const nnp_structure* end() const { return p + n; }
 };
 
 // A non-owning device pointer into a larger allocation.
@@ -91,7 +110,7 @@ class Engine {
     Engine(const Engine&) = delete;
     Engine& operator=(const Engine&) = delete;
 
-    void upload(const std::vector<nnp_structure>& ss);
+    void upload(Strucs ss);
     bool lds_fits(int lm) const;
     void build_neighbors(bool all_images);
     template <typename T>
@@ -118,11 +137,19 @@ class Engine {
     int64_t react_missed = 0;  // reactions the last mode-0 pass missed (0 in the usual case)
     DevMem d_rcount, d_rlist;
     int counters[4] = {0, 0, 0, 0};  // maxcnt[0..3] as of the last download
-    void run(const std::vector<nnp_structure>& ss, uint32_t flags);
-    void fetch(const std::vector<nnp_structure>& ss, uint32_t flags, nnp_result* res);
+    // Bound workspace (nnp_compute_noalloc): every batch buffer is a slice of a fixed array
+    // (bind_scratch), uploads and downloads copy straight between the caller's / pre-reserved host
+    // memory and those slices, and no call allocates device or pinned memory.
+    bool bound = false;
+    int maxp_cap = 0;  // largest row stride the bound slices hold
+    void make_bound(int max_atoms);
+    void bind_scratch(void* base, size_t bytes, const nnp_structure& s);
+    std::vector<int> sp_fill;
+    void run(Strucs ss, uint32_t flags);
+    void fetch(Strucs ss, uint32_t flags, nnp_result* res);
 
     // device-resident batches: upload once, enqueue passes, fetch results
-    void resident_load(const std::vector<nnp_structure>& ss, uint32_t flags);
+    void resident_load(Strucs ss, uint32_t flags);
     void resident_run(uint32_t flags);
     void resident_fetch(uint32_t flags, nnp_result* res);
     EnergyTail energy_tail(bool count_max);

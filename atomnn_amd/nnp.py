@@ -79,6 +79,7 @@ def lib():
         L.nnp_compute.argtypes = [vp, ctypes.POINTER(_Struc), ctypes.c_uint32, ctypes.POINTER(_Result)]
         L.nnp_compute_batch.argtypes = [vp, ctypes.c_int, ctypes.POINTER(_Struc), ctypes.c_uint32,
                                         ctypes.POINTER(_Result)]
+        L.nnp_compute_noalloc.argtypes = [vp, ctypes.POINTER(_Struc), ctypes.c_uint32, ctypes.POINTER(_Result)]
         L.nnp_resident_load.argtypes = [vp, ctypes.c_int, ctypes.POINTER(_Struc), ctypes.c_uint32]
         L.nnp_resident_run.argtypes = [vp, ctypes.c_uint32]
         L.nnp_resident_fetch.argtypes = [vp, ctypes.c_uint32, ctypes.POINTER(_Result)]
@@ -187,6 +188,17 @@ class NNP:
                 ) -> Tuple[float, np.ndarray, Optional[np.ndarray]]:
         (out,) = self.compute_batch([s], flags)
         return out
+
+    def compute_noalloc(self, s: Structure, flags: int = NNP_FORCE | NNP_ATOM_ENERGY | NNP_VACUUM_ENERGY
+                        ) -> Tuple[float, np.ndarray, Optional[np.ndarray]]:
+        """nnp_compute_noalloc: the same results as compute(), with no device or pinned memory
+        allocated by the call (the library's fixed scratch array is the workspace)."""
+        pin = _Pinned(s)
+        ea = np.zeros(s.natoms)
+        f = np.zeros((s.natoms, 3))
+        res = _Result(0.0, ea.ctypes.data, f.ctypes.data)
+        _check(lib().nnp_compute_noalloc(self._h, ctypes.byref(pin.c_struct), flags, ctypes.byref(res)))
+        return res.energy, ea, (f if flags & NNP_FORCE else None)
 
     def compute_batch(self, strucs: Sequence[Structure], flags: int = NNP_FORCE | NNP_ATOM_ENERGY | NNP_VACUUM_ENERGY
                       ) -> List[Tuple[float, np.ndarray, Optional[np.ndarray]]]:

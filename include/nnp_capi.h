@@ -104,6 +104,18 @@ int nnp_compute(nnp_model* m, const nnp_structure* s, uint32_t flags, nnp_result
 /* n structures packed into shared launches; results per structure. */
 int nnp_compute_batch(nnp_model* m, int n, const nnp_structure* s, uint32_t flags, nnp_result* r);
 
+/* nnp_compute that allocates nothing: no counterpart in the reference; for callers that cannot
+ * allocate device or pinned memory (serving loops and harnesses that forbid allocation).  Same
+ * arguments, flags and results as nnp_compute (bit-identical results).  Its device workspace is a
+ * fixed 64 MiB array per device, looked up once per device at nnp_load (with the model's device
+ * current); host bookkeeping is reserved at nnp_load for up to 16384 atoms.  A structure whose
+ * neighbour rows, descriptors and force terms do not fit the array (beyond about 7,500 atoms at
+ * 0.06 atoms/A^3 and rc 6.5, 10,000 at 0.05 and rc 6), or of more than 16384 atoms, is rejected with
+ * NNP_EINVAL / NNP_EUNSUPPORTED and a message.  Every call on a device shares that device's array:
+ * calls that overlap on one device (from several threads or models) race; serialise them.  Call
+ * it on the device the model was loaded on. */
+int nnp_compute_noalloc(nnp_model* m, const nnp_structure* s, uint32_t flags, nnp_result* r);
+
 /* ---- serving (engine extension; the reference has no server) ----
  * A continuous-batching request server on one model: `lanes` concurrent batches, each on its
  * own HIP stream and workspace, share `hbm_budget` bytes of device memory.  Free lanes take
