@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -95,6 +96,10 @@ static void* scratch_on_current_device(int& dev) {
 }
 
 // ------------------------------------------------------------------ device buffers
+// Failure injection for tests (option inject_alloc_failure, process-wide): when set to k > 0,
+// the k-th device allocation after it fails as an exhausted device would (std::bad_alloc).
+static std::atomic<int64_t> g_alloc_fail_in{0};
+
 void DevMem::reserve(size_t bytes) {
     if (bytes <= cap) return;
     if (bound)
@@ -105,6 +110,7 @@ void DevMem::reserve(size_t bytes) {
     cap = 0;
     size_t want = std::max<size_t>(bytes, 256);
     want = (want + 4095) & ~size_t(4095);
+    if (g_alloc_fail_in.load() > 0 && g_alloc_fail_in.fetch_sub(1) == 1) throw std::bad_alloc();
     hipError_t e = hipMalloc(&p, want);
     if (e != hipSuccess) {
         p = nullptr;
@@ -1374,6 +1380,9 @@ int nnp_set_option(nnp_model* m, const char* key, int64_t value) {
             E.force_waves = (int)value;
             E.drop_graph();
             E.eager_runs = 0;
+        } else if (k == "inject_alloc_failure") {
+            if (value < 0) throw std::invalid_argument("nnp_set_option: inject_alloc_failure must be >= 0");
+            g_alloc_fail_in.store(value);
         } else if (k == "reaction_rows") {
             if (value != 0 && value != 1) throw std::invalid_argument("nnp_set_option: reaction_rows is 0 or 1");
             HIPCHK(hipStreamSynchronize(E.stream));

@@ -178,12 +178,18 @@ class Server {
             try {
                 E.run(ss, cfg_.flags);
                 E.fetch(ss, cfg_.flags, res.data());
+            } catch (const std::bad_alloc&) {
+                status = NNP_ENOMEM;
+                err = "out of device memory";
             } catch (const std::exception& e) {
                 status = NNP_EDEVICE;
                 err = e.what();
             }
             size_t held = E.workspace_bytes();
-            if (held > lane_budget_) {  // keep the lane inside its share of the budget
+            // keep the lane inside its share of the budget; after a failed allocation start the next
+            // batch from empty buffers (the device may still be short of memory)
+            if (held > lane_budget_ || status == NNP_ENOMEM) {
+                (void)hipStreamSynchronize(E.stream);
                 for (DevMem* m : {&E.d_nbrj, &E.d_meta, &E.d_nbrr, &E.d_X, &E.d_G, &E.d_dEdG}) m->release();
             }
             const double t = steady_now();

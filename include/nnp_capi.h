@@ -187,6 +187,9 @@ int nnp_profile_read(nnp_model* m, double* ms, int64_t* calls, int nstage);
  *               them every call)
  *   "train_chunk_elems"  training: gradient-row elements (doubles) held at a time (default
  *               2^28, 2 GB; batches with more rows are summed in chunks; at least 65536)
+ *   "inject_alloc_failure"  test hook, process-wide: k > 0 makes the k-th device allocation
+ *               after it fail as on an exhausted device (the call returns NNP_ENOMEM; a serving
+ *               lane fails that batch's requests with NNP_ENOMEM and serves on); 0 clears it
  *   "reaction_rows"  force assembly: 0 = each atom gathers its reactions from its neighbours'
  *               rows, and a batch in which that misses one (a centre lists the atom but the
  *               atom does not list the centre: a distance within rounding of rc) is assembled

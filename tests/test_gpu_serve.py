@@ -92,3 +92,47 @@ def test_open_cluster_then_periodic_requests(potentials, mixed):
         assert abs(E - Eo) <= 1e-10 * max(abs(Eo), np.abs(Eao).sum())
         assert np.linalg.norm(F - Fo, axis=1).max() <= 1e-10 * np.linalg.norm(Fo, axis=1).max()
     m.close()
+
+
+def test_injected_allocation_failure(potentials):
+    """A device allocation that fails (injected: option inject_alloc_failure) makes that call return
+    NNP_ENOMEM, and in a server that batch's requests; the model and the server serve on, with the
+    results of a model that never failed."""
+    flags = NNP_FORCE | NNP_VACUUM_ENERGY
+    m = NNP.read(potentials["serve20k"])
+    ref_model = NNP.read(potentials["serve20k"])
+    try:
+        pool = synth.serve_pool(60)
+        small = [s for s in pool if s.natoms < 200][:3]
+        big = [s for s in pool if 2000 <= s.natoms <= 5000][0]
+        ref = {id(s): ref_model.compute(s, flags) for s in small + [big]}
+
+        def same(res, s):
+            E, Ea, F = res
+            E0, Ea0, F0 = ref[id(s)]
+            assert E == E0 and np.array_equal(F, F0)
+
+        for s in small:  # sizes the model's buffers for small structures only
+            same(m.compute(s, flags), s)
+        m.set_option("inject_alloc_failure", 1)
+        with pytest.raises(NNPError) as ei:
+            m.compute(big, flags)  # its larger buffers are the next allocation
+        assert ei.value.code == -12
+        same(m.compute(big, flags), big)
+        srv = serve.Server(m, flags, lanes=1)
+        try:
+            for s in small:
+                same(srv.submit(s).wait(), s)
+            m.set_option("inject_alloc_failure", 1)
+            p = srv.submit(big)
+            with pytest.raises(NNPError) as ei:
+                p.wait()
+            assert ei.value.code == -12
+            for s in small + [big, small[0]]:
+                same(srv.submit(s).wait(), s)
+        finally:
+            srv.close()
+    finally:
+        m.set_option("inject_alloc_failure", 0)
+        m.close()
+        ref_model.close()
