@@ -1208,9 +1208,12 @@ __host__ __device__ constexpr int mom_chw() {
     return NW == 1 ? 16 : 8;
 }
 
+// partial moments [NW][S][NE][K]; after the adjoint tables the force exchange [NW-1][kWave][3]
+// followed by the cutoff values of every entry [lmaxp][4]
 template <int NE, int NW>
-__host__ __device__ constexpr int force_mom_mp_elems(int ntypes) {
-    return NW * ntypes * NE * kMomK > (NW - 1) * kWave * 3 ? NW * ntypes * NE * kMomK : (NW - 1) * kWave * 3;
+__host__ __device__ constexpr int force_mom_mp_elems(int ntypes, int lmaxp) {
+    return NW * ntypes * NE * kMomK > (NW - 1) * kWave * 3 + 4 * lmaxp ? NW * ntypes * NE * kMomK
+                                                                         : (NW - 1) * kWave * 3 + 4 * lmaxp;
 }
 template <int NE, int NW>
 __host__ __device__ constexpr int force_mom_u_elems(int ntypes) {
@@ -1218,7 +1221,7 @@ __host__ __device__ constexpr int force_mom_u_elems(int ntypes) {
 }
 template <typename T, int NE, int NW>
 __host__ __device__ constexpr size_t force_mom_lds_bytes(int lmaxp, int ldg, int ntypes) {
-    return sizeof(T) * (size_t)(4 * lmaxp + NE * lmaxp + ((ldg + 1) & ~1) + force_mom_mp_elems<NE, NW>(ntypes) +
+    return sizeof(T) * (size_t)(4 * lmaxp + NE * lmaxp + ((ldg + 1) & ~1) + force_mom_mp_elems<NE, NW>(ntypes, lmaxp) +
                                 ntypes * NE + (ntypes * (ntypes + 1) / 2) * NE * (kMomZ + 1) +
                                 force_mom_u_elems<NE, NW>(ntypes)) +
            sizeof(int) * (size_t)lmaxp;
@@ -1238,8 +1241,8 @@ __global__ __launch_bounds__(kWave * NW) __attribute__((amdgpu_waves_per_eu(size
     T* su = (T*)lds_raw;
     T* Rf = su + 4 * lmaxp;
     T* W = Rf + NE * lmaxp;
-    T* Mp = W + ((ldg + 1) & ~1);  // partial moments [NW][S][NE][K]; then the force exchange [NW-1][kWave][3]
-    T* Delta = Mp + force_mom_mp_elems<NE, NW>(ntypes);
+    T* Mp = W + ((ldg + 1) & ~1);  // partial moments; then the force exchange and the cutoff values
+    T* Delta = Mp + force_mom_mp_elems<NE, NW>(ntypes, lmaxp);
     T* wtab = Delta + ntypes * NE;
     T* U = wtab + nblk * NE * (kMomZ + 1);  // power rows [NW][kMomChW][15]; then L [S][K][NE]
     int* orig = (int*)(U + force_mom_u_elems<NE, NW>(ntypes));
@@ -1315,8 +1318,28 @@ __global__ __launch_bounds__(kWave * NW) __attribute__((amdgpu_waves_per_eu(size
     }
     adjoint_moments<T, NE>(plan, ti, ntypes, W, Mp, NW, ntypes * NE * kMomK, Delta, wtab, U, tid, kWave * NW);
 
+    // cutoff value and derivative of every entry, radial then angular, in the partial moments'
+    // place.  Computed ahead of the entry loop: inside it, the polynomial constants of the cosine
+    // are loop invariants that the compiler keeps in registers across the loop, beyond its budget.
+    T* Xx = Mp;
+    T* Cut = Mp + (NW - 1) * kWave * 3;
+    for (int p = tid; p < P; p += kWave * NW) {
+        int s = 0;
+        while (s + 1 < ntypes && p >= sg.start[s + 1]) ++s;
+        const DevRadial& R = plan->rad[ti][s];
+        const T d = su[4 * p + 3];
+        T fc, gc;
+        cutoff_fg<T>(R.cut, d, T(R.rc), T(R.rci), T(R.pirci), fc, gc);
+        T fa = fc, ga = gc;
+        if (A0.cut != R.cut || A0.rc != R.rc) cutoff_fg<T>(A0.cut, d, T(A0.rc), T(A0.rci), T(A0.pirci), fa, ga);
+        Cut[4 * p] = fc;
+        Cut[4 * p + 1] = gc;
+        Cut[4 * p + 2] = fa;
+        Cut[4 * p + 3] = ga;
+    }
+    __syncthreads();
+
     // per entry: this wave's radial functions and exponents; X = -(sum of the waves' dE/dr)
-    T* Xx = Mp;  // the partial moments are no longer needed
     const int64_t row = (int64_t)i * maxp;
     for (int j0 = 0; j0 < P; j0 += kWave) {
         const int p = j0 + lane;
@@ -1328,8 +1351,7 @@ __global__ __launch_bounds__(kWave * NW) __attribute__((amdgpu_waves_per_eu(size
             const T d = su[4 * p + 3], di = T(1) / d;
             // radial term dE/dd u over the functions f = w, w + NW, ...
             const DevRadial& R = plan->rad[ti][s];
-            T fc, gc;
-            cutoff_fg<T>(R.cut, d, T(R.rc), T(R.rci), T(R.pirci), fc, gc);
+            const T fc = Cut[4 * p], gc = Cut[4 * p + 1];
             T sr = T(0);
             for (int f = w; f < R.n; f += NW) {
                 const T eta = T(rfun_eta[R.fbase + f]), rs = T(rfun_rs[R.fbase + f]);
@@ -1341,8 +1363,7 @@ __global__ __launch_bounds__(kWave * NW) __attribute__((amdgpu_waves_per_eu(size
             gy = sr * u[1];
             gz = sr * u[2];
             // angular term of this wave's exponents e = w NK + k
-            T fa = fc, ga = gc;
-            if (A0.cut != R.cut || A0.rc != R.rc) cutoff_fg<T>(A0.cut, d, T(A0.rc), T(A0.rci), T(A0.pirci), fa, ga);
+            const T fa = Cut[4 * p + 2], ga = Cut[4 * p + 3];
             const T dp = ipow(d, A0.alpha);
             const T ddp = T(A0.alpha) * dp * di;
             T m[kMomK3];
