@@ -413,16 +413,21 @@ PlanBuffers build_plan(const HostModel& m) {
             in_real = out_real;
             in_pad = out_pad;
         }
-        // contiguous copy of the layer matrices and biases (one vector copy into LDS)
+        // contiguous copy of the layer matrices (of the output layer only its one real row) and
+        // biases (one vector copy into LDS).  Matrix
+        // element (o, k) sits at o in_pad + (k ^ (o & 15)): the 16 rows a forward MFMA operand load
+        // reads at one k, and the 16 columns a backward one reads in one row, fall in 16 different
+        // LDS banks (row-major, every row of a K-step would start in the same bank).
         align();
         dn.pack_off = (int64_t)W.size();
         for (int l = 0; l < dn.L; ++l) {
             dn.pw_off[l] = (int)((int64_t)W.size() - dn.pack_off);
             const int ip = l == 0 ? dn.nin_pad : dn.wpad[l - 1];
-            for (int q = 0; q < dn.wpad[l] * ip; ++q) {
-                const double v = W[dn.woff[l] + q];
-                W.push_back(v);
-            }
+            const int rows = l == dn.L - 1 ? 1 : dn.wpad[l];  // the output layer's one real row
+            const size_t base = W.size();
+            W.resize(base + (size_t)rows * ip);
+            for (int o = 0; o < rows; ++o)
+                for (int k = 0; k < ip; ++k) W[base + (size_t)o * ip + (k ^ (o & 15))] = W[dn.woff[l] + (int64_t)o * ip + k];
         }
         for (int l = 0; l < dn.L; ++l) {
             dn.pb_off[l] = (int)((int64_t)W.size() - dn.pack_off);

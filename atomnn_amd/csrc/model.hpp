@@ -127,7 +127,9 @@ struct DevNet {
     int64_t wtoff[kMaxLayers]; // the same weights transposed (in_pad x wpad)
     int64_t boff[kMaxLayers];
     int64_t inpw_off, inpb_off;
-    int64_t pack_off;          // contiguous copy [W_0 | W_1 | ... | b_0 | b_1 | ...] (row-major, padded)
+    int64_t pack_off;          // contiguous copy [W_0 | W_1 | ... | b_0 | b_1 | ...] (row-major, padded;
+                               // element (o, k) of W_l at o in_pad + (k ^ (o & 15)); of the
+                               // output layer only row 0)
     int pack_len;              // its length in doubles (even)
     int pw_off[kMaxLayers];    // offsets of W_l and b_l inside the copy
     int pb_off[kMaxLayers];

@@ -30,15 +30,28 @@ namespace atomnn {
 typedef float f4_t __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
 
-// LDS: two activation buffers [16][maxw], the derivatives of every layer [16][wpad[l]], then
-// (FP64 form) the input scale inpw [nin_pad] and the tile's 16 atom indices (16 slots)
+// LDS: two activation buffers [16][lda], the derivatives of every hidden layer [16][wpad[l]] (the
+// output layer is linear: derivative 1), then
+// (FP64 form) the input scale inpw [nin_pad] and the tile's 16 atom indices (16 slots).  The
+// activation rows are padded (lda = widest layer + 1 doubles, + 4 floats) so the 16 rows an MFMA
+// operand load reads start in different LDS banks: with the unpadded stride, a multiple of 16
+// elements, all 16 rows of a K-step fall in one bank.
+static __host__ __device__ int net_lda(int maxw, bool fp64) { return maxw + (fp64 ? 1 : 4); }
 size_t net_lds_bytes(const DevNet& n, bool fp64) {
     int maxw = n.nin_pad, dsum = 0;
     for (int l = 0; l < n.L; ++l) {
         if (n.wpad[l] > maxw) maxw = n.wpad[l];
-        dsum += n.wpad[l];
+        if (l < n.L - 1) dsum += n.wpad[l];
     }
-    return (size_t)(fp64 ? 8 : 4) * (size_t)(2 * 16 * maxw + 16 * dsum + n.nin_pad + 16);
+    return (size_t)(fp64 ? 8 : 4) * (size_t)(2 * 16 * net_lda(maxw, fp64) + 16 * dsum + n.nin_pad + 16);
+}
+
+// LDS copy of a species' network description, staged with the tile's first loads: the layer loops
+// then read each layer's widths and offsets from LDS instead of waiting on a chain of scalar
+// loads from the plan at the start of every layer.
+static_assert(sizeof(DevNet) % 8 == 0, "DevNet is copied in 8-byte words");
+__device__ __forceinline__ void stage_net(const DevNet& src, DevNet& dst, int tid) {
+    if (tid < (int)(sizeof(DevNet) / 8)) reinterpret_cast<int2*>(&dst)[tid] = reinterpret_cast<const int2*>(&src)[tid];
 }
 
 // Elements of the LDS copy of the weights-in-LDS form: every layer's matrix and bias.
@@ -101,17 +114,19 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_f64(const DevPlan* __
     extern __shared__ __align__(16) double shd[];
     const int s = tile_species(nt, plan->T, blockIdx.x);
     const DevNet& N = plan->net[s];
+    __shared__ DevNet snet;
+    stage_net(N, snet, threadIdx.x);
     const int* atoms = spatoms + nt.atom_start[s];
     const int M = nt.atom_start[s + 1] - nt.atom_start[s];
     const int tid = threadIdx.x, wv = tid / kWave, lane = tid % kWave;
     const int r0 = (blockIdx.x - nt.tile_start[s]) * 16;
-    const int maxw = net_maxw(N);
+    const int lda = net_lda(net_maxw(N), true);
     double* cur = shd;
-    double* nxt = shd + 16 * maxw;
-    double* der = shd + 32 * maxw;
+    double* nxt = shd + 16 * lda;
+    double* der = shd + 32 * lda;
     const int li = lane & 15, lk = lane >> 4;
     int dsum = 0;
-    for (int l = 0; l < N.L; ++l) dsum += N.wpad[l];
+    for (int l = 0; l < N.L - 1; ++l) dsum += N.wpad[l];
     double* sinw = der + 16 * dsum;                          // inpw, for dE/dG
     int* sat = reinterpret_cast<int*>(sinw + N.nin_pad);     // atom index of each tile row, -1 past M
     // input staging: thread (gr, gk) fills row gr, columns gk + 16 j; its atom index is loaded
@@ -155,14 +170,14 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_f64(const DevPlan* __
         for (int j = 0; j < kWReg; ++j) dst[min(tid + j * kNetThreads, n2 - 1)] = wr[j];
         for (int idx = tid + kWReg * kNetThreads; idx < n2; idx += kNetThreads) dst[idx] = src[idx];
     }
-    // layer l's weights and biases: LDS copy or global
+    // layer l's weights and biases: LDS copy (bank-swizzled rows, as in the packed copy) or global
     auto wmat = [&](int l) -> const double* {
-        if constexpr (WL) return wl + N.pw_off[l];
-        else return Wb + N.woff[l];
+        if constexpr (WL) return wl + snet.pw_off[l];
+        else return Wb + snet.woff[l];
     };
     auto bvec = [&](int l) -> const double* {
-        if constexpr (WL) return wl + N.pb_off[l];
-        else return Wb + N.boff[l];
+        if constexpr (WL) return wl + snet.pb_off[l];
+        else return Wb + snet.boff[l];
     };
     // scaled input ins = inpw .* (G + inpb), zero past nin and on rows past M
 #pragma unroll
@@ -170,7 +185,7 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_f64(const DevPlan* __
         const int k = gk + 16 * j;
         if (k < N.nin_pad) {
             const bool in = k < N.nin;
-            cur[gr * maxw + k] = in && gat >= 0 ? wq[j] * (gq[j] + bq[j]) : 0.0;
+            cur[gr * lda + k] = in && gat >= 0 ? wq[j] * (gq[j] + bq[j]) : 0.0;
             if (gr == 0) sinw[k] = in ? wq[j] : 0.0;
         }
     }
@@ -180,30 +195,32 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_f64(const DevPlan* __
             w = Wb[N.inpw_off + k];
             if (gat >= 0) v = w * (G[(int64_t)gat * ldg + k] + Wb[N.inpb_off + k]);
         }
-        cur[gr * maxw + k] = v;
+        cur[gr * lda + k] = v;
         if (gr == 0) sinw[k] = w;
     }
     if (gk == 0) sat[gr] = gat;
     __syncthreads();
-    int in_pad = N.nin_pad;
+    const DevNet& NL = snet;
+    const int L1 = NL.L - 1;  // the output layer (linear, one real unit)
+    int in_pad = NL.nin_pad;
     int doff = 0;
-    for (int l = 0; l < N.L; ++l) {
-        const int out_pad = N.wpad[l];
+    // hidden layers
+    for (int l = 0; l < L1; ++l) {
+        const int out_pad = NL.wpad[l];
         const double* W = wmat(l);
         const double* b = bvec(l);
-        const int act = (l == N.L - 1) ? NEU_LINEAR : N.act;
         for (int jt = wv; jt < out_pad / 16; jt += kNetWaves) {
-            const d4_t acc = tile_mm_f64<true>(cur, maxw, W, in_pad, in_pad, jt, li, lk);
+            const d4_t acc = tile_mm_f64<true, WL>(cur, lda, W, in_pad, in_pad, jt, li, lk);
             const int col = jt * 16 + li;
             const double bc = b[col];
             double z[4], av[4], dv[4];
 #pragma unroll
             for (int v = 0; v < 4; ++v) z[v] = acc[v] + bc;
-            net_act4<GEN>(act, z, av, dv);
+            net_act4<GEN>(NL.act, z, av, dv);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int row = 4 * v + lk;
-                nxt[row * maxw + col] = av[v];
+                nxt[row * lda + col] = av[v];
                 der[doff + row * out_pad + col] = dv[v];
             }
         }
@@ -214,35 +231,51 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_f64(const DevPlan* __
         in_pad = out_pad;
         doff += 16 * out_pad;
     }
-    // energies
-    if (tid < 16) {
-        const int at = sat[tid];
-        if (at >= 0) eatom[at] = N.outb + cur[tid * maxw] * N.outw + (vac ? N.evac : 0.0);
+    // The output layer has one real unit and is linear (derivative 1), so neither of its passes
+    // needs the matrix cores: wave 0 forms the energies as row dot products with W_L's first row
+    // (4 lanes per atom), and the other waves the first backward step, which is elementwise:
+    // delta_{L-1} = (outw W_L[0][.]) .* d_{L-1}.  With no hidden layer that step is dE/dG itself.
+    {
+        const double* wo = wmat(L1);  // row 0: unswizzled in the LDS copy as well
+        if (L1 > 0) doff -= 16 * in_pad;  // d_{L-1}
+        if (wv == 0) {
+            const int r = lane >> 2, q = lane & 3;
+            double acc = 0.0;
+            for (int k = q; k < in_pad; k += 4) acc += wo[k] * cur[r * lda + k];
+            acc += __shfl_xor(acc, 1, 4);
+            acc += __shfl_xor(acc, 2, 4);
+            const int at = sat[r];
+            if (q == 0 && at >= 0) eatom[at] = NL.outb + (bvec(L1)[0] + acc) * NL.outw + (vac ? NL.evac : 0.0);
+        } else if (L1 > 0) {
+            for (int r = wv - 1; r < 16; r += kNetWaves - 1)
+                for (int j = lane; j < in_pad; j += kWave) nxt[r * lda + j] = NL.outw * wo[j] * der[doff + r * in_pad + j];
+        } else {
+            for (int r = wv - 1; r < 16; r += kNetWaves - 1) {
+                const int at = sat[r];
+                if (at >= 0)
+                    for (int j = lane; j < NL.nin; j += kWave) dEdG[(int64_t)at * ldg + j] = NL.outw * wo[j] * sinw[j];
+            }
+        }
     }
-    // backward: delta_L = outw .* d_L (single output)
-    const int last = N.wpad[N.L - 1];
-    doff -= 16 * last;
-    for (int idx = tid; idx < 16 * last; idx += kNetThreads) {
-        const int r = idx / last, o = idx % last;
-        nxt[r * maxw + o] = (o == 0) ? N.outw * der[doff + r * last + o] : 0.0;
-    }
+    if (L1 == 0) return;
     __syncthreads();
     {
         double* t = cur;
         cur = nxt;
         nxt = t;
     }
-    for (int l = N.L - 1; l >= 1; --l) {
-        const int K = N.wpad[l], J = N.wpad[l - 1];
+    // backward through the hidden layers: delta_{l-1} = (delta_l W_l) .* d_{l-1}
+    for (int l = L1 - 1; l >= 1; --l) {
+        const int K = NL.wpad[l], J = NL.wpad[l - 1];
         const double* W = wmat(l);  // wpad[l] x wpad[l-1], row-major
         doff -= 16 * J;
         for (int jt = wv; jt < J / 16; jt += kNetWaves) {
-            const d4_t acc = tile_mm_f64<false>(cur, maxw, W, J, K, jt, li, lk);
+            const d4_t acc = tile_mm_f64<false, WL>(cur, lda, W, J, K, jt, li, lk);
             const int col = jt * 16 + li;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int row = 4 * v + lk;
-                nxt[row * maxw + col] = acc[v] * der[doff + row * J + col];
+                nxt[row * lda + col] = acc[v] * der[doff + row * J + col];
             }
         }
         __syncthreads();
@@ -252,15 +285,15 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_f64(const DevPlan* __
     }
     // dE/dG = (delta_0 W_0) .* inpw
     {
-        const int K = N.wpad[0], J = N.nin_pad;
+        const int K = NL.wpad[0], J = NL.nin_pad;
         const double* W = wmat(0);
         for (int jt = wv; jt < J / 16; jt += kNetWaves) {
-            const d4_t acc = tile_mm_f64<false>(cur, maxw, W, J, K, jt, li, lk);
+            const d4_t acc = tile_mm_f64<false, WL>(cur, lda, W, J, K, jt, li, lk);
             const int col = jt * 16 + li;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int at = sat[4 * v + lk];
-                if (at >= 0 && col < N.nin) dEdG[(int64_t)at * ldg + col] = acc[v] * sinw[col];
+                if (at >= 0 && col < NL.nin) dEdG[(int64_t)at * ldg + col] = acc[v] * sinw[col];
             }
         }
     }
@@ -304,15 +337,16 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_mixed(const DevPlan* 
     extern __shared__ __align__(16) float shf[];
     const int s = tile_species(nt, plan->T, blockIdx.x);
     const DevNet& N = plan->net[s];
+    __shared__ DevNet snet;
+    stage_net(N, snet, threadIdx.x);
     const int* atoms = spatoms + nt.atom_start[s];
     const int M = nt.atom_start[s + 1] - nt.atom_start[s];
     const int tid = threadIdx.x, wv = tid / kWave, lane = tid % kWave;
     const int r0 = (blockIdx.x - nt.tile_start[s]) * 16;
-    int maxw = N.nin_pad;
-    for (int l = 0; l < N.L; ++l) maxw = N.wpad[l] > maxw ? N.wpad[l] : maxw;
+    const int lda = net_lda(net_maxw(N), false);
     float* cur = shf;
-    float* nxt = shf + 16 * maxw;
-    float* der = shf + 32 * maxw;
+    float* nxt = shf + 16 * lda;
+    float* der = shf + 32 * lda;
     const int li = lane & 15, lk = lane >> 4;
     for (int idx = tid; idx < 16 * N.nin_pad; idx += kNetThreads) {
         const int r = idx / N.nin_pad, k = idx % N.nin_pad;
@@ -322,16 +356,18 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_mixed(const DevPlan* 
             const int at = atoms[row];
             v = Wf[N.inpw_off + k] * (G[(int64_t)at * ldg + k] + Wf[N.inpb_off + k]);
         }
-        cur[r * maxw + k] = v;
+        cur[r * lda + k] = v;
     }
     __syncthreads();
-    int in_pad = N.nin_pad;
+    const DevNet& NL = snet;
+    const int L1 = NL.L - 1;  // the output layer (linear, one real unit)
+    int in_pad = NL.nin_pad;
     int doff = 0;
-    for (int l = 0; l < N.L; ++l) {
-        const int out_pad = N.wpad[l];
-        const float* W = Wf + N.woff[l];
-        const float* b = Wf + N.boff[l];
-        const int act = (l == N.L - 1) ? NEU_LINEAR : N.act;
+    // hidden layers
+    for (int l = 0; l < L1; ++l) {
+        const int out_pad = NL.wpad[l];
+        const float* W = Wf + NL.woff[l];
+        const float* b = Wf + NL.boff[l];
         for (int jt = wv; jt < out_pad / 16; jt += kNetWaves) {
             f4_t acc = {0.f, 0.f, 0.f, 0.f};
             for (int k0 = 0; k0 < in_pad; k0 += 32) {
@@ -340,7 +376,7 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_mixed(const DevPlan* 
                 for (int q = 0; q < 8; ++q) {
                     const int k = k0 + 8 * lk + q;
                     const bool ok = k < in_pad;
-                    a[q] = ok ? cur[li * maxw + k] : 0.f;
+                    a[q] = ok ? cur[li * lda + k] : 0.f;
                     bv[q] = ok ? W[(int64_t)(jt * 16 + li) * in_pad + k] : 0.f;
                 }
                 acc = mfma_split<TERMS>(a, bv, acc);
@@ -350,11 +386,11 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_mixed(const DevPlan* 
             float z[4], av[4], dv[4];
 #pragma unroll
             for (int v = 0; v < 4; ++v) z[v] = acc[v] + bc;
-            net_act4<GEN>(act, z, av, dv);
+            net_act4<GEN>(NL.act, z, av, dv);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int row = 4 * lk + v;
-                nxt[row * maxw + col] = av[v];
+                nxt[row * lda + col] = av[v];
                 der[doff + row * out_pad + col] = dv[v];
             }
         }
@@ -365,25 +401,43 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_mixed(const DevPlan* 
         in_pad = out_pad;
         doff += 16 * out_pad;
     }
-    if (tid < 16 && r0 + tid < M) {
-        const int at = atoms[r0 + tid];
-        eatom[at] = (double)N.outb + (double)cur[tid * maxw] * N.outw + (vac ? N.evac : 0.0);
+    // output layer without the matrix cores (as in the FP64 kernel): wave 0 forms the energies as
+    // FP32 row dot products with W_L's first row, the other waves the elementwise first backward
+    // step delta_{L-1} = (outw W_L[0][.]) .* d_{L-1} (with no hidden layer: dE/dG itself)
+    {
+        const float* wo = Wf + NL.woff[L1];
+        if (L1 > 0) doff -= 16 * in_pad;  // d_{L-1}
+        if (wv == 0) {
+            const int r = lane >> 2, q = lane & 3;
+            float acc = 0.f;
+            for (int k = q; k < in_pad; k += 4) acc += wo[k] * cur[r * lda + k];
+            acc += __shfl_xor(acc, 1, 4);
+            acc += __shfl_xor(acc, 2, 4);
+            if (q == 0 && r0 + r < M)
+                eatom[atoms[r0 + r]] = (double)NL.outb + (double)(Wf[NL.boff[L1]] + acc) * NL.outw + (vac ? NL.evac : 0.0);
+        } else if (L1 > 0) {
+            for (int r = wv - 1; r < 16; r += kNetWaves - 1)
+                for (int j = lane; j < in_pad; j += kWave) nxt[r * lda + j] = (float)NL.outw * wo[j] * der[doff + r * in_pad + j];
+        } else {
+            for (int r = wv - 1; r < 16; r += kNetWaves - 1) {
+                if (r0 + r < M) {
+                    const int64_t row = (int64_t)atoms[r0 + r] * ldg;
+                    for (int j = lane; j < NL.nin; j += kWave) dEdG[row + j] = (float)NL.outw * wo[j] * Wf[NL.inpw_off + j];
+                }
+            }
+        }
     }
-    const int last = N.wpad[N.L - 1];
-    doff -= 16 * last;
-    for (int idx = tid; idx < 16 * last; idx += kNetThreads) {
-        const int r = idx / last, o = idx % last;
-        nxt[r * maxw + o] = (o == 0) ? (float)N.outw * der[doff + r * last + o] : 0.f;
-    }
+    if (L1 == 0) return;
     __syncthreads();
     {
         float* t = cur;
         cur = nxt;
         nxt = t;
     }
-    for (int l = N.L - 1; l >= 0; --l) {
-        const int K = N.wpad[l], J = (l == 0) ? N.nin_pad : N.wpad[l - 1];
-        const float* W = Wf + N.woff[l];
+    // backward through the hidden layers, then dE/dG = (delta_0 W_0) .* inpw
+    for (int l = L1 - 1; l >= 0; --l) {
+        const int K = NL.wpad[l], J = (l == 0) ? NL.nin_pad : NL.wpad[l - 1];
+        const float* W = Wf + NL.woff[l];
         if (l > 0) doff -= 16 * J;
         for (int jt = wv; jt < J / 16; jt += kNetWaves) {
             f4_t acc = {0.f, 0.f, 0.f, 0.f};
@@ -393,7 +447,7 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_mixed(const DevPlan* 
                 for (int q = 0; q < 8; ++q) {
                     const int k = k0 + 8 * lk + q;
                     const bool ok = k < K;
-                    a[q] = ok ? cur[li * maxw + k] : 0.f;
+                    a[q] = ok ? cur[li * lda + k] : 0.f;
                     bv[q] = ok ? W[(int64_t)k * J + jt * 16 + li] : 0.f;
                 }
                 acc = mfma_split<TERMS>(a, bv, acc);
@@ -403,10 +457,10 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_mixed(const DevPlan* 
             for (int v = 0; v < 4; ++v) {
                 const int rr = 4 * lk + v;
                 if (l > 0) {
-                    nxt[rr * maxw + col] = acc[v] * der[doff + rr * J + col];
+                    nxt[rr * lda + col] = acc[v] * der[doff + rr * J + col];
                 } else {
                     const int row = r0 + rr;
-                    if (row < M && col < N.nin) dEdG[(int64_t)atoms[row] * ldg + col] = acc[v] * Wf[N.inpw_off + col];
+                    if (row < M && col < NL.nin) dEdG[(int64_t)atoms[row] * ldg + col] = acc[v] * Wf[NL.inpw_off + col];
                 }
             }
         }
@@ -451,8 +505,9 @@ hipError_t launch_net(bool fp64, const DevPlan* plan, const DevPlan& hplan, cons
     // the tanh-only kernels when every species' hidden layers are tanh
     bool gen = false;
     for (int s = 0; s < hplan.T; ++s) gen = gen || hplan.net[s].act != NEU_TANH;
-    // weights in LDS: a few tiles per CU at most, and the copy fits beside the activations
-    const bool wl = fp64 && ntiles <= wl_tiles_max && ldsw <= 65536;
+    // weights in LDS: a few tiles per CU at most, and the copy fits beside the activations and the
+    // kernels' static copy of the network description
+    const bool wl = fp64 && ntiles <= wl_tiles_max && ldsw + sizeof(DevNet) <= 65536;
     const size_t bytes = wl ? ldsw : lds;
     auto kern = [&](auto k) { return (const void*)k; };
     const void* f;

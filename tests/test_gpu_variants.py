@@ -27,18 +27,19 @@ def rel_f(F, Fo):
 
 
 def variant_potential(tmp_path, phi_r="GAUSSIAN", phi_a="GAUSS", neuron="TANH", seed=0, zetas=(1.0, 4.0), cut="COS",
-                      norm="UNIT"):
+                      norm="UNIT", hidden=(16, 16)):
     """Writes the potential (preconditioned on the oracle's descriptors of a calibration cell)
     and returns its path."""
     rng = np.random.default_rng(seed)
-    pot = synth.make_potential(["A", "B"], 6.0, 6, [0.005, 0.05], list(zetas), [1, -1], [16, 16], rng)
+    pot = synth.make_potential(["A", "B"], 6.0, 6, [0.005, 0.05], list(zetas), [1, -1], list(hidden), rng)
     # radial functions centred across the cutoff so each variant sees both signs of its argument
     for key in pot.radial:
         pot.radial[key] = [(rs, eta) for rs, eta in zip(np.linspace(0.5, 5.5, 6), np.linspace(0.3, 2.0, 6))]
     pot.phi_r, pot.phi_a, pot.neuron = phi_r, phi_a, neuron
     pot.cut, pot.norm = cut, norm
     pot.outw[0], pot.outb[1] = 0.9, 0.05
-    path = str(tmp_path / f"v_{phi_r}_{phi_a}_{neuron}_{cut}_{norm}_{'_'.join(str(z) for z in zetas)}.ann")
+    path = str(tmp_path / (f"v_{phi_r}_{phi_a}_{neuron}_{cut}_{norm}_{'_'.join(str(z) for z in zetas)}"
+                           f"_h{'x'.join(str(h) for h in hidden)}.ann"))
     pot.write(path)
     cell = synth.cubic_cell(128, 0.05)
     s = synth.Structure(synth.species_exact(128, [0.5, 0.5], rng), synth.random_positions(128, cell, 2.0, seed + 1),
@@ -89,6 +90,36 @@ def test_variant_parity_fp64(tmp_path, case, pipeline):
             assert rel_e(E, Eo, Eao) <= TOL, (case, s.natoms)
             assert np.abs(Ea - Eao).max() <= TOL * np.abs(Eao).sum()
             assert rel_f(F, Fo) <= TOL, (case, s.natoms)
+    finally:
+        m.close()
+
+
+DEPTHS = [(), (16,), (24, 12, 40)]
+
+
+@pytest.mark.parametrize("hidden", DEPTHS, ids=["h" + "x".join(map(str, h)) for h in DEPTHS])
+def test_network_depths(tmp_path, hidden):
+    """Networks with no hidden layer, one, and three of widths that are not multiples of 16: the
+    output layer's energy dot product and elementwise first backward step, FP64 at 1e-10 (weights
+    staged in LDS for these few tiles, read from L2 on a batch of more tiles than two per CU) and
+    the mixed form within 1 meV/atom."""
+    path = variant_potential(tmp_path, hidden=hidden, neuron="SWISH" if len(hidden) == 1 else "TANH")
+    o = ob.OracleModel(path)
+    m = NNP.read(path)
+    try:
+        strucs = structures()
+        if len(hidden) == 3:
+            strucs = strucs + synth.batch512(48)[8:]  # ~13k atoms: over 512 tiles per species pass
+        for s, (E, Ea, F) in zip(strucs, m.compute_batch(strucs)):
+            Eo, Eao, Fo = o.compute(s.types, s.pos, s.cell)
+            assert rel_e(E, Eo, Eao) <= TOL, (hidden, s.natoms)
+            assert np.abs(Ea - Eao).max() <= TOL * np.abs(Eao).sum()
+            assert rel_f(F, Fo) <= TOL, (hidden, s.natoms)
+        s = structures()[-1]
+        E, Ea, F = m.compute(s, NNP_FORCE | NNP_ATOM_ENERGY | NNP_VACUUM_ENERGY | NNP_PREC_MIXED)
+        Eo, Eao, Fo = o.compute(s.types, s.pos, s.cell)
+        assert abs(E - Eo) / s.natoms <= 1e-3, hidden
+        assert np.sqrt(np.mean(np.sum((F - Fo) ** 2, axis=1))) <= 20e-3, hidden
     finally:
         m.close()
 
