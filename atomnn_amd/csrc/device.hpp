@@ -44,6 +44,14 @@ constexpr int kWave = 64;
 // maximum: react[0] = reactions the gather from the neighbours' rows missed (mode 0; cleared by
 // the neighbour launch), react[1] = longest receiver row (mode 1; cleared by that mode's own
 // launches).
+// Launch configuration of the network kernels: wl = FP64 weights copied into LDS (once per block
+// and species), grid = blocks, each taking every grid-th 16-atom tile (0: one block per tile).
+// Every configuration computes the same results.
+struct NetLaunch {
+    int wl = 0;
+    int grid = 0;
+};
+
 struct AssembleArgs {
     const DevStruc* S;
     const int* atom_struc;

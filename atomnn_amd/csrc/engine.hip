@@ -51,7 +51,9 @@ template <typename T>
 hipError_t launch_assemble(const AssembleArgs& a, const T* X, const EnergyTail* tail, hipStream_t st);
 hipError_t launch_net(bool fp64, const DevPlan* plan, const DevPlan& hplan, const void* W, const int* sp_off,
                       const int* spatoms, const void* G, int ldg, int vac, double* eatom, void* dEdG, int bf16_terms,
-                      int wl_tiles_max, hipStream_t st);
+                      NetLaunch cfg, hipStream_t st);
+int net_tiles(const int* sp_off, int ntypes);
+bool net_wl_fits(const DevPlan& hplan);
 hipError_t launch_neighbors(const NeighborArgs& a, hipStream_t st);
 size_t fused_lds(int shape, bool fp64, bool ns, int lmaxp, int ldg, int maxw, int dersum);
 size_t mom_lds(int shape, bool fp64, int lmaxp, int ntypes, int bits);
@@ -593,9 +595,66 @@ void Engine::descriptors() {
 void Engine::networks(bool mixed, bool vac) {
     d_eatom.reserve((size_t)ntot * 8);
     d_dEdG.reserve((size_t)ntot * pb.plan.ldg * (mixed ? 4 : 8));
+    const NetLaunch cfg = net_launch(mixed, vac);
     HIPCHK(launch_net(!mixed, (const DevPlan*)d_plan.p, pb.plan, mixed ? d_w32.p : d_w64.p, sp_off.data(),
                       (const int*)d_spatoms.p, d_G.p, pb.plan.ldg, vac ? 1 : 0, (double*)d_eatom.p, d_dEdG.p,
-                      net_bf16_terms, 2 * num_cus, stream));
+                      net_bf16_terms, cfg, stream));
+}
+
+// Launch configuration of the network kernel.  FP64 batches of at most two 16-atom tiles per CU
+// copy the weights into LDS, one block per tile (the latency-bound small-batch form).  Larger
+// FP64 batches time, the first time their log2 tile count is seen, the weights read through L2
+// (one block per tile) against the weights in LDS on a persistent grid of 1 or 2 blocks per CU,
+// on the batch at hand, and keep the fastest.  Every candidate computes the same results.
+// Serving lanes and captured passes read the cache but do not time kernels.
+NetLaunch Engine::net_launch(bool mixed, bool vac) {
+    NetLaunch cfg;
+    if (mixed || !net_wl_fits(pb.plan)) return cfg;
+    if (net_launch_opt >= 0) {  // fixed by the option
+        cfg.wl = net_launch_opt > 0;
+        cfg.grid = net_launch_opt * num_cus;
+        return cfg;
+    }
+    const int ntiles = net_tiles(sp_off.data(), pb.plan.T);
+    if (ntiles <= 2 * num_cus) {
+        cfg.wl = 1;
+        return cfg;
+    }
+    int lg = 0;
+    while ((1 << (lg + 1)) <= ntiles) ++lg;
+    for (const auto& kv : net_cache)
+        if (kv.first == lg) return kv.second;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIPCHK(hipStreamIsCapturing(stream, &cs));
+    if (!may_tune || cs != hipStreamCaptureStatusNone) return cfg;
+    const NetLaunch cand[] = {{0, 0}, {1, num_cus}, {1, 2 * num_cus}};
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    last_net_tune.clear();
+    double best_ms = 1e300;
+    for (const NetLaunch& c : cand) {
+        double tot = 0;
+        for (int rep = 0; rep < 3; ++rep) {
+            HIPCHK(hipEventRecord(e0, stream));
+            HIPCHK(launch_net(true, (const DevPlan*)d_plan.p, pb.plan, d_w64.p, sp_off.data(), (const int*)d_spatoms.p,
+                              d_G.p, pb.plan.ldg, vac ? 1 : 0, (double*)d_eatom.p, d_dEdG.p, net_bf16_terms, c, stream));
+            HIPCHK(hipEventRecord(e1, stream));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            if (rep > 0) tot += ms;  // the first repetition warms the caches
+        }
+        last_net_tune.push_back({c, tot / 2});
+        if (tot / 2 < best_ms) {
+            best_ms = tot / 2;
+            cfg = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    net_cache.push_back({lg, cfg});
+    return cfg;
 }
 
 template <typename T>
@@ -1353,6 +1412,12 @@ int nnp_set_option(nnp_model* m, const char* key, int64_t value) {
         if (k == "graph") {
             HIPCHK(hipStreamSynchronize(E.stream));
             E.use_graph = value != 0;
+            E.drop_graph();
+            E.eager_runs = 0;
+        } else if (k == "net_launch") {
+            if (value < -1 || value > 2) throw std::invalid_argument("nnp_set_option: net_launch is -1..2");
+            HIPCHK(hipStreamSynchronize(E.stream));
+            E.net_launch_opt = (int)value;
             E.drop_graph();
             E.eager_runs = 0;
         } else if (k == "pipeline") {

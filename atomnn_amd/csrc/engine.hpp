@@ -207,6 +207,12 @@ class Engine {
     std::vector<std::pair<TuneKey, int>> tune_cache;  // key -> blocks per CU
     std::vector<std::pair<int, double>> last_tune;    // candidates and their times (ms) of the last tuning
     int fused_blocks(bool mixed, uint32_t flags);
+    // Network launch configuration per log2 tile count of FP64 batches (tuned once each), and the
+    // candidates and times (ms) of the last tuning
+    std::vector<std::pair<int, NetLaunch>> net_cache;
+    int net_launch_opt = -1;  // option net_launch: -1 tuned, 0 weights through L2, k weights in LDS on k blocks per CU
+    std::vector<std::pair<NetLaunch, double>> last_net_tune;
+    NetLaunch net_launch(bool mixed, bool vac);
     void fused(bool mixed, uint32_t flags, int nblocks);
     void net_dims(int& maxw, int& dersum) const;
     // captured resident pass (replayed with hipGraphLaunch)

@@ -103,23 +103,29 @@ __device__ __forceinline__ int tile_species(const NetTiles& nt, int ntypes, int 
     return s;
 }
 
-// WL: the species' weight matrices are copied into LDS once per tile (all loads in flight
-// together), so the layers read weights from LDS instead of waiting on global loads layer by
-// layer (small batches, where the network kernel is a latency chain of one tile per CU).
+// WL: the species' weight matrices are copied into LDS (all loads in flight together), so the
+// layers read weights from LDS instead of waiting on global loads layer by layer.  A block takes
+// tiles blockIdx.x, blockIdx.x + gridDim.x, ... (species-major), and copies the weights again
+// only when the species changes: with one tile per block this is the small-batch form, with a
+// grid of a few blocks per CU the copy is spread over many tiles.
 template <bool WL, bool GEN>
 __global__ __launch_bounds__(kNetWaves * kWave) void k_net_f64(const DevPlan* __restrict__ plan, const double* __restrict__ Wb,
-                                                   NetTiles nt, const int* __restrict__ spatoms,
+                                                   NetTiles nt, int ntiles, const int* __restrict__ spatoms,
                                                    const double* __restrict__ G, int ldg, int vac,
                                                    double* __restrict__ eatom, double* __restrict__ dEdG) {
     extern __shared__ __align__(16) double shd[];
-    const int s = tile_species(nt, plan->T, blockIdx.x);
-    const DevNet& N = plan->net[s];
     __shared__ DevNet snet;
-    stage_net(N, snet, threadIdx.x);
+    const int tid = threadIdx.x, wv = tid / kWave, lane = tid % kWave;
+    int s_prev = -1;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int s = tile_species(nt, plan->T, tile);
+    const bool reload = s != s_prev;
+    s_prev = s;
+    const DevNet& N = plan->net[s];
+    if (reload) stage_net(N, snet, tid);
     const int* atoms = spatoms + nt.atom_start[s];
     const int M = nt.atom_start[s + 1] - nt.atom_start[s];
-    const int tid = threadIdx.x, wv = tid / kWave, lane = tid % kWave;
-    const int r0 = (blockIdx.x - nt.tile_start[s]) * 16;
+    const int r0 = (tile - nt.tile_start[s]) * 16;
     const int lda = net_lda(net_maxw(N), true);
     double* cur = shd;
     double* nxt = shd + 16 * lda;
@@ -151,8 +157,10 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_f64(const DevPlan* __
         dst = reinterpret_cast<double2*>(wl);
         n2 = N.pack_len / 2;
         // unconditional loads of clamped (in-bounds) indices: no branch, so no wait, between them
+        if (reload) {
 #pragma unroll
-        for (int j = 0; j < kWReg; ++j) wr[j] = src[min(tid + j * kNetThreads, n2 - 1)];
+            for (int j = 0; j < kWReg; ++j) wr[j] = src[min(tid + j * kNetThreads, n2 - 1)];
+        }
     }
     double gq[kGReg], wq[kGReg], bq[kGReg];
     const int64_t grow = (int64_t)(gat >= 0 ? gat : 0) * ldg;  // row past M: any valid row, unused
@@ -166,9 +174,11 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_f64(const DevPlan* __
     if constexpr (WL) {
         // unconditional too (threads past n2 rewrite the last element with its own value), so no
         // load is sunk into a branch of its own
+        if (reload) {
 #pragma unroll
-        for (int j = 0; j < kWReg; ++j) dst[min(tid + j * kNetThreads, n2 - 1)] = wr[j];
-        for (int idx = tid + kWReg * kNetThreads; idx < n2; idx += kNetThreads) dst[idx] = src[idx];
+            for (int j = 0; j < kWReg; ++j) dst[min(tid + j * kNetThreads, n2 - 1)] = wr[j];
+            for (int idx = tid + kWReg * kNetThreads; idx < n2; idx += kNetThreads) dst[idx] = src[idx];
+        }
     }
     // layer l's weights and biases: LDS copy (bank-swizzled rows, as in the packed copy) or global
     auto wmat = [&](int l) -> const double* {
@@ -257,7 +267,7 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_f64(const DevPlan* __
             }
         }
     }
-    if (L1 == 0) return;
+    if (L1 > 0) {
     __syncthreads();
     {
         double* t = cur;
@@ -297,6 +307,9 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_f64(const DevPlan* __
             }
         }
     }
+    }  // L1 > 0
+    __syncthreads();  // the next tile's staging overwrites this tile's LDS
+    }  // tiles
 }
 
 // ------------------------------------------------------------------ mixed (BF16 MFMA, FP32 accumulate)
@@ -491,9 +504,24 @@ static NetTiles make_tiles(const int* sp_off, int ntypes, int& ntiles) {
     return nt;
 }
 
+int net_tiles(const int* sp_off, int ntypes) {
+    int ntiles = 0;
+    (void)make_tiles(sp_off, ntypes, ntiles);
+    return ntiles;
+}
+
+// Whether the FP64 weights-in-LDS form fits: the weights beside the activations and the kernels'
+// static copy of the network description, under 64 KiB.
+bool net_wl_fits(const DevPlan& hplan) {
+    size_t ldsw = 0;
+    for (int s = 0; s < hplan.T; ++s)
+        ldsw = std::max(ldsw, net_lds_bytes(hplan.net[s], true) + 8 * (size_t)net_weight_elems(hplan.net[s]));
+    return ldsw + sizeof(DevNet) <= 65536;
+}
+
 hipError_t launch_net(bool fp64, const DevPlan* plan, const DevPlan& hplan, const void* W, const int* sp_off,
                       const int* spatoms, const void* G, int ldg, int vac, double* eatom, void* dEdG, int bf16_terms,
-                      int wl_tiles_max, hipStream_t st) {
+                      NetLaunch cfg, hipStream_t st) {
     int ntiles = 0;
     const NetTiles nt = make_tiles(sp_off, hplan.T, ntiles);
     if (ntiles == 0) return hipSuccess;
@@ -505,19 +533,21 @@ hipError_t launch_net(bool fp64, const DevPlan* plan, const DevPlan& hplan, cons
     // the tanh-only kernels when every species' hidden layers are tanh
     bool gen = false;
     for (int s = 0; s < hplan.T; ++s) gen = gen || hplan.net[s].act != NEU_TANH;
-    // weights in LDS: a few tiles per CU at most, and the copy fits beside the activations and the
-    // kernels' static copy of the network description
-    const bool wl = fp64 && ntiles <= wl_tiles_max && ldsw + sizeof(DevNet) <= 65536;
+    const bool wl = fp64 && cfg.wl && net_wl_fits(hplan);
     const size_t bytes = wl ? ldsw : lds;
     auto kern = [&](auto k) { return (const void*)k; };
-    const void* f;
-    if (fp64)
-        f = wl ? (gen ? kern(k_net_f64<true, true>) : kern(k_net_f64<true, false>))
-               : (gen ? kern(k_net_f64<false, true>) : kern(k_net_f64<false, false>));
-    else if (bf16_terms == 1)
-        f = gen ? kern(k_net_mixed<1, true>) : kern(k_net_mixed<1, false>);
-    else
-        f = gen ? kern(k_net_mixed<3, true>) : kern(k_net_mixed<3, false>);
+    if (fp64) {
+        const void* f = wl ? (gen ? kern(k_net_f64<true, true>) : kern(k_net_f64<true, false>))
+                           : (gen ? kern(k_net_f64<false, true>) : kern(k_net_f64<false, false>));
+        const int grid = cfg.grid > 0 && cfg.grid < ntiles ? cfg.grid : ntiles;
+        void* args[] = {(void*)&plan,  (void*)&W, (void*)&nt,  (void*)&ntiles, (void*)&spatoms,
+                        (void*)&G,     (void*)&ldg, (void*)&vac, (void*)&eatom,  (void*)&dEdG};
+        const hipError_t e = hipLaunchKernel(f, dim3(grid), dim3(kNetThreads), args, bytes, st);
+        if (e != hipSuccess) return e;
+        return hipGetLastError();
+    }
+    const void* f = bf16_terms == 1 ? (gen ? kern(k_net_mixed<1, true>) : kern(k_net_mixed<1, false>))
+                                    : (gen ? kern(k_net_mixed<3, true>) : kern(k_net_mixed<3, false>));
     void* args[] = {(void*)&plan, (void*)&W, (void*)&nt, (void*)&spatoms, (void*)&G, (void*)&ldg, (void*)&vac,
                     (void*)&eatom, (void*)&dEdG};
     const hipError_t e = hipLaunchKernel(f, dim3(ntiles), dim3(kNetThreads), args, bytes, st);

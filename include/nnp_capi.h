@@ -178,6 +178,11 @@ int nnp_profile_read(nnp_model* m, double* ms, int64_t* calls, int nstage);
  *   "force_waves"  1 = one wavefront per centre in the force-term kernel (default), 2 = two
  *               wavefronts split a centre's exponents, radial functions and moment entries
  *               (moment path on fast-path models only)
+ *   "net_launch"  FP64 network kernel: -1 = tuned (default: weights staged in LDS, one block per
+ *               16-atom tile, up to two tiles per CU; larger batches time the weights read
+ *               through L2 against the weights in LDS on 1 or 2 persistent blocks per CU, once
+ *               per log2 tile count, and keep the fastest), 0 = weights through L2, k = 1, 2:
+ *               weights in LDS on k blocks per CU; every setting gives the same results
  *   "net_bf16_terms"  mixed-precision networks: 3 = BF16 MFMA products of split operands
  *               (hi + lo, about 16 mantissa bits; default), 1 = single BF16 products
  *   "train_one_pass"  energy training: 1 = one forward + backward pass over single-structure

@@ -1,0 +1,48 @@
+"""Launch configurations that must not change any result, bit for bit: the FP64 network kernel's
+weight staging and persistent grid (option net_launch, fixed and tuned), on a batch of more than
+two 16-atom tiles per CU, through nnp_compute and through the resident graph-replayed pass."""
+import numpy as np
+import pytest
+
+from atomnn_amd import synth
+from atomnn_amd.nnp import NNP, NNP_ATOM_ENERGY, NNP_FORCE, NNP_VACUUM_ENERGY
+
+pytestmark = pytest.mark.gpu
+FLAGS = NNP_FORCE | NNP_ATOM_ENERGY | NNP_VACUUM_ENERGY
+
+
+def flat(res):
+    return np.concatenate([np.r_[E, Ea, F.ravel()] for E, Ea, F in res])
+
+
+def test_net_launch_configurations_bitwise(tmp_path):
+    path = str(tmp_path / "batch.ann")
+    synth.workload_potential("batch512").write(path)
+    m = NNP.read(path)
+    try:
+        strucs = synth.batch512(72)  # about 19k atoms: over 512 network tiles
+        assert sum(s.natoms for s in strucs) > 2 * 256 * 16
+        ref = None
+        for opt in (0, 1, 2, -1):
+            m.set_option("net_launch", opt)
+            got = flat(m.compute_batch(strucs))
+            if ref is None:
+                ref = got
+            assert np.array_equal(got, ref), opt
+        # the resident pass sums the structure energies on the device (in another order than
+        # nnp_compute's host sum): per-atom energies and forces bit for bit, totals to rounding
+        per_atom = lambda res: np.concatenate([np.r_[Ea, F.ravel()] for _, Ea, F in res])
+        want = m.compute_batch(strucs)
+        r = m.resident(strucs, FLAGS)
+        for _ in range(3):  # eager first pass (tuned), then graph replays
+            r.run()
+            got = r.fetch()
+            assert np.array_equal(per_atom(got), per_atom(want))
+            assert np.allclose([g[0] for g in got], [w[0] for w in want], rtol=1e-13, atol=0)
+        small = synth.batch512(4)
+        m.set_option("net_launch", 0)
+        a = flat(m.compute_batch(small))
+        m.set_option("net_launch", -1)
+        assert np.array_equal(flat(m.compute_batch(small)), a)
+    finally:
+        m.close()
