@@ -118,197 +118,199 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_f64(const DevPlan* __
     const int tid = threadIdx.x, wv = tid / kWave, lane = tid % kWave;
     int s_prev = -1;
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const int s = tile_species(nt, plan->T, tile);
-    const bool reload = s != s_prev;
-    s_prev = s;
-    const DevNet& N = plan->net[s];
-    if (reload) stage_net(N, snet, tid);
-    const int* atoms = spatoms + nt.atom_start[s];
-    const int M = nt.atom_start[s + 1] - nt.atom_start[s];
-    const int r0 = (tile - nt.tile_start[s]) * 16;
-    const int lda = net_lda(net_maxw(N), true);
-    double* cur = shd;
-    double* nxt = shd + 16 * lda;
-    double* der = shd + 32 * lda;
-    const int li = lane & 15, lk = lane >> 4;
-    int dsum = 0;
-    for (int l = 0; l < N.L - 1; ++l) dsum += N.wpad[l];
-    double* sinw = der + 16 * dsum;                          // inpw, for dE/dG
-    int* sat = reinterpret_cast<int*>(sinw + N.nin_pad);     // atom index of each tile row, -1 past M
-    // input staging: thread (gr, gk) fills row gr, columns gk + 16 j; its atom index is loaded
-    // first, so that load is in flight together with the weight copy
-    static_assert(kNetThreads == 16 * 16, "input staging maps 16 threads to each of the 16 rows");
-    const int gr = tid >> 4, gk = tid & 15;
-    const int gat = r0 + gr < M ? atoms[r0 + gr] : -1;
-    double* wl = nullptr;
-    // Weights and the first 8 column groups of the descriptor row are loaded into registers
-    // before any LDS store, so the gather's loads (which wait only for the atom index) overlap
-    // the weight copy's instead of following them.
-    constexpr int kWReg = 16;  // 16-byte weight loads per thread: n2 <= 16 x 256 under the 64 KB LDS bound
-    constexpr int kGReg = 8;   // column groups of 16 gathered through registers (nin_pad <= 128)
-    double2 wr[kWReg];
-    const double2* src = nullptr;
-    double2* dst = nullptr;
-    int n2 = 0;
-    if constexpr (WL) {
-        wl = sinw + N.nin_pad + 16;
-        // one flat 16-byte copy of [W_0 | W_1 | ... | b_0 | ...]: all loads in flight together
-        src = reinterpret_cast<const double2*>(Wb + N.pack_off);
-        dst = reinterpret_cast<double2*>(wl);
-        n2 = N.pack_len / 2;
-        // unconditional loads of clamped (in-bounds) indices: no branch, so no wait, between them
-        if (reload) {
+        const int s = tile_species(nt, plan->T, tile);
+        const bool reload = s != s_prev;
+        s_prev = s;
+        const DevNet& N = plan->net[s];
+        if (reload) stage_net(N, snet, tid);
+        const int* atoms = spatoms + nt.atom_start[s];
+        const int M = nt.atom_start[s + 1] - nt.atom_start[s];
+        const int r0 = (tile - nt.tile_start[s]) * 16;
+        const int lda = net_lda(net_maxw(N), true);
+        double* cur = shd;
+        double* nxt = shd + 16 * lda;
+        double* der = shd + 32 * lda;
+        const int li = lane & 15, lk = lane >> 4;
+        int dsum = 0;
+        for (int l = 0; l < N.L - 1; ++l) dsum += N.wpad[l];
+        double* sinw = der + 16 * dsum;                          // inpw, for dE/dG
+        int* sat = reinterpret_cast<int*>(sinw + N.nin_pad);     // atom index of each tile row, -1 past M
+        // input staging: thread (gr, gk) fills row gr, columns gk + 16 j; its atom index is loaded
+        // first, so that load is in flight together with the weight copy
+        static_assert(kNetThreads == 16 * 16, "input staging maps 16 threads to each of the 16 rows");
+        const int gr = tid >> 4, gk = tid & 15;
+        const int gat = r0 + gr < M ? atoms[r0 + gr] : -1;
+        double* wl = nullptr;
+        // Weights and the first 8 column groups of the descriptor row are loaded into registers
+        // before any LDS store, so the gather's loads (which wait only for the atom index) overlap
+        // the weight copy's instead of following them.
+        constexpr int kWReg = 16;  // 16-byte weight loads per thread: n2 <= 16 x 256 under the 64 KB LDS bound
+        constexpr int kGReg = 8;   // column groups of 16 gathered through registers (nin_pad <= 128)
+        double2 wr[kWReg];
+        const double2* src = nullptr;
+        double2* dst = nullptr;
+        int n2 = 0;
+        if constexpr (WL) {
+            wl = sinw + N.nin_pad + 16;
+            // one flat 16-byte copy of [W_0 | W_1 | ... | b_0 | ...]: all loads in flight together
+            src = reinterpret_cast<const double2*>(Wb + N.pack_off);
+            dst = reinterpret_cast<double2*>(wl);
+            n2 = N.pack_len / 2;
+            // unconditional loads of clamped (in-bounds) indices: no branch, so no wait, between them
+            if (reload) {
 #pragma unroll
-            for (int j = 0; j < kWReg; ++j) wr[j] = src[min(tid + j * kNetThreads, n2 - 1)];
-        }
-    }
-    double gq[kGReg], wq[kGReg], bq[kGReg];
-    const int64_t grow = (int64_t)(gat >= 0 ? gat : 0) * ldg;  // row past M: any valid row, unused
-#pragma unroll
-    for (int j = 0; j < kGReg; ++j) {
-        const int kc = min(gk + 16 * j, N.nin - 1);
-        wq[j] = Wb[N.inpw_off + kc];
-        bq[j] = Wb[N.inpb_off + kc];
-        gq[j] = G[grow + kc];
-    }
-    if constexpr (WL) {
-        // unconditional too (threads past n2 rewrite the last element with its own value), so no
-        // load is sunk into a branch of its own
-        if (reload) {
-#pragma unroll
-            for (int j = 0; j < kWReg; ++j) dst[min(tid + j * kNetThreads, n2 - 1)] = wr[j];
-            for (int idx = tid + kWReg * kNetThreads; idx < n2; idx += kNetThreads) dst[idx] = src[idx];
-        }
-    }
-    // layer l's weights and biases: LDS copy (bank-swizzled rows, as in the packed copy) or global
-    auto wmat = [&](int l) -> const double* {
-        if constexpr (WL) return wl + snet.pw_off[l];
-        else return Wb + snet.woff[l];
-    };
-    auto bvec = [&](int l) -> const double* {
-        if constexpr (WL) return wl + snet.pb_off[l];
-        else return Wb + snet.boff[l];
-    };
-    // scaled input ins = inpw .* (G + inpb), zero past nin and on rows past M
-#pragma unroll
-    for (int j = 0; j < kGReg; ++j) {
-        const int k = gk + 16 * j;
-        if (k < N.nin_pad) {
-            const bool in = k < N.nin;
-            cur[gr * lda + k] = in && gat >= 0 ? wq[j] * (gq[j] + bq[j]) : 0.0;
-            if (gr == 0) sinw[k] = in ? wq[j] : 0.0;
-        }
-    }
-    for (int k = gk + 16 * kGReg; k < N.nin_pad; k += 16) {
-        double v = 0.0, w = 0.0;
-        if (k < N.nin) {
-            w = Wb[N.inpw_off + k];
-            if (gat >= 0) v = w * (G[(int64_t)gat * ldg + k] + Wb[N.inpb_off + k]);
-        }
-        cur[gr * lda + k] = v;
-        if (gr == 0) sinw[k] = w;
-    }
-    if (gk == 0) sat[gr] = gat;
-    __syncthreads();
-    const DevNet& NL = snet;
-    const int L1 = NL.L - 1;  // the output layer (linear, one real unit)
-    int in_pad = NL.nin_pad;
-    int doff = 0;
-    // hidden layers
-    for (int l = 0; l < L1; ++l) {
-        const int out_pad = NL.wpad[l];
-        const double* W = wmat(l);
-        const double* b = bvec(l);
-        for (int jt = wv; jt < out_pad / 16; jt += kNetWaves) {
-            const d4_t acc = tile_mm_f64<true, WL>(cur, lda, W, in_pad, in_pad, jt, li, lk);
-            const int col = jt * 16 + li;
-            const double bc = b[col];
-            double z[4], av[4], dv[4];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) z[v] = acc[v] + bc;
-            net_act4<GEN>(NL.act, z, av, dv);
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int row = 4 * v + lk;
-                nxt[row * lda + col] = av[v];
-                der[doff + row * out_pad + col] = dv[v];
+                for (int j = 0; j < kWReg; ++j) wr[j] = src[min(tid + j * kNetThreads, n2 - 1)];
             }
         }
+        double gq[kGReg], wq[kGReg], bq[kGReg];
+        const int64_t grow = (int64_t)(gat >= 0 ? gat : 0) * ldg;  // row past M: any valid row, unused
+#pragma unroll
+        for (int j = 0; j < kGReg; ++j) {
+            const int kc = min(gk + 16 * j, N.nin - 1);
+            wq[j] = Wb[N.inpw_off + kc];
+            bq[j] = Wb[N.inpb_off + kc];
+            gq[j] = G[grow + kc];
+        }
+        if constexpr (WL) {
+            // unconditional too (threads past n2 rewrite the last element with its own value), so no
+            // load is sunk into a branch of its own
+            if (reload) {
+#pragma unroll
+                for (int j = 0; j < kWReg; ++j) dst[min(tid + j * kNetThreads, n2 - 1)] = wr[j];
+                for (int idx = tid + kWReg * kNetThreads; idx < n2; idx += kNetThreads) dst[idx] = src[idx];
+            }
+        }
+        // layer l's weights and biases: LDS copy (bank-swizzled rows, as in the packed copy) or global
+        auto wmat = [&](int l) -> const double* {
+            if constexpr (WL) return wl + snet.pw_off[l];
+            else return Wb + snet.woff[l];
+        };
+        auto bvec = [&](int l) -> const double* {
+            if constexpr (WL) return wl + snet.pb_off[l];
+            else return Wb + snet.boff[l];
+        };
+        // scaled input ins = inpw .* (G + inpb), zero past nin and on rows past M
+#pragma unroll
+        for (int j = 0; j < kGReg; ++j) {
+            const int k = gk + 16 * j;
+            if (k < N.nin_pad) {
+                const bool in = k < N.nin;
+                cur[gr * lda + k] = in && gat >= 0 ? wq[j] * (gq[j] + bq[j]) : 0.0;
+                if (gr == 0) sinw[k] = in ? wq[j] : 0.0;
+            }
+        }
+        for (int k = gk + 16 * kGReg; k < N.nin_pad; k += 16) {
+            double v = 0.0, w = 0.0;
+            if (k < N.nin) {
+                w = Wb[N.inpw_off + k];
+                if (gat >= 0) v = w * (G[(int64_t)gat * ldg + k] + Wb[N.inpb_off + k]);
+            }
+            cur[gr * lda + k] = v;
+            if (gr == 0) sinw[k] = w;
+        }
+        if (gk == 0) sat[gr] = gat;
         __syncthreads();
-        double* t = cur;
-        cur = nxt;
-        nxt = t;
-        in_pad = out_pad;
-        doff += 16 * out_pad;
-    }
-    // The output layer has one real unit and is linear (derivative 1), so neither of its passes
-    // needs the matrix cores: wave 0 forms the energies as row dot products with W_L's first row
-    // (4 lanes per atom), and the other waves the first backward step, which is elementwise:
-    // delta_{L-1} = (outw W_L[0][.]) .* d_{L-1}.  With no hidden layer that step is dE/dG itself.
-    {
-        const double* wo = wmat(L1);  // row 0: unswizzled in the LDS copy as well
-        if (L1 > 0) doff -= 16 * in_pad;  // d_{L-1}
-        if (wv == 0) {
-            const int r = lane >> 2, q = lane & 3;
-            double acc = 0.0;
-            for (int k = q; k < in_pad; k += 4) acc += wo[k] * cur[r * lda + k];
-            acc += __shfl_xor(acc, 1, 4);
-            acc += __shfl_xor(acc, 2, 4);
-            const int at = sat[r];
-            if (q == 0 && at >= 0) eatom[at] = NL.outb + (bvec(L1)[0] + acc) * NL.outw + (vac ? NL.evac : 0.0);
-        } else if (L1 > 0) {
-            for (int r = wv - 1; r < 16; r += kNetWaves - 1)
-                for (int j = lane; j < in_pad; j += kWave) nxt[r * lda + j] = NL.outw * wo[j] * der[doff + r * in_pad + j];
-        } else {
-            for (int r = wv - 1; r < 16; r += kNetWaves - 1) {
+        const DevNet& NL = snet;
+        const int L1 = NL.L - 1;  // the output layer (linear, one real unit)
+        int in_pad = NL.nin_pad;
+        int doff = 0;
+        // hidden layers
+        for (int l = 0; l < L1; ++l) {
+            const int out_pad = NL.wpad[l];
+            const double* W = wmat(l);
+            const double* b = bvec(l);
+            for (int jt = wv; jt < out_pad / 16; jt += kNetWaves) {
+                const d4_t acc = tile_mm_f64<true, WL>(cur, lda, W, in_pad, in_pad, jt, li, lk);
+                const int col = jt * 16 + li;
+                const double bc = b[col];
+                double z[4], av[4], dv[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) z[v] = acc[v] + bc;
+                net_act4<GEN>(NL.act, z, av, dv);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int row = 4 * v + lk;
+                    nxt[row * lda + col] = av[v];
+                    der[doff + row * out_pad + col] = dv[v];
+                }
+            }
+            __syncthreads();
+            double* t = cur;
+            cur = nxt;
+            nxt = t;
+            in_pad = out_pad;
+            doff += 16 * out_pad;
+        }
+        // The output layer has one real unit and is linear (derivative 1), so neither of its passes
+        // needs the matrix cores: wave 0 forms the energies as row dot products with W_L's first row
+        // (4 lanes per atom), and the other waves the first backward step, which is elementwise:
+        // delta_{L-1} = (outw W_L[0][.]) .* d_{L-1}.  With no hidden layer that step is dE/dG itself.
+        {
+            const double* wo = wmat(L1);  // row 0: unswizzled in the LDS copy as well
+            if (L1 > 0) doff -= 16 * in_pad;  // d_{L-1}
+            if (wv == 0) {
+                const int r = lane >> 2, q = lane & 3;
+                double acc = 0.0;
+                for (int k = q; k < in_pad; k += 4) acc += wo[k] * cur[r * lda + k];
+                acc += __shfl_xor(acc, 1, 4);
+                acc += __shfl_xor(acc, 2, 4);
                 const int at = sat[r];
-                if (at >= 0)
-                    for (int j = lane; j < NL.nin; j += kWave) dEdG[(int64_t)at * ldg + j] = NL.outw * wo[j] * sinw[j];
+                if (q == 0 && at >= 0) eatom[at] = NL.outb + (bvec(L1)[0] + acc) * NL.outw + (vac ? NL.evac : 0.0);
+            } else if (L1 > 0) {
+                for (int r = wv - 1; r < 16; r += kNetWaves - 1)
+                    for (int j = lane; j < in_pad; j += kWave)
+                        nxt[r * lda + j] = NL.outw * wo[j] * der[doff + r * in_pad + j];
+            } else {
+                for (int r = wv - 1; r < 16; r += kNetWaves - 1) {
+                    const int at = sat[r];
+                    if (at >= 0)
+                        for (int j = lane; j < NL.nin; j += kWave)
+                            dEdG[(int64_t)at * ldg + j] = NL.outw * wo[j] * sinw[j];
+                }
             }
         }
-    }
-    if (L1 > 0) {
-    __syncthreads();
-    {
-        double* t = cur;
-        cur = nxt;
-        nxt = t;
-    }
-    // backward through the hidden layers: delta_{l-1} = (delta_l W_l) .* d_{l-1}
-    for (int l = L1 - 1; l >= 1; --l) {
-        const int K = NL.wpad[l], J = NL.wpad[l - 1];
-        const double* W = wmat(l);  // wpad[l] x wpad[l-1], row-major
-        doff -= 16 * J;
-        for (int jt = wv; jt < J / 16; jt += kNetWaves) {
-            const d4_t acc = tile_mm_f64<false, WL>(cur, lda, W, J, K, jt, li, lk);
-            const int col = jt * 16 + li;
+        if (L1 > 0) {
+            __syncthreads();
+            {
+                double* t = cur;
+                cur = nxt;
+                nxt = t;
+            }
+            // backward through the hidden layers: delta_{l-1} = (delta_l W_l) .* d_{l-1}
+            for (int l = L1 - 1; l >= 1; --l) {
+                const int K = NL.wpad[l], J = NL.wpad[l - 1];
+                const double* W = wmat(l);  // wpad[l] x wpad[l-1], row-major
+                doff -= 16 * J;
+                for (int jt = wv; jt < J / 16; jt += kNetWaves) {
+                    const d4_t acc = tile_mm_f64<false, WL>(cur, lda, W, J, K, jt, li, lk);
+                    const int col = jt * 16 + li;
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int row = 4 * v + lk;
-                nxt[row * lda + col] = acc[v] * der[doff + row * J + col];
+                    for (int v = 0; v < 4; ++v) {
+                        const int row = 4 * v + lk;
+                        nxt[row * lda + col] = acc[v] * der[doff + row * J + col];
+                    }
+                }
+                __syncthreads();
+                double* t = cur;
+                cur = nxt;
+                nxt = t;
             }
-        }
-        __syncthreads();
-        double* t = cur;
-        cur = nxt;
-        nxt = t;
-    }
-    // dE/dG = (delta_0 W_0) .* inpw
-    {
-        const int K = NL.wpad[0], J = NL.nin_pad;
-        const double* W = wmat(0);
-        for (int jt = wv; jt < J / 16; jt += kNetWaves) {
-            const d4_t acc = tile_mm_f64<false, WL>(cur, lda, W, J, K, jt, li, lk);
-            const int col = jt * 16 + li;
+            // dE/dG = (delta_0 W_0) .* inpw
+            {
+                const int K = NL.wpad[0], J = NL.nin_pad;
+                const double* W = wmat(0);
+                for (int jt = wv; jt < J / 16; jt += kNetWaves) {
+                    const d4_t acc = tile_mm_f64<false, WL>(cur, lda, W, J, K, jt, li, lk);
+                    const int col = jt * 16 + li;
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int at = sat[4 * v + lk];
-                if (at >= 0 && col < NL.nin) dEdG[(int64_t)at * ldg + col] = acc[v] * sinw[col];
+                    for (int v = 0; v < 4; ++v) {
+                        const int at = sat[4 * v + lk];
+                        if (at >= 0 && col < NL.nin) dEdG[(int64_t)at * ldg + col] = acc[v] * sinw[col];
+                    }
+                }
             }
-        }
-    }
-    }  // L1 > 0
-    __syncthreads();  // the next tile's staging overwrites this tile's LDS
+        }  // L1 > 0
+        __syncthreads();  // the next tile's staging overwrites this tile's LDS
     }  // tiles
 }
 
@@ -427,15 +429,18 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_mixed(const DevPlan* 
             acc += __shfl_xor(acc, 1, 4);
             acc += __shfl_xor(acc, 2, 4);
             if (q == 0 && r0 + r < M)
-                eatom[atoms[r0 + r]] = (double)NL.outb + (double)(Wf[NL.boff[L1]] + acc) * NL.outw + (vac ? NL.evac : 0.0);
+                eatom[atoms[r0 + r]] =
+                    (double)NL.outb + (double)(Wf[NL.boff[L1]] + acc) * NL.outw + (vac ? NL.evac : 0.0);
         } else if (L1 > 0) {
             for (int r = wv - 1; r < 16; r += kNetWaves - 1)
-                for (int j = lane; j < in_pad; j += kWave) nxt[r * lda + j] = (float)NL.outw * wo[j] * der[doff + r * in_pad + j];
+                for (int j = lane; j < in_pad; j += kWave)
+                    nxt[r * lda + j] = (float)NL.outw * wo[j] * der[doff + r * in_pad + j];
         } else {
             for (int r = wv - 1; r < 16; r += kNetWaves - 1) {
                 if (r0 + r < M) {
                     const int64_t row = (int64_t)atoms[r0 + r] * ldg;
-                    for (int j = lane; j < NL.nin; j += kWave) dEdG[row + j] = (float)NL.outw * wo[j] * Wf[NL.inpw_off + j];
+                    for (int j = lane; j < NL.nin; j += kWave)
+                        dEdG[row + j] = (float)NL.outw * wo[j] * Wf[NL.inpw_off + j];
                 }
             }
         }
