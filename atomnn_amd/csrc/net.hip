@@ -105,10 +105,11 @@ __device__ __forceinline__ int tile_species(const NetTiles& nt, int ntypes, int 
 
 // WL: the species' weight matrices are copied into LDS (all loads in flight together), so the
 // layers read weights from LDS instead of waiting on global loads layer by layer.  A block takes
-// tiles blockIdx.x, blockIdx.x + gridDim.x, ... (species-major), and copies the weights again
-// only when the species changes: with one tile per block this is the small-batch form, with a
-// grid of a few blocks per CU the copy is spread over many tiles.
-template <bool WL, bool GEN>
+// tiles blockIdx.x, blockIdx.x + gridDim.x, ... (species-major, LOOP), and copies the weights
+// again only when the species changes, so with a grid of a few blocks per CU the copy is spread
+// over many tiles; without LOOP a block takes the one tile blockIdx.x (the small-batch form, in
+// straight-line code: the weight loads stay in flight together with the descriptor gather).
+template <bool WL, bool GEN, bool LOOP>
 __global__ __launch_bounds__(kNetWaves * kWave) void k_net_f64(const DevPlan* __restrict__ plan, const double* __restrict__ Wb,
                                                    NetTiles nt, int ntiles, const int* __restrict__ spatoms,
                                                    const double* __restrict__ G, int ldg, int vac,
@@ -117,9 +118,9 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_f64(const DevPlan* __
     __shared__ DevNet snet;
     const int tid = threadIdx.x, wv = tid / kWave, lane = tid % kWave;
     int s_prev = -1;
-    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    for (int tile = blockIdx.x; tile < ntiles; tile += LOOP ? gridDim.x : ntiles) {
         const int s = tile_species(nt, plan->T, tile);
-        const bool reload = s != s_prev;
+        const bool reload = !LOOP || s != s_prev;
         s_prev = s;
         const DevNet& N = plan->net[s];
         if (reload) stage_net(N, snet, tid);
@@ -157,7 +158,8 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_f64(const DevPlan* __
             dst = reinterpret_cast<double2*>(wl);
             n2 = N.pack_len / 2;
             // unconditional loads of clamped (in-bounds) indices: no branch, so no wait, between them
-            if (reload) {
+            // (the persistent form copies directly, on a species change only)
+            if constexpr (!LOOP) {
 #pragma unroll
                 for (int j = 0; j < kWReg; ++j) wr[j] = src[min(tid + j * kNetThreads, n2 - 1)];
             }
@@ -174,10 +176,12 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_f64(const DevPlan* __
         if constexpr (WL) {
             // unconditional too (threads past n2 rewrite the last element with its own value), so no
             // load is sunk into a branch of its own
-            if (reload) {
+            if constexpr (!LOOP) {
 #pragma unroll
                 for (int j = 0; j < kWReg; ++j) dst[min(tid + j * kNetThreads, n2 - 1)] = wr[j];
                 for (int idx = tid + kWReg * kNetThreads; idx < n2; idx += kNetThreads) dst[idx] = src[idx];
+            } else if (reload) {
+                for (int idx = tid; idx < n2; idx += kNetThreads) dst[idx] = src[idx];
             }
         }
         // layer l's weights and biases: LDS copy (bank-swizzled rows, as in the packed copy) or global
@@ -310,8 +314,8 @@ __global__ __launch_bounds__(kNetWaves * kWave) void k_net_f64(const DevPlan* __
                 }
             }
         }  // L1 > 0
-        __syncthreads();  // the next tile's staging overwrites this tile's LDS
-    }  // tiles
+        if constexpr (LOOP) __syncthreads();  // the next tile's staging overwrites this tile's LDS
+    }
 }
 
 // ------------------------------------------------------------------ mixed (BF16 MFMA, FP32 accumulate)
@@ -542,9 +546,10 @@ hipError_t launch_net(bool fp64, const DevPlan* plan, const DevPlan& hplan, cons
     const size_t bytes = wl ? ldsw : lds;
     auto kern = [&](auto k) { return (const void*)k; };
     if (fp64) {
-        const void* f = wl ? (gen ? kern(k_net_f64<true, true>) : kern(k_net_f64<true, false>))
-                           : (gen ? kern(k_net_f64<false, true>) : kern(k_net_f64<false, false>));
-        const int grid = cfg.grid > 0 && cfg.grid < ntiles ? cfg.grid : ntiles;
+        const int grid = wl && cfg.grid > 0 && cfg.grid < ntiles ? cfg.grid : ntiles;
+        const void* f = !wl ? (gen ? kern(k_net_f64<false, true, false>) : kern(k_net_f64<false, false, false>))
+                        : grid < ntiles ? (gen ? kern(k_net_f64<true, true, true>) : kern(k_net_f64<true, false, true>))
+                                        : (gen ? kern(k_net_f64<true, true, false>) : kern(k_net_f64<true, false, false>));
         void* args[] = {(void*)&plan,  (void*)&W, (void*)&nt,  (void*)&ntiles, (void*)&spatoms,
                         (void*)&G,     (void*)&ldg, (void*)&vac, (void*)&eatom,  (void*)&dEdG};
         const hipError_t e = hipLaunchKernel(f, dim3(grid), dim3(kNetThreads), args, bytes, st);
