@@ -66,6 +66,19 @@ hipError_t launch_fused(int shape, int ns, int nblocks, const DevPlan* plan, con
 // The end-of-pass reductions on their own (passes without forces): one wavefront per block.
 __global__ __launch_bounds__(kWave) void k_energy(EnergyTail t) { energy_tail_block(t, blockIdx.x, threadIdx.x); }
 
+// Results of a pass packed into one buffer for a single device-to-host copy: the four counters
+// (16 bytes), then E (per structure), E_i and F at the given 8-byte-word offsets.
+__global__ __launch_bounds__(256) void k_pack_out(const int64_t* __restrict__ cnt, const int64_t* __restrict__ E,
+                                                  int ne, const int64_t* __restrict__ ea, int nea,
+                                                  const int64_t* __restrict__ F, int nf, int64_t* __restrict__ out,
+                                                  int o_e, int o_ea, int o_f) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t < 2) out[t] = cnt[t];
+    if (t < ne) out[o_e + t] = E[t];
+    if (t < nea) out[o_ea + t] = ea[t];
+    if (t < nf) out[o_f + t] = F[t];
+}
+
 // ------------------------------------------------------------------ errors
 static thread_local std::string g_err;
 
@@ -292,7 +305,7 @@ size_t Engine::workspace_bytes() const {
     size_t b = 0;
     for (const DevMem* m : {&d_up, &d_cnt, &d_nbrj, &d_meta, &d_nbrr, &d_maxcnt, &d_abin, &d_bcount, &d_bstart,
                             &d_bfill, &d_batoms, &d_G, &d_dEdG, &d_X, &d_eatom, &d_F, &d_E, &d_sstart, &d_rcount,
-                            &d_rlist})
+                            &d_rlist, &d_out})
         b += m->cap;
     return b;
 }
@@ -524,6 +537,7 @@ void Engine::bind_scratch(void* base, size_t bytes, const nnp_structure& s) {
 
 void Engine::build_neighbors(bool all_images) {
     all_images_ = all_images;
+    speculated = false;
     for (int attempt = 0; attempt < 4; ++attempt) {
         const size_t ent = (size_t)ntot * maxp;
         if (ent >= (size_t)1 << 31) throw std::invalid_argument("batch too large for 32-bit entry indices");
@@ -551,6 +565,14 @@ void Engine::build_neighbors(bool all_images) {
         if (defer_count && lds_fits(maxp)) {
             lmaxp = maxp;
             return;
+        }
+        if (speculate_rows && spec_entries > 0 && spec_entries <= maxp) {
+            const int lm = std::min(maxp, (spec_entries + spec_entries / 8 + 8 + 7) / 8 * 8);
+            if (lds_fits(lm)) {
+                lmaxp = lm;
+                speculated = true;
+                return;
+            }
         }
         int mc = 0;
         HIPCHK(hipMemcpyAsync(&mc, d_maxcnt.p, 4, hipMemcpyDeviceToHost, stream));
@@ -1047,7 +1069,7 @@ void Engine::enqueue_pass(uint32_t flags, bool rows_ready) {
 }
 
 // Staged download: [counters | E per structure | E_i | F] into pinned memory, one sync.
-int Engine::download(bool energies, bool forces) {
+int Engine::download(bool energies, bool forces, bool keep_host) {
     if (bound) {  // straight into the pre-reserved host vectors, no staging buffer
         h_eatom.resize(ntot);
         if (energies) h_E.resize(nstruc);
@@ -1057,17 +1079,23 @@ int Engine::download(bool energies, bool forces) {
         HIPCHK(hipMemcpyAsync(h_eatom.data(), d_eatom.p, (size_t)ntot * 8, hipMemcpyDeviceToHost, stream));
         if (forces) HIPCHK(hipMemcpyAsync(h_F.data(), d_F.p, (size_t)ntot * 24, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
+        down_ea = h_eatom.data();
+        down_f = forces ? h_F.data() : nullptr;
         return counters[0];
     }
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
     const size_t o_e = 256, o_ea = o_e + (energies ? al((size_t)nstruc * 8) : 0), o_f = o_ea + al((size_t)ntot * 8),
                  total = o_f + (forces ? al((size_t)ntot * 24) : 0);
     h_down.reserve(total);
+    d_out.reserve(total);
     char* hd = (char*)h_down.p;
-    HIPCHK(hipMemcpyAsync(hd, d_maxcnt.p, 16, hipMemcpyDeviceToHost, stream));
-    if (energies) HIPCHK(hipMemcpyAsync(hd + o_e, d_E.p, (size_t)nstruc * 8, hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipMemcpyAsync(hd + o_ea, d_eatom.p, (size_t)ntot * 8, hipMemcpyDeviceToHost, stream));
-    if (forces) HIPCHK(hipMemcpyAsync(hd + o_f, d_F.p, (size_t)ntot * 24, hipMemcpyDeviceToHost, stream));
+    const int ne = energies ? nstruc : 0, nf = forces ? 3 * ntot : 0;
+    const int nthr = std::max(std::max(ntot, nf), std::max(ne, 2));
+    hipLaunchKernelGGL(k_pack_out, dim3((nthr + 255) / 256), dim3(256), 0, stream, (const int64_t*)d_maxcnt.p,
+                       (const int64_t*)d_E.p, ne, (const int64_t*)d_eatom.p, ntot, (const int64_t*)d_F.p, nf,
+                       (int64_t*)d_out.p, (int)(o_e / 8), (int)(o_ea / 8), (int)(o_f / 8));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(hd, d_out.p, total, hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
     std::memcpy(counters, hd, 16);
     const int mc = counters[0];
@@ -1075,11 +1103,17 @@ int Engine::download(bool energies, bool forces) {
         h_E.resize(nstruc);
         std::memcpy(h_E.data(), hd + o_e, (size_t)nstruc * 8);
     }
-    h_eatom.resize(ntot);
-    std::memcpy(h_eatom.data(), hd + o_ea, (size_t)ntot * 8);
-    if (forces) {
-        h_F.resize(3 * (size_t)ntot);
-        std::memcpy(h_F.data(), hd + o_f, (size_t)ntot * 24);
+    down_ea = (const double*)(hd + o_ea);
+    down_f = forces ? (const double*)(hd + o_f) : nullptr;
+    if (keep_host) {
+        h_eatom.resize(ntot);
+        std::memcpy(h_eatom.data(), hd + o_ea, (size_t)ntot * 8);
+        if (forces) {
+            h_F.resize(3 * (size_t)ntot);
+            std::memcpy(h_F.data(), hd + o_f, (size_t)ntot * 24);
+        }
+        down_ea = h_eatom.data();
+        down_f = forces ? h_F.data() : nullptr;
     }
     return mc;
 }
@@ -1126,32 +1160,39 @@ void Engine::resident_fetch(uint32_t flags, nnp_result* res) {
 }
 
 void Engine::fetch(Strucs ss, uint32_t flags, nnp_result* res) {
-    const int mc = download(false, flags & NNP_FORCE);
-    if (defer_count && mc > maxp) {  // rows overflowed: redo the batch with the count read back
+    int mc = download(false, flags & NNP_FORCE, false);
+    if ((defer_count && mc > maxp) || (speculated && mc > lmaxp)) {
+        // rows overflowed, or the rows staged at the previous call's maximum were too short: redo
+        // the batch with the count read back
+        const bool dc = defer_count, sp = speculate_rows;
         defer_count = false;
+        speculate_rows = false;
         try {
             run(ss, flags);
         } catch (...) {
-            defer_count = true;
+            defer_count = dc;
+            speculate_rows = sp;
             throw;
         }
-        defer_count = true;
-        (void)download(false, flags & NNP_FORCE);
+        defer_count = dc;
+        speculate_rows = sp;
+        mc = download(false, flags & NNP_FORCE, false);
     }
+    spec_entries = mc;
     // reactions the gather missed (or receiver rows too short): assemble again from receiver rows
     for (int attempt = 0; (flags & NNP_FORCE) && reactions_redo(counters); ++attempt) {
         if (attempt == 3) throw std::runtime_error("force assembly: receiver rows did not converge");
         assemble_prec(flags & NNP_PREC_MIXED);
-        (void)download(false, true);
+        (void)download(false, true, false);
     }
     int a0 = 0;
     for (size_t k = 0; k < ss.size(); ++k) {
         const int n = ss[k].natoms;
         double e = 0;
-        for (int i = 0; i < n; ++i) e += h_eatom[a0 + i];  // nnp.cpp:394-399, atom order
+        for (int i = 0; i < n; ++i) e += down_ea[a0 + i];  // nnp.cpp:394-399, atom order
         res[k].energy = e;
-        if (res[k].energy_atom) std::memcpy(res[k].energy_atom, &h_eatom[a0], (size_t)n * 8);
-        if ((flags & NNP_FORCE) && res[k].force) std::memcpy(res[k].force, &h_F[3 * (size_t)a0], (size_t)n * 24);
+        if (res[k].energy_atom) std::memcpy(res[k].energy_atom, down_ea + a0, (size_t)n * 8);
+        if ((flags & NNP_FORCE) && res[k].force) std::memcpy(res[k].force, down_f + 3 * (size_t)a0, (size_t)n * 24);
         a0 += n;
     }
 }
@@ -1315,6 +1356,11 @@ int nnp_compute_batch(nnp_model* m, int n, const nnp_structure* s, uint32_t flag
         if (!m || !s || !r || n <= 0) throw std::invalid_argument("nnp_compute: invalid argument");
         Engine& E = *m->eng;
         std::vector<nnp_structure> ss(s, s + n);
+        struct Spec {  // row speculation for this call only
+            Engine& e;
+            explicit Spec(Engine& x) : e(x) { e.speculate_rows = true; }
+            ~Spec() { e.speculate_rows = false; }
+        } spec(E);
         E.run(ss, flags);
         E.fetch(ss, flags, r);
         return NNP_OK;

@@ -256,8 +256,20 @@ class Engine {
     DevMem d_abin, d_bcount, d_bstart, d_bfill, d_batoms;
     DevMem d_G, d_dEdG, d_X, d_eatom, d_F;
     // copy E (per structure, optional), E_i, F (optional) and the neighbour maximum to the host
-    // in one staged download; returns the neighbour maximum
-    int download(bool energies, bool forces);
+    // in one staged download (packed on the device into d_out, one copy); returns the neighbour
+    // maximum.  keep_host = false leaves E_i and F in the staging buffer (down_ea, down_f)
+    // instead of copying them into h_eatom / h_F.
+    int download(bool energies, bool forces, bool keep_host = true);
+    DevMem d_out;
+    const double* down_ea = nullptr;
+    const double* down_f = nullptr;
+    // Synchronous calls (nnp_compute): stage the rows at the previous call's realised neighbour
+    // maximum (with head room) instead of reading this call's maximum back before the
+    // descriptors; the maximum comes back with the results and a pass that exceeded the
+    // staged rows is run again with the count read first.
+    bool speculate_rows = false;
+    bool speculated = false;
+    int spec_entries = 0;
 
     std::vector<double> h_pos, h_eatom, h_F;
     std::vector<int> h_type, h_astruc, h_cell_centres, h_pair_centres, h_centres, h_spatoms, sp_off;

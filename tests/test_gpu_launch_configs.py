@@ -46,3 +46,31 @@ def test_net_launch_configurations_bitwise(tmp_path):
         assert np.array_equal(flat(m.compute_batch(small)), a)
     finally:
         m.close()
+
+
+def test_synchronous_calls_with_speculated_rows(tmp_path):
+    """nnp_compute stages the rows at the previous call's neighbour maximum and redoes a call
+    whose maximum exceeds them: a sparse cell, then denser ones (the redo), then the sparse one
+    again, each bit for bit equal to a fresh model's first call (no speculation)."""
+    import numpy as np  # noqa: F811
+
+    path = str(tmp_path / "plain.ann")
+    synth.workload_potential("plain4096").write(path)
+    rng = np.random.default_rng(5)
+    strucs = []
+    for rho in (0.03, 0.05, 0.07, 0.03):
+        cell = synth.cubic_cell(1200, rho)
+        strucs.append(synth.Structure(synth.species_exact(1200, [0.5, 0.5], rng),
+                                      synth.random_positions(1200, cell, 1.8, 11), cell))
+    m = NNP.read(path)
+    try:
+        got = [m.compute(s, FLAGS) for s in strucs]
+    finally:
+        m.close()
+    for s, (E, Ea, F) in zip(strucs, got):
+        f = NNP.read(path)
+        try:
+            E0, Ea0, F0 = f.compute(s, FLAGS)
+        finally:
+            f.close()
+        assert E == E0 and np.array_equal(Ea, Ea0) and np.array_equal(F, F0)
