@@ -1119,11 +1119,11 @@ int Engine::download(bool energies, bool forces, bool keep_host) {
 }
 
 void Engine::resident_fetch(uint32_t flags, nnp_result* res) {
-    int c[4] = {0, 0, 0, 0};
-    HIPCHK(hipMemcpyAsync(c, d_maxcnt.p, 16, hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipStreamSynchronize(stream));
+    // the counters come back with the results (one round trip); a pass they show to be invalid
+    // is redone and fetched again
+    const bool frc = flags & NNP_FORCE;
+    int mc = download(true, frc, false);
     profile_collect();
-    const int mc = c[0];
     if (mc > lmaxp) {  // the rows (or their LDS staging) overflowed: enlarge and redo the pass
         if (mc > maxp) {
             maxp = (mc + mc / 8 + 8 + 31) / 32 * 32;
@@ -1132,29 +1132,24 @@ void Engine::resident_fetch(uint32_t flags, nnp_result* res) {
         lmaxp = std::min(maxp, (mc + mc / 16 + 4 + 7) / 8 * 8);
         drop_graph();
         enqueue_pass(flags);
-        HIPCHK(hipStreamSynchronize(stream));
+        mc = download(true, frc, false);
         profile_collect();
-        HIPCHK(hipMemcpyAsync(c, d_maxcnt.p, 16, hipMemcpyDeviceToHost, stream));
-        HIPCHK(hipStreamSynchronize(stream));
     }
     // reactions the gather missed (or receiver rows too short): the batch takes receiver rows
-    for (int attempt = 0; (flags & NNP_FORCE) && reactions_redo(c); ++attempt) {
+    for (int attempt = 0; frc && reactions_redo(counters); ++attempt) {
         if (attempt == 3) throw std::runtime_error("force assembly: receiver rows did not converge");
         drop_graph();
         eager_runs = 0;
         enqueue_pass(flags);
-        HIPCHK(hipStreamSynchronize(stream));
+        (void)download(true, frc, false);
         profile_collect();
-        HIPCHK(hipMemcpyAsync(c, d_maxcnt.p, 16, hipMemcpyDeviceToHost, stream));
-        HIPCHK(hipStreamSynchronize(stream));
     }
-    (void)download(true, flags & NNP_FORCE);
     int a0 = 0;
     for (int k = 0; k < nstruc; ++k) {
         const int n = h_natoms[k];
         res[k].energy = h_E[k];
-        if (res[k].energy_atom) std::memcpy(res[k].energy_atom, &h_eatom[a0], (size_t)n * 8);
-        if ((flags & NNP_FORCE) && res[k].force) std::memcpy(res[k].force, &h_F[3 * (size_t)a0], (size_t)n * 24);
+        if (res[k].energy_atom) std::memcpy(res[k].energy_atom, down_ea + a0, (size_t)n * 8);
+        if (frc && res[k].force) std::memcpy(res[k].force, down_f + 3 * (size_t)a0, (size_t)n * 24);
         a0 += n;
     }
 }
