@@ -253,7 +253,7 @@ def valu_floor_from_profile(path, kernel):
         with open(path) as f:
             for line in f:
                 cells = [c.strip() for c in line.split("|")]
-                if len(cells) > 8 and cells[1] == kernel:
+                if len(cells) > 8 and cells[1].replace("atomnn::", "") == kernel:
                     return float(cells[7])
     except (OSError, ValueError):
         pass
@@ -456,7 +456,7 @@ def main():
             # 4 cycles on every SIMD; the launch time over it is the issue efficiency
             "valu_issue_floor_us": valu_floor_from_profile(
                 os.path.join(ROOT, "profiles", "plain4096_fp64_sq_counters.md"),
-                "atomnn::k_force_mom<double, 4, 1>" if mom_forces else "atomnn::k_force<double, 4, 4, false>"),
+                "k_force_mom<double, 4, 1>" if mom_forces else "k_force<double, 4, 4, false>"),
             "launch_us": t_force * 1e6,
         },
     }
