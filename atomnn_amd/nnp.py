@@ -213,6 +213,23 @@ class NNP:
             out.append((res[k].energy, eas[k], fs[k] if flags & NNP_FORCE else None))
         return out
 
+    def compute_into(self, s: Structure, flags: int = NNP_FORCE | NNP_ATOM_ENERGY | NNP_VACUUM_ENERGY):
+        """A zero-argument callable that runs nnp_compute on s into fixed result arrays, as a C
+        driver's loop calls it (the same host buffers every call, no per-call marshalling), and
+        returns (energy, per-atom energies, forces) views of those arrays."""
+        pin = _Pinned(s)
+        ea = np.zeros(s.natoms)
+        f = np.zeros((s.natoms, 3))
+        res = _Result(0.0, ea.ctypes.data, f.ctypes.data)
+        fn, h, st, rs = lib().nnp_compute, self._h, ctypes.byref(pin.c_struct), ctypes.byref(res)
+
+        def call():
+            _check(fn(h, st, flags, rs))
+            return res.energy, ea, f
+
+        call.keep = (pin, ea, f, res)  # the arrays live as long as the callable
+        return call
+
     def set_option(self, key: str, value: int) -> None:
         _check(lib().nnp_set_option(self._h, key.encode(), int(value)))
 

@@ -74,3 +74,21 @@ def test_synchronous_calls_with_speculated_rows(tmp_path):
         finally:
             f.close()
         assert E == E0 and np.array_equal(Ea, Ea0) and np.array_equal(F, F0)
+
+
+def test_compute_into_fixed_buffers(tmp_path):
+    """The fixed-buffer caller (the benchmark's nnpce-style loop) returns what compute returns."""
+    import numpy as np  # noqa: F811
+
+    path = str(tmp_path / "plain.ann")
+    synth.workload_potential("plain4096").write(path)
+    m = NNP.read(path)
+    try:
+        s = synth.batch512(3)[2]
+        E0, Ea0, F0 = m.compute(s, FLAGS)
+        call = m.compute_into(s, FLAGS)
+        for _ in range(3):
+            E, Ea, F = call()
+            assert E == E0 and np.array_equal(Ea, Ea0) and np.array_equal(F, F0)
+    finally:
+        m.close()
