@@ -498,6 +498,12 @@ def main():
             st = profiled_steps(Rm, 2)
             res[label] = (mat * steps / secm, got, st, secm / steps)
             del Rm
+            # the same batch through the synchronous call (host positions in, host results out)
+            model_mixed.compute_batch(ms, fl)
+            t0 = time.perf_counter()
+            for _ in range(2):
+                model_mixed.compute_batch(ms, fl)
+            res[label + "_e2e"] = mat * 2 / (time.perf_counter() - t0)
         ref = res["plain_fp64"][1]
         mixed_results = res["mixed"][1]
         e_err = max(abs(a[0] - b[0]) / s.natoms for a, b, s in zip(mixed_results, ref, ms))
@@ -513,6 +519,9 @@ def main():
             "plain_fp64_atoms_per_s": res["plain_fp64"][0],
             "plain_fp64_ms_per_pass": res["plain_fp64"][3] * 1e3,
             "speedup_vs_plain_fp64": res["mixed"][0] / res["plain_fp64"][0],
+            # the same ratio through nnp_compute_batch (upload, pass, download; PCIe included)
+            "end_to_end": {"mixed_atoms_per_s": res["mixed_e2e"], "plain_fp64_atoms_per_s": res["plain_fp64_e2e"],
+                           "speedup_vs_plain_fp64": res["mixed_e2e"] / res["plain_fp64_e2e"]},
             "neighbor_entries": Pm,
             "triplets": Tm,
             "stage_ms": {k: round(float(v), 4) for k, v in
