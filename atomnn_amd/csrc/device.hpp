@@ -39,6 +39,8 @@ struct DevStruc {
 };
 
 constexpr int kWave = 64;
+// XCDs of an MI355X: a launch's blocks are dealt round-robin over them, each with its own L2
+constexpr int kXcds = 8;
 
 // Force assembly (k_assemble).  react points at two per-pass counters beside the neighbour
 // maximum: react[0] = reactions the gather from the neighbours' rows missed (mode 0; cleared by

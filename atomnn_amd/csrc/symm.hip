@@ -1665,7 +1665,6 @@ __device__ __forceinline__ int row_lower_bound(const int* __restrict__ nbr_j, in
     return lo + c;
 }
 
-constexpr int kXcds = 8;
 
 template <typename T, int MODE>
 __global__ __launch_bounds__(kWave) void k_assemble(const DevStruc* __restrict__ S, const int* __restrict__ atom_struc,
