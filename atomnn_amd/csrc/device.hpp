@@ -42,6 +42,16 @@ constexpr int kWave = 64;
 // XCDs of an MI355X: a launch's blocks are dealt round-robin over them, each with its own L2
 constexpr int kXcds = 8;
 
+// XCD-local order of n items (centres): block b, which runs on XCD b mod 8, takes item
+// (b mod 8) chunk + b / 8, so each XCD's blocks cover one contiguous run; -1 for the padding
+// blocks past n.  Launch xcd_grid(n) blocks.
+__device__ __forceinline__ int xcd_slot(int n) {
+    const int chunk = (n + kXcds - 1) / kXcds;
+    const int pos = (int)(blockIdx.x % kXcds) * chunk + (int)(blockIdx.x / kXcds);
+    return pos < n ? pos : -1;
+}
+inline int xcd_grid(int n) { return kXcds * ((n + kXcds - 1) / kXcds); }
+
 // Force assembly (k_assemble).  react points at two per-pass counters beside the neighbour
 // maximum: react[0] = reactions the gather from the neighbours' rows missed (mode 0; cleared by
 // the neighbour launch), react[1] = longest receiver row (mode 1; cleared by that mode's own

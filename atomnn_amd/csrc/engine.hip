@@ -304,7 +304,7 @@ Engine::Engine(const Engine& o, int) : host(o.host), pb(o.pb), shape(o.shape) {
 size_t Engine::workspace_bytes() const {
     size_t b = 0;
     for (const DevMem* m : {&d_up, &d_cnt, &d_nbrj, &d_meta, &d_nbrr, &d_maxcnt, &d_abin, &d_bcount, &d_bstart,
-                            &d_bfill, &d_batoms, &d_G, &d_dEdG, &d_X, &d_eatom, &d_F, &d_E, &d_sstart, &d_rcount,
+                            &d_bfill, &d_G, &d_dEdG, &d_X, &d_eatom, &d_F, &d_E, &d_sstart, &d_rcount,
                             &d_rlist, &d_out})
         b += m->cap;
     return b;
@@ -322,7 +322,7 @@ Engine::~Engine() {
     if (owns_model)
         for (DevMem* m : {&d_plan, &d_rfe, &d_rfr, &d_w64, &d_w32}) m->release();
     for (DevMem* m : {&d_up, &d_cnt, &d_nbrj, &d_meta, &d_nbrr, &d_maxcnt, &d_abin, &d_bcount, &d_bstart,
-                      &d_bfill, &d_batoms, &d_G, &d_dEdG, &d_X, &d_eatom, &d_F, &d_rcount, &d_rlist})
+                      &d_bfill, &d_G, &d_dEdG, &d_X, &d_eatom, &d_F, &d_rcount, &d_rlist})
         m->release();
     h_up.release();
     h_down.release();
@@ -510,7 +510,7 @@ void Engine::bind_scratch(void* base, size_t bytes, const nnp_structure& s) {
     take(d_work, 64);
     take(d_E, 8);
     take(d_sstart, 8);
-    for (DevMem* m : {&d_cnt, &d_abin, &d_batoms, &d_rcount}) take(*m, N * 4);
+    for (DevMem* m : {&d_cnt, &d_abin, &d_rcount}) take(*m, N * 4);
     take(d_eatom, N * 8);
     take(d_F, N * 24);
     for (DevMem* m : {&d_bcount, &d_bstart, &d_bfill}) take(*m, nb * 4);
@@ -555,7 +555,6 @@ void Engine::build_neighbors(bool all_images) {
             d_bcount.reserve((size_t)nbins * 4);
             d_bstart.reserve((size_t)nbins * 4);
             d_bfill.reserve((size_t)nbins * 4);
-            d_batoms.reserve((size_t)ntot * 4);
         }
         const NeighborArgs a = neighbor_args(all_images);
         HIPCHK(launch_neighbors(a, stream));
@@ -567,7 +566,8 @@ void Engine::build_neighbors(bool all_images) {
             return;
         }
         if (speculate_rows && spec_entries > 0 && spec_entries <= maxp) {
-            const int lm = std::min(maxp, (spec_entries + spec_entries / 8 + 8 + 7) / 8 * 8);
+            // the resident path's head room: LDS rows decide the per-centre kernels' occupancy
+            const int lm = std::min(maxp, (spec_entries + spec_entries / 16 + 4 + 7) / 8 * 8);
             if (lds_fits(lm)) {
                 lmaxp = lm;
                 speculated = true;
@@ -723,7 +723,7 @@ void Engine::assemble(const EnergyTail* tail) {
     a.rcap = 0;
     // atoms in bin order (cell-list structures) then the rest, for the XCD-local block order
     a.ncell = nbins > 0 ? (int)h_cell_centres.size() : 0;
-    a.bin_atoms = a.ncell > 0 ? (const int*)d_batoms.p : nullptr;
+    a.bin_atoms = a.ncell > 0 ? (const int*)d_centres.p : nullptr;
     a.pair_centres = (const int*)d_centres.p + a.ncell;
     if (assemble_mode == 1) {
         // a receiver row holds about as many entries as a centre's row
@@ -1011,7 +1011,9 @@ NeighborArgs Engine::neighbor_args(bool all_images) const {
     a.bin_count = (int*)d_bcount.p;
     a.bin_start = (int*)d_bstart.p;
     a.bin_fill = (int*)d_bfill.p;
-    a.bin_atoms = (int*)d_batoms.p;
+    // the binning writes the cell-list atoms in bin order over the first ncell centres, so every
+    // per-centre kernel walking the centres in XCD-local order takes spatially compact runs
+    a.bin_atoms = (int*)d_centres.p;
     return a;
 }
 

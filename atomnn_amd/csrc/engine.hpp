@@ -253,7 +253,7 @@ class Engine {
     HostPinned h_up, h_down;
     DevView d_pos, d_type, d_astruc, d_strucs, d_centres, d_spatoms;
     DevMem d_cnt, d_nbrj, d_meta, d_nbrr, d_maxcnt;
-    DevMem d_abin, d_bcount, d_bstart, d_bfill, d_batoms;
+    DevMem d_abin, d_bcount, d_bstart, d_bfill;
     DevMem d_G, d_dEdG, d_X, d_eatom, d_F;
     // copy E (per structure, optional), E_i, F (optional) and the neighbour maximum to the host
     // in one staged download (packed on the device into d_out, one copy); returns the neighbour

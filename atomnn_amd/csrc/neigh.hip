@@ -475,17 +475,16 @@ __global__ __launch_bounds__(kWave) void k_neigh_cells(const DevStruc* __restric
 // LDS: one 32-bit word per 32 atoms of the largest cell-list structure of the batch.
 __global__ __launch_bounds__(kWave) void k_neigh_cells_bits(
     const DevStruc* __restrict__ S, const int* __restrict__ atom_struc, const double* __restrict__ pos,
-    const int* __restrict__ type, int ncell, int chunk, const int* __restrict__ atom_bin,
+    const int* __restrict__ type, int ncell, const int* __restrict__ atom_bin,
     const int* __restrict__ bin_start, const int* __restrict__ bin_count, const int* __restrict__ bin_atoms,
     double rc2, int maxp, int* __restrict__ cnt, int* __restrict__ nbr_j, int* __restrict__ nbr_meta,
     double4* __restrict__ nbr_r) {
     extern __shared__ unsigned int lds_bits[];
-    // block b takes position (b mod 8) chunk + b / 8 of the bin order of the cell-list atoms: the
-    // blocks of one XCD cover one contiguous, spatially compact run of centres, whose candidates'
-    // positions and bin lists they then share in that XCD's L2 (the centre a block computes does not
-    // change any row)
-    const int slot = (int)(blockIdx.x % kXcds) * chunk + (int)(blockIdx.x / kXcds);
-    if (slot >= ncell) return;
+    // centres in the XCD-local order of the cell-list atoms in bin order: the blocks of one XCD
+    // cover one contiguous, spatially compact run of centres, whose candidates' positions and bin
+    // lists they then share in that XCD's L2 (the centre a block computes does not change any row)
+    const int slot = xcd_slot(ncell);
+    if (slot < 0) return;
     const int i = bin_atoms[slot];
     const DevStruc& s = S[atom_struc[i]];
     const int lane = threadIdx.x;
@@ -642,10 +641,9 @@ hipError_t launch_neighbors(const NeighborArgs& a, hipStream_t st) {
         }
         const size_t bits_lds = (size_t)((a.cell_natoms_max + 31) / 32) * 4;
         if (bits_lds <= 32768) {
-            const int chunk = (a.ncell + kXcds - 1) / kXcds;
-            hipLaunchKernelGGL(k_neigh_cells_bits, dim3(kXcds * chunk), dim3(kWave), bits_lds, st, a.S, a.atom_struc,
-                               a.pos, a.type, a.ncell, chunk, a.atom_bin, a.bin_start, a.bin_count, a.bin_atoms,
-                               a.rc2, a.maxp, a.cnt, a.nbr_j, a.nbr_meta, a.nbr_r);
+            hipLaunchKernelGGL(k_neigh_cells_bits, dim3(xcd_grid(a.ncell)), dim3(kWave), bits_lds, st, a.S,
+                               a.atom_struc, a.pos, a.type, a.ncell, a.atom_bin, a.bin_start, a.bin_count,
+                               a.bin_atoms, a.rc2, a.maxp, a.cnt, a.nbr_j, a.nbr_meta, a.nbr_r);
         } else {  // very large structures: sort the kept candidates instead
             int n2cap = 1;
             while (n2cap < a.maxp) n2cap <<= 1;

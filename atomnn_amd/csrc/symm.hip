@@ -777,7 +777,7 @@ __host__ __device__ constexpr size_t mom_lds_bytes(int lmaxp, int ntypes, int bi
 template <typename T, int NE, int NC, bool GEN>
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <= 16 ? (GEN ? 2 : 3) : 1, 8))) void k_symm(
     const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta, const double* __restrict__ rfun_rs,
-    const int* __restrict__ atom_type, const int* __restrict__ centres, const int* __restrict__ cnt,
+    const int* __restrict__ atom_type, const int* __restrict__ centres, int ncentres, const int* __restrict__ cnt,
     const int* __restrict__ nbr_meta, const double4* __restrict__ nbr_r, int maxp, int lmaxp, T* __restrict__ G,
     int ldg, int use_mom, int red_elems) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -787,7 +787,11 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
     T* Grow = red + red_elems;
     T* nsx = Grow + ((ldg + 1) & ~1);
     T* mom = use_mom ? nsx + (GEN ? 2 * lmaxp : 0) : nullptr;
-    const int i = centres[blockIdx.x];
+    // centres in the XCD-local order of the neighbour kernel, which wrote this centre's row from a
+    // block on the same XCD (its lines can still be in that L2)
+    const int slot = xcd_slot(ncentres);
+    if (slot < 0) return;
+    const int i = centres[slot];
     const int lane = threadIdx.x;
     const int ti = atom_type[i];
     const int P = min(cnt[i], lmaxp);
@@ -807,7 +811,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
 template <typename T, int NE, int NC, bool GEN>
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <= 16 ? (GEN ? 2 : 3) : 1, 8))) void k_force(
     const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta, const double* __restrict__ rfun_rs,
-    const int* __restrict__ atom_type, const int* __restrict__ centres, const int* __restrict__ cnt,
+    const int* __restrict__ atom_type, const int* __restrict__ centres, int ncentres, const int* __restrict__ cnt,
     const int* __restrict__ nbr_meta, const double4* __restrict__ nbr_r, int maxp, int lmaxp,
     const T* __restrict__ dEdG, int ldg, T* __restrict__ Xout, int use_mom) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -821,7 +825,9 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
     int* slot_row = orig + lmaxp;     // [2][kWave]
     T* nsx = (T*)(slot_row + ((2 * kWave + 3) & ~3));
     T* mom = use_mom ? nsx + (GEN ? 2 * lmaxp : 0) : nullptr;
-    const int i = centres[blockIdx.x];
+    const int slot = xcd_slot(ncentres);  // the XCD-local order of the neighbour kernel
+    if (slot < 0) return;
+    const int i = centres[slot];
     const int lane = threadIdx.x;
     const int ti = atom_type[i];
     const int P = min(cnt[i], lmaxp);
@@ -1245,7 +1251,7 @@ __host__ __device__ constexpr size_t force_mom_lds_bytes(int lmaxp, int ldg, int
 template <typename T, int NE, int NW>
 __global__ __launch_bounds__(kWave * NW) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 4 : 6, 8))) void k_force_mom(
     const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta, const double* __restrict__ rfun_rs,
-    const int* __restrict__ atom_type, const int* __restrict__ centres, const int* __restrict__ cnt,
+    const int* __restrict__ atom_type, const int* __restrict__ centres, int ncentres, const int* __restrict__ cnt,
     const int* __restrict__ nbr_meta, const double4* __restrict__ nbr_r, int maxp, int lmaxp,
     const T* __restrict__ dEdG, int ldg, T* __restrict__ Xout) {
     constexpr int NK = NE / NW;
@@ -1263,7 +1269,9 @@ __global__ __launch_bounds__(kWave * NW) __attribute__((amdgpu_waves_per_eu(size
     int* orig = (int*)(U + force_mom_u_elems<NE, NW>(ntypes));
     __shared__ int seg_lds[kMaxSpecies + 1];
     Segs sg{seg_lds};
-    const int i = centres[blockIdx.x];
+    const int slot = xcd_slot(ncentres);  // the XCD-local order of the neighbour kernel
+    if (slot < 0) return;
+    const int i = centres[slot];
     const int ti = atom_type[i];
     const int P = min(cnt[i], lmaxp);
     if (w == 0) stage_entries<T>(i, P, maxp, ntypes, nbr_meta, nbr_r, su, orig, sg, lane);
@@ -1429,8 +1437,8 @@ static hipError_t launch_force_mom_t(const DevPlan* plan, const double* rfe, con
                                      hipStream_t st) {
     const size_t lds = force_mom_lds_bytes<T, NE, NW>(lmaxp, ldg, ntypes);
     if (lds > 65536) return hipErrorInvalidConfiguration;
-    hipLaunchKernelGGL((k_force_mom<T, NE, NW>), dim3(ncentres), dim3(kWave * NW), lds, st, plan, rfe, rfr,
-                       atom_type, centres, cnt, meta, r, maxp, lmaxp, dEdG, ldg, X);
+    hipLaunchKernelGGL((k_force_mom<T, NE, NW>), dim3(xcd_grid(ncentres)), dim3(kWave * NW), lds, st, plan, rfe,
+                       rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp, dEdG, ldg, X);
     return hipGetLastError();
 }
 
@@ -1804,8 +1812,8 @@ static hipError_t launch_symm_t(const DevPlan* plan, const double* rfe, const do
     const size_t lds =
         symm_lds_bytes<T, NE, NC, GEN>(lmaxp, ldg, red) + (use_mom ? mom_lds_bytes<T, NE>(lmaxp, ntypes, 1) : 0);
     if (lds > 65536) return hipErrorInvalidConfiguration;
-    hipLaunchKernelGGL((k_symm<T, NE, NC, GEN>), dim3(ncentres), dim3(kWave), lds, st, plan, rfe, rfr, atom_type,
-                       centres, cnt, meta, r, maxp, lmaxp, G, ldg, use_mom, red);
+    hipLaunchKernelGGL((k_symm<T, NE, NC, GEN>), dim3(xcd_grid(ncentres)), dim3(kWave), lds, st, plan, rfe, rfr,
+                       atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp, G, ldg, use_mom, red);
     return hipGetLastError();
 }
 
@@ -1816,8 +1824,8 @@ static hipError_t launch_force_t(const DevPlan* plan, const double* rfe, const d
                                  hipStream_t st) {
     const size_t lds = force_lds_bytes<T, NE, NC, GEN>(lmaxp, ldg) + (use_mom ? mom_lds_bytes<T, NE>(lmaxp, ntypes, 2) : 0);
     if (lds > 65536) return hipErrorInvalidConfiguration;
-    hipLaunchKernelGGL((k_force<T, NE, NC, GEN>), dim3(ncentres), dim3(kWave), lds, st, plan, rfe, rfr, atom_type,
-                       centres, cnt, meta, r, maxp, lmaxp, dEdG, ldg, X, use_mom);
+    hipLaunchKernelGGL((k_force<T, NE, NC, GEN>), dim3(xcd_grid(ncentres)), dim3(kWave), lds, st, plan, rfe, rfr,
+                       atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp, dEdG, ldg, X, use_mom);
     return hipGetLastError();
 }
 
