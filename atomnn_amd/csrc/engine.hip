@@ -55,6 +55,7 @@ hipError_t launch_net(bool fp64, const DevPlan* plan, const DevPlan& hplan, cons
 int net_tiles(const int* sp_off, int ntypes);
 bool net_wl_fits(const DevPlan& hplan);
 hipError_t launch_neighbors(const NeighborArgs& a, hipStream_t st);
+hipError_t launch_count_max(const int* cnt, int n, int* maxcnt, hipStream_t st);
 size_t fused_lds(int shape, bool fp64, bool ns, int lmaxp, int ldg, int maxw, int dersum);
 size_t mom_lds(int shape, bool fp64, int lmaxp, int ntypes, int bits);
 template <typename T>
@@ -378,49 +379,63 @@ void Engine::upload(Strucs ss) {
         if (!s.pos || !s.type) throw std::invalid_argument("structure without positions or types");
         ntot += s.natoms;
     }
-    h_pos.resize(3 * (size_t)ntot);
-    h_type.resize(ntot);
-    h_astruc.resize(ntot);
+    // per structure: cell, images, cell-list bins
     h_strucs.clear();
-    h_cell_centres.clear();
-    h_pair_centres.clear();
-    int a0 = 0, bins = 0;
-    for (int k = 0; k < nstruc; ++k) {
-        const auto& s = ss[k];
-        DevStruc d = make_struc(s, host.rc, a0);
+    up_modes.clear();
+    int bins = 0;
+    for (int k = 0, a0 = 0; k < nstruc; a0 += ss[k].natoms, ++k) {
+        DevStruc d = make_struc(ss[k], host.rc, a0);
         if (d.mode == 1) {
             d.bin0 = bins;
             bins += d.nbin[0] * d.nbin[1] * d.nbin[2];
-            cell_natoms_max = std::max(cell_natoms_max, s.natoms);
+            cell_natoms_max = std::max(cell_natoms_max, ss[k].natoms);
         }
         h_strucs.push_back(d);
-        std::memcpy(&h_pos[3 * (size_t)a0], s.pos, 3 * sizeof(double) * s.natoms);
-        for (int i = 0; i < s.natoms; ++i) {
-            const int t = s.type[i];
-            if (t < 0 || t >= host.T) throw std::invalid_argument("atom type out of range");
-            h_type[a0 + i] = t;
-            h_astruc[a0 + i] = k;
-            (d.mode == 1 ? h_cell_centres : h_pair_centres).push_back(a0 + i);
-        }
-        a0 += s.natoms;
+        up_modes.push_back((char)d.mode);
     }
     nbins = bins;
-    // per-species atom lists for the network GEMMs
-    sp_off.assign(host.T + 1, 0);
-    for (int i = 0; i < ntot; ++i) sp_off[h_type[i] + 1]++;
-    for (int t = 0; t < host.T; ++t) sp_off[t + 1] += sp_off[t];
-    h_spatoms.resize(ntot);
-    sp_fill.assign(sp_off.begin(), sp_off.end() - 1);
-    for (int i = 0; i < ntot; ++i) h_spatoms[sp_fill[h_type[i]]++] = i;
-    h_centres.assign(h_cell_centres.begin(), h_cell_centres.end());
-    h_centres.insert(h_centres.end(), h_pair_centres.begin(), h_pair_centres.end());
-    // one pinned staging buffer, one copy: positions | species | structure of each atom |
-    // structures | centres (cell-list first) | species-sorted atoms, each 256-byte aligned
+    // per atom (species, structure of each atom, centre lists, species-sorted atoms): kept from
+    // the previous upload when the atom counts, species and search modes are the same, as in a
+    // driver loop over one structure
+    bool same = up_tables && !bound && up_natoms == h_natoms && up_modes_prev == up_modes;
+    for (int k = 0, a0 = 0; same && k < nstruc; a0 += ss[k].natoms, ++k)
+        same = std::memcmp(ss[k].type, &h_type[a0], sizeof(int32_t) * ss[k].natoms) == 0;
+    up_tables = false;
+    if (!same) {
+        h_type.resize(ntot);
+        h_astruc.resize(ntot);
+        h_cell_centres.clear();
+        h_pair_centres.clear();
+        for (int k = 0, a0 = 0; k < nstruc; a0 += ss[k].natoms, ++k) {
+            const auto& st = ss[k];
+            for (int i = 0; i < st.natoms; ++i) {
+                const int t = st.type[i];
+                if (t < 0 || t >= host.T) throw std::invalid_argument("atom type out of range");
+                h_type[a0 + i] = t;
+                h_astruc[a0 + i] = k;
+                (h_strucs[k].mode == 1 ? h_cell_centres : h_pair_centres).push_back(a0 + i);
+            }
+        }
+        // per-species atom lists for the network GEMMs
+        sp_off.assign(host.T + 1, 0);
+        for (int i = 0; i < ntot; ++i) sp_off[h_type[i] + 1]++;
+        for (int t = 0; t < host.T; ++t) sp_off[t + 1] += sp_off[t];
+        h_spatoms.resize(ntot);
+        sp_fill.assign(sp_off.begin(), sp_off.end() - 1);
+        for (int i = 0; i < ntot; ++i) h_spatoms[sp_fill[h_type[i]]++] = i;
+        h_centres.assign(h_cell_centres.begin(), h_cell_centres.end());
+        h_centres.insert(h_centres.end(), h_pair_centres.begin(), h_pair_centres.end());
+    }
+    // one device buffer: positions | structures | species | structure of each atom | centres
+    // (cell-list first) | species-sorted atoms, each 256-byte aligned; the first two change with
+    // every call
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
-    const size_t o_pos = 0, o_type = o_pos + al(h_pos.size() * 8), o_ast = o_type + al((size_t)ntot * 4),
-                 o_st = o_ast + al((size_t)ntot * 4), o_cen = o_st + al(h_strucs.size() * sizeof(DevStruc)),
+    const size_t o_pos = 0, o_st = o_pos + al((size_t)ntot * 24), o_type = o_st + al(h_strucs.size() * sizeof(DevStruc)),
+                 o_ast = o_type + al((size_t)ntot * 4), o_cen = o_ast + al((size_t)ntot * 4),
                  o_spa = o_cen + al((size_t)ntot * 4), total = o_spa + al((size_t)ntot * 4);
+    const void* up_prev = d_up.p;
     d_up.reserve(total);
+    same = same && d_up.p == up_prev;  // a new device buffer takes every table
     char* ds = (char*)d_up.p;
     d_pos.p = ds + o_pos;
     d_type.p = ds + o_type;
@@ -430,6 +445,9 @@ void Engine::upload(Strucs ss) {
     d_spatoms.p = ds + o_spa;
     if (bound) {
         // no staging buffer: each piece goes straight from host memory into its slice
+        h_pos.resize(3 * (size_t)ntot);
+        for (int k = 0, a0 = 0; k < nstruc; a0 += ss[k].natoms, ++k)
+            std::memcpy(&h_pos[3 * (size_t)a0], ss[k].pos, 3 * sizeof(double) * ss[k].natoms);
         HIPCHK(hipMemcpyAsync(d_pos.p, h_pos.data(), h_pos.size() * 8, hipMemcpyHostToDevice, stream));
         HIPCHK(hipMemcpyAsync(d_type.p, h_type.data(), (size_t)ntot * 4, hipMemcpyHostToDevice, stream));
         HIPCHK(hipMemcpyAsync(d_astruc.p, h_astruc.data(), (size_t)ntot * 4, hipMemcpyHostToDevice, stream));
@@ -442,13 +460,19 @@ void Engine::upload(Strucs ss) {
     }
     h_up.reserve(total);
     char* hs = (char*)h_up.p;
-    std::memcpy(hs + o_pos, h_pos.data(), h_pos.size() * 8);
-    std::memcpy(hs + o_type, h_type.data(), (size_t)ntot * 4);
-    std::memcpy(hs + o_ast, h_astruc.data(), (size_t)ntot * 4);
+    for (int k = 0, a0 = 0; k < nstruc; a0 += ss[k].natoms, ++k)
+        std::memcpy(hs + o_pos + 24 * (size_t)a0, ss[k].pos, 3 * sizeof(double) * ss[k].natoms);
     std::memcpy(hs + o_st, h_strucs.data(), h_strucs.size() * sizeof(DevStruc));
-    std::memcpy(hs + o_cen, h_centres.data(), (size_t)ntot * 4);
-    std::memcpy(hs + o_spa, h_spatoms.data(), (size_t)ntot * 4);
-    HIPCHK(hipMemcpyAsync(d_up.p, h_up.p, total, hipMemcpyHostToDevice, stream));
+    if (!same) {
+        std::memcpy(hs + o_type, h_type.data(), (size_t)ntot * 4);
+        std::memcpy(hs + o_ast, h_astruc.data(), (size_t)ntot * 4);
+        std::memcpy(hs + o_cen, h_centres.data(), (size_t)ntot * 4);
+        std::memcpy(hs + o_spa, h_spatoms.data(), (size_t)ntot * 4);
+    }
+    HIPCHK(hipMemcpyAsync(d_up.p, h_up.p, same ? o_type : total, hipMemcpyHostToDevice, stream));
+    up_natoms = h_natoms;
+    up_modes_prev = up_modes;
+    up_tables = true;
     // serving lanes size each batch's rows afresh (a batch of large neighbour counts must not
     // leave every later batch on wide rows); other workspaces keep the stride they reached
     if (maxp == 0 || defer_count) maxp = estimate_maxp(ss, host.rc);
@@ -538,6 +562,7 @@ void Engine::bind_scratch(void* base, size_t bytes, const nnp_structure& s) {
 void Engine::build_neighbors(bool all_images) {
     all_images_ = all_images;
     speculated = false;
+    count_pending = false;
     for (int attempt = 0; attempt < 4; ++attempt) {
         const size_t ent = (size_t)ntot * maxp;
         if (ent >= (size_t)1 << 31) throw std::invalid_argument("batch too large for 32-bit entry indices");
@@ -556,23 +581,25 @@ void Engine::build_neighbors(bool all_images) {
             d_bstart.reserve((size_t)nbins * 4);
             d_bfill.reserve((size_t)nbins * 4);
         }
-        const NeighborArgs a = neighbor_args(all_images);
-        HIPCHK(launch_neighbors(a, stream));
         // serving lanes check the count at the download instead (one host sync per batch), staging
         // full rows in LDS; when full rows do not fit, the count is read here and the rows staged
         // at the realised maximum
-        if (defer_count && lds_fits(maxp)) {
-            lmaxp = maxp;
+        const bool defer = defer_count && lds_fits(maxp);
+        // synchronous calls stage the rows at the previous call's maximum with the resident path's
+        // head room (LDS rows decide the per-centre kernels' occupancy)
+        int lm = 0;
+        const bool spec = !defer && speculate_rows && spec_entries > 0 && spec_entries <= maxp &&
+                          lds_fits(lm = std::min(maxp, (spec_entries + spec_entries / 16 + 4 + 7) / 8 * 8));
+        NeighborArgs a = neighbor_args(all_images);
+        // without a read here, the count maximum is folded into the force assembly's launch (or
+        // taken just before the download)
+        a.count_max = !(defer || spec);
+        HIPCHK(launch_neighbors(a, stream));
+        if (defer || spec) {
+            count_pending = true;
+            lmaxp = defer ? maxp : lm;
+            speculated = spec;
             return;
-        }
-        if (speculate_rows && spec_entries > 0 && spec_entries <= maxp) {
-            // the resident path's head room: LDS rows decide the per-centre kernels' occupancy
-            const int lm = std::min(maxp, (spec_entries + spec_entries / 16 + 4 + 7) / 8 * 8);
-            if (lds_fits(lm)) {
-                lmaxp = lm;
-                speculated = true;
-                return;
-            }
         }
         int mc = 0;
         HIPCHK(hipMemcpyAsync(&mc, d_maxcnt.p, 4, hipMemcpyDeviceToHost, stream));
@@ -735,6 +762,13 @@ void Engine::assemble(const EnergyTail* tail) {
         a.rlist = (int*)d_rlist.p;
         a.rcap = rcap;
     }
+    EnergyTail counts;
+    if (!tail && count_pending) {  // count blocks only: the energies of a synchronous call are summed on the host
+        counts = energy_tail(true);
+        counts.nstruc = 0;
+        tail = &counts;
+    }
+    if (tail && tail->ncount > 0) count_pending = false;
     HIPCHK(launch_assemble<T>(a, (const T*)d_X.p, tail, stream));
 }
 
@@ -883,10 +917,10 @@ void Engine::stage_begin(int s) {
     }
     HIPCHK(hipEventRecord(ev[s][0], stream));
 }
-void Engine::stage_end(int s) {
+void Engine::stage_end(int s, int launches) {
     if (!profiling) return;
     HIPCHK(hipEventRecord(ev[s][1], stream));
-    pending[s] = true;
+    pending[s] = launches;
 }
 void Engine::profile_collect() {
     for (int s = 0; s < ST_COUNT; ++s) {
@@ -895,8 +929,8 @@ void Engine::profile_collect() {
         float ms = 0;
         HIPCHK(hipEventElapsedTime(&ms, ev[s][0], ev[s][1]));
         stage_ms[s] += ms;
-        stage_calls[s] += 1;
-        pending[s] = false;
+        stage_calls[s] += pending[s];
+        pending[s] = 0;
     }
 }
 
@@ -1055,10 +1089,15 @@ void Engine::enqueue_pass(uint32_t flags, bool rows_ready) {
     networks(mixed, flags & NNP_VACUUM_ENERGY);
     stage_end(ST_NET);
     if (flags & NNP_FORCE) {
+        // profiling may repeat the (idempotent) force-term launch back to back inside its events,
+        // so the per-launch time carries a fraction of the events' own cost
+        const int reps = profiling ? profile_repeat : 1;
         stage_begin(ST_FORCE);
-        if (mixed) force_terms<float>();
-        else force_terms<double>();
-        stage_end(ST_FORCE);
+        for (int r = 0; r < reps; ++r) {
+            if (mixed) force_terms<float>();
+            else force_terms<double>();
+        }
+        stage_end(ST_FORCE, reps);
         stage_begin(ST_ASSEMBLE);  // forces, energies and the count check in one launch
         const EnergyTail tail = energy_tail(true);
         assemble_prec(mixed, &tail);
@@ -1072,6 +1111,10 @@ void Engine::enqueue_pass(uint32_t flags, bool rows_ready) {
 
 // Staged download: [counters | E per structure | E_i | F] into pinned memory, one sync.
 int Engine::download(bool energies, bool forces, bool keep_host) {
+    if (count_pending) {  // a pass without the force assembly: the count maximum on its own
+        HIPCHK(launch_count_max((const int*)d_cnt.p, ntot, (int*)d_maxcnt.p, stream));
+        count_pending = false;
+    }
     if (bound) {  // straight into the pre-reserved host vectors, no staging buffer
         h_eatom.resize(ntot);
         if (energies) h_E.resize(nstruc);
@@ -1427,6 +1470,7 @@ int nnp_profile(nnp_model* m, int enable) {
         HIPCHK(hipStreamSynchronize(E.stream));
         E.profile_collect();
         E.profiling = enable != 0;
+        E.profile_repeat = enable > 1 ? enable : 1;
         for (int s = 0; s < Engine::ST_COUNT; ++s) {
             E.stage_ms[s] = 0;
             E.stage_calls[s] = 0;

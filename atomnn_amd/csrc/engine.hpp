@@ -162,9 +162,10 @@ class Engine {
     double stage_ms[ST_COUNT] = {};
     int64_t stage_calls[ST_COUNT] = {};
     void stage_begin(int s);
-    void stage_end(int s);
+    void stage_end(int s, int launches = 1);
     void profile_collect();
-    bool pending[ST_COUNT] = {};
+    int pending[ST_COUNT] = {};   // launches the stage's pending events bracket
+    int profile_repeat = 1;       // force-term launches per profiled pass
     // pipeline: 0 = separate kernels (descriptors, MFMA networks, force terms),
     //           1 = one persistent fused descriptor + network + force kernel
     int pipeline = 0;
@@ -270,6 +271,14 @@ class Engine {
     bool speculate_rows = false;
     bool speculated = false;
     int spec_entries = 0;
+    // the neighbour maximum of the pass is still to be taken (deferred or speculated rows): the
+    // force assembly's launch carries the count blocks, else the download launches them
+    bool count_pending = false;
+    // the per-atom tables of the last upload (species, structure of each atom, centre lists,
+    // species-sorted atoms) are on the device for these atom counts and search modes
+    bool up_tables = false;
+    std::vector<int> up_natoms;
+    std::vector<char> up_modes, up_modes_prev;
 
     std::vector<double> h_pos, h_eatom, h_F;
     std::vector<int> h_type, h_astruc, h_cell_centres, h_pair_centres, h_centres, h_spatoms, sp_off;

@@ -616,6 +616,8 @@ __global__ __launch_bounds__(1024) void k_scan_excl(const int* __restrict__ in, 
 
 namespace atomnn {
 
+hipError_t launch_count_max(const int* cnt, int n, int* maxcnt, hipStream_t st);
+
 hipError_t launch_neighbors(const NeighborArgs& a, hipStream_t st) {
     hipError_t e;
     const bool small = a.ncell > 0 && a.ntot <= kBinsSmallAtoms && a.nbins <= kBinsSmallBins;
@@ -659,9 +661,13 @@ hipError_t launch_neighbors(const NeighborArgs& a, hipStream_t st) {
                            a.pair_centres, a.rc2, a.maxp, a.cnt, a.nbr_j, a.nbr_meta, a.nbr_r, a.maxcnt);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    if (a.ntot > 0 && a.count_max)
-        hipLaunchKernelGGL(k_count_max, dim3(std::min((a.ntot + 255) / 256, 256)), dim3(256), 0, st, a.cnt, a.ntot,
-                           a.maxcnt);
+    if (a.ntot > 0 && a.count_max) return launch_count_max(a.cnt, a.ntot, a.maxcnt, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_count_max(const int* cnt, int n, int* maxcnt, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_count_max, dim3(std::min((n + 255) / 256, 256)), dim3(256), 0, st, cnt, n, maxcnt);
     return hipGetLastError();
 }
 

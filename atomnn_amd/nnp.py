@@ -278,8 +278,10 @@ class Resident:
         _check(lib().nnp_resident_fetch(self.m._h, self.flags, res))
         return [(res[k].energy, eas[k], fs[k] if self.flags & NNP_FORCE else None) for k in range(len(self.natoms))]
 
-    def profile(self, on: bool) -> None:
-        _check(lib().nnp_profile(self.m._h, 1 if on else 0))
+    def profile(self, on: bool, force_repeat: int = 1) -> None:
+        """Stage timing on or off; force_repeat > 1 runs the force-term launch that many times
+        back to back inside its events (the launch is idempotent) and counts each."""
+        _check(lib().nnp_profile(self.m._h, max(1, int(force_repeat)) if on else 0))
 
     def stage_ms(self):
         ms = np.zeros(len(STAGES))

@@ -6,7 +6,8 @@ network per species, energy and forces in FP64, with the structure resident in H
 one full pass of the hot path: neighbour list (cell list), descriptors, per-species networks
 (MFMA), per-entry force terms, force assembly and the per-structure energy.  K steps are timed
 between two stream synchronisations; a second, profiled pass of K steps brackets every stage with
-HIP events on the engine's stream to obtain the force kernel's launch duration for the roofline.
+HIP events on the engine's stream to obtain the force kernel's launch duration for the roofline
+(the force-term launch, which is idempotent, runs 8 times back to back inside its events).
 
 Also reported:
   * "roofline": the force-term kernel's throughput in algorithmic operations (the fixed per-unit
@@ -72,8 +73,10 @@ def timed_steps(R: Resident, steps: int, warmup: int):
     return time.perf_counter() - t0
 
 
-def profiled_steps(R: Resident, steps: int):
-    R.profile(True)
+def profiled_steps(R: Resident, steps: int, force_repeat: int = 8):
+    # the force-term launch runs force_repeat times back to back inside its stage events, so its
+    # per-launch time is the kernel's duration with the events' own cost spread over the launches
+    R.profile(True, force_repeat)
     for _ in range(steps):
         R.run()
         R.sync()
@@ -497,7 +500,7 @@ def main():
             secm = timed_steps(Rm, steps, 1)
             Rm.run()
             got = Rm.fetch()
-            st = profiled_steps(Rm, 2)
+            st = profiled_steps(Rm, 2, 1)
             res[label] = (mat * steps / secm, got, st, secm / steps)
             del Rm
             # the same batch through the synchronous call (host positions in, host results out)

@@ -160,7 +160,10 @@ int nnp_sync(nnp_model* m);
 /* ---- stage timing (engine extension) ----
  * With profiling on, every pass brackets its stages with HIP events on the model's stream.
  * nnp_profile_read returns accumulated milliseconds and call counts per stage:
- * 0 neighbour list, 1 descriptors, 2 networks, 3 force terms, 4 force assembly, 5 energy. */
+ * 0 neighbour list, 1 descriptors, 2 networks, 3 force terms, 4 force assembly, 5 energy.
+ * enable > 1 also runs the force-term launch (idempotent) enable times back to back inside its
+ * events and counts each launch as a call, so its per-call time is the kernel's own duration
+ * with the events' cost spread over the launches. */
 #define NNP_NSTAGES 6
 int nnp_profile(nnp_model* m, int enable);
 int nnp_profile_read(nnp_model* m, double* ms, int64_t* calls, int nstage);

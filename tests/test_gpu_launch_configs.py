@@ -92,3 +92,37 @@ def test_compute_into_fixed_buffers(tmp_path):
             assert E == E0 and np.array_equal(Ea, Ea0) and np.array_equal(F, F0)
     finally:
         m.close()
+
+
+def test_upload_tables_reused_across_calls(tmp_path):
+    """A call with the previous call's atom counts, species and search modes uploads only the
+    positions and cells (the per-atom tables stay on the device); a change of any of them uploads
+    everything.  Each call of one model, in a sequence that alternates between the two, equals
+    a fresh model's call bit for bit."""
+    import numpy as np  # noqa: F811
+
+    path = str(tmp_path / "plain.ann")
+    synth.workload_potential("plain4096").write(path)
+    rng = np.random.default_rng(8)
+    n = 300
+    cell = synth.cubic_cell(n, 0.05)
+    a = synth.Structure(synth.species_exact(n, [0.5, 0.5], rng), synth.random_positions(n, cell, 1.8, 3), cell)
+    moved = synth.Structure(a.types, a.pos + rng.normal(0, 0.05, a.pos.shape), cell)
+    swapped = synth.Structure((1 - a.types).astype(np.int32), a.pos, cell)  # same counts, other species
+    open_a = synth.Structure(a.types, a.pos, None)  # same species, all-pairs search instead of cell list
+    small = synth.batch512(6)[1:3]
+    seq = [[a], [moved], [moved], [swapped], [a], [open_a], [a], small,
+           [small[0], synth.Structure(small[1].types, small[1].pos * 1.01, small[1].cell * 1.01)],
+           [small[0], synth.Structure((1 - small[1].types).astype(np.int32), small[1].pos, small[1].cell)]]
+    m = NNP.read(path)
+    try:
+        got = [m.compute_batch(ss, FLAGS) for ss in seq]
+    finally:
+        m.close()
+    for ss, res in zip(seq, got):
+        f = NNP.read(path)
+        try:
+            want = f.compute_batch(ss, FLAGS)
+        finally:
+            f.close()
+        assert np.array_equal(flat(res), flat(want))
