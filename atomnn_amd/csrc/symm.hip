@@ -1248,6 +1248,58 @@ __host__ __device__ constexpr size_t force_mom_lds_bytes(int lmaxp, int ldg, int
            sizeof(int) * (size_t)lmaxp;
 }
 
+// dE/dr of entry p (the negated force term) over the radial functions f0, f0 + fstep, ... and
+// the NK exponents e0 ... e0 + NK - 1 of the angular term (the caller adds the parts of the
+// other functions and exponents)
+template <typename T, int NE, int NK>
+__device__ __forceinline__ void force_mom_entry(const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta,
+                                                const double* __restrict__ rfun_rs, int ti, int ntypes, const Segs& sg,
+                                                const T* su, const T* Cut, const T* W, const T* U, const T* Delta,
+                                                int p, int f0, int fstep, int e0, T& gx, T& gy, T& gz) {
+    const DevAngular& A0 = plan->ang[ti][0];
+    int s = 0;
+    while (s + 1 < ntypes && p >= sg.start[s + 1]) ++s;
+    const T u[3] = {su[4 * p], su[4 * p + 1], su[4 * p + 2]};
+    const T d = su[4 * p + 3], di = T(1) / d;
+    // radial term dE/dd u
+    const DevRadial& R = plan->rad[ti][s];
+    const T fc = Cut[4 * p], gc = Cut[4 * p + 1];
+    T sr = T(0);
+    for (int f = f0; f < R.n; f += fstep) {
+        const T eta = T(rfun_eta[R.fbase + f]), rs = T(rfun_rs[R.fbase + f]);
+        const T x = d - rs;
+        sr += W[R.goff + f] * kexp(-eta * x * x) * (T(-2) * eta * x * fc + gc);
+    }
+    sr *= T(R.norm);
+    gx = sr * u[0];
+    gy = sr * u[1];
+    gz = sr * u[2];
+    // angular term of the exponents e0 + k
+    const T fa = Cut[4 * p + 2], ga = Cut[4 * p + 3];
+    const T dp = ipow(d, A0.alpha);
+    const T ddp = T(A0.alpha) * dp * di;
+    T m[kMomK3];
+    mom_monos<T>(m, u, std::make_integer_sequence<int, kMomK3>{});
+    T g0[NK], h[3][NK];
+#pragma unroll
+    for (int k = 0; k < NK; ++k) g0[k] = h[0][k] = h[1][k] = h[2][k] = T(0);
+    mom_terms<T, NK, NE, 1>(U + s * kMomK * NE + e0, m, u, g0, h, std::make_integer_sequence<int, kMomK>{});
+#pragma unroll
+    for (int k = 0; k < NK; ++k) {
+        const int e = e0 + k;
+        const T ie = T(A0.ietap[e]);
+        const T ex = kexp(-ie * dp);
+        const T r = fa * ex, rd = (ga - ie * ddp * fa) * ex;
+        const T gu = u[0] * h[0][k] + u[1] * h[1][k] + u[2] * h[2][k];
+        const T dEdR = g0[k] - T(2) * Delta[s * NE + e] * r;
+        const T a1 = dEdR * rd - r * gu * di;
+        const T a2 = r * di;
+        gx += a1 * u[0] + a2 * h[0][k];
+        gy += a1 * u[1] + a2 * h[1][k];
+        gz += a1 * u[2] + a2 * h[2][k];
+    }
+}
+
 template <typename T, int NE, int NW>
 __global__ __launch_bounds__(kWave * NW) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 4 : 6, 8))) void k_force_mom(
     const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta, const double* __restrict__ rfun_rs,
@@ -1365,58 +1417,37 @@ __global__ __launch_bounds__(kWave * NW) __attribute__((amdgpu_waves_per_eu(size
     // per entry: this wave's radial functions and exponents; X = -(sum of the waves' dE/dr)
     const int64_t row = (int64_t)i * maxp;
     for (int j0 = 0; j0 < P; j0 += kWave) {
-        const int p = j0 + lane;
-        T gx = T(0), gy = T(0), gz = T(0);
-        if (p < P) {
-            int s = 0;
-            while (s + 1 < ntypes && p >= sg.start[s + 1]) ++s;
-            const T u[3] = {su[4 * p], su[4 * p + 1], su[4 * p + 2]};
-            const T d = su[4 * p + 3], di = T(1) / d;
-            // radial term dE/dd u over the functions f = w, w + NW, ...
-            const DevRadial& R = plan->rad[ti][s];
-            const T fc = Cut[4 * p], gc = Cut[4 * p + 1];
-            T sr = T(0);
-            for (int f = w; f < R.n; f += NW) {
-                const T eta = T(rfun_eta[R.fbase + f]), rs = T(rfun_rs[R.fbase + f]);
-                const T x = d - rs;
-                sr += W[R.goff + f] * kexp(-eta * x * x) * (T(-2) * eta * x * fc + gc);
-            }
-            sr *= T(R.norm);
-            gx = sr * u[0];
-            gy = sr * u[1];
-            gz = sr * u[2];
-            // angular term of this wave's exponents e = w NK + k
-            const T fa = Cut[4 * p + 2], ga = Cut[4 * p + 3];
-            const T dp = ipow(d, A0.alpha);
-            const T ddp = T(A0.alpha) * dp * di;
-            T m[kMomK3];
-            mom_monos<T>(m, u, std::make_integer_sequence<int, kMomK3>{});
-            T g0[NK], h[3][NK];
-#pragma unroll
-            for (int k = 0; k < NK; ++k) g0[k] = h[0][k] = h[1][k] = h[2][k] = T(0);
-            mom_terms<T, NK, NE, 1>(U + s * kMomK * NE + w * NK, m, u, g0, h, std::make_integer_sequence<int, kMomK>{});
-#pragma unroll
-            for (int k = 0; k < NK; ++k) {
-                const int e = w * NK + k;
-                const T ie = T(A0.ietap[e]);
-                const T ex = kexp(-ie * dp);
-                const T r = fa * ex, rd = (ga - ie * ddp * fa) * ex;
-                const T gu = u[0] * h[0][k] + u[1] * h[1][k] + u[2] * h[2][k];
-                const T dEdR = g0[k] - T(2) * Delta[s * NE + e] * r;
-                const T a1 = dEdR * rd - r * gu * di;
-                const T a2 = r * di;
-                gx += a1 * u[0] + a2 * h[0][k];
-                gy += a1 * u[1] + a2 * h[1][k];
-                gz += a1 * u[2] + a2 * h[2][k];
+        if constexpr (NW == 1 && NK % 2 == 0 && sizeof(T) == 4 && NE <= 4) {
+            if (P - j0 <= kWave / 2) {
+                // at most 32 entries left: two lanes per entry, the even radial functions and the
+                // first half of the exponents in lanes 0-31, the rest in lanes 32-63, added by a
+                // shuffle across the halves (the last pass of a centre with 64 < P <= 96 costs
+                // half a pass instead of a whole one).  FP32 only: in FP64 the second form of the
+                // body pushes the kernel past its register budget.
+                const int hf = lane >> 5, p = j0 + (lane & 31);
+                T gx = T(0), gy = T(0), gz = T(0);
+                if (p < P)
+                    force_mom_entry<T, NE, NK / 2>(plan, rfun_eta, rfun_rs, ti, ntypes, sg, su, Cut, W, U, Delta, p, hf,
+                                                   2, hf * (NK / 2), gx, gy, gz);
+                gx += __shfl_xor(gx, kWave / 2);
+                gy += __shfl_xor(gy, kWave / 2);
+                gz += __shfl_xor(gz, kWave / 2);
+                if (hf == 0 && p < P) store_x4<T>(Xout + 4 * (row + orig[p]), -gx, -gy, -gz);
+                continue;
             }
         }
+        const int p = j0 + lane;
+        T gx = T(0), gy = T(0), gz = T(0);
+        if (p < P)
+            force_mom_entry<T, NE, NK>(plan, rfun_eta, rfun_rs, ti, ntypes, sg, su, Cut, W, U, Delta, p, w, NW, w * NK,
+                                       gx, gy, gz);
         if (w > 0) {
             T* x = Xx + ((w - 1) * kWave + lane) * 3;
             x[0] = gx;
             x[1] = gy;
             x[2] = gz;
         }
-        __syncthreads();
+        if (NW > 1) __syncthreads();
         if (w == 0 && p < P) {
             for (int v = 1; v < NW; ++v) {
                 const T* x = Xx + ((v - 1) * kWave + lane) * 3;
@@ -1426,7 +1457,7 @@ __global__ __launch_bounds__(kWave * NW) __attribute__((amdgpu_waves_per_eu(size
             }
             store_x4<T>(Xout + 4 * (row + orig[p]), -gx, -gy, -gz);
         }
-        __syncthreads();
+        if (NW > 1) __syncthreads();
     }
 }
 
