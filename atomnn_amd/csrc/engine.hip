@@ -433,9 +433,9 @@ void Engine::upload(Strucs ss) {
     const size_t o_pos = 0, o_st = o_pos + al((size_t)ntot * 24), o_type = o_st + al(h_strucs.size() * sizeof(DevStruc)),
                  o_ast = o_type + al((size_t)ntot * 4), o_cen = o_ast + al((size_t)ntot * 4),
                  o_spa = o_cen + al((size_t)ntot * 4), total = o_spa + al((size_t)ntot * 4);
-    const void* up_prev = d_up.p;
+    const size_t up_cap = d_up.cap;
     d_up.reserve(total);
-    same = same && d_up.p == up_prev;  // a new device buffer takes every table
+    same = same && d_up.cap == up_cap;  // a new device buffer (it only grows) takes every table
     char* ds = (char*)d_up.p;
     d_pos.p = ds + o_pos;
     d_type.p = ds + o_type;
