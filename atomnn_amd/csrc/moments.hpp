@@ -18,10 +18,11 @@
 // using grad_r u^a = (grad_u u^a - |a| u^a u) / d and Euler's theorem sum_a |a| L u^a = u . h.
 //
 // GPU mapping (one wavefront per centre):
-//  * moments: lane q < 35 owns monomial q and sums R_e(p) u_p^q over the species' entries in a
-//    fixed order, reading a per-entry power table (1, u, u^2, u^3, u^4 per axis, 15 values)
-//    from LDS: three broadcast-row reads and two products per entry instead of a stored
-//    35-column monomial table;
+//  * moments: the 32 monomials q < 32 are summed directly, lane 32 h + q taking monomial q over
+//    the entries of parity h, reading a per-entry power table (1, u, u^2, u^3, u^4 per axis, 15
+//    values) from LDS: three broadcast-row reads and two products per entry instead of a stored
+//    35-column monomial table; the halves meet by a shuffle, and the last three monomials of
+//    degree 4 follow from the others because |u| = 1;
 //  * forces: lane p owns entry p, keeps the 20 monomials of degree <= 3 in registers and walks
 //    the 35 monomials unrolled at compile time, so each adjoint row L[s][q][0..NE) is read once
 //    and feeds the value and all three gradient components.
@@ -120,44 +121,81 @@ __device__ __forceinline__ void power_rows(const T* su, int c0, int c1, T* pw, i
     }
 }
 
+// Monomials summed directly: q < 32.  The last three of degree 4 follow from |u| = 1
+// (x^2 + y^2 + z^2 = 1 for every entry, so each moment identity below holds term by term):
+//   y^2 z^2 = y^2 - x^2 y^2 - y^4,  y z^3 = y z - x^2 y z - y^3 z,  z^4 = z^2 - x^2 z^2 - y^2 z^2.
+constexpr int kMomDirect = 32;
+constexpr int mom_index(int ax, int ay, int az) {
+    for (int q = 0; q < kMomK; ++q)
+        if (kMomTabC.a[q][0] == ax && kMomTabC.a[q][1] == ay && kMomTabC.a[q][2] == az) return q;
+    return -1;
+}
+static_assert(mom_index(0, 2, 2) == 32 && mom_index(0, 1, 3) == 33 && mom_index(0, 0, 4) == 34,
+              "the derived monomials are the last three");
+
+// Moment rows M[q] (stride 1) of one (species, exponent): fill q = 32, 33, 34 from the others.
+template <typename T>
+__device__ __forceinline__ void mom_derive(T* M) {
+    constexpr int y2 = mom_index(0, 2, 0), x2y2 = mom_index(2, 2, 0), y4 = mom_index(0, 4, 0);
+    constexpr int yz = mom_index(0, 1, 1), x2yz = mom_index(2, 1, 1), y3z = mom_index(0, 3, 1);
+    constexpr int z2 = mom_index(0, 0, 2), x2z2 = mom_index(2, 0, 2);
+    static_assert(y2 >= 0 && x2y2 >= 0 && y4 >= 0 && yz >= 0 && x2yz >= 0 && y3z >= 0 && z2 >= 0 && x2z2 >= 0 &&
+                      x2y2 < kMomDirect && y4 < kMomDirect && x2yz < kMomDirect && y3z < kMomDirect &&
+                      x2z2 < kMomDirect,
+                  "the identities use directly summed monomials");
+    M[32] = M[y2] - M[x2y2] - M[y4];
+    M[33] = M[yz] - M[x2yz] - M[y3z];
+    M[34] = M[z2] - M[x2z2] - M[32];
+}
+
+// One chunk's contribution to the moments of the entries [c0, c1) whose power rows are in pw:
+// lane 32 h + q sums monomial q over the entries of parity h of each species' part of the chunk,
+// the two halves are added by a shuffle and lane q (h = 0) adds the sum to Mom[s][e][q].  Every
+// moment is a fixed-order sum.
+template <typename T, int NE>
+__device__ __forceinline__ void moments_chunk(const T* Rf, int c0, int c1, int ntypes, const int* seg, const T* pw,
+                                              T* Mom, int lane) {
+    const int q = lane & 31, hf = lane >> 5;
+    const int ox = kMomTab.a[q][0], oy = 5 + kMomTab.a[q][1], oz = 10 + kMomTab.a[q][2];
+    for (int s = 0; s < ntypes; ++s) {
+        const int lo = max(seg[s], c0), hi = min(seg[s + 1], c1);
+        if (lo >= hi) continue;
+        T a[NE];
+#pragma unroll
+        for (int e = 0; e < NE; ++e) a[e] = T(0);
+#pragma unroll 2
+        for (int p = lo + hf; p < hi; p += 2) {
+            const T* row = pw + (p - c0) * kMomPw;
+            const T m = row[ox] * row[oy] * row[oz];
+#pragma unroll
+            for (int e = 0; e < NE; ++e) a[e] += Rf[p * NE + e] * m;
+        }
+#pragma unroll
+        for (int e = 0; e < NE; ++e) a[e] += __shfl_xor(a[e], kWave / 2);
+        if (hf == 0)
+#pragma unroll
+            for (int e = 0; e < NE; ++e) Mom[(s * NE + e) * kMomK + q] += a[e];
+    }
+}
+
 // Per-species moments Mom[s][e][q] of the staged entries (Rf holds R_e(p) for every entry) and,
 // when D is given, the diagonals D[s][e] = sum_{p in s} R_e(p)^2.  The power table pw holds
-// `ch` entries at a time.  Every sum runs over the entries in index order.
+// `ch` entries at a time.
 template <typename T, int NE>
 __device__ __forceinline__ void build_moments(const T* su, const T* Rf, int P, int ch, int ntypes, const int* seg,
                                               T* pw, T* Mom, T* D, int lane) {
     ch = max(ch, 1);
-    const int q = lane;
-    int ox = 0, oy = 5, oz = 10;
-    if (q < kMomK) {
-        ox = kMomTab.a[q][0];
-        oy = 5 + kMomTab.a[q][1];
-        oz = 10 + kMomTab.a[q][2];
-    }
+    for (int it = lane; it < ntypes * NE * kMomK; it += kWave) Mom[it] = T(0);
     for (int c0 = 0; c0 < P || c0 == 0; c0 += ch) {
         const int c1 = min(P, c0 + ch);
-        __syncthreads();  // the previous chunk's readers are done
+        __syncthreads();  // the previous chunk's readers are done (and the moments cleared)
         power_rows<T>(su, c0, c1, pw, lane);
         __syncthreads();
-        if (q < kMomK) {
-            for (int s = 0; s < ntypes; ++s) {
-                const int lo = max(seg[s], c0), hi = min(seg[s + 1], c1);
-                T a[NE];
-#pragma unroll
-                for (int e = 0; e < NE; ++e) a[e] = c0 == 0 ? T(0) : Mom[(s * NE + e) * kMomK + q];
-#pragma unroll 4
-                for (int p = lo; p < hi; ++p) {
-                    const T* row = pw + (p - c0) * kMomPw;
-                    const T m = row[ox] * row[oy] * row[oz];
-#pragma unroll
-                    for (int e = 0; e < NE; ++e) a[e] += Rf[p * NE + e] * m;
-                }
-#pragma unroll
-                for (int e = 0; e < NE; ++e) Mom[(s * NE + e) * kMomK + q] = a[e];
-            }
-        }
+        moments_chunk<T, NE>(Rf, c0, c1, ntypes, seg, pw, Mom, lane);
         if (c1 >= P) break;
     }
+    __syncthreads();
+    for (int it = lane; it < ntypes * NE; it += kWave) mom_derive<T>(Mom + it * kMomK);
     if (D) {
         for (int it = lane; it < ntypes * NE; it += kWave) {
             const int e = it % NE, s = it / NE;

@@ -1348,47 +1348,25 @@ __global__ __launch_bounds__(kWave * NW) __attribute__((amdgpu_waves_per_eu(size
     }
     __syncthreads();
 
-    // partial moments of this wave's entries [lo, hi): lane q < 35 owns monomial q
+    // partial moments of this wave's entries [lo, hi) (32 monomials summed over the two half-waves,
+    // the last three derived from |u| = 1)
     {
-        const int q = lane;
-        int ox = 0, oy = 5, oz = 10;
-        if (q < kMomK) {
-            ox = kMomTab.a[q][0];
-            oy = 5 + kMomTab.a[q][1];
-            oz = 10 + kMomTab.a[q][2];
-        }
         T* Mw = Mp + w * ntypes * NE * kMomK;
         constexpr int kMomChW = mom_chw<NW>();
         T* pw = U + w * kMomChW * kMomPw;
-        if (q < kMomK)
-            for (int it = 0; it < ntypes * NE; ++it) Mw[it * kMomK + q] = T(0);
+        for (int it = lane; it < ntypes * NE * kMomK; it += kWave) Mw[it] = T(0);
         const int per = (P + NW - 1) / NW;
         const int lo = min(P, w * per), hi = min(P, lo + per);
         const int nch = (per + kMomChW - 1) / kMomChW;  // the same for every wave
         for (int c = 0; c < nch; ++c) {
             const int c0 = min(hi, lo + c * kMomChW), c1 = min(hi, c0 + kMomChW);
-            __syncthreads();  // the previous chunk's readers are done
+            __syncthreads();  // the previous chunk's readers are done (and the moments cleared)
             power_rows<T>(su, c0, c1, pw, lane);
             __syncthreads();
-            if (q < kMomK) {
-                for (int s = 0; s < ntypes; ++s) {
-                    const int a0 = max(sg.start[s], c0), a1 = min(sg.start[s + 1], c1);
-                    if (a0 >= a1) continue;
-                    T a[NE];
-#pragma unroll
-                    for (int e = 0; e < NE; ++e) a[e] = Mw[(s * NE + e) * kMomK + q];
-#pragma unroll 2
-                    for (int p = a0; p < a1; ++p) {
-                        const T* row = pw + (p - c0) * kMomPw;
-                        const T m = row[ox] * row[oy] * row[oz];
-#pragma unroll
-                        for (int e = 0; e < NE; ++e) a[e] += Rf[p * NE + e] * m;
-                    }
-#pragma unroll
-                    for (int e = 0; e < NE; ++e) Mw[(s * NE + e) * kMomK + q] = a[e];
-                }
-            }
+            moments_chunk<T, NE>(Rf, c0, c1, ntypes, sg.start, pw, Mw, lane);
         }
+        __syncthreads();
+        for (int it = lane; it < ntypes * NE; it += kWave) mom_derive<T>(Mw + it * kMomK);
         __syncthreads();
     }
     adjoint_moments<T, NE>(plan, ti, ntypes, W, Mp, NW, ntypes * NE * kMomK, Delta, wtab, U, tid, kWave * NW);
