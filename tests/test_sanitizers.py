@@ -100,9 +100,9 @@ def test_parser_under_sanitizers_on_corrupted_files(tools, tmp_path):
     out = run([tools[1], src] + bad)
     lines = out.strip().split("\n")
     assert len(lines) == len(bad) + 2 and lines[0].startswith(f"parsed {src}")
-    verdict = [l.split(" ", 1)[0] for l in lines[1:-1]]
+    outcome = [l.split(" ", 1)[0] for l in lines[1:-1]]
     # every truncation short of the whole file is rejected
-    assert all(v == "rejected" for v in verdict[:ntrunc - 1]), lines[1:ntrunc]
+    assert all(v == "rejected" for v in outcome[:ntrunc - 1]), lines[1:ntrunc]
     # every structural corruption (bad counts, cutoff, names, kinds) is rejected; a replaced weight
     # or parameter may parse, as the reference's own number reading accepts it
-    assert all(v == "rejected" for v in verdict[nstruct:]), lines[nstruct + 1:]
+    assert all(v == "rejected" for v in outcome[nstruct:]), lines[nstruct + 1:]
