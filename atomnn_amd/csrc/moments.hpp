@@ -78,9 +78,9 @@ constexpr MomTab make_mom_tab() {
     return t;
 }
 
-// compile-time copy (folded in unrolled code) and device copy (lane-varying lookups)
+// compile-time table (folded in unrolled code); lane-varying look-ups use mom_exps / mom_deg /
+// mom_mult below, which compute the same entries without a memory access
 constexpr MomTab kMomTabC = make_mom_tab();
-__constant__ const MomTab kMomTab = make_mom_tab();
 
 constexpr int mom_first_dir(int q) {
     return kMomTabC.a[q][0] > 0 ? 0 : (kMomTabC.a[q][1] > 0 ? 1 : 2);
@@ -121,6 +121,52 @@ __device__ __forceinline__ void power_rows(const T* su, int c0, int c1, T* pw, i
     }
 }
 
+// Exponents, degree and multinomial coefficient of monomial q < 35 by integer arithmetic (the
+// order of the table above): lane-varying look-ups in the constant table are global-memory loads,
+// a round trip each time a loop or barrier separates them from their use.
+__host__ __device__ constexpr void mom_exps(int q, int& ax, int& ay, int& az) {
+    const int d = (q >= 1) + (q >= 4) + (q >= 10) + (q >= 20);
+    const int r = q - d * (d + 1) * (d + 2) / 6;  // position inside the degree-d block
+    const int t = (r >= 1) + (r >= 3) + (r >= 6) + (r >= 10);  // d - ax
+    const int j = r - t * (t + 1) / 2;
+    ax = d - t;
+    ay = t - j;
+    az = j;
+}
+__host__ __device__ constexpr int mom_deg(int q) {
+    return (q >= 1) + (q >= 4) + (q >= 10) + (q >= 20);
+}
+__host__ __device__ constexpr int small_fact(int n) {
+    return n < 2 ? 1 : (n == 2 ? 2 : (n == 3 ? 6 : 24));
+}
+// the coefficient as four bits of one of three 64-bit constants (no division, no memory access)
+__host__ __device__ constexpr unsigned long long mom_mult_word(int w) {
+    unsigned long long v = 0;
+    for (int k = 0; k < 16 && 16 * w + k < kMomK; ++k) {
+        const int q = 16 * w + k;
+        int ax = 0, ay = 0, az = 0;
+        mom_exps(q, ax, ay, az);
+        v |= (unsigned long long)(small_fact(ax + ay + az) / (small_fact(ax) * small_fact(ay) * small_fact(az)))
+             << (4 * k);
+    }
+    return v;
+}
+__host__ __device__ constexpr int mom_mult(int q) {
+    constexpr unsigned long long w0 = mom_mult_word(0), w1 = mom_mult_word(1), w2 = mom_mult_word(2);
+    const unsigned long long w = q < 16 ? w0 : (q < 32 ? w1 : w2);
+    return (int)((w >> (4 * (q & 15))) & 15);
+}
+constexpr bool mom_arith_matches_table() {
+    for (int q = 0; q < kMomK; ++q) {
+        int ax = 0, ay = 0, az = 0;
+        mom_exps(q, ax, ay, az);
+        if (ax != kMomTabC.a[q][0] || ay != kMomTabC.a[q][1] || az != kMomTabC.a[q][2]) return false;
+        if (mom_deg(q) != kMomTabC.deg[q] || mom_mult(q) != kMomTabC.mult[q]) return false;
+    }
+    return true;
+}
+static_assert(mom_arith_matches_table(), "monomial arithmetic and table agree");
+
 // Monomials summed directly: q < 32.  The last three of degree 4 follow from |u| = 1
 // (x^2 + y^2 + z^2 = 1 for every entry, so each moment identity below holds term by term):
 //   y^2 z^2 = y^2 - x^2 y^2 - y^4,  y z^3 = y z - x^2 y z - y^3 z,  z^4 = z^2 - x^2 z^2 - y^2 z^2.
@@ -156,7 +202,9 @@ template <typename T, int NE>
 __device__ __forceinline__ void moments_chunk(const T* Rf, int c0, int c1, int ntypes, const int* seg, const T* pw,
                                               T* Mom, int lane) {
     const int q = lane & 31, hf = lane >> 5;
-    const int ox = kMomTab.a[q][0], oy = 5 + kMomTab.a[q][1], oz = 10 + kMomTab.a[q][2];
+    int ax = 0, ay = 0, az = 0;
+    mom_exps(q, ax, ay, az);
+    const int ox = ax, oy = 5 + ay, oz = 10 + az;
     for (int s = 0; s < ntypes; ++s) {
         const int lo = max(seg[s], c0), hi = min(seg[s + 1], c1);
         if (lo >= hi) continue;
@@ -232,7 +280,7 @@ __device__ __forceinline__ void symm_angular_moments(const DevPlan* __restrict__
         const T* Mb = Mom + (bb * NE + e) * kMomK;
         T acc = T(0);
         const int q0 = pd * (pd + 1) * (pd + 2) / 6, q1 = (pd + 1) * (pd + 2) * (pd + 3) / 6;
-        for (int q = q0; q < q1; ++q) acc += T(kMomTab.mult[q]) * Ma[q] * Mb[q];
+        for (int q = q0; q < q1; ++q) acc += T(mom_mult(q)) * Ma[q] * Mb[q];
         S[it] = acc;
     }
     __syncthreads();
@@ -273,9 +321,7 @@ __device__ __forceinline__ void build_moments_jvp(const double* su, const double
     const int q = lane;
     int ax = 0, ay = 0, az = 0;
     if (q < kMomK) {
-        ax = kMomTab.a[q][0];
-        ay = kMomTab.a[q][1];
-        az = kMomTab.a[q][2];
+        mom_exps(q, ax, ay, az);
     }
     // power-table offsets of u_c^(a_c) and u_c^(a_c - 1) (the latter only read when a_c > 0)
     const int ox = ax, oy = 5 + ay, oz = 10 + az;
@@ -351,7 +397,7 @@ __device__ __forceinline__ void symm_angular_moments_jvp(const DevPlan* __restri
         const double* dMb = dMom + (bb * NE + e) * kMomK;
         double acc = 0.0;
         const int q0 = pd * (pd + 1) * (pd + 2) / 6, q1 = (pd + 1) * (pd + 2) * (pd + 3) / 6;
-        for (int q = q0; q < q1; ++q) acc += double(kMomTab.mult[q]) * (dMa[q] * Mb[q] + Ma[q] * dMb[q]);
+        for (int q = q0; q < q1; ++q) acc += double(mom_mult(q)) * (dMa[q] * Mb[q] + Ma[q] * dMb[q]);
         S[it] = acc;
     }
     __syncthreads();
@@ -408,7 +454,7 @@ __device__ __forceinline__ void adjoint_moments(const DevPlan* __restrict__ plan
     __syncthreads();
     for (int it = tid; it < ntypes * kMomK * NE; it += nthreads) {
         const int e = it % NE, q = (it / NE) % kMomK, s = it / (NE * kMomK);
-        const int pd = kMomTab.deg[q];
+        const int pd = mom_deg(q);
         T acc = T(0);
         for (int b = 0; b < ntypes; ++b) {
             const int blk = b >= s ? b * (b + 1) / 2 + s : s * (s + 1) / 2 + b;
@@ -418,7 +464,7 @@ __device__ __forceinline__ void adjoint_moments(const DevPlan* __restrict__ plan
             for (int k = 1; k < nparts; ++k) mb += Mom[k * pstride + mi];
             acc += (b == s ? T(2) : T(1)) * wv * mb;
         }
-        L[it] = T(kMomTab.mult[q]) * acc;  // [s][q][e]
+        L[it] = T(mom_mult(q)) * acc;  // [s][q][e]
     }
     for (int it = tid; it < ntypes * NE; it += nthreads) {
         const int e = it % NE, s = it / NE;
