@@ -30,13 +30,15 @@ namespace atomnn {
 
 // launchers (symm.hip, net.hip, neigh.hip kernels)
 int grid_shape(int ne, int nc);
+int shape_ne(int shape);  // exponents per centre of a grid shape
+constexpr int kMomK = 35;  // monomials of degree <= 4: the moment table's row length
 size_t symm_lds(int shape, bool fp64, bool ns, int lmaxp, int ldg);
 size_t force_lds(int shape, bool fp64, bool ns, int lmaxp, int ldg);
 size_t net_lds_bytes(const DevNet& n, bool fp64);
 template <typename T>
 hipError_t launch_symm(int shape, int ns, const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
                        const int* centres, int ncentres, const int* cnt, const int* meta, const double4* r, int maxp,
-                       int lmaxp, T* G, int ldg, int use_mom, int ntypes, hipStream_t st);
+                       int lmaxp, T* G, int ldg, int use_mom, int ntypes, hipStream_t st, T* mom_out);
 template <typename T>
 hipError_t launch_force(int shape, int ns, const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
                         const int* centres, int ncentres, const int* cnt, const int* meta, const double4* r, int maxp,
@@ -46,7 +48,7 @@ template <typename T>
 hipError_t launch_force_mom(int shape, int waves, const DevPlan* plan, const double* rfe, const double* rfr,
                             const int* atom_type, const int* centres, int ncentres, const int* cnt, const int* meta,
                             const double4* r, int maxp, int lmaxp, const T* dEdG, int ldg, T* X, int ntypes,
-                            hipStream_t st);
+                            hipStream_t st, const T* mom_in);
 template <typename T>
 hipError_t launch_assemble(const AssembleArgs& a, const T* X, const EnergyTail* tail, hipStream_t st);
 hipError_t launch_net(bool fp64, const DevPlan* plan, const DevPlan& hplan, const void* W, const int* sp_off,
@@ -306,7 +308,7 @@ size_t Engine::workspace_bytes() const {
     size_t b = 0;
     for (const DevMem* m : {&d_up, &d_cnt, &d_nbrj, &d_meta, &d_nbrr, &d_maxcnt, &d_abin, &d_bcount, &d_bstart,
                             &d_bfill, &d_G, &d_dEdG, &d_X, &d_eatom, &d_F, &d_E, &d_sstart, &d_rcount,
-                            &d_rlist, &d_out})
+                            &d_rlist, &d_out, &d_mom})
         b += m->cap;
     return b;
 }
@@ -323,7 +325,7 @@ Engine::~Engine() {
     if (owns_model)
         for (DevMem* m : {&d_plan, &d_rfe, &d_rfr, &d_w64, &d_w32}) m->release();
     for (DevMem* m : {&d_up, &d_cnt, &d_nbrj, &d_meta, &d_nbrr, &d_maxcnt, &d_abin, &d_bcount, &d_bstart,
-                      &d_bfill, &d_G, &d_dEdG, &d_X, &d_eatom, &d_F, &d_rcount, &d_rlist})
+                      &d_bfill, &d_G, &d_dEdG, &d_X, &d_eatom, &d_F, &d_rcount, &d_rlist, &d_mom})
         m->release();
     h_up.release();
     h_down.release();
@@ -367,6 +369,7 @@ void Engine::upload(Strucs ss) {
     resident_valid = false;
     drop_graph();
     eager_runs = 0;
+    mom_fresh = 0;
     assemble_mode = react_rows;
     react_missed = 0;
     nstruc = (int)ss.size();
@@ -563,6 +566,7 @@ void Engine::build_neighbors(bool all_images) {
     all_images_ = all_images;
     speculated = false;
     count_pending = false;
+    mom_fresh = 0;  // new rows
     for (int attempt = 0; attempt < 4; ++attempt) {
         const size_t ent = (size_t)ntot * maxp;
         if (ent >= (size_t)1 << 31) throw std::invalid_argument("batch too large for 32-bit entry indices");
@@ -635,10 +639,16 @@ void Engine::descriptors() {
     for (int t = 0; t < pb.plan.T; ++t) all = all && pb.plan.mom_ok[t];
     const int mom = mom_flag(lds, sizeof(T) == 8, 1) ? (all ? 2 : 1) : 0;
     d_G.reserve((size_t)ntot * pb.plan.ldg * sizeof(T));
+    // the moment tables go to the force-term kernel when it takes the moment-only form (not in the
+    // workspace of calls that allocate nothing)
+    const bool keep = mom == 2 && !bound && !pb.plan.generic && (moments & 2);
+    mom_fresh = 0;
+    if (keep) d_mom.reserve((size_t)ntot * pb.plan.T * shape_ne(shape) * kMomK * sizeof(T));
     HIPCHK(launch_symm<T>(shape, pb.plan.generic, (const DevPlan*)d_plan.p, (const double*)d_rfe.p, (const double*)d_rfr.p,
                           (const int*)d_type.p, (const int*)d_centres.p, ntot, (const int*)d_cnt.p,
                           (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp, lmaxp, (T*)d_G.p, pb.plan.ldg, mom,
-                          pb.plan.T, stream));
+                          pb.plan.T, stream, keep ? (T*)d_mom.p : nullptr));
+    if (keep) mom_fresh = (int)sizeof(T);
 }
 
 void Engine::networks(bool mixed, bool vac) {
@@ -718,7 +728,8 @@ void Engine::force_terms() {
         HIPCHK(launch_force_mom<T>(shape, force_waves, (const DevPlan*)d_plan.p, (const double*)d_rfe.p, (const double*)d_rfr.p,
                                    (const int*)d_type.p, (const int*)d_centres.p, ntot, (const int*)d_cnt.p,
                                    (const int*)d_meta.p, (const double4*)d_nbrr.p, maxp, lmaxp, (const T*)d_dEdG.p,
-                                   pb.plan.ldg, (T*)d_X.p, pb.plan.T, stream));
+                                   pb.plan.ldg, (T*)d_X.p, pb.plan.T, stream,
+                                   mom_fresh == (int)sizeof(T) ? (const T*)d_mom.p : nullptr));
         return;
     }
     const size_t lds = force_lds(shape, fp64, pb.plan.generic, lmaxp, pb.plan.ldg);

@@ -256,6 +256,11 @@ class Engine {
     DevMem d_cnt, d_nbrj, d_meta, d_nbrr, d_maxcnt;
     DevMem d_abin, d_bcount, d_bstart, d_bfill;
     DevMem d_G, d_dEdG, d_X, d_eatom, d_F;
+    // per-centre moment tables [S][NE][35] the moment-only descriptor kernel leaves for the
+    // force-term kernel of the same rows (which then skips its own moment sums); mom_fresh is the
+    // element size of the tables that match the current rows, 0 when there are none
+    DevMem d_mom;
+    int mom_fresh = 0;
     // copy E (per structure, optional), E_i, F (optional) and the neighbour maximum to the host
     // in one staged download (packed on the device into d_out, one copy); returns the neighbour
     // maximum.  keep_host = false leaves E_i and F in the staging buffer (down_ea, down_f)

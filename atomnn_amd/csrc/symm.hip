@@ -335,7 +335,7 @@ template <typename T, int NE, int NC, bool GEN>
 __device__ __forceinline__ void symm_body(const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta,
                                           const double* __restrict__ rfun_rs, int ti, int ntypes, int P, int maxp,
                                           const T* su, T* Rf, T* red, int red_elems, T* Grow, const Segs& sg,
-                                          T* nsx, T* mom, int lane) {
+                                          T* nsx, T* mom, int lane, T* mom_out = nullptr) {
     // radial: cutoff once per neighbour, then (function, part) items over 64 lanes; the parts of
     // a function are summed in a fixed order
     {
@@ -400,6 +400,10 @@ __device__ __forceinline__ void symm_body(const DevPlan* __restrict__ plan, cons
         radial_tables<T, NE, GEN>(plan->ang[ti][0], su, 0, P, Rf, nullptr, lane);
         __syncthreads();
         build_moments<T, NE>(su, Rf, P, ch, ntypes, sg.start, red, Mom, D, lane);
+        // the centre's moment table for the force-term kernel of the same pass, which then skips
+        // its own moment sums
+        if (mom_out)
+            for (int it = lane; it < ntypes * NE * kMomK; it += kWave) mom_out[it] = Mom[it];
         symm_angular_moments<T, NE>(plan, ti, ntypes, Mom, D, S, Grow, lane);
         __syncthreads();
         return;
@@ -779,7 +783,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
     const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta, const double* __restrict__ rfun_rs,
     const int* __restrict__ atom_type, const int* __restrict__ centres, int ncentres, const int* __restrict__ cnt,
     const int* __restrict__ nbr_meta, const double4* __restrict__ nbr_r, int maxp, int lmaxp, T* __restrict__ G,
-    int ldg, int use_mom, int red_elems) {
+    int ldg, int use_mom, int red_elems, T* __restrict__ Mout) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     T* su = (T*)lds_raw;
     T* Rf = su + 4 * lmaxp;
@@ -802,7 +806,7 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(NE * NC <
     for (int k = lane; k < ldg; k += kWave) Grow[k] = T(0);
     __syncthreads();
     symm_body<T, NE, NC, GEN>(plan, rfun_eta, rfun_rs, ti, ntypes, P, lmaxp, su, Rf, red, red_elems, Grow, sg, nsx,
-                              mom, lane);
+                              mom, lane, Mout ? Mout + (int64_t)i * (ntypes * NE * kMomK) : nullptr);
     __syncthreads();
     for (int k = lane; k < ldg; k += kWave) G[(int64_t)i * ldg + k] = Grow[k];
 }
@@ -1305,7 +1309,7 @@ __global__ __launch_bounds__(kWave * NW) __attribute__((amdgpu_waves_per_eu(size
     const DevPlan* __restrict__ plan, const double* __restrict__ rfun_eta, const double* __restrict__ rfun_rs,
     const int* __restrict__ atom_type, const int* __restrict__ centres, int ncentres, const int* __restrict__ cnt,
     const int* __restrict__ nbr_meta, const double4* __restrict__ nbr_r, int maxp, int lmaxp,
-    const T* __restrict__ dEdG, int ldg, T* __restrict__ Xout) {
+    const T* __restrict__ dEdG, int ldg, T* __restrict__ Xout, const T* __restrict__ Min) {
     constexpr int NK = NE / NW;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int ntypes = plan->T;
@@ -1331,10 +1335,17 @@ __global__ __launch_bounds__(kWave * NW) __attribute__((amdgpu_waves_per_eu(size
         const int t0 = NW == 1 ? tid : tid - kWave, nt = NW == 1 ? kWave : kWave * (NW - 1);
         for (int k = t0; k < ldg; k += nt) W[k] = dEdG[(int64_t)i * ldg + k];
     }
+    // the descriptor kernel of the same pass left this centre's moments: they are read with the
+    // entries, and the exponent factors and moment sums below are skipped
+    if (Min) {
+        const T* src = Min + (int64_t)i * (ntypes * NE * kMomK);
+        for (int it = tid; it < ntypes * NE * kMomK; it += kWave * NW) Mp[it] = src[it];
+    }
     __syncthreads();
 
-    // angular radial factors R_e(p) of this wave's exponents (one table for every block)
     const DevAngular& A0 = plan->ang[ti][0];
+    if (!Min) {
+    // angular radial factors R_e(p) of this wave's exponents (one table for every block)
     for (int p = lane; p < P; p += kWave) {
         const T d = su[4 * p + 3];
         T fc, gc;
@@ -1369,7 +1380,8 @@ __global__ __launch_bounds__(kWave * NW) __attribute__((amdgpu_waves_per_eu(size
         for (int it = lane; it < ntypes * NE; it += kWave) mom_derive<T>(Mw + it * kMomK);
         __syncthreads();
     }
-    adjoint_moments<T, NE>(plan, ti, ntypes, W, Mp, NW, ntypes * NE * kMomK, Delta, wtab, U, tid, kWave * NW);
+    }
+    adjoint_moments<T, NE>(plan, ti, ntypes, W, Mp, Min ? 1 : NW, ntypes * NE * kMomK, Delta, wtab, U, tid, kWave * NW);
 
     // cutoff value and derivative of every entry, radial then angular, in the partial moments'
     // place.  Computed ahead of the entry loop: inside it, the polynomial constants of the cosine
@@ -1443,11 +1455,11 @@ template <typename T, int NE, int NW>
 static hipError_t launch_force_mom_t(const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
                                      const int* centres, int ncentres, const int* cnt, const int* meta,
                                      const double4* r, int maxp, int lmaxp, const T* dEdG, int ldg, T* X, int ntypes,
-                                     hipStream_t st) {
+                                     hipStream_t st, const T* Min) {
     const size_t lds = force_mom_lds_bytes<T, NE, NW>(lmaxp, ldg, ntypes);
     if (lds > 65536) return hipErrorInvalidConfiguration;
     hipLaunchKernelGGL((k_force_mom<T, NE, NW>), dim3(xcd_grid(ncentres)), dim3(kWave * NW), lds, st, plan, rfe,
-                       rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp, dEdG, ldg, X);
+                       rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp, dEdG, ldg, X, Min);
     return hipGetLastError();
 }
 
@@ -1814,7 +1826,7 @@ __global__ __launch_bounds__(kWave) void k_react_fill(const DevStruc* __restrict
 template <typename T, int NE, int NC, bool GEN>
 static hipError_t launch_symm_t(const DevPlan* plan, const double* rfe, const double* rfr, const int* atom_type,
                                 const int* centres, int ncentres, const int* cnt, const int* meta, const double4* r,
-                                int maxp, int lmaxp, T* G, int ldg, int use_mom, int ntypes, hipStream_t st) {
+                                int maxp, int lmaxp, T* G, int ldg, int use_mom, int ntypes, hipStream_t st, T* Mout) {
     // use_mom 2: every species takes the moments, the reduction buffer only holds power-table
     // chunks of 16 entries (and the radial partial sums, kWave values)
     const int red = use_mom == 2 ? 16 * kMomPw : kWave * NE * NC;
@@ -1822,7 +1834,8 @@ static hipError_t launch_symm_t(const DevPlan* plan, const double* rfe, const do
         symm_lds_bytes<T, NE, NC, GEN>(lmaxp, ldg, red) + (use_mom ? mom_lds_bytes<T, NE>(lmaxp, ntypes, 1) : 0);
     if (lds > 65536) return hipErrorInvalidConfiguration;
     hipLaunchKernelGGL((k_symm<T, NE, NC, GEN>), dim3(xcd_grid(ncentres)), dim3(kWave), lds, st, plan, rfe, rfr,
-                       atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp, G, ldg, use_mom, red);
+                       atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp, G, ldg, use_mom, red,
+                       use_mom == 2 ? Mout : nullptr);
     return hipGetLastError();
 }
 
@@ -1843,6 +1856,9 @@ int grid_shape(int ne, int nc) {
     if (ne <= 2 && nc <= 4) return 0;
     if (ne <= 4 && nc <= 4) return 1;
     return 2;
+}
+int shape_ne(int shape) {
+    return shape == 0 ? 2 : (shape == 1 ? 4 : 8);
 }
 
 template <typename T, bool GEN>
@@ -1872,27 +1888,27 @@ template <typename T, bool GEN>
 static hipError_t launch_symm_s(int shape, const DevPlan* plan, const double* rfe, const double* rfr,
                                 const int* atom_type, const int* centres, int ncentres, const int* cnt, const int* meta,
                                 const double4* r, int maxp, int lmaxp, T* G, int ldg, int use_mom, int ntypes,
-                                hipStream_t st) {
+                                hipStream_t st, T* Mout) {
     if (shape == 0)
         return launch_symm_t<T, 2, 4, GEN>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp, G, ldg,
-                                          use_mom, ntypes, st);
+                                          use_mom, ntypes, st, Mout);
     if (shape == 1)
         return launch_symm_t<T, 4, 4, GEN>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp, G, ldg,
-                                          use_mom, ntypes, st);
+                                          use_mom, ntypes, st, Mout);
     return launch_symm_t<T, 8, 8, GEN>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp, G, ldg, use_mom,
-                                      ntypes, st);
+                                      ntypes, st, Mout);
 }
 
 template <typename T>
 hipError_t launch_symm(int shape, int ns, const DevPlan* plan, const double* rfe, const double* rfr,
                        const int* atom_type, const int* centres, int ncentres, const int* cnt, const int* meta,
                        const double4* r, int maxp, int lmaxp, T* G, int ldg, int use_mom, int ntypes,
-                                hipStream_t st) {
+                       hipStream_t st, T* Mout) {
     if (ns)
         return launch_symm_s<T, true>(shape, plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp, G, ldg,
-                                      use_mom, ntypes, st);
+                                      use_mom, ntypes, st, Mout);
     return launch_symm_s<T, false>(shape, plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp, G, ldg,
-                                   use_mom, ntypes, st);
+                                   use_mom, ntypes, st, Mout);
 }
 
 template <typename T, bool GEN>
@@ -1938,34 +1954,34 @@ template <typename T, int NW>
 static hipError_t launch_force_mom_w(int shape, const DevPlan* plan, const double* rfe, const double* rfr,
                                      const int* atom_type, const int* centres, int ncentres, const int* cnt,
                                      const int* meta, const double4* r, int maxp, int lmaxp, const T* dEdG, int ldg,
-                                     T* X, int ntypes, hipStream_t st) {
+                                     T* X, int ntypes, hipStream_t st, const T* Min) {
     if (shape == 0)
         return launch_force_mom_t<T, 2, NW>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp,
-                                            dEdG, ldg, X, ntypes, st);
+                                            dEdG, ldg, X, ntypes, st, Min);
     if (shape == 1)
         return launch_force_mom_t<T, 4, NW>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp,
-                                            dEdG, ldg, X, ntypes, st);
+                                            dEdG, ldg, X, ntypes, st, Min);
     return launch_force_mom_t<T, 8, NW>(plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp, dEdG,
-                                        ldg, X, ntypes, st);
+                                        ldg, X, ntypes, st, Min);
 }
 
 template <typename T>
 hipError_t launch_force_mom(int shape, int waves, const DevPlan* plan, const double* rfe, const double* rfr,
                             const int* atom_type, const int* centres, int ncentres, const int* cnt, const int* meta,
                             const double4* r, int maxp, int lmaxp, const T* dEdG, int ldg, T* X, int ntypes,
-                            hipStream_t st) {
+                            hipStream_t st, const T* Min) {
     if (waves == 1)
         return launch_force_mom_w<T, 1>(shape, plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp,
-                                        dEdG, ldg, X, ntypes, st);
+                                        dEdG, ldg, X, ntypes, st, Min);
     return launch_force_mom_w<T, 2>(shape, plan, rfe, rfr, atom_type, centres, ncentres, cnt, meta, r, maxp, lmaxp, dEdG,
-                                    ldg, X, ntypes, st);
+                                    ldg, X, ntypes, st, Min);
 }
 template hipError_t launch_force_mom<double>(int, int, const DevPlan*, const double*, const double*, const int*,
                                              const int*, int, const int*, const int*, const double4*, int, int,
-                                             const double*, int, double*, int, hipStream_t);
+                                             const double*, int, double*, int, hipStream_t, const double*);
 template hipError_t launch_force_mom<float>(int, int, const DevPlan*, const double*, const double*, const int*,
                                             const int*, int, const int*, const int*, const double4*, int, int,
-                                            const float*, int, float*, int, hipStream_t);
+                                            const float*, int, float*, int, hipStream_t, const float*);
 
 template <typename T>
 hipError_t launch_assemble(const AssembleArgs& a, const T* X, const EnergyTail* tail, hipStream_t st) {
@@ -1994,10 +2010,10 @@ hipError_t launch_assemble(const AssembleArgs& a, const T* X, const EnergyTail* 
 
 template hipError_t launch_symm<double>(int, int, const DevPlan*, const double*, const double*, const int*, const int*,
                                         int, const int*, const int*, const double4*, int, int, double*, int, int, int,
-                                        hipStream_t);
+                                        hipStream_t, double*);
 template hipError_t launch_symm<float>(int, int, const DevPlan*, const double*, const double*, const int*, const int*,
                                        int, const int*, const int*, const double4*, int, int, float*, int, int, int,
-                                       hipStream_t);
+                                       hipStream_t, float*);
 template hipError_t launch_force<double>(int, int, const DevPlan*, const double*, const double*, const int*,
                                          const int*, int, const int*, const int*, const double4*, int, int,
                                          const double*, int, double*, int, int, hipStream_t);
