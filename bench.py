@@ -102,13 +102,15 @@ def force_ops_triplets(P, T, nR, nA):
     return P * (30 + 13 * nR) + T * (75 + 28 * nA)
 
 
-def force_ops_moments(P, N, S, nR, NE):
-    """Algorithmic ops of the moment force pass (35 monomials of degree <= 4, NE radial exponents, S
+def force_ops_moments(P, N, S, nR, NE, stored_moments=True):
+    """Executed ops of the moment force pass (35 monomials of degree <= 4, NE radial exponents, S
     species).  Per entry: 30 + 13 nR (cutoff, radial functions), 13 NE (angular radial factors),
-    35 (2 + 2 NE) (moments: monomial and one FMA per exponent), 64 + 216 NE (monomials, the adjoint
-    polynomial's value and gradient per exponent: 35 + 60 FMAs, and the force assembly); per centre
-    35 NE S (3 S + 1) (adjoint moments)."""
-    return P * (30 + 13 * nR + 13 * NE + 35 * (2 + 2 * NE) + 64 + 216 * NE) + N * 35 * NE * S * (3 * S + 1)
+    64 + 216 NE (monomials, the adjoint polynomial's value and gradient per exponent: 35 + 60 FMAs,
+    and the force assembly), and 35 (2 + 2 NE) (moments: monomial and one FMA per exponent) unless
+    the force kernel reads the moment tables the descriptor kernel left; per centre 35 NE S (3 S + 1)
+    (adjoint moments)."""
+    per_entry = 30 + 13 * nR + 13 * NE + 64 + 216 * NE + (0 if stored_moments else 35 * (2 + 2 * NE))
+    return P * per_entry + N * 35 * NE * S * (3 * S + 1)
 
 
 def traffic_from_profile(path, key="traffic_bytes_per_launch"):
