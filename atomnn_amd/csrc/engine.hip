@@ -32,6 +32,7 @@ namespace atomnn {
 int grid_shape(int ne, int nc);
 int shape_ne(int shape);  // exponents per centre of a grid shape
 constexpr int kMomK = 35;  // monomials of degree <= 4: the moment table's row length
+constexpr int kMomZ = 4;   // their highest degree
 size_t symm_lds(int shape, bool fp64, bool ns, int lmaxp, int ldg);
 size_t force_lds(int shape, bool fp64, bool ns, int lmaxp, int ldg);
 size_t net_lds_bytes(const DevNet& n, bool fp64);
@@ -272,6 +273,7 @@ Engine::Engine(HostModel&& hm) : host(std::move(hm)) {
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&num_cus, hipDeviceAttributeMultiprocessorCount, dev));
+    build_adjoint_table();
     d_plan.reserve(sizeof(DevPlan));
     HIPCHK(hipMemcpy(d_plan.p, &pb.plan, sizeof(DevPlan), hipMemcpyHostToDevice));
     d_rfe.reserve(pb.rfun_eta.size() * 8);
@@ -283,6 +285,50 @@ Engine::Engine(HostModel&& hm) : host(std::move(hm)) {
     std::vector<float> wf(pb.weights.begin(), pb.weights.end());
     d_w32.reserve(wf.size() * 4);
     HIPCHK(hipMemcpy(d_w32.p, wf.data(), wf.size() * 4, hipMemcpyHostToDevice));
+}
+
+// The w table of the moment path's adjoint (moments.hpp adjoint_moments) as a linear map of the
+// dE/dG row: w[blk][e][p] = norm (1/2 for a same-species block) sum_c dE/dG[f(e, c)] C(zeta_c, p)
+// lambda_c^p 2^-zeta_c over the block's angular terms c with f(e, c) defined and p <= zeta_c.
+// Stored per item as kMaxC (index, coefficient) pairs, so the kernel reads one row of constants
+// instead of walking the block's description.
+void Engine::build_adjoint_table() {
+    const DevPlan& P = pb.plan;
+    const int ne = shape_ne(shape), T = P.T, nblk = T * (T + 1) / 2;
+    const size_t items = (size_t)T * nblk * ne * (kMomZ + 1);
+    std::vector<double> coef(items * kMaxC, 0.0);
+    std::vector<int> index(items * kMaxC, 0);
+    auto binom = [](int z, int p) { int r = 1; for (int k = 0; k < p; ++k) r = r * (z - k) / (k + 1); return r; };
+    for (int t = 0; t < T; ++t) {
+        if (!P.mom_ok[t]) continue;
+        for (int blk = 0; blk < nblk; ++blk) {
+            int bb = 0;
+            while ((bb + 1) * (bb + 2) / 2 <= blk) ++bb;
+            const int a = blk - bb * (bb + 1) / 2;
+            const DevAngular& A = P.ang[t][blk];
+            for (int e = 0; e < ne && e < A.ne; ++e)
+                for (int p = 0; p <= kMomZ; ++p) {
+                    const size_t base = ((((size_t)t * nblk + blk) * ne + e) * (kMomZ + 1) + p) * kMaxC;
+                    int k = 0;
+                    for (int c = 0; c < A.nc; ++c) {
+                        const int f = A.fmap[e * kMaxC + c], z = A.zint[c];
+                        if (f < 0 || p > z) continue;
+                        double lp = 1.0;
+                        for (int q = 0; q < p; ++q) lp *= A.lambda[c];
+                        coef[base + k] = binom(z, p) * lp * (1.0 / (double)(1 << z)) * A.norm * (a == bb ? 0.5 : 1.0);
+                        index[base + k] = A.goff + f;
+                        ++k;
+                    }
+                }
+        }
+    }
+    d_adjc.reserve(coef.size() * 8);
+    d_adji.reserve(index.size() * 4);
+    HIPCHK(hipMemcpy(d_adjc.p, coef.data(), coef.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_adji.p, index.data(), index.size() * 4, hipMemcpyHostToDevice));
+    pb.plan.adj_coef = (const double*)d_adjc.p;
+    pb.plan.adj_index = (const int*)d_adji.p;
+    pb.plan.adj_ne = ne;
 }
 
 Engine::Engine(const Engine& o, int) : host(o.host), pb(o.pb), shape(o.shape) {
@@ -323,7 +369,7 @@ Engine::~Engine() {
     d_sstart.release();
     d_work.release();
     if (owns_model)
-        for (DevMem* m : {&d_plan, &d_rfe, &d_rfr, &d_w64, &d_w32}) m->release();
+        for (DevMem* m : {&d_plan, &d_rfe, &d_rfr, &d_w64, &d_w32, &d_adjc, &d_adji}) m->release();
     for (DevMem* m : {&d_up, &d_cnt, &d_nbrj, &d_meta, &d_nbrr, &d_maxcnt, &d_abin, &d_bcount, &d_bstart,
                       &d_bfill, &d_G, &d_dEdG, &d_X, &d_eatom, &d_F, &d_rcount, &d_rlist, &d_mom})
         m->release();

@@ -113,6 +113,7 @@ class Engine {
     void upload(Strucs ss);
     bool lds_fits(int lm) const;
     void build_neighbors(bool all_images);
+    void build_adjoint_table();
     template <typename T>
     void descriptors();
     void networks(bool mixed, bool vac);
@@ -244,7 +245,7 @@ class Engine {
     bool all_images_ = false;
 
     // model (owned by the first engine; workspaces alias it)
-    DevMem d_plan, d_rfe, d_rfr, d_w64, d_w32;
+    DevMem d_plan, d_rfe, d_rfr, d_w64, d_w32, d_adjc, d_adji;
     bool owns_model = true;
     // bytes of device memory held by the batch buffers
     size_t workspace_bytes() const;

@@ -157,6 +157,12 @@ struct DevPlan {
     DevRadial rad[kMaxSpecies][kMaxSpecies];
     DevAngular ang[kMaxSpecies][kMaxSpecies * (kMaxSpecies + 1) / 2];
     DevNet net[kMaxSpecies];
+    // adjoint coefficients of the moment path (device arrays, set by the engine): for centre
+    // species t and item (block, exponent e < adj_ne, power p) of the w table, kMaxC pairs
+    // (dE/dG index, coefficient) with w = sum_c dE/dG[index] coefficient (unused pairs: 0, 0.0)
+    const double* adj_coef;
+    const int* adj_index;
+    int adj_ne;
 };
 
 struct PlanBuffers {

@@ -425,12 +425,25 @@ __device__ __forceinline__ void symm_angular_moments_jvp(const DevPlan* __restri
 // Adjoint tables from dE/dG (row W): w table, Delta[s][e] (into Delta) and L[s][q][e] (into L,
 // which held the power table: the caller's barrier after the moments separates the two uses).
 // The moments are the sum of `nparts` partial tables `pstride` elements apart, added in order.
-// Threads tid of nthreads share the items; the barriers are block-wide.
-template <typename T, int NE>
+// Threads tid of nthreads share the items; the barriers are block-wide.  TAB: the w table from the
+// engine's coefficient rows (plan->adj_coef) when present.
+template <typename T, int NE, bool TAB = false>
 __device__ __forceinline__ void adjoint_moments(const DevPlan* __restrict__ plan, int ti, int ntypes, const T* W,
                                                 const T* Mom, int nparts, int pstride, T* Delta, T* wtab, T* L,
                                                 int tid, int nthreads) {
     const int nblk = ntypes * (ntypes + 1) / 2;
+    if (TAB && plan->adj_coef && plan->adj_ne == NE) {
+        // the w table from the engine's coefficient rows: one row of constants per item
+        const size_t b0 = (size_t)ti * nblk * NE * (kMomZ + 1);
+        for (int it = tid; it < nblk * NE * (kMomZ + 1); it += nthreads) {
+            const double* cf = plan->adj_coef + (b0 + it) * kMaxC;
+            const int* ix = plan->adj_index + (b0 + it) * kMaxC;
+            T acc = T(0);
+#pragma unroll
+            for (int c = 0; c < kMaxC; ++c) acc += W[ix[c]] * T(cf[c]);
+            wtab[it] = acc;
+        }
+    } else
     for (int it = tid; it < nblk * NE * (kMomZ + 1); it += nthreads) {
         const int pd = it % (kMomZ + 1), e = (it / (kMomZ + 1)) % NE, blk = it / ((kMomZ + 1) * NE);
         const DevAngular& A = plan->ang[ti][blk];

@@ -1381,7 +1381,8 @@ __global__ __launch_bounds__(kWave * NW) __attribute__((amdgpu_waves_per_eu(size
         __syncthreads();
     }
     }
-    adjoint_moments<T, NE>(plan, ti, ntypes, W, Mp, Min ? 1 : NW, ntypes * NE * kMomK, Delta, wtab, U, tid, kWave * NW);
+    adjoint_moments<T, NE, true>(plan, ti, ntypes, W, Mp, Min ? 1 : NW, ntypes * NE * kMomK, Delta, wtab, U, tid,
+                                 kWave * NW);
 
     // cutoff value and derivative of every entry, radial then angular, in the partial moments'
     // place.  Computed ahead of the entry loop: inside it, the polynomial constants of the cosine
