@@ -329,6 +329,35 @@ void Engine::build_adjoint_table() {
     pb.plan.adj_coef = (const double*)d_adjc.p;
     pb.plan.adj_index = (const int*)d_adji.p;
     pb.plan.adj_ne = ne;
+    // the angular descriptors from the moment products: G_f = norm 2^-zeta sum_p C(zeta, p)
+    // lambda^p X_p for the function f of each (block, exponent, term)
+    const size_t gitems = (size_t)T * nblk * ne * kMaxC;
+    std::vector<double> gcoef(gitems * (kMomZ + 1), 0.0);
+    std::vector<int> gindex(gitems, -1);
+    for (int t = 0; t < T; ++t) {
+        if (!P.mom_ok[t]) continue;
+        for (int blk = 0; blk < nblk; ++blk) {
+            const DevAngular& A = P.ang[t][blk];
+            for (int e = 0; e < ne && e < A.ne; ++e)
+                for (int c = 0; c < A.nc; ++c) {
+                    const int f = A.fmap[e * kMaxC + c], z = A.zint[c];
+                    if (f < 0) continue;
+                    const size_t it = (((size_t)t * nblk + blk) * ne + e) * kMaxC + c;
+                    gindex[it] = A.goff + f;
+                    double lp = 1.0;
+                    for (int p = 0; p <= z && p <= kMomZ; ++p) {
+                        gcoef[it * (kMomZ + 1) + p] = binom(z, p) * lp * A.norm * (1.0 / (double)(1 << z));
+                        lp *= A.lambda[c];
+                    }
+                }
+        }
+    }
+    d_angc.reserve(gcoef.size() * 8);
+    d_angi.reserve(gindex.size() * 4);
+    HIPCHK(hipMemcpy(d_angc.p, gcoef.data(), gcoef.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_angi.p, gindex.data(), gindex.size() * 4, hipMemcpyHostToDevice));
+    pb.plan.ang_coef = (const double*)d_angc.p;
+    pb.plan.ang_index = (const int*)d_angi.p;
 }
 
 Engine::Engine(const Engine& o, int) : host(o.host), pb(o.pb), shape(o.shape) {
@@ -369,7 +398,7 @@ Engine::~Engine() {
     d_sstart.release();
     d_work.release();
     if (owns_model)
-        for (DevMem* m : {&d_plan, &d_rfe, &d_rfr, &d_w64, &d_w32, &d_adjc, &d_adji}) m->release();
+        for (DevMem* m : {&d_plan, &d_rfe, &d_rfr, &d_w64, &d_w32, &d_adjc, &d_adji, &d_angc, &d_angi}) m->release();
     for (DevMem* m : {&d_up, &d_cnt, &d_nbrj, &d_meta, &d_nbrr, &d_maxcnt, &d_abin, &d_bcount, &d_bstart,
                       &d_bfill, &d_G, &d_dEdG, &d_X, &d_eatom, &d_F, &d_rcount, &d_rlist, &d_mom})
         m->release();

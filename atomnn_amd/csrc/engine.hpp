@@ -245,7 +245,7 @@ class Engine {
     bool all_images_ = false;
 
     // model (owned by the first engine; workspaces alias it)
-    DevMem d_plan, d_rfe, d_rfr, d_w64, d_w32, d_adjc, d_adji;
+    DevMem d_plan, d_rfe, d_rfr, d_w64, d_w32, d_adjc, d_adji, d_angc, d_angi;
     bool owns_model = true;
     // bytes of device memory held by the batch buffers
     size_t workspace_bytes() const;

@@ -163,6 +163,11 @@ struct DevPlan {
     const double* adj_coef;
     const int* adj_index;
     int adj_ne;
+    // angular descriptors from the moment sums: for centre species t and item (block, exponent
+    // e < adj_ne, term c < kMaxC), the descriptor index (-1: none) and five coefficients with
+    // G = sum_p coefficient_p X_p over the powers p of the block's moment products
+    const double* ang_coef;
+    const int* ang_index;
 };
 
 struct PlanBuffers {

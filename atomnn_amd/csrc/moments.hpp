@@ -285,6 +285,26 @@ __device__ __forceinline__ void symm_angular_moments(const DevPlan* __restrict__
     }
     __syncthreads();
     // G_f = norm 2^-zeta sum_p C(zeta,p) lambda^p X_p, X_p = S_ab or (S_aa - D_a)/2
+    if (plan->ang_coef && plan->adj_ne == NE) {  // the engine's rows: index and five coefficients
+        const size_t r0 = (size_t)ti * nblk * NE * kMaxC;
+        for (int it = lane; it < nblk * NE * kMaxC; it += kWave) {
+            const int gi = plan->ang_index[r0 + it];
+            if (gi < 0) continue;
+            const int e = (it / kMaxC) % NE, blk = it / (kMaxC * NE);
+            int a, bb;
+            blk_species(blk, a, bb);
+            const double* cf = plan->ang_coef + (r0 + it) * (kMomZ + 1);
+            T acc = T(0);
+#pragma unroll
+            for (int p = 0; p <= kMomZ; ++p) {
+                T x = S[(blk * NE + e) * (kMomZ + 1) + p];
+                if (a == bb) x = T(0.5) * (x - D[a * NE + e]);
+                acc += T(cf[p]) * x;
+            }
+            Grow[gi] = acc;
+        }
+        return;
+    }
     for (int it = lane; it < nblk * NE * kMaxC; it += kWave) {
         const int c = it % kMaxC, e = (it / kMaxC) % NE, blk = it / (kMaxC * NE);
         const DevAngular& A = plan->ang[ti][blk];
