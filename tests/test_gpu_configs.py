@@ -149,3 +149,25 @@ def test_batch_fp32_leg(potentials, pipeline):
         print(f"FP32 leg ({'fused' if pipeline else 'separate'}): worst {worst[0]:.5f} meV/atom, {worst[1]:.4f} meV/A")
     finally:
         m.close()
+
+
+@pytest.mark.parametrize("waves", [1, 2])
+def test_precision_cell_force_waves(potentials, precision_cell, waves):
+    """The moment force kernel with one or two wavefronts per centre (option force_waves), which
+    reads the moment tables the descriptor kernel stored, against the oracle: FP64 within 1e-10,
+    mixed within the gates; a model that sums its moments itself (the option moments without the
+    descriptor bit: triplet descriptors, moment forces) gives the same forces to rounding."""
+    s, ref = precision_cell
+    m = NNP.read(potentials["mixed64x4096"])
+    try:
+        m.set_option("force_waves", waves)
+        got = m.compute(s, FLAGS)
+        check_fp64(got, ref, f"C3 fp64 force_waves {waves}")
+        de, df = mixed_errors(m.compute(s, FLAGS | NNP_PREC_MIXED), ref, s.natoms)
+        assert de <= 1.0 and df <= 20.0, (waves, de, df)
+        m.set_option("moments", 2)
+        own = m.compute(s, FLAGS)
+        check_fp64(own, ref, f"C3 fp64 force_waves {waves}, moments summed in the force kernel")
+        assert np.linalg.norm(own[2] - got[2], axis=1).max() <= 1e-12 * np.linalg.norm(got[2], axis=1).max()
+    finally:
+        m.close()
