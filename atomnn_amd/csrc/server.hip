@@ -73,10 +73,13 @@ class Server {
     }
 
     // Estimated device bytes of a batch of `atoms` atoms with about `p` entries per atom:
-    // neighbour rows (40 B), force terms (32 B) and per-atom arrays.
+    // neighbour rows (40 B), force terms (32 B) and per-atom arrays (descriptor and dE/dG rows, the
+    // moment table of S NE 35 values).
     size_t estimate(size_t atoms, double p) const {
         const size_t maxp = (size_t)std::ceil(p * 1.5 + 16);
-        return atoms * (maxp * 72 + (size_t)owner_.pb.plan.ldg * 16 + 128) + (1 << 20);
+        const int ne = owner_.shape == 0 ? 2 : (owner_.shape == 1 ? 4 : 8);  // exponents of the grid shape
+        const size_t mom = (size_t)owner_.pb.plan.T * ne * 35 * 8;           // FP64 bound
+        return atoms * (maxp * 72 + (size_t)owner_.pb.plan.ldg * 16 + mom + 128) + (1 << 20);
     }
 
     int64_t submit(const nnp_structure& s, nnp_result* out) {
@@ -190,7 +193,8 @@ class Server {
             // batch from empty buffers (the device may still be short of memory)
             if (held > lane_budget_ || status == NNP_ENOMEM) {
                 (void)hipStreamSynchronize(E.stream);
-                for (DevMem* m : {&E.d_nbrj, &E.d_meta, &E.d_nbrr, &E.d_X, &E.d_G, &E.d_dEdG}) m->release();
+                for (DevMem* m : {&E.d_nbrj, &E.d_meta, &E.d_nbrr, &E.d_X, &E.d_G, &E.d_dEdG, &E.d_mom}) m->release();
+                E.mom_fresh = 0;
             }
             const double t = steady_now();
             {
